@@ -1,0 +1,194 @@
+#!/usr/bin/env python
+"""Benchmark of the implicit co-kriging hot path on one MI355X.
+
+Workload (default): the precision-form model -- one folded stack of 20,000
+interface points on 10 surfaces and 5,000 orientations (system N = 34,993),
+evaluated on a 200^3 regular grid (8,000,000 points).  One step = assemble +
+solve + surface thresholds + grid evaluation (scalar field + ids) with the
+inputs already resident in HBM.  Two solve forms are timed on the same GPU:
+FP32 Cholesky (MFMA) + FP64 GMRES refinement (the headline form) and the plain
+FP64 blocked LU (the FP64 direct form); the grid is evaluated in FP64 in both
+(FP32 pair sums cannot reach 1e-6 on this workload).
+
+Prints ONE JSON line.  `value` = grid points modelled per second, end to end,
+in the FP32+refinement form.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "model s & grid Gpts/s at N=35k: FP32+refine/learned vs FP64 direct, err≤1e-6"
+PEAK = {"fp64_valu": 78.6, "fp32_valu": 157.3, "fp32_mfma": 157.3, "fp64_mfma": 78.6, "hbm": 8000.0}
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--scale", type=float, default=1.0, help="scale of point/orientation counts")
+    ap.add_argument("--res", type=int, default=200, help="grid resolution per axis")
+    ap.add_argument("--eval-precision", choices=["f64", "f32"], default="f64")
+    ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 direct form")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    return ap.parse_args()
+
+
+def main():
+    args = parse()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: every box in this setup has one GPU and the co-kriging "
+              "path is not sharded here (see DESIGN.md for the multi-GPU design).", file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import torch
+    from gempy_amd import _lib, synthetic
+    from gempy_amd.engine import DeviceStack
+
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    dev = torch.device("cuda", 0)
+    n_pts, n_ori = int(round(20000 * args.scale)), int(round(5000 * args.scale))
+    model = synthetic.config3(n_points=n_pts, n_ori=n_ori, resolution=(args.res,) * 3)
+    st, grid, k = model.stacks[0], model.grid, model.options.kernel_options
+    center = [(grid.extent[2 * d] + grid.extent[2 * d + 1]) / 2 for d in range(3)]
+    ds = DeviceStack(st, k, center)
+    N, P = ds.N, grid.n_points
+    R, S, no = ds.sys.n_rest, ds.sys.n_surf, ds.sys.n_ori
+    Nc = 3 * no + R
+    eprec = _lib.GK_F64 if args.eval_precision == "f64" else _lib.GK_F32
+    log(f"config3 N={N} P={P} R={R} S={S} n_ori={no} eval={args.eval_precision}")
+
+    def step(form, record=None):
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+        ev[0].record()
+        w, info = ds.solve(form)
+        ev[1].record()
+        thr, _ = ds.surface_thresholds(w)
+        ev[2].record()
+        Sg, _, ids = ds.evaluate_lattice(grid, w, eprec, False, thr)
+        ev[3].record()
+        if record is not None:
+            torch.cuda.synchronize()
+            record.append(dict(solve=ev[0].elapsed_time(ev[1]), thr=ev[1].elapsed_time(ev[2]),
+                               eval=ev[2].elapsed_time(ev[3]), info=info))
+        return w, thr, Sg, ids
+
+    def run(form, name):
+        for i in range(args.warmup):
+            log(f"{name}: warmup {i + 1}/{args.warmup}")
+            step(form)
+        torch.cuda.synchronize()
+        rec = []
+        t0 = time.perf_counter()
+        for i in range(args.steps):
+            out = step(form, rec)
+            log(f"{name}: step {i + 1}/{args.steps} solve {rec[-1]['solve']:.1f} ms eval {rec[-1]['eval']:.1f} ms "
+                f"info {rec[-1]['info']}")
+        torch.cuda.synchronize()
+        t = (time.perf_counter() - t0) / args.steps
+        return t, rec, out
+
+    t_fast, rec_fast, out_fast = run(_lib.GK_SOLVE_FP32_REFINE, "fp32_refine")
+    res = {}
+    if not args.no_fp64:
+        t_f64, rec_f64, out_f64 = run(_lib.GK_SOLVE_FP64_LU, "fp64_lu")
+        s_fast, s_ref = out_fast[2].cpu().numpy(), out_f64[2].cpu().numpy()
+        rng = float(s_ref.max() - s_ref.min())
+        band = np.zeros(P, dtype=bool)
+        thr_ref = out_f64[1]
+        for t in thr_ref:
+            band |= np.abs(s_ref - t) <= 1e-6 * rng
+        ids_f, ids_r = out_fast[3].cpu().numpy(), out_f64[3].cpu().numpy()
+        res.update(model_s_fp64=t_f64, speedup_vs_fp64=t_f64 / t_fast,
+                   err_vs_fp64_max_rel=float(np.abs(s_fast - s_ref).max() / rng),
+                   err_vs_fp64_rms_rel=float(np.sqrt(np.mean((s_fast - s_ref) ** 2)) / rng),
+                   ids_mismatch_outside_band=int(((ids_f != ids_r) & ~band).sum()),
+                   ids_band_voxels=int(band.sum()),
+                   fp64_solve_ms=float(np.mean([r["solve"] for r in rec_f64])),
+                   fp64_eval_ms=float(np.mean([r["eval"] for r in rec_f64])),
+                   fp64_lu_factor_ms=float(np.mean([r["info"]["t_factor_ms"] for r in rec_f64])))
+
+    solve_ms = float(np.mean([r["solve"] for r in rec_fast]))
+    eval_ms = float(np.mean([r["eval"] for r in rec_fast]))
+    factor_ms = float(np.mean([r["info"]["t_factor_ms"] for r in rec_fast]))
+    # grid-evaluation kernel roofline: 21 flop per (point, interface/reference
+    # source) pair and 24 per (point, orientation) pair, FMA = 2.
+    flop_eval = P * (21.0 * (R + S) + 24.0 * no)
+    ach_eval = flop_eval / (eval_ms * 1e-3) / 1e12
+    peak_eval = PEAK["fp64_valu"] if eprec == _lib.GK_F64 else PEAK["fp32_valu"]
+    chol_flop = Nc ** 3 / 3.0
+    out = {
+        "metric": METRIC, "value": P / t_fast / 1e9, "unit": "Gpts/s (model: assemble+solve+evaluate)",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": t_fast * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 factor + f64 refine/eval" if eprec == _lib.GK_F64 else "f32",
+        "data": "synthetic (seeded config-3 generator)",
+        "config": {"workload": "config3 precision form: 1 stack, 10 surfaces, N=%d, grid %d^3" % (N, args.res),
+                   "N": N, "grid_points": P, "interface_points": n_pts, "orientations": n_ori},
+        "model_s": t_fast, "grid_gpts_s": P / (eval_ms * 1e-3) / 1e9, "solve_ms": solve_ms,
+        "eval_ms": eval_ms, "factor_ms": factor_ms,
+        "gmres_iterations": rec_fast[-1]["info"]["iterations"], "relres": rec_fast[-1]["info"]["relres"],
+        "fallback": rec_fast[-1]["info"]["fallback"],
+        "roofline": {"bound": "valu", "kernel": "eval_kernel (grid, lattice)",
+                     "achieved": ach_eval, "peak": peak_eval, "unit": "TFLOP/s",
+                     "frac": ach_eval / peak_eval, "traffic": None},
+        "roofline_factor": {"bound": "mfma", "kernel": "FP32 Cholesky (potrf+trsm+syrk)",
+                            "achieved": chol_flop / (factor_ms * 1e-3) / 1e12, "peak": PEAK["fp32_mfma"],
+                            "unit": "TFLOP/s", "frac": chol_flop / (factor_ms * 1e-3) / 1e12 / PEAK["fp32_mfma"]},
+    }
+    out.update(res)
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(model, out_fast[0].cpu().numpy(), args.cpu_seconds, N, P)
+    print(json.dumps(out), flush=True)
+
+
+def cpu_baseline(model, weights, budget_s, N, P):
+    """FP64 NumPy oracle on the host cores, on a bounded sample of the same
+    workload, extrapolated: solve time of a sub-sampled system scaled by
+    (N/Ns)^3 and evaluation time of a sample of grid points scaled by P/Ps."""
+    import numpy as np
+    from gempy_amd import synthetic
+    from oracle import cokriging as ok
+    cores = os.cpu_count() or 1
+    k = model.options.kernel_options
+    st = model.stacks[0]
+    # solve sample
+    sub = synthetic.config3(n_points=2400, n_ori=600, resolution=(8, 8, 8)).stacks[0]
+    t0 = time.perf_counter()
+    A, b = ok.build_system(sub.sp_xyz, sub.counts, sub.sp_nugget, sub.ori_xyz, sub.ori_grad, sub.ori_nugget,
+                           k.range, k.c_o)
+    ok.solve_system(A, b)
+    t_solve_s = time.perf_counter() - t0
+    Ns = len(b)
+    # evaluation sample with the full system's weights
+    X = model.grid.values()[:: max(1, P // 20000)][:20000]
+    t0 = time.perf_counter()
+    ok.evaluate(X[:2000], st.sp_xyz, st.counts, st.ori_xyz, weights, k.range, k.c_o)
+    t_probe = time.perf_counter() - t0
+    n_eval = int(min(len(X), max(2000, 2000 * (0.6 * budget_s) / max(t_probe, 1e-3))))
+    t0 = time.perf_counter()
+    ok.evaluate(X[:n_eval], st.sp_xyz, st.counts, st.ori_xyz, weights, k.range, k.c_o)
+    t_eval_s = time.perf_counter() - t0
+    t_model = t_solve_s * (N / Ns) ** 3 + t_eval_s * P / n_eval
+    return {"value": P / t_model / 1e9, "unit": "Gpts/s (model, extrapolated)", "cores": cores,
+            "kind": "port",
+            "sample": f"FP64 NumPy oracle: assemble+solve of a N={Ns} system ({t_solve_s:.2f} s, x(N/Ns)^3) "
+                      f"+ evaluation of {n_eval} grid points with the full N={N} weights "
+                      f"({t_eval_s:.2f} s, xP/{n_eval}); extrapolated model time {t_model:.0f} s; "
+                      f"BLAS threads = all host cores, elementwise NumPy single-threaded"}
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,776 @@
+// Tiled kernel-vector products of the cubic co-kriging interpolant.
+//
+// One kernel family evaluates S(x) = sum over sources of weight * kernel(x, source)
+// (+ drift) and optionally grad S, on a regular lattice regenerated in-kernel or
+// on explicit points.  It serves three callers:
+//   * grid / octree / custom-point evaluation (the engine's exported fields,
+//     test_example_models_I.py:20-21; chunking 2-layers.approved.txt:172 is
+//     unnecessary here: the lattice is never materialised),
+//   * the matrix-free operator y = A x (the rows of A are the gradient of the
+//     interpolant at orientation points and its increments at surface points),
+//   * surface-point thresholds.
+// Sources are staged through LDS in tiles of 256 and broadcast to the
+// wavefront; every thread owns PPT points (a z-run on the lattice, where
+// rho^2 is updated incrementally).
+#include <algorithm>
+#include <cmath>
+#include "gk_internal.h"
+
+namespace {
+
+constexpr int TS = 256;   // sources per LDS tile == threads per block
+
+template <typename T> struct alignas(sizeof(T) * 4) V4 { T x, y, z, w; };
+template <typename T> struct alignas(sizeof(T) * 4) V8 { T x, y, z, pad0, wx, wy, wz, pad1; };
+
+struct EvalArgs {
+  int64_t n_points;
+  int64_t n_runs;
+  double ox, oy, oz, hx, hy, hz;
+  int64_t nx, ny, nz, nzr;
+  const double* xyz;
+  double cx, cy, cz, inv_a;
+  const void* pts;
+  int64_t n_pts;
+  const void* ori;
+  int64_t n_ori;
+  int32_t uni_degree, n_fault;
+  const double* lam;          // drift then fault weights
+  double gi_res;
+  const double* fault_values;
+  double* scalar;
+  double* grad;
+  int32_t* ids;
+  int32_t n_thr, thr_rev;
+  double thr[GK_MAX_THRESHOLDS];
+  // source splitting across blockIdx.y (small point sets): partial sums go to
+  // part_s [nsplit][n_points] (and part_g [nsplit][n_points][3]) and a
+  // finalize kernel adds them in a fixed order.
+  int32_t nsplit;
+  int64_t pts_chunk, ori_chunk;
+  double* part_s;
+  double* part_g;
+};
+
+__device__ __forceinline__ void drift_terms(const EvalArgs& a, double X, double Y, double Z,
+                                            int64_t pidx, double& s, double* g) {
+  if (a.uni_degree >= 1) {
+    const double* l = a.lam;
+    s += a.gi_res * (l[0] * X + l[1] * Y + l[2] * Z);
+    if (g) { g[0] += a.gi_res * l[0]; g[1] += a.gi_res * l[1]; g[2] += a.gi_res * l[2]; }
+    if (a.uni_degree >= 2) {
+      s += a.gi_res * (l[3] * X * X + l[4] * Y * Y + l[5] * Z * Z + l[6] * X * Y + l[7] * X * Z +
+                       l[8] * Y * Z);
+      if (g) {
+        g[0] += a.gi_res * (2 * l[3] * X + l[6] * Y + l[7] * Z);
+        g[1] += a.gi_res * (2 * l[4] * Y + l[6] * X + l[8] * Z);
+        g[2] += a.gi_res * (2 * l[5] * Z + l[7] * X + l[8] * Y);
+      }
+    }
+  }
+  if (a.n_fault > 0 && a.fault_values) {
+    const int L = a.uni_degree == 0 ? 0 : (a.uni_degree == 1 ? 3 : 9);
+    for (int f = 0; f < a.n_fault; ++f)
+      s += a.gi_res * a.lam[L + f] * a.fault_values[pidx * a.n_fault + f];
+  }
+}
+
+__device__ __forceinline__ int32_t segment_id(const EvalArgs& a, double s) {
+  int32_t id = 1;
+  if (a.thr_rev) {
+    for (int k = 0; k < a.n_thr; ++k) id += (s > a.thr[k]) ? 1 : 0;
+  } else {
+    for (int k = 0; k < a.n_thr; ++k) id += (s < a.thr[k]) ? 1 : 0;
+  }
+  return id;
+}
+
+template <typename T, bool LATTICE, bool GRAD, bool MASKED, int PPT>
+__global__ __launch_bounds__(TS) void eval_kernel(EvalArgs a) {
+  __shared__ V8<T> s_src[TS];
+  const int tid = threadIdx.x;
+  const int64_t run = (int64_t)blockIdx.x * TS + tid;
+  const bool run_ok = run < a.n_runs;
+
+  // point coordinates (normalised frame) --------------------------------
+  T px[PPT], py[PPT], pz[PPT];
+  int64_t pidx[PPT];
+  bool pok[PPT];
+  T x0 = 0, y0 = 0, z0 = 0, hz = 0;
+  int64_t ix = 0, iy = 0, iz0 = 0;
+  if (LATTICE) {
+    int64_t r = run_ok ? run : 0;
+    iz0 = (r % a.nzr) * PPT;
+    int64_t rest = r / a.nzr;
+    iy = rest % a.ny;
+    ix = rest / a.ny;
+    x0 = (T)((a.ox + ix * a.hx - a.cx) * a.inv_a);
+    y0 = (T)((a.oy + iy * a.hy - a.cy) * a.inv_a);
+    z0 = (T)((a.oz + iz0 * a.hz - a.cz) * a.inv_a);
+    hz = (T)(a.hz * a.inv_a);
+#pragma unroll
+    for (int q = 0; q < PPT; ++q) {
+      pok[q] = run_ok && (iz0 + q) < a.nz;
+      pidx[q] = (ix * a.ny + iy) * a.nz + iz0 + q;
+    }
+  } else {
+#pragma unroll
+    for (int q = 0; q < PPT; ++q) {
+      int64_t p = (run_ok ? run : 0) * PPT + q;
+      pok[q] = run_ok && p < a.n_points;
+      pidx[q] = pok[q] ? p : 0;
+      const double* X = a.xyz + 3 * pidx[q];
+      px[q] = (T)((X[0] - a.cx) * a.inv_a);
+      py[q] = (T)((X[1] - a.cy) * a.inv_a);
+      pz[q] = (T)((X[2] - a.cz) * a.inv_a);
+    }
+  }
+
+  double acc[PPT];
+  double gacc[PPT][3];
+  T part[PPT];
+  T gpart[PPT][3];
+#pragma unroll
+  for (int q = 0; q < PPT; ++q) {
+    acc[q] = 0.0;
+    part[q] = T(0);
+    gacc[q][0] = gacc[q][1] = gacc[q][2] = 0.0;
+    gpart[q][0] = gpart[q][1] = gpart[q][2] = T(0);
+  }
+
+  // ---- interface + reference sources: weight * C(|x - p|) -----------------
+  const V4<T>* pts = reinterpret_cast<const V4<T>*>(a.pts);
+  V4<T>* s4 = reinterpret_cast<V4<T>*>(s_src);
+  const int64_t pb = (int64_t)blockIdx.y * a.pts_chunk;
+  const int64_t pe = min(a.n_pts, pb + a.pts_chunk);
+  const int64_t obeg = (int64_t)blockIdx.y * a.ori_chunk;
+  const int64_t oend = min(a.n_ori, obeg + a.ori_chunk);
+  for (int64_t base = pb; base < pe; base += TS) {
+    __syncthreads();
+    {
+      int64_t j = base + tid;
+      V4<T> v;
+      if (j < pe) v = pts[j];
+      else { v.x = T(0); v.y = T(0); v.z = T(0); v.w = T(0); }
+      s4[tid] = v;
+    }
+    __syncthreads();
+#pragma unroll 2
+    for (int k = 0; k < TS; ++k) {
+      const V4<T> s = s4[k];
+      if (LATTICE) {
+        const T dx = x0 - s.x, dy = y0 - s.y;
+        const T dxy = fma(dx, dx, dy * dy);
+        T e = z0 - s.z;
+#pragma unroll
+        for (int q = 0; q < PPT; ++q) {
+          const T t = fma(e, e, dxy);
+          const T y = gk_rsqrt(t);
+          const T rho = t * y;
+          T c = gk_cov_poly(t, rho);
+          if (MASKED) c = t < T(1) ? c : T(0);
+          part[q] = fma(s.w, c, part[q]);
+          if (GRAD) {
+            T dp = gk_dcov_poly(t, rho);
+            if (MASKED) dp = t < T(1) ? dp : T(0);
+            const T wd = s.w * dp;
+            gpart[q][0] = fma(wd, dx, gpart[q][0]);
+            gpart[q][1] = fma(wd, dy, gpart[q][1]);
+            gpart[q][2] = fma(wd, e, gpart[q][2]);
+          }
+          e += hz;
+        }
+      } else {
+#pragma unroll
+        for (int q = 0; q < PPT; ++q) {
+          const T dx = px[q] - s.x, dy = py[q] - s.y, dz = pz[q] - s.z;
+          const T t = fma(dz, dz, fma(dy, dy, dx * dx));
+          const T y = gk_rsqrt(t);
+          const T rho = t * y;
+          T c = gk_cov_poly(t, rho);
+          if (MASKED) c = t < T(1) ? c : T(0);
+          part[q] = fma(s.w, c, part[q]);
+          if (GRAD) {
+            T dp = gk_dcov_poly(t, rho);
+            if (MASKED) dp = t < T(1) ? dp : T(0);
+            const T wd = s.w * dp;
+            gpart[q][0] = fma(wd, dx, gpart[q][0]);
+            gpart[q][1] = fma(wd, dy, gpart[q][1]);
+            gpart[q][2] = fma(wd, dz, gpart[q][2]);
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < PPT; ++q) {
+      acc[q] += (double)part[q];
+      part[q] = T(0);
+      if (GRAD) {
+        for (int u = 0; u < 3; ++u) { gacc[q][u] += (double)gpart[q][u]; gpart[q][u] = T(0); }
+      }
+    }
+  }
+
+  // ---- orientation sources: (w . (q - x)) C'(r)/r --------------------------
+  const V8<T>* ori = reinterpret_cast<const V8<T>*>(a.ori);
+  for (int64_t base = obeg; base < oend; base += TS) {
+    __syncthreads();
+    {
+      int64_t j = base + tid;
+      V8<T> v;
+      if (j < oend) v = ori[j];
+      else { v.x = v.y = v.z = v.pad0 = v.wx = v.wy = v.wz = v.pad1 = T(0); }
+      s_src[tid] = v;
+    }
+    __syncthreads();
+#pragma unroll 2
+    for (int k = 0; k < TS; ++k) {
+      const V8<T> s = s_src[k];
+      if (LATTICE) {
+        const T dx = s.x - x0, dy = s.y - y0;          // q - x
+        const T dxy = fma(dx, dx, dy * dy);
+        const T base_dot = fma(s.wx, dx, s.wy * dy);
+        T e = z0 - s.z;                                // x - q (z)
+#pragma unroll
+        for (int q = 0; q < PPT; ++q) {
+          const T t = fma(e, e, dxy);
+          const T y = gk_rsqrt(t);
+          const T rho = t * y;
+          T p = gk_dcov_poly(t, rho);
+          if (MASKED) p = t < T(1) ? p : T(0);
+          const T dot = fma(-s.wz, e, base_dot);
+          part[q] = fma(dot, p, part[q]);
+          if (GRAD) {
+            const T om = T(1) - t;
+            T dd = T(26.25) * om * om * y;
+            if (MASKED) dd = t < T(1) ? dd : T(0);
+            const T hw = dot * dd;
+            gpart[q][0] -= fma(s.wx, p, hw * dx);
+            gpart[q][1] -= fma(s.wy, p, hw * dy);
+            gpart[q][2] -= fma(s.wz, p, -hw * e);
+          }
+          e += hz;
+        }
+      } else {
+#pragma unroll
+        for (int q = 0; q < PPT; ++q) {
+          const T dx = s.x - px[q], dy = s.y - py[q], dz = s.z - pz[q];   // q - x
+          const T t = fma(dz, dz, fma(dy, dy, dx * dx));
+          const T y = gk_rsqrt(t);
+          const T rho = t * y;
+          T p = gk_dcov_poly(t, rho);
+          if (MASKED) p = t < T(1) ? p : T(0);
+          const T dot = fma(s.wz, dz, fma(s.wy, dy, s.wx * dx));
+          part[q] = fma(dot, p, part[q]);
+          if (GRAD) {
+            const T om = T(1) - t;
+            T dd = T(26.25) * om * om * y;
+            if (MASKED) dd = t < T(1) ? dd : T(0);
+            const T hw = dot * dd;
+            gpart[q][0] -= fma(s.wx, p, hw * dx);
+            gpart[q][1] -= fma(s.wy, p, hw * dy);
+            gpart[q][2] -= fma(s.wz, p, hw * dz);
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < PPT; ++q) {
+      acc[q] += (double)part[q];
+      part[q] = T(0);
+      if (GRAD) {
+        for (int u = 0; u < 3; ++u) { gacc[q][u] += (double)gpart[q][u]; gpart[q][u] = T(0); }
+      }
+    }
+  }
+
+  // ---- drift (FP64, raw coordinates), ids, store -----------------------------
+  if (a.nsplit > 1) {
+#pragma unroll
+    for (int q = 0; q < PPT; ++q) {
+      if (!pok[q]) continue;
+      const int64_t o = (int64_t)blockIdx.y * a.n_points + pidx[q];
+      a.part_s[o] = acc[q];
+      if (GRAD && a.part_g) {
+        a.part_g[3 * o + 0] = gacc[q][0];
+        a.part_g[3 * o + 1] = gacc[q][1];
+        a.part_g[3 * o + 2] = gacc[q][2];
+      }
+    }
+    return;
+  }
+#pragma unroll
+  for (int q = 0; q < PPT; ++q) {
+    if (!pok[q]) continue;
+    double X, Y, Z;
+    if (LATTICE) {
+      X = a.ox + ix * a.hx; Y = a.oy + iy * a.hy; Z = a.oz + (iz0 + q) * a.hz;
+    } else {
+      const double* P = a.xyz + 3 * pidx[q];
+      X = P[0]; Y = P[1]; Z = P[2];
+    }
+    double s = acc[q];
+    double g[3] = {gacc[q][0] * a.inv_a, gacc[q][1] * a.inv_a, gacc[q][2] * a.inv_a};
+    drift_terms(a, X, Y, Z, pidx[q], s, GRAD ? g : nullptr);
+    a.scalar[pidx[q]] = s;
+    if (GRAD && a.grad) {
+      a.grad[3 * pidx[q] + 0] = g[0];
+      a.grad[3 * pidx[q] + 1] = g[1];
+      a.grad[3 * pidx[q] + 2] = g[2];
+    }
+    if (a.ids) a.ids[pidx[q]] = segment_id(a, s);
+  }
+}
+
+// Sum the per-chunk partials (fixed order), add drift, segment, store.
+__global__ void eval_finalize_kernel(EvalArgs a, int grad) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= a.n_points) return;
+  double s = 0.0, g[3] = {0.0, 0.0, 0.0};
+  for (int y = 0; y < a.nsplit; ++y) {
+    const int64_t o = (int64_t)y * a.n_points + p;
+    s += a.part_s[o];
+    if (grad) { g[0] += a.part_g[3 * o]; g[1] += a.part_g[3 * o + 1]; g[2] += a.part_g[3 * o + 2]; }
+  }
+  g[0] *= a.inv_a; g[1] *= a.inv_a; g[2] *= a.inv_a;
+  const double* P = a.xyz + 3 * p;
+  drift_terms(a, P[0], P[1], P[2], p, s, grad ? g : nullptr);
+  a.scalar[p] = s;
+  if (grad && a.grad) { a.grad[3 * p] = g[0]; a.grad[3 * p + 1] = g[1]; a.grad[3 * p + 2] = g[2]; }
+  if (a.ids) a.ids[p] = segment_id(a, s);
+}
+
+// ---------------------------------------------------------------- source prep
+// Interface sources: rest points with weight i_res c_o w_i, reference points
+// with weight -i_res c_o W_s (W_s = sum of w_i over the surface): the
+// rest-minus-reference differencing of the interface covariance collapses
+// into per-surface reference sources.
+__global__ void surface_sums_kernel(const double* wI, const int32_t* surf, int64_t R, double* W) {
+  __shared__ double red[256];
+  const int s = blockIdx.x;
+  double acc = 0.0;
+  for (int64_t i = threadIdx.x; i < R; i += 256)
+    if (surf[i] == s) acc += wI[i];
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) W[s] = red[0];
+}
+
+template <typename T>
+__global__ void prep_pts_kernel(gk_system sys, const double* w, const double* W, V4<T>* out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t R = sys.n_rest, S = sys.n_surf;
+  if (i >= R + S) return;
+  const double inv_a = 1.0 / sys.range;
+  const double* p;
+  double wt;
+  if (i < R) { p = sys.rest_xyz + 3 * i; wt = sys.i_res * sys.c_o * w[3 * sys.n_ori + i]; }
+  else { p = sys.ref_xyz + 3 * (i - R); wt = -sys.i_res * sys.c_o * W[i - R]; }
+  V4<T> v;
+  v.x = (T)((p[0] - sys.center[0]) * inv_a);
+  v.y = (T)((p[1] - sys.center[1]) * inv_a);
+  v.z = (T)((p[2] - sys.center[2]) * inv_a);
+  v.w = (T)wt;
+  out[i] = v;
+}
+
+template <typename T>
+__global__ void prep_ori_kernel(gk_system sys, const double* w, V8<T>* out) {
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t n = sys.n_ori;
+  if (g >= n) return;
+  const double inv_a = 1.0 / sys.range;
+  const double f = sys.gi_res * sys.c_o * inv_a;
+  const double* q = sys.ori_xyz + 3 * g;
+  V8<T> v;
+  v.x = (T)((q[0] - sys.center[0]) * inv_a);
+  v.y = (T)((q[1] - sys.center[1]) * inv_a);
+  v.z = (T)((q[2] - sys.center[2]) * inv_a);
+  v.pad0 = T(0);
+  v.wx = (T)(f * w[g]);
+  v.wy = (T)(f * w[n + g]);
+  v.wz = (T)(f * w[2 * n + g]);
+  v.pad1 = T(0);
+  out[g] = v;
+}
+
+int n_drift_of(int deg) { return deg == 0 ? 0 : (deg == 1 ? 3 : 9); }
+
+template <typename T, bool LATTICE, bool GRAD, bool MASKED>
+void launch_eval(const EvalArgs& a, int ppt_runs_div, hipStream_t st) {
+  (void)ppt_runs_div;
+  constexpr int PPT = LATTICE ? (sizeof(T) == 4 ? 8 : 4) : (sizeof(T) == 4 ? 4 : 2);
+  EvalArgs b = a;
+  if (LATTICE) {
+    b.nzr = gk_ceil_div(a.nz, PPT);
+    b.n_runs = a.nx * a.ny * b.nzr;
+  } else {
+    b.n_runs = gk_ceil_div(a.n_points, PPT);
+  }
+  const int64_t blocks = gk_ceil_div(b.n_runs, TS);
+  if (blocks == 0) return;
+  hipLaunchKernelGGL((eval_kernel<T, LATTICE, GRAD, MASKED, PPT>), dim3((unsigned)blocks, (unsigned)b.nsplit),
+                     dim3(TS), 0, st, b);
+  if (b.nsplit > 1)
+    hipLaunchKernelGGL(eval_finalize_kernel, dim3((unsigned)gk_ceil_div(b.n_points, 256)), dim3(256), 0, st,
+                       b, GRAD ? 1 : 0);
+}
+
+template <typename T>
+void dispatch_eval(const EvalArgs& a, bool lattice, bool grad, bool masked, hipStream_t st) {
+  if (lattice) {
+    if (grad) { if (masked) launch_eval<T, true, true, true>(a, 0, st); else launch_eval<T, true, true, false>(a, 0, st); }
+    else { if (masked) launch_eval<T, true, false, true>(a, 0, st); else launch_eval<T, true, false, false>(a, 0, st); }
+  } else {
+    if (grad) { if (masked) launch_eval<T, false, true, true>(a, 0, st); else launch_eval<T, false, true, false>(a, 0, st); }
+    else { if (masked) launch_eval<T, false, false, true>(a, 0, st); else launch_eval<T, false, false, false>(a, 0, st); }
+  }
+}
+
+}  // namespace
+
+// Prepare the packed sources for `weights` in the handle workspace.
+// Layout: [W_s (n_surf doubles, 256-B aligned) | pts V4<T> (R+S) | ori V8<T> (n_ori)].
+static int prepare_sources(gk_handle h, const gk_system* sys, const double* weights, int precision,
+                           hipStream_t st, const void** pts, const void** ori, void* ws) {
+  const int64_t R = sys->n_rest, S = sys->n_surf, no = sys->n_ori;
+  const size_t tsz = precision == GK_F64 ? 8 : 4;
+  char* base = static_cast<char*>(ws);
+  double* W = reinterpret_cast<double*>(base);
+  size_t off = gk_round_up(S * 8, 256);
+  void* p4 = base + off;
+  off += gk_round_up((R + S) * 4 * tsz, 256);
+  void* p8 = base + off;
+  (void)h;
+  if (S > 0)
+    hipLaunchKernelGGL(surface_sums_kernel, dim3((unsigned)S), dim3(256), 0, st,
+                       weights + 3 * no, sys->rest_surf, R, W);
+  if (R + S > 0) {
+    unsigned nb = (unsigned)gk_ceil_div(R + S, 256);
+    if (precision == GK_F64)
+      hipLaunchKernelGGL(prep_pts_kernel<double>, dim3(nb), dim3(256), 0, st, *sys, weights, W,
+                         reinterpret_cast<V4<double>*>(p4));
+    else
+      hipLaunchKernelGGL(prep_pts_kernel<float>, dim3(nb), dim3(256), 0, st, *sys, weights, W,
+                         reinterpret_cast<V4<float>*>(p4));
+  }
+  if (no > 0) {
+    unsigned nb = (unsigned)gk_ceil_div(no, 256);
+    if (precision == GK_F64)
+      hipLaunchKernelGGL(prep_ori_kernel<double>, dim3(nb), dim3(256), 0, st, *sys, weights,
+                         reinterpret_cast<V8<double>*>(p8));
+    else
+      hipLaunchKernelGGL(prep_ori_kernel<float>, dim3(nb), dim3(256), 0, st, *sys, weights,
+                         reinterpret_cast<V8<float>*>(p8));
+  }
+  *pts = p4;
+  *ori = p8;
+  GK_HIP(hipGetLastError());
+  return GK_OK;
+}
+
+static size_t sources_bytes(const gk_system* sys) {
+  const int64_t R = sys->n_rest, S = sys->n_surf, no = sys->n_ori;
+  return gk_round_up(S * 8, 256) + gk_round_up((R + S) * 32, 256) + gk_round_up(no * 64, 256) + 256;
+}
+
+static int check_system(const gk_system* sys) {
+  GK_REQUIRE(sys != nullptr, GK_ERR_ARG, "gk: system is NULL");
+  GK_REQUIRE(sys->n_ori >= 0 && sys->n_rest >= 0 && sys->n_surf >= 0, GK_ERR_ARG,
+             "gk: negative sizes in system");
+  GK_REQUIRE(sys->uni_degree >= 0 && sys->uni_degree <= 2, GK_ERR_ARG,
+             "gk: uni_degree must be 0, 1 or 2 (got %d)", sys->uni_degree);
+  GK_REQUIRE(sys->n_fault >= 0 && sys->n_fault <= 8, GK_ERR_ARG, "gk: n_fault must be 0..8");
+  GK_REQUIRE(sys->range > 0 && sys->c_o > 0, GK_ERR_ARG, "gk: range and c_o must be positive");
+  GK_REQUIRE(sys->n_ori == 0 || (sys->ori_xyz && sys->ori_grad && sys->ori_nugget), GK_ERR_ARG,
+             "gk: orientation arrays missing");
+  GK_REQUIRE(sys->n_rest == 0 || (sys->rest_xyz && sys->rest_surf && sys->rest_nugget), GK_ERR_ARG,
+             "gk: rest-point arrays missing");
+  GK_REQUIRE(sys->n_surf == 0 || sys->ref_xyz, GK_ERR_ARG, "gk: reference points missing");
+  GK_REQUIRE(sys->n_fault == 0 || sys->fault_rest, GK_ERR_ARG, "gk: fault_rest missing");
+  return GK_OK;
+}
+
+// Largest distance between the sources and a box, in range units: when it is
+// below 1 the cubic covariance needs no r < a mask (its polynomial is exact).
+static bool needs_mask(const gk_system* sys, const double* lo, const double* hi, double src_lo[3],
+                       double src_hi[3]) {
+  double d2 = 0.0;
+  for (int k = 0; k < 3; ++k) {
+    double a = std::max(std::fabs(hi[k] - src_lo[k]), std::fabs(src_hi[k] - lo[k]));
+    d2 += a * a;
+  }
+  return d2 >= (1.0 - 1e-9) * sys->range * sys->range;
+}
+
+extern "C" int gk_eval_impl(gk_handle h, const gk_system* sys, const double* weights,
+                            gk_eval_desc* d, hipStream_t st, bool force_mask,
+                            const double* src_box /*6 or null*/) {
+  GK_TRY(check_system(sys));
+  GK_REQUIRE(d && weights && d->scalar, GK_ERR_ARG, "gk_eval: desc, weights and scalar required");
+  GK_REQUIRE(d->precision == GK_F32 || d->precision == GK_F64, GK_ERR_ARG, "gk_eval: bad precision");
+  GK_REQUIRE(d->n_thresholds >= 0 && d->n_thresholds <= GK_MAX_THRESHOLDS, GK_ERR_ARG,
+             "gk_eval: n_thresholds must be 0..%d", GK_MAX_THRESHOLDS);
+  GK_REQUIRE(!d->ids || d->n_thresholds > 0, GK_ERR_ARG, "gk_eval: ids need thresholds");
+  GK_REQUIRE(sys->n_fault == 0 || d->fault_values, GK_ERR_ARG,
+             "gk_eval: fault_values required when the system has fault columns");
+  EvalArgs a{};
+  double lo[3], hi[3];
+  if (d->kind == GK_POINTS_LATTICE) {
+    GK_REQUIRE(d->res[0] > 0 && d->res[1] > 0 && d->res[2] > 0, GK_ERR_ARG, "gk_eval: bad lattice");
+    a.nx = d->res[0]; a.ny = d->res[1]; a.nz = d->res[2];
+    a.ox = d->origin[0]; a.oy = d->origin[1]; a.oz = d->origin[2];
+    a.hx = d->spacing[0]; a.hy = d->spacing[1]; a.hz = d->spacing[2];
+    d->n_points = a.nx * a.ny * a.nz;
+    for (int k = 0; k < 3; ++k) {
+      lo[k] = d->origin[k];
+      hi[k] = d->origin[k] + (d->res[k] - 1) * d->spacing[k];
+    }
+  } else {
+    GK_REQUIRE(d->kind == GK_POINTS_EXPLICIT, GK_ERR_ARG, "gk_eval: bad kind");
+    GK_REQUIRE(d->n_points >= 0 && (d->n_points == 0 || d->xyz), GK_ERR_ARG, "gk_eval: points missing");
+    for (int k = 0; k < 3; ++k) { lo[k] = -1e300; hi[k] = 1e300; }
+  }
+  a.n_points = d->n_points;
+  if (a.n_points == 0) return GK_OK;
+  a.xyz = d->xyz;
+  a.cx = sys->center[0]; a.cy = sys->center[1]; a.cz = sys->center[2];
+  a.inv_a = 1.0 / sys->range;
+  a.n_pts = sys->n_rest + sys->n_surf;
+  a.n_ori = sys->n_ori;
+  a.uni_degree = sys->uni_degree;
+  a.n_fault = sys->n_fault;
+  a.lam = weights + 3 * sys->n_ori + sys->n_rest;
+  a.gi_res = sys->gi_res;
+  a.fault_values = d->fault_values;
+  a.scalar = d->scalar;
+  a.grad = d->gradient;
+  a.ids = d->ids;
+  a.n_thr = d->n_thresholds;
+  a.thr_rev = (d->n_thresholds >= 2 && d->thresholds[0] < d->thresholds[d->n_thresholds - 1]) ? 1 : 0;
+  for (int k = 0; k < d->n_thresholds; ++k) a.thr[k] = d->thresholds[k];
+
+  void* ws = nullptr;
+  GK_TRY(gk_workspace(h, WS_EVAL_SRC, sources_bytes(sys), &ws));
+  GK_TRY(prepare_sources(h, sys, weights, d->precision, st, &a.pts, &a.ori, ws));
+
+  // Split the sources over blockIdx.y when the points alone cannot fill the GPU.
+  a.nsplit = 1;
+  a.pts_chunk = std::max<int64_t>(a.n_pts, 1);
+  a.ori_chunk = std::max<int64_t>(a.n_ori, 1);
+  if (d->kind == GK_POINTS_EXPLICIT) {
+    const int ppt = d->precision == GK_F64 ? 2 : 4;
+    const int64_t blocks = gk_ceil_div(gk_ceil_div(a.n_points, ppt), TS);
+    const int64_t tiles = std::max(gk_ceil_div(a.n_pts, TS), gk_ceil_div(a.n_ori, TS));
+    int64_t want = std::min<int64_t>(std::min<int64_t>(gk_ceil_div(2048, blocks), 64), tiles);
+    if (want > 1) {
+      a.pts_chunk = gk_round_up(gk_ceil_div(a.n_pts, want), TS);
+      a.ori_chunk = gk_round_up(gk_ceil_div(a.n_ori, want), TS);
+      a.nsplit = (int32_t)std::max(gk_ceil_div(a.n_pts, a.pts_chunk), gk_ceil_div(a.n_ori, a.ori_chunk));
+      if (a.nsplit > 1) {
+        void* pw = nullptr;
+        const size_t per = (size_t)a.n_points * (d->gradient ? 4 : 1);
+        GK_TRY(gk_workspace(h, WS_EVAL_PART, per * a.nsplit * sizeof(double) + 512, &pw));
+        a.part_s = static_cast<double*>(pw);
+        a.part_g = d->gradient ? a.part_s + (size_t)a.nsplit * a.n_points : nullptr;
+      } else {
+        a.pts_chunk = std::max<int64_t>(a.n_pts, 1);
+        a.ori_chunk = std::max<int64_t>(a.n_ori, 1);
+      }
+    }
+  }
+
+  bool masked = true;
+  if (!force_mask && d->kind == GK_POINTS_LATTICE && src_box) {
+    double sl[3] = {src_box[0], src_box[1], src_box[2]}, sh[3] = {src_box[3], src_box[4], src_box[5]};
+    masked = needs_mask(sys, lo, hi, sl, sh);
+  }
+  const bool grad = d->gradient != nullptr;
+  if (d->precision == GK_F64) dispatch_eval<double>(a, d->kind == GK_POINTS_LATTICE, grad, masked, st);
+  else dispatch_eval<float>(a, d->kind == GK_POINTS_LATTICE, grad, masked, st);
+  GK_HIP(hipGetLastError());
+  return GK_OK;
+}
+
+// ------------------------------------------------------------------ matvec
+namespace {
+// y rows from the evaluated field at the data points.
+__global__ void matvec_combine_kernel(gk_system sys, const double* x, const double* s_data,
+                                      const double* g_ori, double* y) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t no = sys.n_ori, R = sys.n_rest;
+  const int64_t NG = 3 * no;
+  if (i < NG) {
+    const int64_t u = i / no, g = i % no;
+    y[i] = g_ori[3 * g + u] / sys.gi_res + sys.ori_nugget[g] * x[i];
+  } else if (i < NG + R) {
+    const int64_t r = i - NG;
+    double v = s_data[r] - s_data[R + sys.rest_surf[r]] + sys.rest_nugget[r] * x[i];
+    y[i] = v;
+  }
+}
+
+// Drift / fault rows: deterministic block reductions, one block per row.
+__global__ void matvec_drift_rows_kernel(gk_system sys, const double* x, double* y) {
+  __shared__ double red[256];
+  const int l = blockIdx.x;                  // 0..L+F-1
+  const int64_t no = sys.n_ori, R = sys.n_rest, NG = 3 * no;
+  const int L = sys.uni_degree == 0 ? 0 : (sys.uni_degree == 1 ? 3 : 9);
+  double acc = 0.0;
+  for (int64_t i = threadIdx.x; i < NG + R; i += 256) {
+    double v = 0.0;
+    if (l < L) {
+      if (i < NG) {
+        const int64_t u = i / no, g = i % no;
+        const double* q = sys.ori_xyz + 3 * g;
+        double df;
+        if (l < 3) df = (l == u) ? 1.0 : 0.0;
+        else if (l < 6) df = (l - 3 == u) ? 2.0 * q[u] : 0.0;
+        else if (l == 6) df = (u == 0) ? q[1] : (u == 1 ? q[0] : 0.0);
+        else if (l == 7) df = (u == 0) ? q[2] : (u == 2 ? q[0] : 0.0);
+        else df = (u == 1) ? q[2] : (u == 2 ? q[1] : 0.0);
+        v = df * x[i];
+      } else {
+        const int64_t r = i - NG;
+        const double* p = sys.rest_xyz + 3 * r;
+        const double* p0 = sys.ref_xyz + 3 * sys.rest_surf[r];
+        double f1, f0;
+        if (l < 3) { f1 = p[l]; f0 = p0[l]; }
+        else if (l < 6) { f1 = p[l - 3] * p[l - 3]; f0 = p0[l - 3] * p0[l - 3]; }
+        else if (l == 6) { f1 = p[0] * p[1]; f0 = p0[0] * p0[1]; }
+        else if (l == 7) { f1 = p[0] * p[2]; f0 = p0[0] * p0[2]; }
+        else { f1 = p[1] * p[2]; f0 = p0[1] * p0[2]; }
+        v = sys.gi_res * (f1 - f0) * x[i];
+      }
+    } else if (i >= NG) {
+      const int64_t r = i - NG;
+      v = sys.gi_res * sys.fault_rest[r * sys.n_fault + (l - L)] * x[i];
+    }
+    acc += v;
+  }
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) y[NG + R + l] = red[0];
+}
+
+// Fault-drift contribution to the interface rows (not part of the field eval).
+__global__ void matvec_fault_rows_kernel(gk_system sys, const double* x, double* y) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= sys.n_rest) return;
+  const int L = sys.uni_degree == 0 ? 0 : (sys.uni_degree == 1 ? 3 : 9);
+  const int64_t NG = 3 * sys.n_ori, R = sys.n_rest;
+  double v = 0.0;
+  for (int f = 0; f < sys.n_fault; ++f)
+    v += sys.gi_res * sys.fault_rest[r * sys.n_fault + f] * x[NG + R + L + f];
+  y[NG + r] += v;
+}
+
+__global__ void gather_data_points_kernel(gk_system sys, double* pts) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t R = sys.n_rest, S = sys.n_surf;
+  if (i >= R + S) return;
+  const double* p = i < R ? sys.rest_xyz + 3 * i : sys.ref_xyz + 3 * (i - R);
+  pts[3 * i] = p[0]; pts[3 * i + 1] = p[1]; pts[3 * i + 2] = p[2];
+}
+}  // namespace
+
+extern "C" int gk_matvec_impl(gk_handle h, const gk_system* sys, int32_t precision, const double* x,
+                              double* y, hipStream_t st, void* scratch) {
+  // scratch: >= (R+S)*3 + (R+S) + n_ori*4 doubles
+  const int64_t R = sys->n_rest, S = sys->n_surf, no = sys->n_ori;
+  double* pts = static_cast<double*>(scratch);
+  double* s_data = pts + 3 * (R + S);
+  double* s_ori = s_data + (R + S);
+  double* g_ori = s_ori + no;
+  if (R + S > 0)
+    hipLaunchKernelGGL(gather_data_points_kernel, dim3((unsigned)gk_ceil_div(R + S, 256)), dim3(256),
+                       0, st, *sys, pts);
+  gk_system nofault = *sys;
+  nofault.n_fault = 0;  // fault drift enters the interface rows separately
+  if (R + S > 0) {
+    gk_eval_desc d{};
+    d.kind = GK_POINTS_EXPLICIT;
+    d.precision = precision;
+    d.n_points = R + S;
+    d.xyz = pts;
+    d.scalar = s_data;
+    GK_TRY(gk_eval_impl(h, &nofault, x, &d, st, true, nullptr));
+  }
+  if (no > 0) {
+    gk_eval_desc d{};
+    d.kind = GK_POINTS_EXPLICIT;
+    d.precision = precision;
+    d.n_points = no;
+    d.xyz = sys->ori_xyz;
+    d.scalar = s_ori;
+    d.gradient = g_ori;
+    GK_TRY(gk_eval_impl(h, &nofault, x, &d, st, true, nullptr));
+  }
+  const int64_t NG = 3 * no;
+  if (NG + R > 0)
+    hipLaunchKernelGGL(matvec_combine_kernel, dim3((unsigned)gk_ceil_div(NG + R, 256)), dim3(256), 0,
+                       st, *sys, x, s_data, g_ori, y);
+  if (sys->n_fault > 0 && R > 0)
+    hipLaunchKernelGGL(matvec_fault_rows_kernel, dim3((unsigned)gk_ceil_div(R, 256)), dim3(256), 0,
+                       st, *sys, x, y);
+  const int L = n_drift_of(sys->uni_degree);
+  if (L + sys->n_fault > 0)
+    hipLaunchKernelGGL(matvec_drift_rows_kernel, dim3(L + sys->n_fault), dim3(256), 0, st, *sys, x, y);
+  GK_HIP(hipGetLastError());
+  return GK_OK;
+}
+
+extern "C" size_t gk_matvec_scratch_doubles(const gk_system* sys) {
+  return (size_t)(4 * (sys->n_rest + sys->n_surf) + 4 * sys->n_ori + 64);
+}
+
+// ------------------------------------------------------------ stack combination
+namespace {
+// lith = id of the youngest non-fault stack whose domain (above its last
+// surface, id <= n_surf) holds the point, offset to global element ids;
+// basement otherwise.  ids: [n_stacks][n] int32 (stack order youngest first).
+__global__ void combine_lith_kernel(const int32_t* ids, int32_t n_stacks, int64_t n, const int32_t* nsurf,
+                                    const int32_t* offset, int32_t basement, int32_t* out) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  int32_t v = basement;
+  for (int s = 0; s < n_stacks; ++s) {
+    const int32_t id = ids[(int64_t)s * n + p];
+    if (id <= nsurf[s]) { v = id + offset[s]; break; }
+  }
+  out[p] = v;
+}
+}  // namespace
+
+extern "C" int gk_combine_lith(gk_handle h, int32_t n_stacks, const int32_t* ids, int64_t n,
+                               const int32_t* n_surf, int32_t* lith, gk_stream stream) {
+  GK_REQUIRE(h && ids && lith && n_surf && n_stacks > 0 && n_stacks <= 64 && n >= 0, GK_ERR_ARG,
+             "gk_combine_lith: bad arguments");
+  hipStream_t st = (hipStream_t)stream;
+  int32_t hbuf[128];
+  int32_t off = 0;
+  for (int s = 0; s < n_stacks; ++s) {
+    GK_REQUIRE(n_surf[s] > 0, GK_ERR_ARG, "gk_combine_lith: stack %d has no surface", s);
+    hbuf[s] = n_surf[s];
+    hbuf[64 + s] = off;
+    off += n_surf[s];
+  }
+  void* ws = nullptr;
+  GK_TRY(gk_workspace(h, WS_MATRIX2, 128 * sizeof(int32_t), &ws));
+  int32_t* d = static_cast<int32_t*>(ws);
+  GK_HIP(hipMemcpyAsync(d, hbuf, sizeof(hbuf), hipMemcpyHostToDevice, st));
+  if (n > 0)
+    hipLaunchKernelGGL(combine_lith_kernel, dim3((unsigned)gk_ceil_div(n, 256)), dim3(256), 0, st, ids,
+                       n_stacks, n, d, d + 64, off + 1, lith);
+  GK_HIP(hipGetLastError());
+  GK_HIP(hipStreamSynchronize(st));   // hbuf is a stack buffer
+  return GK_OK;
+}

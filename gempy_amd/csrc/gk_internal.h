@@ -1,0 +1,90 @@
+// Internal definitions shared by the gk HIP translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <cstdio>
+#include <cstdarg>
+#include <vector>
+#include "../../include/gk.h"
+
+void gk_set_error(const char* fmt, ...);
+
+#define GK_HIP(x)                                                                  \
+  do {                                                                             \
+    hipError_t e_ = (x);                                                           \
+    if (e_ != hipSuccess) {                                                        \
+      gk_set_error("%s:%d %s -> %s", __FILE__, __LINE__, #x, hipGetErrorString(e_)); \
+      return GK_ERR_HIP;                                                           \
+    }                                                                              \
+  } while (0)
+
+#define GK_REQUIRE(cond, code, ...)                                                \
+  do {                                                                             \
+    if (!(cond)) {                                                                 \
+      gk_set_error(__VA_ARGS__);                                                   \
+      return code;                                                                 \
+    }                                                                              \
+  } while (0)
+
+#define GK_TRY(x)                                                                  \
+  do {                                                                             \
+    int rc_ = (x);                                                                 \
+    if (rc_ != GK_OK) return rc_;                                                  \
+  } while (0)
+
+// Per-device context: growable device workspaces owned by the handle, one per
+// purpose so that nested calls (the solver calling the matvec calling the
+// evaluator) never alias each other's buffers.
+enum gk_ws_slot {
+  WS_EVAL_SRC = 0,   // packed sources of one evaluation
+  WS_EVAL_PART,      // per-chunk partial sums of a split evaluation
+  WS_MATVEC,         // data points and field values of the matrix-free operator
+  WS_FACTOR,         // panel buffers of a factorization
+  WS_SOLVER,         // Krylov basis, preconditioner blocks
+  WS_MATRIX,         // assembled matrices of gk_solve
+  WS_MATRIX2,
+  WS_COUNT
+};
+struct gk_context {
+  int device = 0;
+  void* ws[WS_COUNT] = {};
+  size_t ws_bytes[WS_COUNT] = {};
+};
+
+int gk_workspace(gk_handle h, int slot, size_t bytes, void** out);
+
+static inline int64_t gk_ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
+static inline int64_t gk_round_up(int64_t a, int64_t b) { return gk_ceil_div(a, b) * b; }
+
+// ------------------------------------------------------------------ covariance
+// All device code works in range-normalised coordinates (x / a), so the cubic
+// covariance is a polynomial in t = rho^2 and rho.
+//   C/c_o            = 1 - 7 t + rho t (35/4 - 7/2 t + 3/4 t^2)
+//   (C'/r) a^2/c_o   = -14 + rho (105/4 + t (-35/2 + 21/4 t))
+//   (C''-C'/r) a^2/c_o = 105/4 rho (1 - t)^2
+template <typename T>
+__device__ __forceinline__ T gk_cov_poly(T t, T rho) {
+  T q = fma(t, T(0.75), T(-3.5));
+  q = fma(q, t, T(8.75));
+  q = fma(q, rho, T(-7.0));
+  return fma(q, t, T(1.0));
+}
+
+template <typename T>
+__device__ __forceinline__ T gk_dcov_poly(T t, T rho) {
+  T p = fma(t, T(5.25), T(-17.5));
+  p = fma(p, t, T(26.25));
+  return fma(p, rho, T(-14.0));
+}
+
+// 1/sqrt(t) to working precision; t is clamped away from 0 (rho = t * y = 0 there).
+__device__ __forceinline__ float gk_rsqrt(float t) {
+  return __builtin_amdgcn_rsqf(fmaxf(t, 1e-30f));
+}
+__device__ __forceinline__ double gk_rsqrt(double t) {
+  t = fmax(t, 1e-280);
+  double y = __builtin_amdgcn_rsq(t);
+  // one Newton step: y <- y (3 - t y^2) / 2  (relative error squares)
+  double r = fma(-t * y, y, 1.0);
+  return fma(0.5 * y, r, y);
+}

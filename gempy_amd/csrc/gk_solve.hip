@@ -1,0 +1,486 @@
+// Solve of one stack's co-kriging system (the engine's kernel_solver).
+//
+// GK_SOLVE_FP64_LU    : FP64 assembly + blocked LU (FP64 MFMA) + triangular solves.
+// GK_SOLVE_FP32_REFINE: FP32 equilibrated Cholesky (FP32 MFMA) of the SPD
+//   covariance block, exact Schur complement on the drift block, used as the
+//   right preconditioner of FP64 restarted flexible GMRES whose matvec is the
+//   matrix-free FP64 operator (kernel tiles recomputed, no FP64 matrix).  The
+//   iteration stops on the true FP64 residual (tolerance reached, or no more
+//   progress at the FP64 rounding floor); if it is still above `accept` the
+//   solve falls back to the FP64 LU.
+#include <cmath>
+#include <vector>
+#include <algorithm>
+#include "gk_internal.h"
+
+extern "C" int gk_assemble_impl(gk_handle h, const gk_system* sys, int32_t dtype, void* A, int64_t lda,
+                                double* rhs, double* dscale, hipStream_t st, int lower_only);
+extern "C" int gk_assemble_drift_impl(const gk_system* sys, double* U, hipStream_t st);
+extern "C" int gk_rhs_impl(const gk_system* sys, double* b, hipStream_t st);
+extern "C" int gk_matvec(gk_handle h, const gk_system* sys, int32_t precision, const double* x, double* y,
+                         gk_stream stream);
+
+namespace {
+
+constexpr int RED_ROWS = 4096;   // rows per block of the column-dot kernels
+
+__global__ void dots_partial_kernel(const double* V, int64_t ldv, int ncols, const double* w, int64_t n,
+                                    double* partial) {
+  __shared__ double red[256];
+  const int64_t r0 = (int64_t)blockIdx.x * RED_ROWS;
+  const int64_t r1 = std::min<int64_t>(n, r0 + RED_ROWS);
+  for (int c = 0; c < ncols; ++c) {
+    double acc = 0.0;
+    const double* v = V + (int64_t)c * ldv;
+    for (int64_t i = r0 + threadIdx.x; i < r1; i += 256) acc = fma(v[i], w[i], acc);
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+      if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[(int64_t)blockIdx.x * ncols + c] = red[0];
+    __syncthreads();
+  }
+}
+
+__global__ void reduce_cols_kernel(const double* partial, int nblk, int ncols, double* out) {
+  const int c = threadIdx.x;
+  if (c >= ncols) return;
+  double s = 0.0;
+  for (int b = 0; b < nblk; ++b) s += partial[(int64_t)b * ncols + c];
+  out[c] = s;
+}
+
+// w -= V h   (V columns of length n, h on device)
+__global__ void gemv_sub_kernel(const double* V, int64_t ldv, int ncols, const double* h, double* w,
+                                int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double s = w[i];
+  for (int c = 0; c < ncols; ++c) s = fma(-V[(int64_t)c * ldv + i], h[c], s);
+  w[i] = s;
+}
+
+__global__ void scale_kernel(const double* w, double s, double* v, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) v[i] = w[i] * s;
+}
+
+__global__ void sub_kernel(const double* b, const double* ax, double* r, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) r[i] = b[i] - ax[i];
+}
+
+__global__ void to_f32_kernel(const double* r, const double* d, float* u, int64_t nc, int64_t npad) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < npad) u[i] = i < nc ? (float)(d[i] * r[i]) : 0.0f;
+}
+
+__global__ void from_f32_kernel(const float* u, const double* d, double* y, int64_t nc) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nc) y[i] = d[i] * (double)u[i];
+}
+
+__global__ void add_diag_kernel(float* H, int64_t lda, int64_t n, float s) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) H[i * lda + i] += s;
+}
+
+// U^T y partial sums (U row-major Nc x Lt)
+__global__ void drift_dots_kernel(const double* U, int Lt, const double* y, int64_t nc, double* partial) {
+  __shared__ double red[256];
+  const int64_t r0 = (int64_t)blockIdx.x * RED_ROWS;
+  const int64_t r1 = std::min<int64_t>(nc, r0 + RED_ROWS);
+  for (int l = 0; l < Lt; ++l) {
+    double acc = 0.0;
+    for (int64_t i = r0 + threadIdx.x; i < r1; i += 256) acc = fma(U[i * Lt + l], y[i], acc);
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+      if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[(int64_t)blockIdx.x * Lt + l] = red[0];
+    __syncthreads();
+  }
+}
+
+// lambda = Sinv (sum_b partial - r2)
+__global__ void drift_lambda_kernel(const double* partial, int nblk, int Lt, const double* r2,
+                                    const double* Sinv, double* lam) {
+  __shared__ double t[32];
+  const int l = threadIdx.x;
+  if (l < Lt) {
+    double s = 0.0;
+    for (int b = 0; b < nblk; ++b) s += partial[(int64_t)b * Lt + l];
+    t[l] = s - (r2 ? r2[l] : 0.0);
+  }
+  __syncthreads();
+  if (l < Lt) {
+    double v = 0.0;
+    for (int k = 0; k < Lt; ++k) v = fma(Sinv[l * Lt + k], t[k], v);
+    lam[l] = v;
+  }
+}
+
+// x[:Nc] = y - Y lam ; x[Nc:] = lam
+__global__ void saddle_final_kernel(const double* y, const double* Y, int Lt, const double* lam, double* x,
+                                    int64_t nc) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nc) {
+    double s = y[i];
+    for (int l = 0; l < Lt; ++l) s = fma(-Y[i * Lt + l], lam[l], s);
+    x[i] = s;
+  } else if (i < nc + Lt) {
+    x[i] = lam[i - nc];
+  }
+}
+
+__global__ void set_col_kernel(const double* src, double* Y, int Lt, int l, int64_t nc) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nc) Y[i * Lt + l] = src[i];
+}
+__global__ void get_col_kernel(const double* U, int Lt, int l, double* dst, int64_t nc) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nc) dst[i] = U[i * Lt + l];
+}
+
+inline unsigned nblk(int64_t n, int b = 256) { return (unsigned)std::max<int64_t>(1, gk_ceil_div(n, b)); }
+
+struct Solver {
+  gk_handle h;
+  const gk_system* sys;
+  hipStream_t st;
+  int64_t N, Nc, Lt, Npad, Ncpad;
+  // preconditioner
+  float* H = nullptr;
+  float* dinv = nullptr;
+  double* dscale = nullptr;
+  double* U = nullptr;
+  double* Y = nullptr;
+  double* Sinv = nullptr;
+  float* u32 = nullptr;
+  double* ytmp = nullptr;
+  double* lam = nullptr;
+  double* partial = nullptr;
+  // Krylov
+  int m = 50;
+  double* V = nullptr;
+  double* Z = nullptr;
+  double* w = nullptr;
+  double* r = nullptr;
+  double* hdev = nullptr;
+  double* b = nullptr;
+
+  int host_dots(const double* Vc, int64_t ldv, int ncols, const double* vec, int64_t n, double* out_dev,
+                double* out_host) {
+    const unsigned nb = nblk(n, RED_ROWS);
+    hipLaunchKernelGGL(dots_partial_kernel, dim3(nb), dim3(256), 0, st, Vc, ldv, ncols, vec, n, partial);
+    hipLaunchKernelGGL(reduce_cols_kernel, dim3(1), dim3(128), 0, st, partial, (int)nb, ncols, out_dev);
+    if (out_host) {
+      GK_HIP(hipMemcpyAsync(out_host, out_dev, ncols * sizeof(double), hipMemcpyDeviceToHost, st));
+      GK_HIP(hipStreamSynchronize(st));
+    }
+    return GK_OK;
+  }
+
+  int norm(const double* v, int64_t n, double* out) {
+    GK_TRY(host_dots(v, n, 1, v, n, hdev + 120, out));
+    *out = std::sqrt(std::max(*out, 0.0));
+    return GK_OK;
+  }
+
+  // K^-1 r (covariance block) through the FP32 Cholesky of the equilibrated block
+  int apply_K(const double* rin, double* yout) {
+    hipLaunchKernelGGL(to_f32_kernel, dim3(nblk(Ncpad)), dim3(256), 0, st, rin, dscale, u32, Nc, Ncpad);
+    GK_TRY(gk_cholesky_solve_f32(h, H, dinv, Ncpad, u32, st));
+    hipLaunchKernelGGL(from_f32_kernel, dim3(nblk(Nc)), dim3(256), 0, st, u32, dscale, yout, Nc);
+    return GK_OK;
+  }
+
+  // M^-1 r for the saddle system
+  int apply_M(const double* rin, double* xout) {
+    GK_TRY(apply_K(rin, ytmp));
+    if (Lt > 0) {
+      const unsigned nb = nblk(Nc, RED_ROWS);
+      hipLaunchKernelGGL(drift_dots_kernel, dim3(nb), dim3(256), 0, st, U, (int)Lt, ytmp, Nc, partial);
+      hipLaunchKernelGGL(drift_lambda_kernel, dim3(1), dim3(32), 0, st, partial, (int)nb, (int)Lt, rin + Nc,
+                         Sinv, lam);
+    }
+    hipLaunchKernelGGL(saddle_final_kernel, dim3(nblk(Nc + Lt)), dim3(256), 0, st, ytmp, Y, (int)Lt, lam, xout,
+                       Nc);
+    GK_HIP(hipGetLastError());
+    return GK_OK;
+  }
+
+  int setup_preconditioner(int* info_out) {
+    void* p = nullptr;
+    GK_TRY(gk_workspace(h, WS_MATRIX, (size_t)Ncpad * Ncpad * sizeof(float), &p));
+    H = static_cast<float*>(p);
+    float shift = 0.0f;
+    int32_t info = 0;
+    for (int attempt = 0; attempt < 5; ++attempt) {
+      GK_TRY(gk_assemble_impl(h, sys, GK_F32, H, Ncpad, nullptr, dscale, st, 1));
+      if (shift > 0.0f)
+        hipLaunchKernelGGL(add_diag_kernel, dim3(nblk(Nc)), dim3(256), 0, st, H, Ncpad, Nc, shift);
+      int rc = gk_cholesky_f32(h, H, Nc, Ncpad, dinv, &info, st);
+      if (rc == GK_OK) break;
+      if (rc != GK_ERR_NOT_SPD) return rc;
+      shift = shift == 0.0f ? 1e-6f : shift * 10.0f;
+    }
+    *info_out = info;
+    if (info != 0) return GK_ERR_NOT_SPD;
+    if (Lt > 0) {
+      GK_TRY(gk_assemble_drift_impl(sys, U, st));
+      for (int l = 0; l < Lt; ++l) {
+        hipLaunchKernelGGL(get_col_kernel, dim3(nblk(Nc)), dim3(256), 0, st, U, (int)Lt, l, r, Nc);
+        GK_TRY(apply_K(r, w));
+        hipLaunchKernelGGL(set_col_kernel, dim3(nblk(Nc)), dim3(256), 0, st, w, Y, (int)Lt, l, Nc);
+      }
+      // S = U^T Y (Lt x Lt), inverted on the host (Lt <= 17)
+      std::vector<double> S(Lt * Lt), col(Lt);
+      for (int l = 0; l < Lt; ++l) {
+        hipLaunchKernelGGL(get_col_kernel, dim3(nblk(Nc)), dim3(256), 0, st, Y, (int)Lt, l, w, Nc);
+        const unsigned nb = nblk(Nc, RED_ROWS);
+        hipLaunchKernelGGL(drift_dots_kernel, dim3(nb), dim3(256), 0, st, U, (int)Lt, w, Nc, partial);
+        hipLaunchKernelGGL(reduce_cols_kernel, dim3(1), dim3(128), 0, st, partial, (int)nb, (int)Lt, hdev);
+        GK_HIP(hipMemcpyAsync(col.data(), hdev, Lt * sizeof(double), hipMemcpyDeviceToHost, st));
+        GK_HIP(hipStreamSynchronize(st));
+        for (int k = 0; k < Lt; ++k) S[k * Lt + l] = col[k];
+      }
+      std::vector<double> inv(Lt * Lt, 0.0);
+      for (int i = 0; i < Lt; ++i) inv[i * Lt + i] = 1.0;
+      for (int c = 0; c < Lt; ++c) {           // Gauss-Jordan with partial pivoting
+        int piv = c;
+        for (int i = c + 1; i < Lt; ++i)
+          if (std::fabs(S[i * Lt + c]) > std::fabs(S[piv * Lt + c])) piv = i;
+        GK_REQUIRE(S[piv * Lt + c] != 0.0, GK_ERR_SINGULAR, "gk_solve: singular drift Schur complement");
+        for (int k = 0; k < Lt; ++k) { std::swap(S[c * Lt + k], S[piv * Lt + k]); std::swap(inv[c * Lt + k], inv[piv * Lt + k]); }
+        const double d = 1.0 / S[c * Lt + c];
+        for (int k = 0; k < Lt; ++k) { S[c * Lt + k] *= d; inv[c * Lt + k] *= d; }
+        for (int i = 0; i < Lt; ++i) {
+          if (i == c) continue;
+          const double f = S[i * Lt + c];
+          if (f == 0.0) continue;
+          for (int k = 0; k < Lt; ++k) { S[i * Lt + k] -= f * S[c * Lt + k]; inv[i * Lt + k] -= f * inv[c * Lt + k]; }
+        }
+      }
+      GK_HIP(hipMemcpyAsync(Sinv, inv.data(), Lt * Lt * sizeof(double), hipMemcpyHostToDevice, st));
+      GK_HIP(hipStreamSynchronize(st));
+    }
+    return GK_OK;
+  }
+
+  // Right-preconditioned restarted flexible GMRES on the FP64 matrix-free operator.
+  int fgmres(double* x, double tol, double accept, int max_iter, int* iters, double* relres, bool* ok) {
+    double nb = 0.0;
+    GK_TRY(norm(b, N, &nb));
+    if (nb == 0.0) nb = 1.0;
+    GK_HIP(hipMemsetAsync(x, 0, N * sizeof(double), st));
+    int total = 0;
+    double prev = 1e300, rel = 1.0;
+    std::vector<double> Hh((m + 1) * m), cs(m), sn(m), g(m + 1), hv(m + 2), yv(m);
+    *ok = false;
+    for (int cycle = 0; cycle < 1000; ++cycle) {
+      GK_TRY(gk_matvec(h, sys, GK_F64, x, w, st));
+      hipLaunchKernelGGL(sub_kernel, dim3(nblk(N)), dim3(256), 0, st, b, w, r, N);
+      double beta = 0.0;
+      GK_TRY(norm(r, N, &beta));
+      rel = beta / nb;
+      if (rel <= tol) { *ok = true; break; }
+      if (cycle > 0 && rel > 0.5 * prev) { *ok = rel <= accept; break; }   // FP64 floor reached
+      if (total >= max_iter) { *ok = rel <= accept; break; }
+      prev = rel;
+      hipLaunchKernelGGL(scale_kernel, dim3(nblk(N)), dim3(256), 0, st, r, 1.0 / beta, V, N);
+      std::fill(g.begin(), g.end(), 0.0);
+      g[0] = beta;
+      int j = 0;
+      for (; j < m && total < max_iter; ++j) {
+        double* vj = V + (int64_t)j * Npad;
+        double* zj = Z + (int64_t)j * Npad;
+        GK_TRY(apply_M(vj, zj));
+        GK_TRY(gk_matvec(h, sys, GK_F64, zj, w, st));
+        // classical Gram-Schmidt, two passes
+        std::vector<double> hc(j + 1, 0.0);
+        for (int pass = 0; pass < 2; ++pass) {
+          GK_TRY(host_dots(V, Npad, j + 1, w, N, hdev, hv.data()));
+          hipLaunchKernelGGL(gemv_sub_kernel, dim3(nblk(N)), dim3(256), 0, st, V, Npad, j + 1, hdev, w, N);
+          for (int i = 0; i <= j; ++i) hc[i] += hv[i];
+        }
+        double hn = 0.0;
+        GK_TRY(norm(w, N, &hn));
+        for (int i = 0; i <= j; ++i) Hh[i * m + j] = hc[i];
+        Hh[(j + 1) * m + j] = hn;
+        if (hn > 0.0)
+          hipLaunchKernelGGL(scale_kernel, dim3(nblk(N)), dim3(256), 0, st, w, 1.0 / hn, V + (int64_t)(j + 1) * Npad, N);
+        for (int i = 0; i < j; ++i) {          // previous rotations
+          const double a = Hh[i * m + j], c = Hh[(i + 1) * m + j];
+          Hh[i * m + j] = cs[i] * a + sn[i] * c;
+          Hh[(i + 1) * m + j] = -sn[i] * a + cs[i] * c;
+        }
+        const double a = Hh[j * m + j], c = Hh[(j + 1) * m + j];
+        const double den = std::hypot(a, c);
+        cs[j] = den > 0 ? a / den : 1.0;
+        sn[j] = den > 0 ? c / den : 0.0;
+        Hh[j * m + j] = den;
+        Hh[(j + 1) * m + j] = 0.0;
+        g[j + 1] = -sn[j] * g[j];
+        g[j] = cs[j] * g[j];
+        ++total;
+        if (std::fabs(g[j + 1]) / nb <= 0.1 * tol || hn == 0.0) { ++j; break; }
+      }
+      // back substitution and update
+      for (int i = j - 1; i >= 0; --i) {
+        double s = g[i];
+        for (int k = i + 1; k < j; ++k) s -= Hh[i * m + k] * yv[k];
+        yv[i] = s / Hh[i * m + i];
+      }
+      // x += Z y  (as x -= Z (-y))
+      for (int i = 0; i < j; ++i) yv[i] = -yv[i];
+      GK_HIP(hipMemcpyAsync(hdev + 64, yv.data(), j * sizeof(double), hipMemcpyHostToDevice, st));
+      hipLaunchKernelGGL(gemv_sub_kernel, dim3(nblk(N)), dim3(256), 0, st, Z, Npad, j, hdev + 64, x, N);
+      GK_HIP(hipStreamSynchronize(st));
+    }
+    *iters = total;
+    *relres = rel;
+    return GK_OK;
+  }
+
+  // ||b - A x|| / ||b|| with the FP64 matrix-free operator
+  int true_relres(const double* x, double* out) {
+    double nb = 0.0, nr = 0.0;
+    GK_TRY(norm(b, N, &nb));
+    GK_TRY(gk_matvec(h, sys, GK_F64, x, w, st));
+    hipLaunchKernelGGL(sub_kernel, dim3(nblk(N)), dim3(256), 0, st, b, w, r, N);
+    GK_TRY(norm(r, N, &nr));
+    *out = nb > 0 ? nr / nb : nr;
+    return GK_OK;
+  }
+};
+
+double elapsed_ms(hipEvent_t a, hipEvent_t b) {
+  float ms = 0.0f;
+  (void)hipEventElapsedTime(&ms, a, b);
+  return ms;
+}
+
+}  // namespace
+
+static int solve_lu(gk_handle h, const gk_system* sys, hipStream_t st, int64_t N, int64_t Npad, double* x,
+                    gk_solve_info* info, hipEvent_t* ev) {
+  void* p = nullptr;
+  GK_TRY(gk_workspace(h, WS_MATRIX2, (size_t)Npad * Npad * sizeof(double) + (size_t)3 * Npad * GK_BLOCK * sizeof(double), &p));
+  double* A = static_cast<double*>(p);
+  double* dinv = A + (size_t)Npad * Npad;
+  double* bpad = dinv + (size_t)2 * Npad * GK_BLOCK;
+  GK_HIP(hipEventRecord(ev[0], st));
+  GK_HIP(hipMemsetAsync(bpad, 0, Npad * sizeof(double), st));
+  GK_TRY(gk_assemble_impl(h, sys, GK_F64, A, Npad, bpad, nullptr, st, 0));
+  GK_HIP(hipEventRecord(ev[1], st));
+  int32_t finfo = 0;
+  int rc = gk_lu_f64(h, A, N, Npad, dinv, &finfo, st);
+  info->info = finfo;
+  if (rc != GK_OK) return rc;
+  GK_HIP(hipEventRecord(ev[2], st));
+  GK_TRY(gk_lu_solve_f64(h, A, dinv, Npad, bpad, st));
+  GK_HIP(hipMemcpyAsync(x, bpad, N * sizeof(double), hipMemcpyDeviceToDevice, st));
+  GK_HIP(hipEventRecord(ev[3], st));
+  GK_HIP(hipEventSynchronize(ev[3]));
+  info->t_assemble_ms += elapsed_ms(ev[0], ev[1]);
+  info->t_factor_ms += elapsed_ms(ev[1], ev[2]);
+  info->t_refine_ms += elapsed_ms(ev[2], ev[3]);
+  info->form = GK_SOLVE_FP64_LU;
+  return GK_OK;
+}
+
+extern "C" int gk_solve(gk_handle h, const gk_system* sys, int32_t form, double tol, int32_t max_iter,
+                        double* x, gk_solve_info* info, gk_stream stream) {
+  GK_REQUIRE(h && sys && x && info, GK_ERR_ARG, "gk_solve: NULL argument");
+  GK_REQUIRE(form == GK_SOLVE_FP64_LU || form == GK_SOLVE_FP32_REFINE, GK_ERR_ARG, "gk_solve: bad form");
+  hipStream_t st = (hipStream_t)stream;
+  *info = gk_solve_info{};
+  const int64_t N = gk_system_n(sys);
+  GK_REQUIRE(N > 0, GK_ERR_ARG, "gk_solve: empty system");
+  const int64_t Npad = gk_round_up(N, GK_BLOCK);
+  const int L = sys->uni_degree == 0 ? 0 : (sys->uni_degree == 1 ? 3 : 9);
+  const int64_t Lt = L + sys->n_fault;
+  const int64_t Nc = N - Lt;
+  hipEvent_t ev[4];
+  for (auto& e : ev) GK_HIP(hipEventCreate(&e));
+  struct EvGuard { hipEvent_t* e; ~EvGuard() { for (int i = 0; i < 4; ++i) (void)hipEventDestroy(e[i]); } } guard{ev};
+  if (tol <= 0) tol = 1e-10;
+  if (max_iter <= 0) max_iter = 300;
+
+  Solver S;
+  S.h = h; S.sys = sys; S.st = st;
+  S.N = N; S.Nc = Nc; S.Lt = Lt; S.Npad = Npad; S.Ncpad = gk_round_up(std::max<int64_t>(Nc, 1), GK_BLOCK);
+  const bool fast = form == GK_SOLVE_FP32_REFINE;
+  const int m = fast ? S.m : 1;
+  size_t need = 0;
+  auto take = [&](size_t bytes) { size_t o = need; need += gk_round_up(bytes, 256); return o; };
+  const size_t o_dinv = take(fast ? (size_t)S.Ncpad * GK_BLOCK * sizeof(float) : 0);
+  const size_t o_dsc = take(S.Ncpad * sizeof(double));
+  const size_t o_U = take((size_t)S.Ncpad * std::max<int64_t>(Lt, 1) * sizeof(double));
+  const size_t o_Y = take((size_t)S.Ncpad * std::max<int64_t>(Lt, 1) * sizeof(double));
+  const size_t o_Si = take(32 * 32 * sizeof(double));
+  const size_t o_u32 = take(S.Ncpad * sizeof(float));
+  const size_t o_yt = take(Npad * sizeof(double));
+  const size_t o_lam = take(64 * sizeof(double));
+  const size_t o_part = take((size_t)(gk_ceil_div(Npad, RED_ROWS) + 1) * 64 * sizeof(double));
+  const size_t o_V = take((size_t)(m + 1) * Npad * sizeof(double));
+  const size_t o_Z = take((size_t)m * Npad * sizeof(double));
+  const size_t o_w = take(Npad * sizeof(double));
+  const size_t o_r = take(Npad * sizeof(double));
+  const size_t o_h = take(256 * sizeof(double));
+  const size_t o_b = take(Npad * sizeof(double));
+  void* p = nullptr;
+  GK_TRY(gk_workspace(h, WS_SOLVER, need, &p));
+  char* base = static_cast<char*>(p);
+  S.dinv = reinterpret_cast<float*>(base + o_dinv);
+  S.dscale = reinterpret_cast<double*>(base + o_dsc);
+  S.U = reinterpret_cast<double*>(base + o_U);
+  S.Y = reinterpret_cast<double*>(base + o_Y);
+  S.Sinv = reinterpret_cast<double*>(base + o_Si);
+  S.u32 = reinterpret_cast<float*>(base + o_u32);
+  S.ytmp = reinterpret_cast<double*>(base + o_yt);
+  S.lam = reinterpret_cast<double*>(base + o_lam);
+  S.partial = reinterpret_cast<double*>(base + o_part);
+  S.V = reinterpret_cast<double*>(base + o_V);
+  S.Z = reinterpret_cast<double*>(base + o_Z);
+  S.w = reinterpret_cast<double*>(base + o_w);
+  S.r = reinterpret_cast<double*>(base + o_r);
+  S.hdev = reinterpret_cast<double*>(base + o_h);
+  S.b = reinterpret_cast<double*>(base + o_b);
+  GK_TRY(gk_rhs_impl(sys, S.b, st));
+
+  if (!fast) {
+    GK_TRY(solve_lu(h, sys, st, N, Npad, x, info, ev));
+    GK_TRY(S.true_relres(x, &info->relres));
+    return GK_OK;
+  }
+  GK_HIP(hipEventRecord(ev[0], st));
+  GK_HIP(hipEventRecord(ev[1], st));
+  int32_t finfo = 0;
+  int rc = S.setup_preconditioner(&finfo);
+  info->info = finfo;
+  GK_HIP(hipEventRecord(ev[2], st));
+  bool ok = false;
+  if (rc == GK_OK) {
+    GK_TRY(S.fgmres(x, tol, 1e-6, max_iter, &info->iterations, &info->relres, &ok));
+  } else if (rc != GK_ERR_NOT_SPD) {
+    return rc;
+  }
+  GK_HIP(hipEventRecord(ev[3], st));
+  GK_HIP(hipEventSynchronize(ev[3]));
+  info->t_assemble_ms = 0.0;
+  info->t_factor_ms = elapsed_ms(ev[1], ev[2]);
+  info->t_refine_ms = elapsed_ms(ev[2], ev[3]);
+  info->form = GK_SOLVE_FP32_REFINE;
+  if (!ok) {
+    info->fallback = 1;
+    GK_TRY(solve_lu(h, sys, st, N, Npad, x, info, ev));
+    GK_TRY(S.true_relres(x, &info->relres));
+  }
+  return GK_OK;
+}

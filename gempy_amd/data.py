@@ -1,0 +1,224 @@
+"""Input and output records of the interpolator.
+
+The names follow the records the reference passes across its engine boundary
+(``gempy/modules/data_manipulation/_engine_factory.py:27-58``:
+``SurfacePoints(sp_coords, nugget_effect_scalar)``,
+``Orientations(dip_positions, dip_gradients, nugget_effect_grad)``, the engine
+grid, ``unit_values``) and the option records serialized in
+``2-layers.approved.txt:149-182`` (``kernel_options``, ``evaluation_options``,
+``sigmoid_slope``).  One :class:`StackInput` holds the data of one structural
+group (``structural_frame.py:305-317``: points per surface, orientations per
+stack, relation type).
+"""
+from __future__ import annotations
+
+import copy
+import enum
+from dataclasses import dataclass, field
+from typing import List, Optional, Sequence
+
+import numpy as np
+
+
+class StackRelationType(enum.IntEnum):
+    """Relation of a structural group to the older ones (values as serialized in
+    the reference's approved JSON: ERODE = 1, FAULT = 3)."""
+    ERODE = 1
+    ONLAP = 2
+    FAULT = 3
+    BASEMENT = 4
+
+
+class SolverForm(str, enum.Enum):
+    """How the co-kriging system is solved on the device."""
+    FP64_DIRECT = "fp64_direct"        # FP64 assembly + FP64 blocked LU (MFMA)
+    FP32_REFINE = "fp32_refine"        # FP32 Cholesky (MFMA) + FP64 iterative refinement
+    GMRES_LEARNED = "gmres_learned"    # matrix-free FP32 GMRES + learned preconditioner
+
+
+@dataclass
+class KernelOptions:
+    """``kernel_options`` record (2-layers.approved.txt:150-162)."""
+    range: float = 1.7
+    c_o: float = 10.0
+    uni_degree: int = 1
+    i_res: float = 4.0
+    gi_res: float = 2.0
+    number_dimensions: int = 3
+    kernel_function: str = "cubic"
+    kernel_solver: int = 1
+    compute_condition_number: bool = False
+
+
+@dataclass
+class EvaluationOptions:
+    """``evaluation_options`` record (2-layers.approved.txt:163-176)."""
+    number_octree_levels: int = 1
+    evaluation_chunk_size: int = 500_000
+    compute_scalar_gradient: bool = False
+
+
+@dataclass
+class InterpolationOptions:
+    kernel_options: KernelOptions = field(default_factory=KernelOptions)
+    evaluation_options: EvaluationOptions = field(default_factory=EvaluationOptions)
+    sigmoid_slope: float = 5_000_000.0
+    solver: SolverForm = SolverForm.FP32_REFINE
+
+    def __init__(self, kernel_options: Optional[KernelOptions] = None,
+                 evaluation_options: Optional[EvaluationOptions] = None,
+                 sigmoid_slope: float = 5_000_000.0, solver: SolverForm = SolverForm.FP32_REFINE,
+                 number_octree_levels: Optional[int] = None,
+                 compute_scalar_gradient: Optional[bool] = None):
+        self.kernel_options = kernel_options or KernelOptions()
+        self.evaluation_options = evaluation_options or EvaluationOptions()
+        if number_octree_levels is not None:
+            self.evaluation_options.number_octree_levels = int(number_octree_levels)
+        if compute_scalar_gradient is not None:
+            self.evaluation_options.compute_scalar_gradient = bool(compute_scalar_gradient)
+        self.sigmoid_slope = sigmoid_slope
+        self.solver = SolverForm(solver)
+
+    def as_dict(self):
+        k = self.kernel_options
+        return {"range": k.range, "c_o": k.c_o, "uni_degree": k.uni_degree,
+                "i_res": k.i_res, "gi_res": k.gi_res}
+
+
+@dataclass
+class RegularGrid:
+    """Cell-centred lattice: ``extent`` (xmin, xmax, ymin, ymax, zmin, zmax) and
+    ``resolution`` (nx, ny, nz); ravel order ``ij`` with z fastest
+    (regular_grid.py:58-71,191-201)."""
+    extent: Sequence[float]
+    resolution: Sequence[int]
+
+    def __post_init__(self):
+        self.extent = tuple(float(v) for v in self.extent)
+        self.resolution = tuple(int(v) for v in self.resolution)
+
+    @property
+    def n_points(self) -> int:
+        nx, ny, nz = self.resolution
+        return nx * ny * nz
+
+    @property
+    def spacing(self):
+        e, r = self.extent, self.resolution
+        return tuple((e[2 * d + 1] - e[2 * d]) / r[d] for d in range(3))
+
+    @property
+    def origin(self):
+        """Coordinates of the first cell centre."""
+        e, dx = self.extent, self.spacing
+        return tuple(e[2 * d] + dx[d] / 2 for d in range(3))
+
+    def values(self) -> np.ndarray:
+        o, dx, r = self.origin, self.spacing, self.resolution
+        axes = [o[d] + dx[d] * np.arange(r[d]) for d in range(3)]
+        g = np.meshgrid(*axes, indexing="ij")
+        return np.stack([c.ravel() for c in g], axis=1)
+
+
+@dataclass
+class StackInput:
+    """Data of one structural group.
+
+    ``sp_xyz`` rows are grouped by surface (``counts`` points each, surfaces in
+    stratigraphic order, youngest first); the first point of every surface is
+    its reference point.  ``faults`` lists the indices (in the model's stack
+    list) of the fault stacks whose block enters this stack as a drift column.
+    """
+    name: str
+    sp_xyz: np.ndarray
+    counts: List[int]
+    sp_nugget: np.ndarray
+    ori_xyz: np.ndarray
+    ori_grad: np.ndarray
+    ori_nugget: np.ndarray
+    relation: StackRelationType = StackRelationType.ERODE
+    surface_names: List[str] = field(default_factory=list)
+    faults: List[int] = field(default_factory=list)
+
+    def __post_init__(self):
+        self.sp_xyz = np.ascontiguousarray(np.asarray(self.sp_xyz, dtype=np.float64).reshape(-1, 3))
+        self.counts = [int(c) for c in self.counts]
+        self.sp_nugget = np.ascontiguousarray(np.asarray(self.sp_nugget, dtype=np.float64).reshape(-1))
+        self.ori_xyz = np.ascontiguousarray(np.asarray(self.ori_xyz, dtype=np.float64).reshape(-1, 3))
+        self.ori_grad = np.ascontiguousarray(np.asarray(self.ori_grad, dtype=np.float64).reshape(-1, 3))
+        self.ori_nugget = np.ascontiguousarray(np.asarray(self.ori_nugget, dtype=np.float64).reshape(-1))
+        self.relation = StackRelationType(self.relation)
+        self.faults = [int(f) for f in self.faults]
+
+    @property
+    def n_surfaces(self) -> int:
+        return len(self.counts)
+
+    @property
+    def n_orientations(self) -> int:
+        return self.ori_xyz.shape[0]
+
+    @property
+    def n_rest(self) -> int:
+        return self.sp_xyz.shape[0] - len(self.counts)
+
+    def copy(self) -> "StackInput":
+        return copy.deepcopy(self)
+
+    def as_oracle_dict(self):
+        return {"sp_xyz": self.sp_xyz, "counts": self.counts, "sp_nugget": self.sp_nugget,
+                "ori_xyz": self.ori_xyz, "ori_grad": self.ori_grad,
+                "ori_nugget": self.ori_nugget,
+                "relation": "fault" if self.relation == StackRelationType.FAULT else "erode",
+                "faults": list(self.faults)}
+
+
+@dataclass
+class GeoModelInput:
+    """Everything ``compute_model`` needs: stacks (youngest first), options, grid."""
+    stacks: List[StackInput]
+    options: InterpolationOptions
+    grid: Optional[RegularGrid] = None
+    name: str = "model"
+    custom_grid: Optional[np.ndarray] = None
+
+    def system_size(self, k: int = 0) -> int:
+        st = self.stacks[k]
+        L = {0: 0, 1: 3, 2: 9}[self.options.kernel_options.uni_degree]
+        return 3 * st.n_orientations + st.n_rest + L + len(st.faults)
+
+
+@dataclass
+class StackSolution:
+    """Per-stack output (the engine's ``InterpOutput``: weights, exported fields,
+    ids block; ``test_example_models_I.py:20-21``)."""
+    weights: np.ndarray
+    scalar_field_at_surface_points: np.ndarray
+    thresholds: np.ndarray
+    scalar_field: Optional[np.ndarray] = None
+    gradient: Optional[np.ndarray] = None
+    ids_block: Optional[np.ndarray] = None
+    solver_info: dict = field(default_factory=dict)
+
+
+@dataclass
+class OctreeLevel:
+    """Points evaluated at one octree level: voxel indices at that level's
+    resolution, ids at the voxel centres and the mask of voxels refined next."""
+    resolution: tuple
+    voxel_index: np.ndarray
+    ids: np.ndarray
+    scalar: np.ndarray
+    refine: np.ndarray
+
+
+@dataclass
+class Solutions:
+    """Model output (``Solutions`` contract consumed at geo_model.py:100-127)."""
+    stacks: List[StackSolution]
+    lith_block: Optional[np.ndarray] = None
+    fault_blocks: List[np.ndarray] = field(default_factory=list)
+    scalar_field_matrix: Optional[np.ndarray] = None
+    octrees_output: List[OctreeLevel] = field(default_factory=list)
+    custom: Optional[np.ndarray] = None
+    timings: dict = field(default_factory=dict)

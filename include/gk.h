@@ -1,0 +1,177 @@
+/*
+ * gk.h -- C ABI of libgkhip.so, the MI355X (gfx950) implicit co-kriging engine.
+ *
+ * The reference (gempy) has no C ABI: every numeric step of gp.compute_model is
+ * a Python call into the external engine (gempy/API/compute_API.py:68-73).
+ * Each entry point below replaces one stage of that engine, as cited, and is
+ * reached from the package's `hip` backend (gempy_amd/_lib.py) through ctypes.
+ *
+ * Conventions
+ *  - All array arguments are DEVICE pointers owned by the caller, row-major,
+ *    unless stated otherwise; the library never frees caller memory.
+ *  - Calls are stream-ordered on `stream` (a hipStream_t; NULL = default).
+ *    Calls that return scalars to the host (info, iteration counts) synchronise
+ *    that stream before returning.
+ *  - Every function returns GK_OK (0) or a negative GK_ERR_* code; the message
+ *    of the last error on the calling thread is in gk_last_error().
+ *  - A handle owns a growable device workspace; one handle must not be used by
+ *    two host threads at once.
+ */
+#ifndef GK_H_
+#define GK_H_
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct gk_context* gk_handle;
+typedef void* gk_stream;
+
+enum {
+  GK_OK = 0,
+  GK_ERR_ARG = -1,
+  GK_ERR_HIP = -2,
+  GK_ERR_NOT_SPD = -3,
+  GK_ERR_SINGULAR = -4,
+  GK_ERR_NOT_CONVERGED = -5,
+  GK_ERR_UNSUPPORTED = -6
+};
+
+enum { GK_F32 = 0, GK_F64 = 1 };
+enum { GK_POINTS_EXPLICIT = 0, GK_POINTS_LATTICE = 1 };
+enum { GK_SOLVE_FP64_LU = 0, GK_SOLVE_FP32_REFINE = 1 };
+#define GK_MAX_THRESHOLDS 64
+#define GK_BLOCK 256   /* factorization block size; lda of factored matrices is a multiple */
+
+/* One stack's co-kriging data in the engine's layout
+ * (SurfacePoints / Orientations records built at
+ * gempy/modules/data_manipulation/_engine_factory.py:27-37, split into the
+ * reference point (first point) and rest points of every surface).
+ * Unknown order: [w_G: x comps of all orientations, then y, then z | w_I per
+ * rest point | universal drift (0, 3 or 9) | one per fault column]. */
+typedef struct {
+  int64_t n_ori;
+  int64_t n_rest;
+  int64_t n_surf;
+  int32_t uni_degree;          /* 0, 1 or 2 (kernel_options.uni_degree) */
+  int32_t n_fault;             /* fault-drift columns */
+  double range;                /* kernel_options.range */
+  double c_o;                  /* kernel_options.c_o */
+  double i_res;                /* kernel_options.i_res (4) */
+  double gi_res;               /* kernel_options.gi_res (2) */
+  const double* ori_xyz;       /* [n_ori*3] */
+  const double* ori_grad;      /* [n_ori*3] */
+  const double* ori_nugget;    /* [n_ori]   nugget_effect_grad */
+  const double* rest_xyz;      /* [n_rest*3] */
+  const int32_t* rest_surf;    /* [n_rest]  surface index 0..n_surf-1 */
+  const double* rest_nugget;   /* [n_rest]  nu_i + nu_ref(i) */
+  const double* ref_xyz;       /* [n_surf*3] */
+  const double* fault_rest;    /* [n_rest*n_fault] F(p_i)-F(p_ref(i)); NULL if n_fault==0 */
+  double center[3];            /* origin of the device's normalised frame (x-center)/range */
+} gk_system;
+
+/* Where and what to evaluate (the engine's exported fields: scalar field,
+ * its gradient when compute_scalar_gradient (2-layers.approved.txt:174), and
+ * the ids block from the surface thresholds). */
+typedef struct {
+  int32_t kind;                /* GK_POINTS_EXPLICIT or GK_POINTS_LATTICE */
+  int32_t precision;           /* GK_F32 or GK_F64 arithmetic of the pair sums */
+  int64_t n_points;            /* explicit: rows of xyz; lattice: filled by the library */
+  const double* xyz;           /* explicit points [n_points*3] */
+  double origin[3];            /* lattice: first cell centre */
+  double spacing[3];           /* lattice: cell size */
+  int64_t res[3];              /* lattice: nx, ny, nz (z fastest, regular_grid.py:58-71) */
+  const double* fault_values;  /* [n_points*n_fault] fault-block values at the points, or NULL */
+  double* scalar;              /* out [n_points] (required) */
+  double* gradient;            /* out [n_points*3] or NULL */
+  int32_t* ids;                /* out [n_points] or NULL */
+  int32_t n_thresholds;        /* host-side thresholds (<= GK_MAX_THRESHOLDS) for ids */
+  double thresholds[GK_MAX_THRESHOLDS];
+} gk_eval_desc;
+
+typedef struct {
+  int32_t form;                /* GK_SOLVE_* actually used (after any fallback) */
+  int32_t iterations;          /* Krylov iterations of the refinement */
+  int32_t fallback;            /* 1 if FP32_REFINE stagnated and fell back to FP64 LU */
+  int32_t info;                /* factorization info (0 = ok, k>0 = failed pivot) */
+  double relres;               /* final ||b - A x|| / ||b|| (FP64, matrix-free) */
+  double t_assemble_ms, t_factor_ms, t_refine_ms;
+} gk_solve_info;
+
+/* ---- library ---------------------------------------------------------- */
+int gk_version(void);
+const char* gk_last_error(void);
+int gk_create(int device, gk_handle* out);
+void gk_destroy(gk_handle h);
+int64_t gk_system_n(const gk_system* sys);
+int64_t gk_system_padded(const gk_system* sys);  /* N rounded up to GK_BLOCK */
+
+/* ---- (a) kernel_constructor: assembly -----------------------------------
+ * Replaces the engine's covariance/drift assembly behind
+ * compute_API.py:68-73 (kernel_options at 2-layers.approved.txt:150-162).
+ * dtype GK_F64: full saddle-point matrix N x N into A (lda >= N), rows/cols
+ *   N..lda-1 set to the identity (lda a multiple of GK_BLOCK).
+ * dtype GK_F32: covariance block only (Nc = 3 n_ori + n_rest), symmetrically
+ *   equilibrated H = D A_cc D with D = diag(A_cc)^-1/2 written to dscale[Nc];
+ *   padding rows/cols to lda are the identity.
+ * rhs (nullable) receives b = [G_x | G_y | G_z | 0 ...] (length N). */
+int gk_assemble(gk_handle h, const gk_system* sys, int32_t dtype, void* A, int64_t lda,
+                double* rhs, double* dscale, gk_stream stream);
+
+/* Drift block U (Nc x (L + n_fault)) in FP64, row-major. */
+int gk_assemble_drift(gk_handle h, const gk_system* sys, double* U, gk_stream stream);
+
+/* y = A x for the full system, matrix-free (the kernel tiles are recomputed;
+ * the reference builds A densely).  precision GK_F64 or GK_F32 pair sums. */
+int gk_matvec(gk_handle h, const gk_system* sys, int32_t precision, const double* x, double* y,
+              gk_stream stream);
+
+/* ---- (b) solver (kernel_solver, 2-layers.approved.txt:158) -------------- */
+/* In-place lower Cholesky of an lda x lda FP32 matrix (lda multiple of GK_BLOCK;
+ * rows/cols >= n are reset to the identity).  dinv [lda*GK_BLOCK] receives the
+ * inverses of the diagonal blocks.  info = 0 or the first failing block+1. */
+int gk_cholesky_f32(gk_handle h, float* A, int64_t n, int64_t lda, float* dinv, int32_t* info,
+                    gk_stream stream);
+/* Solve (L L^T) x = b in place with the output of gk_cholesky_f32 (FP32 vector of length lda). */
+int gk_cholesky_solve_f32(gk_handle h, const float* L, const float* dinv, int64_t lda, float* b,
+                          gk_stream stream);
+/* In-place LU without pivoting (the co-kriging matrix has an SPD leading
+ * block) of an lda x lda FP64 matrix; dinv [2*lda*GK_BLOCK] receives the
+ * inverses of the unit-lower and upper diagonal blocks. */
+int gk_lu_f64(gk_handle h, double* A, int64_t n, int64_t lda, double* dinv, int32_t* info,
+              gk_stream stream);
+int gk_lu_solve_f64(gk_handle h, const double* LU, const double* dinv, int64_t lda, double* b,
+                    gk_stream stream);
+/* C = beta C - A op(B): MFMA GEMM building block (exported for testing).
+ * trans_b = 1: B is n x k (C -= A B^T); 0: B is k x n.  lower_only: update
+ * only 256-blocks on or below the diagonal (SYRK). */
+int gk_gemm_f32(gk_handle h, int64_t m, int64_t n, int64_t k, const float* A, int64_t lda,
+                const float* B, int64_t ldb, int32_t trans_b, float beta, float* C, int64_t ldc,
+                int32_t lower_only, gk_stream stream);
+int gk_gemm_f64(gk_handle h, int64_t m, int64_t n, int64_t k, const double* A, int64_t lda,
+                const double* B, int64_t ldb, int32_t trans_b, double beta, double* C, int64_t ldc,
+                int32_t lower_only, gk_stream stream);
+
+/* Whole solve of one stack: assemble, factor, solve and (FP32 form) refine
+ * with FP64 right-preconditioned GMRES on the matrix-free FP64 operator,
+ * falling back to the FP64 LU if the refinement stagnates above tol. */
+int gk_solve(gk_handle h, const gk_system* sys, int32_t form, double tol, int32_t max_iter,
+             double* x, gk_solve_info* info, gk_stream stream);
+
+/* ---- (c) evaluation: scalar field, gradient, ids ------------------------ */
+int gk_eval(gk_handle h, const gk_system* sys, const double* weights, gk_eval_desc* desc,
+            gk_stream stream);
+
+/* Stack combination (the engine's erosion masking; relations at
+ * gempy/core/data/structural_frame.py:243-273): ids [n_stacks][n] of the
+ * non-fault stacks, youngest first, with n_surf[s] surfaces each (host array);
+ * lith[p] = global element id of the youngest stack whose domain (above its
+ * last surface) holds p, basement = sum(n_surf) + 1 otherwise. */
+int gk_combine_lith(gk_handle h, int32_t n_stacks, const int32_t* ids, int64_t n, const int32_t* n_surf,
+                    int32_t* lith, gk_stream stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GK_H_ */

@@ -1,0 +1,311 @@
+"""HIP path vs the FP64 oracle (and numpy for the dense building blocks).
+
+Tolerances: FP64 kernels 1e-10 relative (assembly, matvec, evaluation);
+FP32 MFMA GEMM 2e-5 relative to the operand scale; solved fields <= 1e-6 of
+the field range; ids bit-exact away from thresholds.
+"""
+import ctypes as C
+import os
+import sys
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import cokriging as ok
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def say(msg):
+    print(f"[gpu-case] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+def _lib():
+    from gempy_amd import _lib as L
+    return L
+
+
+def _h():
+    from gempy_amd.engine import handle
+    return handle(0)
+
+
+def _st():
+    return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _p(t):
+    return C.c_void_p(t.data_ptr())
+
+
+# ------------------------------------------------------------ dense blocks
+@pytest.mark.parametrize("dtype", ["f32", "f64"])
+@pytest.mark.parametrize("trans_b", [0, 1])
+def test_gemm_matches_numpy(gpu, dtype, trans_b):
+    say(f"gemm {dtype} trans_b={trans_b} m=384 n=256 k=256")
+    L = _lib()
+    rng = np.random.default_rng(0)
+    m, n, k = 384, 256, 256
+    tdt = torch.float32 if dtype == "f32" else torch.float64
+    A = rng.standard_normal((m, k))
+    B = rng.standard_normal((n, k) if trans_b else (k, n))
+    Cm = rng.standard_normal((m, n))
+    tA, tB, tC = (torch.tensor(x, dtype=tdt, device=gpu).contiguous() for x in (A, B, Cm))
+    fn = L.load().gk_gemm_f32 if dtype == "f32" else L.load().gk_gemm_f64
+    L.check("gemm", fn(_h(), m, n, k, _p(tA), k, _p(tB), tB.shape[1], trans_b, 1.0, _p(tC), n, 0, _st()))
+    torch.cuda.synchronize()
+    ref = Cm - A @ (B.T if trans_b else B)
+    tol = 2e-5 * np.sqrt(k) if dtype == "f32" else 1e-12 * np.sqrt(k)
+    np.testing.assert_allclose(tC.cpu().numpy(), ref, rtol=0, atol=tol * 4)
+
+
+def test_gemm_lower_only(gpu):
+    say("gemm f32 lower_only m=n=512 k=256")
+    L = _lib()
+    rng = np.random.default_rng(1)
+    m, k = 512, 256
+    W = rng.standard_normal((m, k))
+    Cm = rng.standard_normal((m, m))
+    tW = torch.tensor(W, dtype=torch.float32, device=gpu)
+    tC = torch.tensor(Cm, dtype=torch.float32, device=gpu)
+    L.check("gemm", L.load().gk_gemm_f32(_h(), m, m, k, _p(tW), k, _p(tW), k, 1, 1.0, _p(tC), m, 1, _st()))
+    out = tC.cpu().numpy()
+    ref = Cm - W @ W.T
+    for bi in range(4):
+        for bj in range(4):
+            blk = (slice(bi * 128, bi * 128 + 128), slice(bj * 128, bj * 128 + 128))
+            if bi >= bj:
+                np.testing.assert_allclose(out[blk], ref[blk], atol=1e-3)
+            else:
+                np.testing.assert_array_equal(out[blk], Cm[blk].astype(np.float32))
+
+
+def _spd(n, seed, cond_scale=1.0):
+    rng = np.random.default_rng(seed)
+    M = rng.standard_normal((n, n))
+    return M @ M.T / n + cond_scale * np.eye(n)
+
+
+def test_cholesky_f32_and_solve(gpu):
+    say("cholesky f32 n=700 lda=768")
+    L = _lib()
+    n, lda = 700, 768
+    A = _spd(n, 2)
+    tA = torch.zeros((lda, lda), dtype=torch.float32, device=gpu)
+    tA[:n, :n] = torch.tensor(A, dtype=torch.float32)
+    dinv = torch.empty(lda * 256, dtype=torch.float32, device=gpu)
+    info = C.c_int32(-1)
+    L.check("chol", L.load().gk_cholesky_f32(_h(), _p(tA), n, lda, _p(dinv), C.byref(info), _st()))
+    assert info.value == 0
+    Lf = torch.tril(tA[:n, :n]).double().cpu().numpy()
+    np.testing.assert_allclose(Lf @ Lf.T, A, atol=2e-5 * np.abs(A).max())
+    b = np.random.default_rng(3).standard_normal(n)
+    tb = torch.zeros(lda, dtype=torch.float32, device=gpu)
+    tb[:n] = torch.tensor(b, dtype=torch.float32)
+    L.check("chol_solve", L.load().gk_cholesky_solve_f32(_h(), _p(tA), _p(dinv), lda, _p(tb), _st()))
+    x = tb[:n].double().cpu().numpy()
+    np.testing.assert_allclose(x, np.linalg.solve(A, b), rtol=0, atol=1e-4 * np.abs(np.linalg.solve(A, b)).max())
+    np.testing.assert_array_equal(tb[n:].cpu().numpy(), 0.0)
+
+
+def test_lu_f64_and_solve(gpu):
+    say("lu f64 n=600 lda=768 (saddle-point like)")
+    L = _lib()
+    n, lda = 600, 768
+    K = _spd(n - 3, 4)
+    U = np.random.default_rng(5).standard_normal((n - 3, 3))
+    A = np.zeros((n, n)); A[:n - 3, :n - 3] = K; A[:n - 3, n - 3:] = U; A[n - 3:, :n - 3] = U.T
+    tA = torch.zeros((lda, lda), dtype=torch.float64, device=gpu)
+    tA[:n, :n] = torch.tensor(A)
+    dinv = torch.empty(2 * lda * 256, dtype=torch.float64, device=gpu)
+    info = C.c_int32(-1)
+    L.check("lu", L.load().gk_lu_f64(_h(), _p(tA), n, lda, _p(dinv), C.byref(info), _st()))
+    assert info.value == 0
+    b = np.random.default_rng(6).standard_normal(n)
+    tb = torch.zeros(lda, dtype=torch.float64, device=gpu)
+    tb[:n] = torch.tensor(b)
+    L.check("lu_solve", L.load().gk_lu_solve_f64(_h(), _p(tA), _p(dinv), lda, _p(tb), _st()))
+    np.testing.assert_allclose(tb[:n].cpu().numpy(), np.linalg.solve(A, b), rtol=1e-9, atol=1e-10)
+
+
+# ------------------------------------------------------ co-kriging stages
+def _small_stack(seed=0, n_surf=3, per=40, n_ori=30, fault=False):
+    from gempy_amd.data import StackInput
+    rng = np.random.default_rng(seed)
+    pts, counts = [], []
+    for k in range(n_surf):
+        xy = rng.uniform(0, 1000, (per, 2))
+        z = 700 - 200 * k + 60 * np.sin(xy[:, 0] / 150.0) * np.cos(xy[:, 1] / 170.0) + rng.normal(0, 3, per)
+        pts.append(np.column_stack([xy, z])); counts.append(per)
+    sp = np.concatenate(pts)
+    q = np.column_stack([rng.uniform(0, 1000, (n_ori, 2)), rng.uniform(200, 800, n_ori)])
+    g = rng.normal(size=(n_ori, 3)) * 0.2 + [0, 0, 1]
+    g /= np.linalg.norm(g, axis=1, keepdims=True)
+    st = StackInput("s", sp, counts, np.full(len(sp), 1e-6), q, g, np.full(n_ori, 0.01))
+    fsp = (sp[:, 0] > 500).astype(float)[:, None] + 1.0 if fault else None
+    return st, fsp
+
+
+def _kopts(deg=1):
+    from gempy_amd.data import KernelOptions
+    a = float(np.sqrt(3) * 1000)
+    return KernelOptions(range=a, c_o=a * a / 42, uni_degree=deg)
+
+
+def _oracle_A(st, k, fsp=None):
+    return ok.build_system(st.sp_xyz, st.counts, st.sp_nugget, st.ori_xyz, st.ori_grad, st.ori_nugget,
+                           k.range, k.c_o, k.uni_degree, k.i_res, k.gi_res, fsp)
+
+
+@pytest.mark.parametrize("deg,fault", [(1, False), (2, False), (1, True), (0, False)])
+def test_assembly_f64_matches_oracle(gpu, deg, fault):
+    say(f"assemble f64 deg={deg} fault={fault}")
+    from gempy_amd.engine import DeviceStack
+    st, fsp = _small_stack(1, fault=fault)
+    k = _kopts(deg)
+    ds = DeviceStack(st, k, [500, 500, 500], fsp)
+    A, b, _ = ds.assemble()
+    Ao, bo = _oracle_A(st, k, fsp)
+    N = len(bo)
+    Ag = A[:N, :N].cpu().numpy()
+    np.testing.assert_allclose(Ag, Ao, rtol=1e-10, atol=1e-10 * np.abs(Ao).max())
+    np.testing.assert_allclose(b.cpu().numpy(), bo)
+    pad = A[N:, N:].cpu().numpy()
+    np.testing.assert_array_equal(pad, np.eye(pad.shape[0]))
+
+
+def test_assembly_f32_equilibrated(gpu):
+    say("assemble f32 equilibrated")
+    from gempy_amd.engine import DeviceStack
+    st, _ = _small_stack(2)
+    k = _kopts()
+    ds = DeviceStack(st, k, [500, 500, 500])
+    H, _, d = ds.assemble(dtype=_lib().GK_F32)
+    Ao, bo = _oracle_A(st, k)
+    Nc = len(bo) - 3
+    dd = 1 / np.sqrt(np.diag(Ao)[:Nc])
+    np.testing.assert_allclose(d.cpu().numpy(), dd, rtol=1e-12)
+    Hn = (dd[:, None] * Ao[:Nc, :Nc] * dd[None, :])
+    Hg = H[:Nc, :Nc].double().cpu().numpy()
+    low = np.tril_indices(Nc)
+    np.testing.assert_allclose(Hg[low], Hn[low], rtol=0, atol=2e-7 * np.abs(Hn).max())
+
+
+@pytest.mark.parametrize("deg,fault", [(1, False), (2, True)])
+def test_matvec_matches_oracle(gpu, deg, fault):
+    say(f"matvec deg={deg} fault={fault}")
+    from gempy_amd.engine import DeviceStack
+    st, fsp = _small_stack(3, fault=fault)
+    k = _kopts(deg)
+    ds = DeviceStack(st, k, [500, 500, 500], fsp)
+    Ao, bo = _oracle_A(st, k, fsp)
+    x = np.random.default_rng(7).standard_normal(len(bo))
+    y = ds.matvec(torch.tensor(x, device=gpu)).cpu().numpy()
+    ref = Ao @ x
+    scale = np.abs(Ao).dot(np.abs(x))
+    assert np.all(np.abs(y - ref) <= 1e-11 * scale + 1e-300)
+    y32 = ds.matvec(torch.tensor(x, device=gpu), precision=_lib().GK_F32).cpu().numpy()
+    assert np.all(np.abs(y32 - ref) <= 1e-5 * scale)
+
+
+@pytest.mark.parametrize("precision", ["f64", "f32"])
+def test_eval_points_and_lattice_match_oracle(gpu, precision):
+    say(f"eval {precision} explicit + lattice with gradient")
+    from gempy_amd.data import RegularGrid
+    from gempy_amd.engine import DeviceStack
+    L = _lib()
+    st, _ = _small_stack(4)
+    k = _kopts()
+    Ao, bo = _oracle_A(st, k)
+    w = np.linalg.solve(Ao, bo)
+    ds = DeviceStack(st, k, [500, 500, 500])
+    tw = torch.tensor(w, device=gpu)
+    grid = RegularGrid((0, 1000, 0, 1000, 0, 1000), (12, 10, 14))
+    X = grid.values()
+    So, Go = ok.evaluate(X, st.sp_xyz, st.counts, st.ori_xyz, w, k.range, k.c_o, gradient=True)
+    prec = L.GK_F64 if precision == "f64" else L.GK_F32
+    rng_s = So.max() - So.min()
+    tol = 1e-10 if precision == "f64" else 5e-3
+    S, G, _ = ds.evaluate_points(torch.tensor(X, device=gpu), tw, prec, gradient=True)
+    assert np.abs(S.cpu().numpy() - So).max() <= tol * rng_s
+    assert np.abs(G.cpu().numpy() - Go).max() <= tol * np.abs(Go).max() * 10
+    S2, G2, _ = ds.evaluate_lattice(grid, tw, prec, gradient=True)
+    assert np.abs(S2.cpu().numpy() - So).max() <= tol * rng_s
+    assert np.abs(G2.cpu().numpy() - Go).max() <= tol * np.abs(Go).max() * 10
+    say(f"eval {precision}: max|dS|/range explicit={np.abs(S.cpu().numpy() - So).max() / rng_s:.3e} "
+        f"lattice={np.abs(S2.cpu().numpy() - So).max() / rng_s:.3e}")
+
+
+@pytest.mark.parametrize("form", ["fp64_lu", "fp32_refine"])
+def test_solve_matches_oracle(gpu, form):
+    say(f"solve {form}")
+    from gempy_amd.engine import DeviceStack
+    L = _lib()
+    st, _ = _small_stack(5, n_surf=4, per=120, n_ori=90)
+    k = _kopts()
+    Ao, bo = _oracle_A(st, k)
+    wo = np.linalg.solve(Ao, bo)
+    ds = DeviceStack(st, k, [500, 500, 500])
+    f = L.GK_SOLVE_FP64_LU if form == "fp64_lu" else L.GK_SOLVE_FP32_REFINE
+    w, info = ds.solve(f)
+    say(f"solve {form}: {info}")
+    assert info["relres"] < 1e-8
+    X = np.random.default_rng(8).uniform(0, 1000, (4000, 3))
+    So = ok.evaluate(X, st.sp_xyz, st.counts, st.ori_xyz, wo, k.range, k.c_o)
+    S, _, _ = ds.evaluate_points(torch.tensor(X, device=gpu), w, L.GK_F64)
+    err = np.abs(S.cpu().numpy() - So).max() / (So.max() - So.min())
+    say(f"solve {form}: max|dS|/range = {err:.3e}")
+    assert err <= 1e-6
+
+
+def test_compute_model_config1_matches_fixture(gpu):
+    say("compute_model config1 vs golden")
+    from gempy_amd import GemPyEngineConfig, compute_model, synthetic
+    d = np.load(os.path.join(GOLD, "config1_oracle.npz"))
+    m = synthetic.config1()
+    for dt in ("float64", "float32"):
+        sol = compute_model(m, GemPyEngineConfig(dtype=dt))
+        S = sol.stacks[0].scalar_field.cpu().numpy()
+        rng_s = d["scalar"].max() - d["scalar"].min()
+        err = np.abs(S - d["scalar"]).max() / rng_s
+        say(f"config1 {dt}: max|dS|/range = {err:.3e} solver={sol.stacks[0].solver_info}")
+        assert err <= 1e-6
+        np.testing.assert_allclose(sol.stacks[0].thresholds, d["thresholds"], rtol=1e-9)
+        ids = sol.lith_block.cpu().numpy()
+        band = np.min(np.abs(d["scalar"][:, None] - d["thresholds"][None, :]), 1) <= 1e-6 * rng_s
+        assert np.array_equal(ids[~band], d["ids"][~band])
+
+
+def test_compute_model_at_anticline_corners(gpu):
+    say("compute_model_at anticline corners")
+    from gempy_amd import GeoModelInput, InterpolationOptions, KernelOptions, StackInput, compute_model_at
+    d = np.load(os.path.join(GOLD, "anticline.npz"))
+    st = StackInput("Strat_Series", d["sp_xyz"], list(d["counts"]), d["sp_nugget"], d["ori_xyz"],
+                    d["ori_grad"], d["ori_nugget"])
+    m = GeoModelInput([st], InterpolationOptions(KernelOptions(range=1.7, c_o=10.0)))
+    ids = compute_model_at(m, d["corners"])
+    np.testing.assert_array_equal(ids, d["expected_ids"])
+
+
+def test_compute_model_multistack_fault_matches_oracle(gpu):
+    say("compute_model config2 scale=0.05 grid 24^3 vs oracle")
+    from gempy_amd import GemPyEngineConfig, compute_model, synthetic
+    m = synthetic.config2(scale=0.05, resolution=(24, 24, 24))
+    X = m.grid.values()
+    res = ok.compute_model([s.as_oracle_dict() for s in m.stacks], m.options.as_dict(), X)
+    sol = compute_model(m, GemPyEngineConfig(dtype="float64"))
+    lith = sol.lith_block.cpu().numpy()
+    band = np.zeros(len(X), dtype=bool)
+    for r in res["stacks"]:
+        rng_s = r["scalar"].max() - r["scalar"].min()
+        band |= np.min(np.abs(r["scalar"][:, None] - r["thresholds"][None, :]), 1) <= 1e-6 * rng_s
+    mism = (lith != res["lith_block"]) & ~band
+    say(f"config2 small: lith mismatches outside band = {mism.sum()} / {len(X)}, band = {band.sum()}")
+    assert mism.sum() == 0
+    for s_gpu, r in zip(sol.stacks, res["stacks"]):
+        S = s_gpu.scalar_field.cpu().numpy()
+        assert np.abs(S - r["scalar"]).max() <= 1e-6 * (r["scalar"].max() - r["scalar"].min())
+    np.testing.assert_array_equal(sol.fault_blocks[0].cpu().numpy(), res["fault_blocks"][0])
