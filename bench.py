@@ -103,6 +103,9 @@ def main():
     t_fast, rec_fast, out_fast = run(_lib.GK_SOLVE_FP32_REFINE, "fp32_refine")
     res = {}
     if not args.no_fp64:
+        t_chol, rec_chol, out_chol = run(_lib.GK_SOLVE_FP64_CHOL, "fp64_chol")
+        res.update(model_s_fp64_chol=t_chol, fp64_chol_factor_ms=float(np.mean([r["info"]["t_factor_ms"] for r in rec_chol])),
+                   fp64_chol_solve_ms=float(np.mean([r["solve"] for r in rec_chol])))
         t_f64, rec_f64, out_f64 = run(_lib.GK_SOLVE_FP64_LU, "fp64_lu")
         s_fast, s_ref = out_fast[2].cpu().numpy(), out_f64[2].cpu().numpy()
         rng = float(s_ref.max() - s_ref.min())

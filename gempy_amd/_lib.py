@@ -15,7 +15,7 @@ GK_ERR = {-1: "GK_ERR_ARG", -2: "GK_ERR_HIP", -3: "GK_ERR_NOT_SPD", -4: "GK_ERR_
           -5: "GK_ERR_NOT_CONVERGED", -6: "GK_ERR_UNSUPPORTED"}
 GK_F32, GK_F64 = 0, 1
 GK_POINTS_EXPLICIT, GK_POINTS_LATTICE = 0, 1
-GK_SOLVE_FP64_LU, GK_SOLVE_FP32_REFINE = 0, 1
+GK_SOLVE_FP64_LU, GK_SOLVE_FP32_REFINE, GK_SOLVE_FP64_CHOL = 0, 1, 2
 GK_MAX_THRESHOLDS = 64
 GK_BLOCK = 256
 
@@ -63,6 +63,8 @@ SIGNATURES = {
     "gk_matvec": (C.c_int, [P, C.POINTER(GkSystem), C.c_int32, P, P, P]),
     "gk_cholesky_f32": (C.c_int, [P, P, C.c_int64, C.c_int64, P, C.POINTER(C.c_int32), P]),
     "gk_cholesky_solve_f32": (C.c_int, [P, P, P, C.c_int64, P, P]),
+    "gk_cholesky_f64": (C.c_int, [P, P, C.c_int64, C.c_int64, P, C.POINTER(C.c_int32), P]),
+    "gk_cholesky_solve_f64": (C.c_int, [P, P, P, C.c_int64, P, P]),
     "gk_lu_f64": (C.c_int, [P, P, C.c_int64, C.c_int64, P, C.POINTER(C.c_int32), P]),
     "gk_lu_solve_f64": (C.c_int, [P, P, P, C.c_int64, P, P]),
     "gk_gemm_f32": (C.c_int, [P, C.c_int64, C.c_int64, C.c_int64, P, C.c_int64, P, C.c_int64,
@@ -73,6 +75,10 @@ SIGNATURES = {
                            C.POINTER(GkSolveInfo), P]),
     "gk_eval": (C.c_int, [P, C.POINTER(GkSystem), P, C.POINTER(GkEvalDesc), P]),
     "gk_combine_lith": (C.c_int, [P, C.c_int32, P, C.c_int64, C.POINTER(C.c_int32), P, P]),
+    "gk_octree_points": (C.c_int, [P, C.POINTER(C.c_double), C.POINTER(C.c_int64), P, C.c_int64, C.c_int32,
+                                   P, P]),
+    "gk_octree_refine": (C.c_int, [P, C.c_int32, P, C.c_int64, P, C.POINTER(C.c_int64), P, P,
+                                   C.POINTER(C.c_int64), P]),
 }
 
 _lib = None

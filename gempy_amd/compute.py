@@ -6,7 +6,9 @@ ValueError for a backend this library does not provide), package the input,
 then run, per stack (youngest first; fault stacks before the stacks they
 offset): assemble -> solve -> evaluate at the surface points (thresholds) ->
 evaluate on the grid (scalar field, optional gradient, ids), and finally the
-erosion combination of the non-fault stacks into the lith block.
+erosion combination of the non-fault stacks into the lith block.  With more
+than one octree level the grid is also evaluated level by level on the
+voxels whose probe ids differ (``Solutions.octrees_output``).
 """
 from __future__ import annotations
 
@@ -19,18 +21,22 @@ import torch
 
 from . import _lib
 from .config import AvailableBackends, BackendTensor, GemPyEngineConfig
-from .data import (GeoModelInput, RegularGrid, Solutions, SolverForm, StackRelationType,
+from .data import (GeoModelInput, OctreeLevel, Solutions, SolverForm, StackRelationType,
                    StackSolution)
 from .engine import DeviceStack, _ptr, _stream, handle
 
+_FORMS = {SolverForm.FP64_DIRECT: 0, SolverForm.FP32_REFINE: 1, SolverForm.FP64_CHOL: 2}
+
 
 def _form(model: GeoModelInput, cfg: GemPyEngineConfig) -> int:
+    """engine dtype 'float32' -> FP32 factor + FP64 refinement; 'float64' -> the
+    options' FP64 form (Cholesky + Schur unless the LU is asked for)."""
     if cfg.dtype == "float32":
         return _lib.GK_SOLVE_FP32_REFINE
     if cfg.dtype == "float64":
-        return _lib.GK_SOLVE_FP64_LU
-    return (_lib.GK_SOLVE_FP64_LU if model.options.solver == SolverForm.FP64_DIRECT
-            else _lib.GK_SOLVE_FP32_REFINE)
+        return (_lib.GK_SOLVE_FP64_LU if model.options.solver == SolverForm.FP64_DIRECT
+                else _lib.GK_SOLVE_FP64_CHOL)
+    return _FORMS[model.options.solver]
 
 
 def _validate(model: GeoModelInput) -> None:
@@ -41,13 +47,60 @@ def _validate(model: GeoModelInput) -> None:
     for k, st in enumerate(model.stacks):
         if st.n_surfaces == 0:
             raise ValueError(f"stack {st.name!r} has no surface")
-        if st.n_orientations == 0 and st.relation != StackRelationType.FAULT and st.n_rest == 0:
+        if st.n_orientations == 0 and st.n_rest == 0:
             raise ValueError(f"stack {st.name!r} has neither orientations nor rest points")
         if st.n_surfaces > _lib.GK_MAX_THRESHOLDS:
             raise ValueError(f"stack {st.name!r}: at most {_lib.GK_MAX_THRESHOLDS} surfaces")
         for f in st.faults:
             if not (0 <= f < k) or model.stacks[f].relation != StackRelationType.FAULT:
                 raise ValueError(f"stack {st.name!r}: fault index {f} must name an earlier fault stack")
+    if model.grid is not None:
+        L = model.options.evaluation_options.number_octree_levels
+        if L > 1 and any(r % (1 << (L - 1)) for r in model.grid.resolution):
+            raise ValueError("with octree levels the grid resolution must be base * 2^(levels-1)")
+
+
+class _Model:
+    """The solved stacks of one model on the device."""
+
+    def __init__(self, model: GeoModelInput, form: int):
+        self.model = model
+        self.stacks: List[DeviceStack] = []
+        self.sols: List[StackSolution] = []
+        center = _center(model)
+        for st in model.stacks:
+            fault_sp = None
+            if st.faults:
+                pts = torch.tensor(st.sp_xyz, dtype=torch.float64, device="cuda")
+                fault_sp = np.stack([self._fault_ids(j, pts).cpu().numpy() for j in st.faults], 1)
+            ds = DeviceStack(st, model.options.kernel_options, center, fault_sp)
+            w, info = ds.solve(form)
+            thr, s_sp = ds.surface_thresholds(w)
+            self.stacks.append(ds)
+            self.sols.append(StackSolution(weights=w, scalar_field_at_surface_points=s_sp.cpu().numpy(),
+                                           thresholds=thr, solver_info=info))
+
+    def _fault_ids(self, j, pts):
+        ds = self.stacks[j]
+        _, _, fid = ds.evaluate_points(pts, precision=_lib.GK_F64, thresholds=ds.thresholds)
+        return fid.double()
+
+    def at_points(self, pts: torch.Tensor, precision=_lib.GK_F64, keep_scalar=False):
+        """Ids of every stack, the combined lith block and the fault blocks at pts."""
+        ids_rows, nsurf, faults, fault_of, scalars = [], [], [], {}, []
+        for k, (st, ds) in enumerate(zip(self.model.stacks, self.stacks)):
+            fv = (torch.stack([fault_of[j] for j in st.faults], 1).contiguous() if st.faults else None)
+            S, _, ids = ds.evaluate_points(pts, precision=precision, thresholds=ds.thresholds, fault_values=fv)
+            if keep_scalar:
+                scalars.append(S)
+            if st.relation == StackRelationType.FAULT:
+                fault_of[k] = ids.double()
+                faults.append(ids)
+            else:
+                ids_rows.append(ids)
+                nsurf.append(st.n_surfaces)
+        lith = _combine(ids_rows, nsurf, pts.shape[0]) if ids_rows else None
+        return lith, faults, scalars
 
 
 def compute_model(model: GeoModelInput, engine_config: Optional[GemPyEngineConfig] = None,
@@ -60,41 +113,16 @@ def compute_model(model: GeoModelInput, engine_config: Optional[GemPyEngineConfi
                                            dtype=cfg.dtype, grads=cfg.compute_grads)
     else:
         raise ValueError(f"Backend {cfg.backend} not supported")
-    form = _form(model, cfg)
-    grid = model.grid
-    opts = model.options
-    gradient = opts.evaluation_options.compute_scalar_gradient
-    center = _center(model)
     t0 = time.perf_counter()
-    stack_sols: List[StackSolution] = []
-    dev_stacks: List[DeviceStack] = []
-    fault_grid = {}         # fault stack index -> fault block on the grid (float64, device)
-    fault_custom = {}
-    custom = None if model.custom_grid is None else torch.tensor(
-        np.asarray(model.custom_grid, dtype=np.float64).reshape(-1, 3), dtype=torch.float64, device="cuda")
-    ids_rows, nsurf, custom_rows = [], [], []
-    for k, st in enumerate(model.stacks):
-        fault_sp = None
-        fv_grid = fv_custom = None
-        if st.faults:
-            cols = []
-            for j in st.faults:
-                ds = dev_stacks[j]
-                pts = torch.tensor(st.sp_xyz, dtype=torch.float64, device="cuda")
-                _, _, fid = ds.evaluate_points(pts, precision=_lib.GK_F64, thresholds=ds.thresholds)
-                cols.append(fid.double().cpu().numpy())
-            fault_sp = np.stack(cols, 1)
-            if grid is not None:
-                fv_grid = torch.stack([fault_grid[j] for j in st.faults], 1).contiguous()
-            if custom is not None:
-                fv_custom = torch.stack([fault_custom[j] for j in st.faults], 1).contiguous()
-        ds = DeviceStack(st, opts.kernel_options, center, fault_sp)
-        w, info = ds.solve(form)
-        thr, s_sp = ds.surface_thresholds(w)
-        sol = StackSolution(weights=w, scalar_field_at_surface_points=s_sp.cpu().numpy(),
-                            thresholds=thr, solver_info=info)
-        if grid is not None:
-            S, G, ids = ds.evaluate_lattice(grid, w, eval_precision, gradient, thr, fv_grid)
+    grid, opts = model.grid, model.options
+    gradient = opts.evaluation_options.compute_scalar_gradient
+    m = _Model(model, _form(model, cfg))
+    out = Solutions(stacks=m.sols)
+    if grid is not None:
+        ids_rows, nsurf, fault_grid = [], [], {}
+        for k, (st, ds, sol) in enumerate(zip(model.stacks, m.stacks, m.sols)):
+            fv = torch.stack([fault_grid[j] for j in st.faults], 1).contiguous() if st.faults else None
+            S, G, ids = ds.evaluate_lattice(grid, sol.weights, eval_precision, gradient, sol.thresholds, fv)
             sol.scalar_field = S if keep_scalar else None
             sol.gradient = G
             sol.ids_block = ids
@@ -102,37 +130,69 @@ def compute_model(model: GeoModelInput, engine_config: Optional[GemPyEngineConfi
                 fault_grid[k] = ids.double()
             else:
                 ids_rows.append(ids)
-        if custom is not None:
-            _, _, cids = ds.evaluate_points(custom, w, _lib.GK_F64, thresholds=thr, fault_values=fv_custom)
-            if st.relation == StackRelationType.FAULT:
-                fault_custom[k] = cids.double()
-            else:
-                custom_rows.append(cids)
-        if st.relation != StackRelationType.FAULT:
-            nsurf.append(st.n_surfaces)
-        dev_stacks.append(ds)
-        stack_sols.append(sol)
-    out = Solutions(stacks=stack_sols)
-    if grid is not None and ids_rows:
-        out.lith_block = _combine(ids_rows, nsurf, grid.n_points)
-        out.fault_blocks = [stack_sols[k].ids_block for k in sorted(fault_grid)]
+                nsurf.append(st.n_surfaces)
+        if ids_rows:
+            out.lith_block = _combine(ids_rows, nsurf, grid.n_points)
+        out.fault_blocks = [m.sols[k].ids_block for k in sorted(fault_grid)]
         if keep_scalar:
-            out.scalar_field_matrix = torch.stack([s.scalar_field for s in stack_sols])
-    if custom is not None and custom_rows:
-        out.custom = _combine(custom_rows, nsurf, custom.shape[0])
+            out.scalar_field_matrix = torch.stack([s.scalar_field for s in m.sols])
+        levels = opts.evaluation_options.number_octree_levels
+        if levels > 1:
+            out.octrees_output = compute_octree(m, levels)
+    if model.custom_grid is not None:
+        pts = torch.tensor(np.asarray(model.custom_grid, dtype=np.float64).reshape(-1, 3), dtype=torch.float64,
+                           device="cuda")
+        out.custom, _, _ = m.at_points(pts)
     torch.cuda.synchronize()
     out.timings["model_s"] = time.perf_counter() - t0
+    return out
+
+
+def compute_octree(m: _Model, levels: int) -> List[OctreeLevel]:
+    """Level-by-level evaluation from the base grid (resolution / 2^(levels-1))
+    down to the finest level, probing centre + corners of each active voxel."""
+    lib = _lib.load()
+    grid = m.model.grid
+    ext = (C.c_double * 6)(*grid.extent)
+    res = [r >> (levels - 1) for r in grid.resolution]
+    vox = torch.arange(res[0] * res[1] * res[2], dtype=torch.int64, device="cuda")
+    out: List[OctreeLevel] = []
+    for lvl in range(levels):
+        n = int(vox.numel())
+        cres = (C.c_int64 * 3)(*res)
+        last = lvl == levels - 1
+        npts = n if last else 9 * n
+        pts = torch.empty((max(npts, 1), 3), dtype=torch.float64, device="cuda")
+        _lib.check("gk_octree_points", lib.gk_octree_points(handle(0), ext, cres, _ptr(vox), n, int(last),
+                                                            _ptr(pts), _stream()))
+        pts = pts[:npts]
+        lith, faults, scal = m.at_points(pts, keep_scalar=True)
+        if last:
+            out.append(OctreeLevel(tuple(res), vox, lith, scal[-1] if scal else None, None))
+            break
+        fields = torch.stack([lith] + faults).contiguous() if lith is not None else torch.stack(faults)
+        refine = torch.empty(max(n, 1), dtype=torch.uint8, device="cuda")
+        children = torch.empty(max(8 * n, 1), dtype=torch.int64, device="cuda")
+        nch = C.c_int64(0)
+        _lib.check("gk_octree_refine", lib.gk_octree_refine(handle(0), fields.shape[0], _ptr(fields), n, _ptr(vox),
+                                                            cres, _ptr(refine), _ptr(children), C.byref(nch),
+                                                            _stream()))
+        centre_scalar = scal[-1][0::9] if scal else None
+        out.append(OctreeLevel(tuple(res), vox, lith[0::9] if lith is not None else None, centre_scalar,
+                               refine[:n].bool()))
+        vox = children[:nch.value].clone()
+        res = [2 * r for r in res]
     return out
 
 
 def _combine(rows, nsurf, n) -> torch.Tensor:
     lib = _lib.load()
     ids = torch.stack(rows).contiguous()
-    lith = torch.empty(n, dtype=torch.int32, device="cuda")
+    lith = torch.empty(max(n, 1), dtype=torch.int32, device="cuda")
     ns = (C.c_int32 * len(nsurf))(*nsurf)
     _lib.check("gk_combine_lith", lib.gk_combine_lith(handle(0), len(rows), _ptr(ids), n, ns, _ptr(lith),
                                                       _stream()))
-    return lith
+    return lith[:n]
 
 
 def _center(model: GeoModelInput):

@@ -199,35 +199,30 @@ extern "C" int gk_assemble_impl(gk_handle h, const gk_system* sys, int32_t dtype
                                 double* rhs, double* dscale, hipStream_t st, int lower_only) {
   (void)h;
   Geo G = make_geo(sys);
+  GK_REQUIRE(dtype == GK_F32 || dtype == GK_F64, GK_ERR_ARG, "gk_assemble: dtype must be GK_F32 or GK_F64");
+  GK_REQUIRE(A != nullptr, GK_ERR_ARG, "gk_assemble: A is NULL");
+  const bool equil = dscale != nullptr;
+  GK_REQUIRE(equil || dtype == GK_F64, GK_ERR_ARG, "gk_assemble: the FP32 matrix is the equilibrated block");
+  const int64_t need = equil ? G.Nc : G.N;
+  GK_REQUIRE(lda >= need && lda % GK_BLOCK == 0, GK_ERR_ARG,
+             "gk_assemble: lda (%lld) must be >= %lld and a multiple of %d", (long long)lda, (long long)need,
+             GK_BLOCK);
+  if (equil && G.Nc > 0)
+    hipLaunchKernelGGL(diag_scale_kernel, dim3((unsigned)gk_ceil_div(G.Nc, 256)), dim3(256), 0, st, G, dscale);
+  dim3 grid((unsigned)gk_ceil_div(lda, 64), (unsigned)gk_ceil_div(lda, 64));
   if (dtype == GK_F64) {
-    GK_REQUIRE(A != nullptr, GK_ERR_ARG, "gk_assemble: A is NULL");
-    GK_REQUIRE(lda >= G.N && lda % GK_BLOCK == 0, GK_ERR_ARG,
-               "gk_assemble: lda (%lld) must be >= N (%lld) and a multiple of %d", (long long)lda,
-               (long long)G.N, GK_BLOCK);
-    dim3 grid((unsigned)gk_ceil_div(lda, 64), (unsigned)gk_ceil_div(lda, 64));
-    if (lower_only)
-      hipLaunchKernelGGL((assemble_kernel<double, false, true>), grid, dim3(64, 4), 0, st, G,
-                         static_cast<double*>(A), lda, lda, nullptr);
-    else
-      hipLaunchKernelGGL((assemble_kernel<double, false, false>), grid, dim3(64, 4), 0, st, G,
-                         static_cast<double*>(A), lda, lda, nullptr);
-  } else if (dtype == GK_F32) {
-    GK_REQUIRE(A != nullptr && dscale != nullptr, GK_ERR_ARG, "gk_assemble: F32 needs A and dscale");
-    GK_REQUIRE(lda >= G.Nc && lda % GK_BLOCK == 0, GK_ERR_ARG,
-               "gk_assemble: lda (%lld) must be >= Nc (%lld) and a multiple of %d", (long long)lda,
-               (long long)G.Nc, GK_BLOCK);
-    if (G.Nc > 0)
-      hipLaunchKernelGGL(diag_scale_kernel, dim3((unsigned)gk_ceil_div(G.Nc, 256)), dim3(256), 0, st,
-                         G, dscale);
-    dim3 grid((unsigned)gk_ceil_div(lda, 64), (unsigned)gk_ceil_div(lda, 64));
-    if (lower_only)
-      hipLaunchKernelGGL((assemble_kernel<float, true, true>), grid, dim3(64, 4), 0, st, G,
-                         static_cast<float*>(A), lda, lda, dscale);
-    else
-      hipLaunchKernelGGL((assemble_kernel<float, true, false>), grid, dim3(64, 4), 0, st, G,
-                         static_cast<float*>(A), lda, lda, dscale);
+    double* M = static_cast<double*>(A);
+    if (equil) {
+      if (lower_only) hipLaunchKernelGGL((assemble_kernel<double, true, true>), grid, dim3(64, 4), 0, st, G, M, lda, lda, dscale);
+      else hipLaunchKernelGGL((assemble_kernel<double, true, false>), grid, dim3(64, 4), 0, st, G, M, lda, lda, dscale);
+    } else {
+      if (lower_only) hipLaunchKernelGGL((assemble_kernel<double, false, true>), grid, dim3(64, 4), 0, st, G, M, lda, lda, nullptr);
+      else hipLaunchKernelGGL((assemble_kernel<double, false, false>), grid, dim3(64, 4), 0, st, G, M, lda, lda, nullptr);
+    }
   } else {
-    GK_REQUIRE(false, GK_ERR_ARG, "gk_assemble: dtype must be GK_F32 or GK_F64");
+    float* M = static_cast<float*>(A);
+    if (lower_only) hipLaunchKernelGGL((assemble_kernel<float, true, true>), grid, dim3(64, 4), 0, st, G, M, lda, lda, dscale);
+    else hipLaunchKernelGGL((assemble_kernel<float, true, false>), grid, dim3(64, 4), 0, st, G, M, lda, lda, dscale);
   }
   if (rhs && G.N > 0)
     hipLaunchKernelGGL(rhs_kernel, dim3((unsigned)gk_ceil_div(G.N, 256)), dim3(256), 0, st, G, rhs);

@@ -5,9 +5,10 @@
 //    permuted so that each lane reads its 4 (f64) / 8 (f32) consecutive k
 //    values with 128-bit LDS loads; MFMA only needs A and B to agree on it.
 //  * Right-looking blocked Cholesky (FP32) and LU without pivoting (FP64) with
-//    256-wide panels: the diagonal block is factored and inverted by one
-//    workgroup, the panel is a GEMM with the inverted diagonal block, and the
-//    trailing update is the MFMA GEMM (lower tiles only for Cholesky).
+//    256-wide panels: the diagonal block is factored and inverted as two
+//    LDS-resident 128-wide halves joined by 128^3 GEMMs, the panel is a GEMM
+//    with the inverted diagonal block, and the trailing update is the MFMA
+//    GEMM (lower tiles only for Cholesky).
 //  * Blocked triangular solves with the inverted diagonal blocks.
 #include <cmath>
 #include "gk_internal.h"
@@ -167,14 +168,17 @@ __global__ __launch_bounds__(256) void gemm_kernel(int64_t m, int64_t n, int64_t
     __syncthreads();
   }
 
-  // epilogue: lane holds rows 4*(lane/16)+r, column lane%16 of each 16x16 tile
+  // epilogue: lane holds column lane%16 of each 16x16 tile; its rows are
+  // 4*(lane/16)+r for the f32 instruction and lane/16+4*r for the f64 one
+  // (both checked with an in-bounds lane-layout probe on gfx950).
 #pragma unroll
   for (int i = 0; i < 4; ++i)
 #pragma unroll
     for (int j = 0; j < 4; ++j)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int64_t row = m0 + wm * 64 + i * 16 + lk * 4 + r;
+        const int rr = sizeof(T) == 8 ? lk + 4 * r : lk * 4 + r;
+        const int64_t row = m0 + wm * 64 + i * 16 + rr;
         const int64_t col = n0 + wn * 64 + j * 16 + lrow;
         T* pc = C + row * ldc + col;
         const T prod = acc[i][j][r];
@@ -211,103 +215,198 @@ int gemm_launch(int64_t m, int64_t n, int64_t k, const T* A, int64_t lda, const 
 }
 
 // ------------------------------------------------------- diagonal blocks
-// Cholesky of one NB x NB block in place (lower), then its inverse into Dinv.
-// One workgroup of 1024 threads; the block stays in L2.
-__global__ __launch_bounds__(1024) void potrf_diag_kernel(float* A, int64_t lda, int64_t k0,
-                                                          float* Dinv, int32_t* info) {
-  float* D = A + k0 * lda + k0;
-  const int tid = threadIdx.x;
-  __shared__ float piv;
-  __shared__ float col[NB];
-  for (int j = 0; j < NB; ++j) {
+// A 256-wide diagonal block is factored as two 128-wide halves: each half is
+// factored and inverted by one workgroup with the block resident in LDS, and
+// the halves are joined with 128^3 MFMA GEMMs (trsm/syrk and the off-diagonal
+// quadrant of the inverse).
+constexpr int DB = 128;
+
+// Cholesky of a DB x DB FP32 block (lower) in LDS and the inverse of its factor.
+__global__ __launch_bounds__(1024) void potrf128_kernel(float* A, int64_t lda, float* X, int64_t ldx,
+                                                        int32_t* info, int64_t gbase) {
+  __shared__ float L[DB][DB + 1];
+  __shared__ float Xs[DB][DB + 1];
+  const int tid = threadIdx.x, tx = tid & 31, ty = tid >> 5;
+  for (int e = tid; e < DB * DB; e += 1024) {
+    const int i = e / DB, c = e % DB;
+    L[i][c] = (c <= i) ? A[i * lda + c] : 0.0f;
+    Xs[i][c] = (i == c) ? 1.0f : 0.0f;
+  }
+  __syncthreads();
+  for (int j = 0; j < DB; ++j) {
+    float d = L[j][j];
+    const bool bad = !(d > 0.0f);
+    if (bad) d = 1.0f;
+    const float piv = sqrtf(d), ip = 1.0f / piv;
+    __syncthreads();
     if (tid == 0) {
-      float d = D[j * lda + j];
-      if (!(d > 0.0f)) {
-        atomicCAS(info, 0, (int32_t)(k0 + j + 1));
-        d = 1.0f;
-      }
-      piv = sqrtf(d);
-      D[j * lda + j] = piv;
+      L[j][j] = piv;
+      if (bad) atomicCAS(info, 0, (int32_t)(gbase + j + 1));
     }
+    if (tid > j && tid < DB) L[tid][j] *= ip;
     __syncthreads();
-    const float ip = 1.0f / piv;
-    for (int i = j + 1 + tid; i < NB; i += 1024) {
-      float v = D[i * lda + j] * ip;
-      D[i * lda + j] = v;
-      col[i] = v;
-    }
-    __syncthreads();
-    // trailing lower update: rows i > j, cols j < c <= i
-    const int rem = NB - j - 1;
-    const int64_t total = (int64_t)rem * (rem + 1) / 2;
-    for (int64_t e = tid; e < total; e += 1024) {
-      int ii = (int)((sqrtf(8.0f * (float)e + 1.0f) - 1.0f) * 0.5f);
-      while ((int64_t)(ii + 1) * (ii + 2) / 2 <= e) ++ii;
-      while ((int64_t)ii * (ii + 1) / 2 > e) --ii;
-      const int cc = (int)(e - (int64_t)ii * (ii + 1) / 2);
-      const int i = j + 1 + ii, c = j + 1 + cc;
-      D[i * lda + c] -= col[i] * col[c];
-    }
+    for (int i = j + 1 + ty; i < DB; i += 32)
+      for (int c = j + 1 + tx; c <= i; c += 32) L[i][c] -= L[i][j] * L[c][j];
     __syncthreads();
   }
-  // inverse of the lower factor: thread c computes column c of L^-1
-  float* X = Dinv + k0 * NB;     // NB x NB row-major block
-  for (int c = tid; c < NB; c += 1024) {
-    for (int i = 0; i < NB; ++i) {
-      if (i < c) { X[i * NB + c] = 0.0f; continue; }
-      float s = (i == c) ? 1.0f : 0.0f;
-      for (int kk = c; kk < i; ++kk) s -= D[i * lda + kk] * X[kk * NB + c];
-      X[i * NB + c] = s / D[i * lda + i];
-    }
+  // X = L^-1, right-looking forward substitution
+  for (int k = 0; k < DB; ++k) {
+    const float ip = 1.0f / L[k][k];
+    if (tid <= k) Xs[k][tid] *= ip;
+    __syncthreads();
+    for (int i = k + 1 + ty; i < DB; i += 32)
+      for (int c = tx; c <= k; c += 32) Xs[i][c] -= L[i][k] * Xs[k][c];
+    __syncthreads();
+  }
+  for (int e = tid; e < DB * DB; e += 1024) {
+    const int i = e / DB, c = e % DB;
+    if (c <= i) A[i * lda + c] = L[i][c];
+    X[i * ldx + c] = Xs[i][c];
   }
 }
 
-// LU without pivoting of one NB x NB FP64 block, then inverses of the unit
-// lower factor (Dinv) and of the upper factor (Dinv + lda*NB).
-__global__ __launch_bounds__(1024) void getrf_diag_kernel(double* A, int64_t lda, int64_t k0,
-                                                          double* Dinv, int64_t ld_total, int32_t* info) {
-  double* D = A + k0 * lda + k0;
-  const int tid = threadIdx.x;
-  __shared__ double rowp[NB];
-  __shared__ double colp[NB];
-  for (int j = 0; j < NB; ++j) {
-    __syncthreads();
-    double p = D[j * lda + j];
-    if (!(fabs(p) > 0.0)) {
-      if (tid == 0) atomicCAS(info, 0, (int32_t)(k0 + j + 1));
-      p = 1.0;
-    }
+// LU without pivoting of a DB x DB FP64 block in LDS, then the inverses of its
+// unit-lower factor (Li) and of its upper factor (Ui), each computed with the
+// factor and the inverse packed in the same LDS buffer.
+__global__ __launch_bounds__(1024) void getrf128_kernel(double* A, int64_t lda, double* Li, double* Ui,
+                                                        int64_t ldx, int32_t* info, int64_t gbase) {
+  __shared__ double S[DB * (DB + 1)];
+  const int tid = threadIdx.x, tx = tid & 31, ty = tid >> 5;
+  constexpr int LD = DB + 1;
+  for (int e = tid; e < DB * DB; e += 1024) S[(e / DB) * LD + e % DB] = A[(e / DB) * lda + e % DB];
+  __syncthreads();
+  for (int j = 0; j < DB; ++j) {
+    double p = S[j * LD + j];
+    const bool bad = !(fabs(p) > 0.0);
+    if (bad) p = 1.0;
     const double ip = 1.0 / p;
-    for (int i = j + 1 + tid; i < NB; i += 1024) {
-      const double v = D[i * lda + j] * ip;
-      colp[i] = v;
-      rowp[i] = D[j * lda + i];
-    }
     __syncthreads();
-    for (int i = j + 1 + tid; i < NB; i += 1024) D[i * lda + j] = colp[i];
-    const int rem = NB - j - 1;
-    for (int64_t e = tid; e < (int64_t)rem * rem; e += 1024) {
-      const int i = j + 1 + (int)(e / rem), c = j + 1 + (int)(e % rem);
-      D[i * lda + c] -= colp[i] * rowp[c];
-    }
+    if (tid == 0 && bad) atomicCAS(info, 0, (int32_t)(gbase + j + 1));
+    if (tid > j && tid < DB) S[tid * LD + j] *= ip;
+    __syncthreads();
+    for (int i = j + 1 + ty; i < DB; i += 32)
+      for (int c = j + 1 + tx; c < DB; c += 32) S[i * LD + c] -= S[i * LD + j] * S[j * LD + c];
+    __syncthreads();
+  }
+  for (int e = tid; e < DB * DB; e += 1024) A[(e / DB) * lda + e % DB] = S[(e / DB) * LD + e % DB];
+  __syncthreads();
+  // --- unit-lower inverse: strict lower L packed at [0, 8128), X packed lower at [8128, ...)
+  double* Lp = S;
+  double* Xp = S + DB * (DB - 1) / 2;
+  auto psl = [](int i, int c) { return i * (i - 1) / 2 + c; };
+  auto pl = [](int i, int c) { return i * (i + 1) / 2 + c; };
+  for (int e = tid; e < DB * DB; e += 1024) {
+    const int i = e / DB, c = e % DB;
+    if (c < i) Lp[psl(i, c)] = A[i * lda + c];
   }
   __syncthreads();
-  double* Li = Dinv + k0 * NB;               // unit lower inverse
-  double* Ui = Dinv + ld_total * NB + k0 * NB; // upper inverse
-  for (int c = tid; c < NB; c += 1024) {
-    for (int i = 0; i < NB; ++i) {            // L^-1 column c (forward)
-      if (i < c) { Li[i * NB + c] = 0.0; continue; }
-      double s = (i == c) ? 1.0 : 0.0;
-      for (int kk = c; kk < i; ++kk) s -= D[i * lda + kk] * Li[kk * NB + c];
-      Li[i * NB + c] = s;
-    }
-    for (int i = NB - 1; i >= 0; --i) {       // U^-1 column c (backward)
-      if (i > c) { Ui[i * NB + c] = 0.0; continue; }
-      double s = (i == c) ? 1.0 : 0.0;
-      for (int kk = i + 1; kk <= c; ++kk) s -= D[i * lda + kk] * Ui[kk * NB + c];
-      Ui[i * NB + c] = s / D[i * lda + i];
-    }
+  for (int e = tid; e < DB * DB; e += 1024) {
+    const int i = e / DB, c = e % DB;
+    if (c <= i) Xp[pl(i, c)] = (i == c) ? 1.0 : 0.0;
   }
+  __syncthreads();
+  for (int k = 0; k < DB; ++k) {
+    for (int i = k + 1 + ty; i < DB; i += 32) {
+      const double l = Lp[psl(i, k)];
+      for (int c = tx; c <= k; c += 32) Xp[pl(i, c)] -= l * Xp[pl(k, c)];
+    }
+    __syncthreads();
+  }
+  for (int e = tid; e < DB * DB; e += 1024) {
+    const int i = e / DB, c = e % DB;
+    Li[i * ldx + c] = (c <= i) ? Xp[pl(i, c)] : 0.0;
+  }
+  __syncthreads();
+  // --- upper inverse: U packed upper at [0, 8256), Y packed upper at [8256, ...)
+  double* Up = S;
+  double* Yp = S + DB * (DB + 1) / 2;
+  auto pu = [](int i, int c) { return i * (2 * DB - i + 1) / 2 + (c - i); };
+  for (int e = tid; e < DB * DB; e += 1024) {
+    const int i = e / DB, c = e % DB;
+    if (c >= i) { Up[pu(i, c)] = A[i * lda + c]; Yp[pu(i, c)] = (i == c) ? 1.0 : 0.0; }
+  }
+  __syncthreads();
+  for (int k = DB - 1; k >= 0; --k) {
+    const double ip = 1.0 / Up[pu(k, k)];
+    if (tid >= k && tid < DB) Yp[pu(k, tid)] *= ip;
+    __syncthreads();
+    for (int i = ty; i < k; i += 32) {
+      const double u = Up[pu(i, k)];
+      for (int c = k + tx; c < DB; c += 32) Yp[pu(i, c)] -= u * Yp[pu(k, c)];
+    }
+    __syncthreads();
+  }
+  for (int e = tid; e < DB * DB; e += 1024) {
+    const int i = e / DB, c = e % DB;
+    Ui[i * ldx + c] = (c >= i) ? Yp[pu(i, c)] : 0.0;
+  }
+}
+
+// Cholesky of a DB x DB FP64 block and the inverse of its factor, both packed
+// (lower triangle) in one LDS buffer.
+__global__ __launch_bounds__(1024) void potrf128_f64_kernel(double* A, int64_t lda, double* X, int64_t ldx,
+                                                            int32_t* info, int64_t gbase) {
+  __shared__ double S[DB * (DB + 1)];
+  double* Lp = S;                         // packed lower incl. diagonal
+  double* Xp = S + DB * (DB + 1) / 2;
+  auto pl = [](int i, int c) { return i * (i + 1) / 2 + c; };
+  const int tid = threadIdx.x, tx = tid & 31, ty = tid >> 5;
+  for (int e = tid; e < DB * DB; e += 1024) {
+    const int i = e / DB, c = e % DB;
+    if (c <= i) { Lp[pl(i, c)] = A[i * lda + c]; Xp[pl(i, c)] = (i == c) ? 1.0 : 0.0; }
+  }
+  __syncthreads();
+  for (int j = 0; j < DB; ++j) {
+    double d = Lp[pl(j, j)];
+    const bool bad = !(d > 0.0);
+    if (bad) d = 1.0;
+    const double piv = sqrt(d), ip = 1.0 / piv;
+    __syncthreads();
+    if (tid == 0) {
+      Lp[pl(j, j)] = piv;
+      if (bad) atomicCAS(info, 0, (int32_t)(gbase + j + 1));
+    }
+    if (tid > j && tid < DB) Lp[pl(tid, j)] *= ip;
+    __syncthreads();
+    for (int i = j + 1 + ty; i < DB; i += 32) {
+      const double lij = Lp[pl(i, j)];
+      for (int c = j + 1 + tx; c <= i; c += 32) Lp[pl(i, c)] -= lij * Lp[pl(c, j)];
+    }
+    __syncthreads();
+  }
+  for (int k = 0; k < DB; ++k) {
+    const double ip = 1.0 / Lp[pl(k, k)];
+    if (tid <= k) Xp[pl(k, tid)] *= ip;
+    __syncthreads();
+    for (int i = k + 1 + ty; i < DB; i += 32) {
+      const double l = Lp[pl(i, k)];
+      for (int c = tx; c <= k; c += 32) Xp[pl(i, c)] -= l * Xp[pl(k, c)];
+    }
+    __syncthreads();
+  }
+  for (int e = tid; e < DB * DB; e += 1024) {
+    const int i = e / DB, c = e % DB;
+    if (c <= i) A[i * lda + c] = Lp[pl(i, c)];
+    X[i * ldx + c] = (c <= i) ? Xp[pl(i, c)] : 0.0;
+  }
+}
+
+template <typename T> struct Potrf128;
+template <> struct Potrf128<float> {
+  static void launch(float* A, int64_t lda, float* X, int64_t ldx, int32_t* info, int64_t g, hipStream_t st) {
+    hipLaunchKernelGGL(potrf128_kernel, dim3(1), dim3(1024), 0, st, A, lda, X, ldx, info, g);
+  }
+};
+template <> struct Potrf128<double> {
+  static void launch(double* A, int64_t lda, double* X, int64_t ldx, int32_t* info, int64_t g, hipStream_t st) {
+    hipLaunchKernelGGL(potrf128_f64_kernel, dim3(1), dim3(1024), 0, st, A, lda, X, ldx, info, g);
+  }
+};
+
+template <typename T>
+__global__ void zero_block_kernel(T* X, int64_t ldx, int rows, int cols) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < rows * cols) X[(int64_t)(e / cols) * ldx + e % cols] = T(0);
 }
 
 // Identity padding of rows/cols [n, lda).
@@ -385,6 +484,57 @@ __global__ __launch_bounds__(256) void diag_apply_kernel(const T* Dk, T* b) {
   b[i] = s;
 }
 
+// Factor + invert one 256-wide diagonal block from two 128-wide halves.
+// Cholesky: L = [L11 0; L21 L22], L^-1 = [X11 0; -X22 L21 X11  X22].
+template <typename T>
+int diag256_chol(T* A, int64_t lda, int64_t k0, T* dinv, int32_t* dinfo, T* T1, T* T2, hipStream_t st) {
+  T* A11 = A + k0 * lda + k0;
+  T* A21 = A + (k0 + DB) * lda + k0;
+  T* A22 = A21 + DB;
+  T* D = dinv + k0 * NB;                 // 256 x 256, ld NB
+  Potrf128<T>::launch(A11, lda, D, NB, dinfo, k0, st);
+  GK_TRY(gemm_launch<T>(DB, DB, DB, A21, lda, D, NB, true, T(1), T(0), T1, DB, false, st));        // L21
+  hipLaunchKernelGGL(copy_block_kernel<T>, dim3(DB * DB / 256), dim3(256), 0, st, T1, (int64_t)DB, A21,
+                     lda, (int64_t)DB, (int64_t)DB);
+  GK_TRY(gemm_launch<T>(DB, DB, DB, T1, DB, T1, DB, true, T(-1), T(1), A22, lda, false, st));      // syrk
+  Potrf128<T>::launch(A22, lda, D + DB * NB + DB, NB, dinfo, k0 + DB, st);
+  GK_TRY(gemm_launch<T>(DB, DB, DB, T1, DB, D, NB, false, T(1), T(0), T2, DB, false, st));         // L21 X11
+  GK_TRY(gemm_launch<T>(DB, DB, DB, D + DB * NB + DB, NB, T2, DB, false, T(-1), T(0), D + DB * NB, NB,
+                        false, st));
+  hipLaunchKernelGGL(zero_block_kernel<T>, dim3(DB * DB / 256), dim3(256), 0, st, D + DB, (int64_t)NB, DB, DB);
+  return GK_OK;
+}
+
+// LU: L^-1 = [Li11 0; -Li22 L21 Li11  Li22], U^-1 = [Ui11  -Ui11 U12 Ui22; 0 Ui22].
+int diag256_lu(double* A, int64_t lda, int64_t k0, double* dinv, int32_t* dinfo, double* T, hipStream_t st) {
+  double* A11 = A + k0 * lda + k0;
+  double* A12 = A11 + DB;
+  double* A21 = A + (k0 + DB) * lda + k0;
+  double* A22 = A21 + DB;
+  double* Li = dinv + k0 * NB;
+  double* Ui = dinv + lda * NB + k0 * NB;
+  double *T1 = T, *T2 = T + DB * DB, *T3 = T + 2 * DB * DB, *T4 = T + 3 * DB * DB;
+  hipLaunchKernelGGL(getrf128_kernel, dim3(1), dim3(1024), 0, st, A11, lda, Li, Ui, (int64_t)NB, dinfo, k0);
+  GK_TRY(gemm_launch<double>(DB, DB, DB, Li, NB, A12, lda, false, 1.0, 0.0, T1, DB, false, st));    // U12
+  GK_TRY(gemm_launch<double>(DB, DB, DB, A21, lda, Ui, NB, false, 1.0, 0.0, T2, DB, false, st));    // L21
+  hipLaunchKernelGGL(copy_block_kernel<double>, dim3(DB * DB / 256), dim3(256), 0, st, T1, (int64_t)DB, A12,
+                     lda, (int64_t)DB, (int64_t)DB);
+  hipLaunchKernelGGL(copy_block_kernel<double>, dim3(DB * DB / 256), dim3(256), 0, st, T2, (int64_t)DB, A21,
+                     lda, (int64_t)DB, (int64_t)DB);
+  GK_TRY(gemm_launch<double>(DB, DB, DB, T2, DB, T1, DB, false, -1.0, 1.0, A22, lda, false, st));
+  hipLaunchKernelGGL(getrf128_kernel, dim3(1), dim3(1024), 0, st, A22, lda, Li + DB * NB + DB,
+                     Ui + DB * NB + DB, (int64_t)NB, dinfo, k0 + DB);
+  GK_TRY(gemm_launch<double>(DB, DB, DB, T2, DB, Li, NB, false, 1.0, 0.0, T3, DB, false, st));      // L21 Li11
+  GK_TRY(gemm_launch<double>(DB, DB, DB, Li + DB * NB + DB, NB, T3, DB, false, -1.0, 0.0, Li + DB * NB, NB,
+                             false, st));
+  GK_TRY(gemm_launch<double>(DB, DB, DB, T1, DB, Ui + DB * NB + DB, NB, false, 1.0, 0.0, T4, DB, false, st));
+  GK_TRY(gemm_launch<double>(DB, DB, DB, Ui, NB, T4, DB, false, -1.0, 0.0, Ui + DB, NB, false, st));
+  hipLaunchKernelGGL(zero_block_kernel<double>, dim3(DB * DB / 256), dim3(256), 0, st, Li + DB, (int64_t)NB, DB, DB);
+  hipLaunchKernelGGL(zero_block_kernel<double>, dim3(DB * DB / 256), dim3(256), 0, st, Ui + DB * NB, (int64_t)NB,
+                     DB, DB);
+  return GK_OK;
+}
+
 }  // namespace
 
 // ------------------------------------------------------------------ API
@@ -404,36 +554,40 @@ extern "C" int gk_gemm_f64(gk_handle h, int64_t m, int64_t n, int64_t k, const d
                              (hipStream_t)stream);
 }
 
-extern "C" int gk_cholesky_f32(gk_handle h, float* A, int64_t n, int64_t lda, float* dinv, int32_t* info,
-                               gk_stream stream) {
-  hipStream_t st = (hipStream_t)stream;
-  GK_REQUIRE(A && dinv, GK_ERR_ARG, "gk_cholesky_f32: A and dinv required");
+template <typename T>
+static int cholesky_impl(gk_handle h, T* A, int64_t n, int64_t lda, T* dinv, int32_t* info, hipStream_t st) {
+  GK_REQUIRE(A && dinv, GK_ERR_ARG, "gk_cholesky: A and dinv required");
   GK_REQUIRE(lda % NB == 0 && n <= lda && n >= 0, GK_ERR_ARG,
-             "gk_cholesky_f32: lda must be a multiple of %d and >= n", NB);
+             "gk_cholesky: lda must be a multiple of %d and >= n", NB);
   const int64_t nb = lda / NB;
   void* ws = nullptr;
-  // workspace: info (256 B) + W panel (lda x NB floats)
-  GK_TRY(gk_workspace(h, WS_FACTOR, 256 + (size_t)lda * NB * sizeof(float), &ws));
+  GK_TRY(gk_workspace(h, WS_FACTOR, 256 + ((size_t)lda * NB + 2 * DB * DB) * sizeof(T), &ws));
   int32_t* dinfo = static_cast<int32_t*>(ws);
-  float* W = reinterpret_cast<float*>(static_cast<char*>(ws) + 256);
+  T* W = reinterpret_cast<T*>(static_cast<char*>(ws) + 256);
+  T* Tb = W + (size_t)lda * NB;
   GK_HIP(hipMemsetAsync(dinfo, 0, sizeof(int32_t), st));
   if (n < lda) {
     const int64_t tot = (lda - n) * lda + n * (lda - n);
-    hipLaunchKernelGGL(pad_identity_kernel<float>, dim3((unsigned)gk_ceil_div(tot, 256)), dim3(256), 0, st,
-                       A, lda, n);
+    hipLaunchKernelGGL(pad_identity_kernel<T>, dim3((unsigned)gk_ceil_div(tot, 256)), dim3(256), 0, st, A, lda, n);
   }
   for (int64_t kb = 0; kb < nb; ++kb) {
     const int64_t k0 = kb * NB;
-    hipLaunchKernelGGL(potrf_diag_kernel, dim3(1), dim3(1024), 0, st, A, lda, k0, dinv, dinfo);
+    GK_TRY(diag256_chol<T>(A, lda, k0, dinv, dinfo, Tb, Tb + DB * DB, st));
     const int64_t rem = lda - k0 - NB;
     if (rem <= 0) break;
     // W = A21 Linv^T ;  A21 <- W ; A22 -= W W^T (lower)
-    GK_TRY(gemm_launch<float>(rem, NB, NB, A + (k0 + NB) * lda + k0, lda, dinv + k0 * NB, NB, true, 1.0f,
-                              0.0f, W, NB, false, st));
-    hipLaunchKernelGGL(copy_block_kernel<float>, dim3((unsigned)gk_ceil_div(rem * NB, 256)), dim3(256), 0,
-                       st, W, (int64_t)NB, A + (k0 + NB) * lda + k0, lda, rem, (int64_t)NB);
-    GK_TRY(gemm_launch<float>(rem, rem, NB, W, NB, W, NB, true, -1.0f, 1.0f, A + (k0 + NB) * lda + k0 + NB,
-                              lda, true, st));
+    GK_TRY(gemm_launch<T>(rem, NB, NB, A + (k0 + NB) * lda + k0, lda, dinv + k0 * NB, NB, true, T(1), T(0), W,
+                          NB, false, st));
+    hipLaunchKernelGGL(copy_block_kernel<T>, dim3((unsigned)gk_ceil_div(rem * NB, 256)), dim3(256), 0, st, W,
+                       (int64_t)NB, A + (k0 + NB) * lda + k0, lda, rem, (int64_t)NB);
+    GK_TRY(gemm_launch<T>(rem, rem, NB, W, NB, W, NB, true, T(-1), T(1), A + (k0 + NB) * lda + k0 + NB, lda, true,
+                          st));
+    if (kb % 32 == 31) {      // stop early once a pivot has failed
+      int32_t hinfo = 0;
+      GK_HIP(hipMemcpyAsync(&hinfo, dinfo, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+      GK_HIP(hipStreamSynchronize(st));
+      if (hinfo != 0) break;
+    }
   }
   GK_HIP(hipGetLastError());
   int32_t hinfo = 0;
@@ -443,31 +597,46 @@ extern "C" int gk_cholesky_f32(gk_handle h, float* A, int64_t n, int64_t lda, fl
   return hinfo == 0 ? GK_OK : GK_ERR_NOT_SPD;
 }
 
-extern "C" int gk_cholesky_solve_f32(gk_handle h, const float* L, const float* dinv, int64_t lda, float* b,
-                                     gk_stream stream) {
-  (void)h;
-  hipStream_t st = (hipStream_t)stream;
-  GK_REQUIRE(L && dinv && b && lda % NB == 0, GK_ERR_ARG, "gk_cholesky_solve_f32: bad arguments");
+template <typename T>
+static int cholesky_solve_impl(const T* L, const T* dinv, int64_t lda, T* b, hipStream_t st) {
+  GK_REQUIRE(L && dinv && b && lda % NB == 0, GK_ERR_ARG, "gk_cholesky_solve: bad arguments");
   const int64_t nb = lda / NB;
-  // forward: L y = b
-  for (int64_t kb = 0; kb < nb; ++kb) {
+  for (int64_t kb = 0; kb < nb; ++kb) {          // forward: L y = b
     const int64_t k0 = kb * NB;
-    hipLaunchKernelGGL((diag_apply_kernel<float, false>), dim3(1), dim3(NB), 0, st, dinv + k0 * NB, b + k0);
+    hipLaunchKernelGGL((diag_apply_kernel<T, false>), dim3(1), dim3(NB), 0, st, dinv + k0 * NB, b + k0);
     const int64_t r0 = k0 + NB;
     if (r0 < lda)
-      hipLaunchKernelGGL(gemv_panel_kernel<float>, dim3((unsigned)gk_ceil_div(lda - r0, 32)), dim3(256), 0,
-                         st, L, lda, r0, lda, k0, b + k0, b);
+      hipLaunchKernelGGL(gemv_panel_kernel<T>, dim3((unsigned)gk_ceil_div(lda - r0, 32)), dim3(256), 0, st, L,
+                         lda, r0, lda, k0, b + k0, b);
   }
-  // backward: L^T x = y
-  for (int64_t kb = nb - 1; kb >= 0; --kb) {
+  for (int64_t kb = nb - 1; kb >= 0; --kb) {     // backward: L^T x = y
     const int64_t k0 = kb * NB;
-    hipLaunchKernelGGL((diag_apply_kernel<float, true>), dim3(1), dim3(NB), 0, st, dinv + k0 * NB, b + k0);
+    hipLaunchKernelGGL((diag_apply_kernel<T, true>), dim3(1), dim3(NB), 0, st, dinv + k0 * NB, b + k0);
     if (k0 > 0)
-      hipLaunchKernelGGL(gemv_panelT_kernel<float>, dim3((unsigned)gk_ceil_div(k0, 256)), dim3(256), 0, st,
-                         L, lda, k0, k0, b + k0, b);
+      hipLaunchKernelGGL(gemv_panelT_kernel<T>, dim3((unsigned)gk_ceil_div(k0, 256)), dim3(256), 0, st, L, lda,
+                         k0, k0, b + k0, b);
   }
   GK_HIP(hipGetLastError());
   return GK_OK;
+}
+
+extern "C" int gk_cholesky_f32(gk_handle h, float* A, int64_t n, int64_t lda, float* dinv, int32_t* info,
+                               gk_stream stream) {
+  return cholesky_impl<float>(h, A, n, lda, dinv, info, (hipStream_t)stream);
+}
+extern "C" int gk_cholesky_f64(gk_handle h, double* A, int64_t n, int64_t lda, double* dinv, int32_t* info,
+                               gk_stream stream) {
+  return cholesky_impl<double>(h, A, n, lda, dinv, info, (hipStream_t)stream);
+}
+extern "C" int gk_cholesky_solve_f32(gk_handle h, const float* L, const float* dinv, int64_t lda, float* b,
+                                     gk_stream stream) {
+  (void)h;
+  return cholesky_solve_impl<float>(L, dinv, lda, b, (hipStream_t)stream);
+}
+extern "C" int gk_cholesky_solve_f64(gk_handle h, const double* L, const double* dinv, int64_t lda, double* b,
+                                     gk_stream stream) {
+  (void)h;
+  return cholesky_solve_impl<double>(L, dinv, lda, b, (hipStream_t)stream);
 }
 
 extern "C" int gk_lu_f64(gk_handle h, double* A, int64_t n, int64_t lda, double* dinv, int32_t* info,
@@ -478,10 +647,11 @@ extern "C" int gk_lu_f64(gk_handle h, double* A, int64_t n, int64_t lda, double*
              "gk_lu_f64: lda must be a multiple of %d and >= n", NB);
   const int64_t nb = lda / NB;
   void* ws = nullptr;
-  GK_TRY(gk_workspace(h, WS_FACTOR, 256 + 2 * (size_t)lda * NB * sizeof(double), &ws));
+  GK_TRY(gk_workspace(h, WS_FACTOR, 256 + (2 * (size_t)lda * NB + 4 * DB * DB) * sizeof(double), &ws));
   int32_t* dinfo = static_cast<int32_t*>(ws);
   double* W1 = reinterpret_cast<double*>(static_cast<char*>(ws) + 256);   // U12: NB x rem
   double* W2 = W1 + (size_t)lda * NB;                                        // L21: rem x NB
+  double* T = W2 + (size_t)lda * NB;                                         // 4 x DB x DB
   GK_HIP(hipMemsetAsync(dinfo, 0, sizeof(int32_t), st));
   if (n < lda) {
     const int64_t tot = (lda - n) * lda + n * (lda - n);
@@ -490,7 +660,7 @@ extern "C" int gk_lu_f64(gk_handle h, double* A, int64_t n, int64_t lda, double*
   }
   for (int64_t kb = 0; kb < nb; ++kb) {
     const int64_t k0 = kb * NB;
-    hipLaunchKernelGGL(getrf_diag_kernel, dim3(1), dim3(1024), 0, st, A, lda, k0, dinv, lda, dinfo);
+    GK_TRY(diag256_lu(A, lda, k0, dinv, dinfo, T, st));
     const int64_t rem = lda - k0 - NB;
     if (rem <= 0) break;
     const double* Li = dinv + k0 * NB;

@@ -19,6 +19,10 @@ extern "C" int gk_assemble_drift_impl(const gk_system* sys, double* U, hipStream
 extern "C" int gk_rhs_impl(const gk_system* sys, double* b, hipStream_t st);
 extern "C" int gk_matvec(gk_handle h, const gk_system* sys, int32_t precision, const double* x, double* y,
                          gk_stream stream);
+extern "C" int gk_cholesky_f64(gk_handle h, double* A, int64_t n, int64_t lda, double* dinv, int32_t* info,
+                               gk_stream stream);
+extern "C" int gk_cholesky_solve_f64(gk_handle h, const double* L, const double* dinv, int64_t lda, double* b,
+                                     gk_stream stream);
 
 namespace {
 
@@ -72,17 +76,25 @@ __global__ void sub_kernel(const double* b, const double* ax, double* r, int64_t
   if (i < n) r[i] = b[i] - ax[i];
 }
 
-__global__ void to_f32_kernel(const double* r, const double* d, float* u, int64_t nc, int64_t npad) {
+template <typename T>
+__global__ void to_scaled_kernel(const double* r, const double* d, T* u, int64_t nc, int64_t npad) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < npad) u[i] = i < nc ? (float)(d[i] * r[i]) : 0.0f;
+  if (i < npad) u[i] = i < nc ? (T)(d[i] * r[i]) : T(0);
 }
 
-__global__ void from_f32_kernel(const float* u, const double* d, double* y, int64_t nc) {
+template <typename T>
+__global__ void from_scaled_kernel(const T* u, const double* d, double* y, int64_t nc) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < nc) y[i] = d[i] * (double)u[i];
 }
 
-__global__ void add_diag_kernel(float* H, int64_t lda, int64_t n, float s) {
+__global__ void axpy_kernel(const double* a, double* x, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) x[i] += a[i];
+}
+
+template <typename T>
+__global__ void add_diag_kernel(T* H, int64_t lda, int64_t n, T s) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) H[i * lda + i] += s;
 }
@@ -153,14 +165,15 @@ struct Solver {
   const gk_system* sys;
   hipStream_t st;
   int64_t N, Nc, Lt, Npad, Ncpad;
-  // preconditioner
-  float* H = nullptr;
-  float* dinv = nullptr;
+  // preconditioner (FP32 or FP64 factor of the equilibrated covariance block)
+  bool f64 = false;
+  void* H = nullptr;
+  void* dinv = nullptr;
   double* dscale = nullptr;
   double* U = nullptr;
   double* Y = nullptr;
   double* Sinv = nullptr;
-  float* u32 = nullptr;
+  void* uvec = nullptr;
   double* ytmp = nullptr;
   double* lam = nullptr;
   double* partial = nullptr;
@@ -172,6 +185,7 @@ struct Solver {
   double* r = nullptr;
   double* hdev = nullptr;
   double* b = nullptr;
+  double* xt = nullptr;
 
   int host_dots(const double* Vc, int64_t ldv, int ncols, const double* vec, int64_t n, double* out_dev,
                 double* out_host) {
@@ -191,11 +205,19 @@ struct Solver {
     return GK_OK;
   }
 
-  // K^-1 r (covariance block) through the FP32 Cholesky of the equilibrated block
+  // K^-1 r (covariance block) through the Cholesky factor of the equilibrated block
   int apply_K(const double* rin, double* yout) {
-    hipLaunchKernelGGL(to_f32_kernel, dim3(nblk(Ncpad)), dim3(256), 0, st, rin, dscale, u32, Nc, Ncpad);
-    GK_TRY(gk_cholesky_solve_f32(h, H, dinv, Ncpad, u32, st));
-    hipLaunchKernelGGL(from_f32_kernel, dim3(nblk(Nc)), dim3(256), 0, st, u32, dscale, yout, Nc);
+    if (f64) {
+      double* u = static_cast<double*>(uvec);
+      hipLaunchKernelGGL(to_scaled_kernel<double>, dim3(nblk(Ncpad)), dim3(256), 0, st, rin, dscale, u, Nc, Ncpad);
+      GK_TRY(gk_cholesky_solve_f64(h, static_cast<double*>(H), static_cast<double*>(dinv), Ncpad, u, st));
+      hipLaunchKernelGGL(from_scaled_kernel<double>, dim3(nblk(Nc)), dim3(256), 0, st, u, dscale, yout, Nc);
+    } else {
+      float* u = static_cast<float*>(uvec);
+      hipLaunchKernelGGL(to_scaled_kernel<float>, dim3(nblk(Ncpad)), dim3(256), 0, st, rin, dscale, u, Nc, Ncpad);
+      GK_TRY(gk_cholesky_solve_f32(h, static_cast<float*>(H), static_cast<float*>(dinv), Ncpad, u, st));
+      hipLaunchKernelGGL(from_scaled_kernel<float>, dim3(nblk(Nc)), dim3(256), 0, st, u, dscale, yout, Nc);
+    }
     return GK_OK;
   }
 
@@ -214,20 +236,31 @@ struct Solver {
     return GK_OK;
   }
 
-  int setup_preconditioner(int* info_out) {
+  // Assemble and factor the equilibrated covariance block in FP32 (no retry:
+  // a breakdown means the block is too ill-conditioned for an FP32 factor) or
+  // FP64 (a tiny diagonal shift is tried if the factorization breaks down).
+  int setup_preconditioner(bool use_f64, int* info_out) {
+    f64 = use_f64;
+    const size_t esz = f64 ? sizeof(double) : sizeof(float);
     void* p = nullptr;
-    GK_TRY(gk_workspace(h, WS_MATRIX, (size_t)Ncpad * Ncpad * sizeof(float), &p));
-    H = static_cast<float*>(p);
-    float shift = 0.0f;
+    GK_TRY(gk_workspace(h, WS_MATRIX, (size_t)Ncpad * Ncpad * esz, &p));
+    H = p;
     int32_t info = 0;
-    for (int attempt = 0; attempt < 5; ++attempt) {
-      GK_TRY(gk_assemble_impl(h, sys, GK_F32, H, Ncpad, nullptr, dscale, st, 1));
-      if (shift > 0.0f)
-        hipLaunchKernelGGL(add_diag_kernel, dim3(nblk(Nc)), dim3(256), 0, st, H, Ncpad, Nc, shift);
-      int rc = gk_cholesky_f32(h, H, Nc, Ncpad, dinv, &info, st);
+    double shift = 0.0;
+    for (int attempt = 0; attempt < (f64 ? 3 : 1); ++attempt) {
+      GK_TRY(gk_assemble_impl(h, sys, f64 ? GK_F64 : GK_F32, H, Ncpad, nullptr, dscale, st, 1));
+      int rc;
+      if (f64) {
+        if (shift > 0.0)
+          hipLaunchKernelGGL(add_diag_kernel<double>, dim3(nblk(Nc)), dim3(256), 0, st, static_cast<double*>(H),
+                             Ncpad, Nc, shift);
+        rc = gk_cholesky_f64(h, static_cast<double*>(H), Nc, Ncpad, static_cast<double*>(dinv), &info, st);
+      } else {
+        rc = gk_cholesky_f32(h, static_cast<float*>(H), Nc, Ncpad, static_cast<float*>(dinv), &info, st);
+      }
       if (rc == GK_OK) break;
       if (rc != GK_ERR_NOT_SPD) return rc;
-      shift = shift == 0.0f ? 1e-6f : shift * 10.0f;
+      shift = shift == 0.0 ? 1e-13 : shift * 100.0;
     }
     *info_out = info;
     if (info != 0) return GK_ERR_NOT_SPD;
@@ -273,37 +306,61 @@ struct Solver {
   }
 
   // Right-preconditioned restarted flexible GMRES on the FP64 matrix-free operator.
+  // The true FP64 residual is checked every `check` inner iterations: the
+  // iteration stops when it is below tol, or when it no longer halves between
+  // checks (the FP64 rounding floor of the operator has been reached).
   int fgmres(double* x, double tol, double accept, int max_iter, int* iters, double* relres, bool* ok) {
+    const int check = 8;
     double nb = 0.0;
     GK_TRY(norm(b, N, &nb));
     if (nb == 0.0) nb = 1.0;
     GK_HIP(hipMemsetAsync(x, 0, N * sizeof(double), st));
     int total = 0;
-    double prev = 1e300, rel = 1.0;
+    double last_true = 1.0;
     std::vector<double> Hh((m + 1) * m), cs(m), sn(m), g(m + 1), hv(m + 2), yv(m);
     *ok = false;
-    for (int cycle = 0; cycle < 1000; ++cycle) {
-      GK_TRY(gk_matvec(h, sys, GK_F64, x, w, st));
+    bool done = false;
+    // out = x + Z y(j), y from the first j columns of the rotated Hessenberg
+    auto combine = [&](int j, double* out) -> int {
+      for (int i = j - 1; i >= 0; --i) {
+        double sacc = g[i];
+        for (int k2 = i + 1; k2 < j; ++k2) sacc -= Hh[i * m + k2] * yv[k2];
+        yv[i] = sacc / Hh[i * m + i];
+      }
+      std::vector<double> ny(j);
+      for (int i = 0; i < j; ++i) ny[i] = -yv[i];
+      GK_HIP(hipMemcpyAsync(hdev + 64, ny.data(), j * sizeof(double), hipMemcpyHostToDevice, st));
+      if (out != x) GK_HIP(hipMemcpyAsync(out, x, N * sizeof(double), hipMemcpyDeviceToDevice, st));
+      hipLaunchKernelGGL(gemv_sub_kernel, dim3(nblk(N)), dim3(256), 0, st, Z, Npad, j, hdev + 64, out, N);
+      GK_HIP(hipStreamSynchronize(st));
+      return GK_OK;
+    };
+    auto true_rel = [&](const double* xv, double* out) -> int {
+      GK_TRY(gk_matvec(h, sys, GK_F64, xv, w, st));
       hipLaunchKernelGGL(sub_kernel, dim3(nblk(N)), dim3(256), 0, st, b, w, r, N);
-      double beta = 0.0;
-      GK_TRY(norm(r, N, &beta));
-      rel = beta / nb;
-      if (rel <= tol) { *ok = true; break; }
-      if (cycle > 0 && rel > 0.5 * prev) { *ok = rel <= accept; break; }   // FP64 floor reached
-      if (total >= max_iter) { *ok = rel <= accept; break; }
-      prev = rel;
+      double nr = 0.0;
+      GK_TRY(norm(r, N, &nr));
+      *out = nr / nb;
+      return GK_OK;
+    };
+    while (!done) {
+      double rel0 = 0.0;
+      GK_TRY(true_rel(x, &rel0));         // r = b - A x
+      last_true = rel0;
+      if (rel0 <= tol || total >= max_iter) { *ok = rel0 <= accept; break; }
+      const double beta = rel0 * nb;
       hipLaunchKernelGGL(scale_kernel, dim3(nblk(N)), dim3(256), 0, st, r, 1.0 / beta, V, N);
       std::fill(g.begin(), g.end(), 0.0);
       g[0] = beta;
       int j = 0;
-      for (; j < m && total < max_iter; ++j) {
+      bool restart = false;
+      while (!done && !restart) {
         double* vj = V + (int64_t)j * Npad;
         double* zj = Z + (int64_t)j * Npad;
         GK_TRY(apply_M(vj, zj));
         GK_TRY(gk_matvec(h, sys, GK_F64, zj, w, st));
-        // classical Gram-Schmidt, two passes
         std::vector<double> hc(j + 1, 0.0);
-        for (int pass = 0; pass < 2; ++pass) {
+        for (int pass = 0; pass < 2; ++pass) {      // classical Gram-Schmidt, two passes
           GK_TRY(host_dots(V, Npad, j + 1, w, N, hdev, hv.data()));
           hipLaunchKernelGGL(gemv_sub_kernel, dim3(nblk(N)), dim3(256), 0, st, V, Npad, j + 1, hdev, w, N);
           for (int i = 0; i <= j; ++i) hc[i] += hv[i];
@@ -313,8 +370,9 @@ struct Solver {
         for (int i = 0; i <= j; ++i) Hh[i * m + j] = hc[i];
         Hh[(j + 1) * m + j] = hn;
         if (hn > 0.0)
-          hipLaunchKernelGGL(scale_kernel, dim3(nblk(N)), dim3(256), 0, st, w, 1.0 / hn, V + (int64_t)(j + 1) * Npad, N);
-        for (int i = 0; i < j; ++i) {          // previous rotations
+          hipLaunchKernelGGL(scale_kernel, dim3(nblk(N)), dim3(256), 0, st, w, 1.0 / hn,
+                             V + (int64_t)(j + 1) * Npad, N);
+        for (int i = 0; i < j; ++i) {
           const double a = Hh[i * m + j], c = Hh[(i + 1) * m + j];
           Hh[i * m + j] = cs[i] * a + sn[i] * c;
           Hh[(i + 1) * m + j] = -sn[i] * a + cs[i] * c;
@@ -328,21 +386,53 @@ struct Solver {
         g[j + 1] = -sn[j] * g[j];
         g[j] = cs[j] * g[j];
         ++total;
-        if (std::fabs(g[j + 1]) / nb <= 0.1 * tol || hn == 0.0) { ++j; break; }
+        ++j;
+        const bool full = (j == m) || hn == 0.0 || total >= max_iter;
+        if (full || j % check == 0) {
+          GK_TRY(combine(j, xt));
+          double rel = 0.0;
+          GK_TRY(true_rel(xt, &rel));
+          const bool converged = rel <= tol;
+          const bool stalled = rel > 0.5 * last_true;
+          last_true = rel;
+          if (converged || stalled || total >= max_iter || hn == 0.0) {
+            GK_HIP(hipMemcpyAsync(x, xt, N * sizeof(double), hipMemcpyDeviceToDevice, st));
+            *ok = rel <= accept;
+            done = true;
+          } else if (full) {
+            GK_HIP(hipMemcpyAsync(x, xt, N * sizeof(double), hipMemcpyDeviceToDevice, st));
+            restart = true;
+          }
+        }
       }
-      // back substitution and update
-      for (int i = j - 1; i >= 0; --i) {
-        double s = g[i];
-        for (int k = i + 1; k < j; ++k) s -= Hh[i * m + k] * yv[k];
-        yv[i] = s / Hh[i * m + i];
-      }
-      // x += Z y  (as x -= Z (-y))
-      for (int i = 0; i < j; ++i) yv[i] = -yv[i];
-      GK_HIP(hipMemcpyAsync(hdev + 64, yv.data(), j * sizeof(double), hipMemcpyHostToDevice, st));
-      hipLaunchKernelGGL(gemv_sub_kernel, dim3(nblk(N)), dim3(256), 0, st, Z, Npad, j, hdev + 64, x, N);
-      GK_HIP(hipStreamSynchronize(st));
     }
+    GK_HIP(hipStreamSynchronize(st));
     *iters = total;
+    *relres = last_true;
+    return GK_OK;
+  }
+
+  // x = M^-1 b with the FP64 factor, then refinement steps x += M^-1 (b - A x)
+  // while the FP64 residual keeps decreasing.
+  int direct_refine(double* x, int max_steps, int* steps, double* relres) {
+    double nb = 0.0;
+    GK_TRY(norm(b, N, &nb));
+    if (nb == 0.0) nb = 1.0;
+    GK_TRY(apply_M(b, x));
+    double prev = 1e300, rel = 0.0;
+    int k = 0;
+    for (; k < max_steps; ++k) {
+      GK_TRY(gk_matvec(h, sys, GK_F64, x, w, st));
+      hipLaunchKernelGGL(sub_kernel, dim3(nblk(N)), dim3(256), 0, st, b, w, r, N);
+      double nr = 0.0;
+      GK_TRY(norm(r, N, &nr));
+      rel = nr / nb;
+      if (rel > 0.5 * prev) break;
+      prev = rel;
+      GK_TRY(apply_M(r, xt));
+      hipLaunchKernelGGL(axpy_kernel, dim3(nblk(N)), dim3(256), 0, st, xt, x, N);
+    }
+    *steps = k;
     *relres = rel;
     return GK_OK;
   }
@@ -397,7 +487,8 @@ static int solve_lu(gk_handle h, const gk_system* sys, hipStream_t st, int64_t N
 extern "C" int gk_solve(gk_handle h, const gk_system* sys, int32_t form, double tol, int32_t max_iter,
                         double* x, gk_solve_info* info, gk_stream stream) {
   GK_REQUIRE(h && sys && x && info, GK_ERR_ARG, "gk_solve: NULL argument");
-  GK_REQUIRE(form == GK_SOLVE_FP64_LU || form == GK_SOLVE_FP32_REFINE, GK_ERR_ARG, "gk_solve: bad form");
+  GK_REQUIRE(form == GK_SOLVE_FP64_LU || form == GK_SOLVE_FP32_REFINE || form == GK_SOLVE_FP64_CHOL, GK_ERR_ARG,
+             "gk_solve: bad form");
   hipStream_t st = (hipStream_t)stream;
   *info = gk_solve_info{};
   const int64_t N = gk_system_n(sys);
@@ -415,16 +506,16 @@ extern "C" int gk_solve(gk_handle h, const gk_system* sys, int32_t form, double 
   Solver S;
   S.h = h; S.sys = sys; S.st = st;
   S.N = N; S.Nc = Nc; S.Lt = Lt; S.Npad = Npad; S.Ncpad = gk_round_up(std::max<int64_t>(Nc, 1), GK_BLOCK);
-  const bool fast = form == GK_SOLVE_FP32_REFINE;
+  const bool fast = form == GK_SOLVE_FP32_REFINE;   // needs the Krylov basis
   const int m = fast ? S.m : 1;
   size_t need = 0;
   auto take = [&](size_t bytes) { size_t o = need; need += gk_round_up(bytes, 256); return o; };
-  const size_t o_dinv = take(fast ? (size_t)S.Ncpad * GK_BLOCK * sizeof(float) : 0);
+  const size_t o_dinv = take(form != GK_SOLVE_FP64_LU ? (size_t)S.Ncpad * GK_BLOCK * sizeof(double) : 0);
   const size_t o_dsc = take(S.Ncpad * sizeof(double));
   const size_t o_U = take((size_t)S.Ncpad * std::max<int64_t>(Lt, 1) * sizeof(double));
   const size_t o_Y = take((size_t)S.Ncpad * std::max<int64_t>(Lt, 1) * sizeof(double));
   const size_t o_Si = take(32 * 32 * sizeof(double));
-  const size_t o_u32 = take(S.Ncpad * sizeof(float));
+  const size_t o_u32 = take(S.Ncpad * sizeof(double));
   const size_t o_yt = take(Npad * sizeof(double));
   const size_t o_lam = take(64 * sizeof(double));
   const size_t o_part = take((size_t)(gk_ceil_div(Npad, RED_ROWS) + 1) * 64 * sizeof(double));
@@ -434,15 +525,16 @@ extern "C" int gk_solve(gk_handle h, const gk_system* sys, int32_t form, double 
   const size_t o_r = take(Npad * sizeof(double));
   const size_t o_h = take(256 * sizeof(double));
   const size_t o_b = take(Npad * sizeof(double));
+  const size_t o_xt = take(Npad * sizeof(double));
   void* p = nullptr;
   GK_TRY(gk_workspace(h, WS_SOLVER, need, &p));
   char* base = static_cast<char*>(p);
-  S.dinv = reinterpret_cast<float*>(base + o_dinv);
+  S.dinv = base + o_dinv;
   S.dscale = reinterpret_cast<double*>(base + o_dsc);
   S.U = reinterpret_cast<double*>(base + o_U);
   S.Y = reinterpret_cast<double*>(base + o_Y);
   S.Sinv = reinterpret_cast<double*>(base + o_Si);
-  S.u32 = reinterpret_cast<float*>(base + o_u32);
+  S.uvec = base + o_u32;
   S.ytmp = reinterpret_cast<double*>(base + o_yt);
   S.lam = reinterpret_cast<double*>(base + o_lam);
   S.partial = reinterpret_cast<double*>(base + o_part);
@@ -452,35 +544,54 @@ extern "C" int gk_solve(gk_handle h, const gk_system* sys, int32_t form, double 
   S.r = reinterpret_cast<double*>(base + o_r);
   S.hdev = reinterpret_cast<double*>(base + o_h);
   S.b = reinterpret_cast<double*>(base + o_b);
+  S.xt = reinterpret_cast<double*>(base + o_xt);
   GK_TRY(gk_rhs_impl(sys, S.b, st));
 
-  if (!fast) {
+  if (form == GK_SOLVE_FP64_LU) {
     GK_TRY(solve_lu(h, sys, st, N, Npad, x, info, ev));
     GK_TRY(S.true_relres(x, &info->relres));
     return GK_OK;
   }
+  auto run_chol64 = [&]() -> int {
+    GK_HIP(hipEventRecord(ev[0], st));
+    int32_t finfo = 0;
+    GK_TRY(S.setup_preconditioner(true, &finfo));
+    info->info = finfo;
+    GK_HIP(hipEventRecord(ev[1], st));
+    int steps = 0;
+    GK_TRY(S.direct_refine(x, 3, &steps, &info->relres));
+    GK_HIP(hipEventRecord(ev[2], st));
+    GK_HIP(hipEventSynchronize(ev[2]));
+    info->t_factor_ms += elapsed_ms(ev[0], ev[1]);
+    info->t_refine_ms += elapsed_ms(ev[1], ev[2]);
+    info->iterations += steps;
+    info->form = GK_SOLVE_FP64_CHOL;
+    return GK_OK;
+  };
+  if (form == GK_SOLVE_FP64_CHOL) {
+    GK_TRY(run_chol64());
+    return GK_OK;
+  }
+  // FP32 factor + FP64 GMRES refinement
   GK_HIP(hipEventRecord(ev[0], st));
-  GK_HIP(hipEventRecord(ev[1], st));
   int32_t finfo = 0;
-  int rc = S.setup_preconditioner(&finfo);
+  int rc = S.setup_preconditioner(false, &finfo);
   info->info = finfo;
-  GK_HIP(hipEventRecord(ev[2], st));
+  GK_HIP(hipEventRecord(ev[1], st));
   bool ok = false;
   if (rc == GK_OK) {
     GK_TRY(S.fgmres(x, tol, 1e-6, max_iter, &info->iterations, &info->relres, &ok));
   } else if (rc != GK_ERR_NOT_SPD) {
     return rc;
   }
-  GK_HIP(hipEventRecord(ev[3], st));
-  GK_HIP(hipEventSynchronize(ev[3]));
-  info->t_assemble_ms = 0.0;
-  info->t_factor_ms = elapsed_ms(ev[1], ev[2]);
-  info->t_refine_ms = elapsed_ms(ev[2], ev[3]);
+  GK_HIP(hipEventRecord(ev[2], st));
+  GK_HIP(hipEventSynchronize(ev[2]));
+  info->t_factor_ms = elapsed_ms(ev[0], ev[1]);
+  info->t_refine_ms = elapsed_ms(ev[1], ev[2]);
   info->form = GK_SOLVE_FP32_REFINE;
   if (!ok) {
     info->fallback = 1;
-    GK_TRY(solve_lu(h, sys, st, N, Npad, x, info, ev));
-    GK_TRY(S.true_relres(x, &info->relres));
+    GK_TRY(run_chol64());
   }
   return GK_OK;
 }

@@ -31,8 +31,9 @@ class StackRelationType(enum.IntEnum):
 
 class SolverForm(str, enum.Enum):
     """How the co-kriging system is solved on the device."""
-    FP64_DIRECT = "fp64_direct"        # FP64 assembly + FP64 blocked LU (MFMA)
-    FP32_REFINE = "fp32_refine"        # FP32 Cholesky (MFMA) + FP64 iterative refinement
+    FP64_DIRECT = "fp64_direct"        # FP64 assembly + FP64 blocked LU (MFMA), the reference's solve class
+    FP64_CHOL = "fp64_chol"            # FP64 Cholesky of the covariance block + Schur complement on the drift
+    FP32_REFINE = "fp32_refine"        # FP32 Cholesky (MFMA) + FP64 GMRES refinement, FP64_CHOL fallback
     GMRES_LEARNED = "gmres_learned"    # matrix-free FP32 GMRES + learned preconditioner
 
 

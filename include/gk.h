@@ -40,7 +40,7 @@ enum {
 
 enum { GK_F32 = 0, GK_F64 = 1 };
 enum { GK_POINTS_EXPLICIT = 0, GK_POINTS_LATTICE = 1 };
-enum { GK_SOLVE_FP64_LU = 0, GK_SOLVE_FP32_REFINE = 1 };
+enum { GK_SOLVE_FP64_LU = 0, GK_SOLVE_FP32_REFINE = 1, GK_SOLVE_FP64_CHOL = 2 };
 #define GK_MAX_THRESHOLDS 64
 #define GK_BLOCK 256   /* factorization block size; lda of factored matrices is a multiple */
 
@@ -112,9 +112,10 @@ int64_t gk_system_padded(const gk_system* sys);  /* N rounded up to GK_BLOCK */
  * compute_API.py:68-73 (kernel_options at 2-layers.approved.txt:150-162).
  * dtype GK_F64: full saddle-point matrix N x N into A (lda >= N), rows/cols
  *   N..lda-1 set to the identity (lda a multiple of GK_BLOCK).
- * dtype GK_F32: covariance block only (Nc = 3 n_ori + n_rest), symmetrically
- *   equilibrated H = D A_cc D with D = diag(A_cc)^-1/2 written to dscale[Nc];
- *   padding rows/cols to lda are the identity.
+ * dscale != NULL (required for GK_F32, optional for GK_F64): covariance
+ *   block only (Nc = 3 n_ori + n_rest), symmetrically equilibrated
+ *   H = D A_cc D with D = diag(A_cc)^-1/2 written to dscale[Nc]; padding
+ *   rows/cols to lda are the identity.
  * rhs (nullable) receives b = [G_x | G_y | G_z | 0 ...] (length N). */
 int gk_assemble(gk_handle h, const gk_system* sys, int32_t dtype, void* A, int64_t lda,
                 double* rhs, double* dscale, gk_stream stream);
@@ -136,6 +137,11 @@ int gk_cholesky_f32(gk_handle h, float* A, int64_t n, int64_t lda, float* dinv, 
 /* Solve (L L^T) x = b in place with the output of gk_cholesky_f32 (FP32 vector of length lda). */
 int gk_cholesky_solve_f32(gk_handle h, const float* L, const float* dinv, int64_t lda, float* b,
                           gk_stream stream);
+/* FP64 counterparts (FP64 MFMA trailing updates). */
+int gk_cholesky_f64(gk_handle h, double* A, int64_t n, int64_t lda, double* dinv, int32_t* info,
+                    gk_stream stream);
+int gk_cholesky_solve_f64(gk_handle h, const double* L, const double* dinv, int64_t lda, double* b,
+                          gk_stream stream);
 /* In-place LU without pivoting (the co-kriging matrix has an SPD leading
  * block) of an lda x lda FP64 matrix; dinv [2*lda*GK_BLOCK] receives the
  * inverses of the unit-lower and upper diagonal blocks. */
@@ -153,9 +159,15 @@ int gk_gemm_f64(gk_handle h, int64_t m, int64_t n, int64_t k, const double* A, i
                 const double* B, int64_t ldb, int32_t trans_b, double beta, double* C, int64_t ldc,
                 int32_t lower_only, gk_stream stream);
 
-/* Whole solve of one stack: assemble, factor, solve and (FP32 form) refine
- * with FP64 right-preconditioned GMRES on the matrix-free FP64 operator,
- * falling back to the FP64 LU if the refinement stagnates above tol. */
+/* Whole solve of one stack.
+ *  GK_SOLVE_FP64_LU    : FP64 full-system assembly + blocked LU (the reference's
+ *                        dense direct solve, numpy.linalg.solve class).
+ *  GK_SOLVE_FP64_CHOL  : FP64 equilibrated Cholesky of the SPD covariance block +
+ *                        Schur complement on the drift block + FP64 refinement steps.
+ *  GK_SOLVE_FP32_REFINE: FP32 equilibrated Cholesky used as the preconditioner of
+ *                        FP64 right-preconditioned GMRES on the matrix-free FP64
+ *                        operator; if the FP32 factor breaks down or the iteration
+ *                        stagnates above 1e-6, falls back to GK_SOLVE_FP64_CHOL. */
 int gk_solve(gk_handle h, const gk_system* sys, int32_t form, double tol, int32_t max_iter,
              double* x, gk_solve_info* info, gk_stream stream);
 
@@ -170,6 +182,22 @@ int gk_eval(gk_handle h, const gk_system* sys, const double* weights, gk_eval_de
  * last surface) holds p, basement = sum(n_surf) + 1 otherwise. */
 int gk_combine_lith(gk_handle h, int32_t n_stacks, const int32_t* ids, int64_t n, const int32_t* n_surf,
                     int32_t* lith, gk_stream stream);
+
+/* ---- octree evaluation (the engine's octree levels: _number_octree_levels,
+ * fault.approved.txt:210-214; resolution = base * 2^(levels-1),
+ * gempy/core/data/grid_modules/regular_grid.py:36-43) ------------------------
+ * Probe points of `n` voxels (linear indices (ix*ny+iy)*nz+iz at resolution
+ * res[3] of the box extent[6]): the centre and the 8 corners of every voxel
+ * (xyz [n*9*3], centre first) or, with centres_only, the centres [n*3]. */
+int gk_octree_points(gk_handle h, const double* extent, const int64_t* res, const int64_t* voxels, int64_t n,
+                     int32_t centres_only, double* xyz, gk_stream stream);
+/* ids [n_fields][n*9] (int32) at the probe points; refine[v] = 1 when any field
+ * differs among voxel v's 9 points; children [8*n] receives the 8 children (at
+ * resolution 2*res) of every refined voxel in voxel order; *n_children their
+ * count (host, the call synchronises the stream). */
+int gk_octree_refine(gk_handle h, int32_t n_fields, const int32_t* ids, int64_t n, const int64_t* voxels,
+                     const int64_t* res, uint8_t* refine, int64_t* children, int64_t* n_children,
+                     gk_stream stream);
 
 #ifdef __cplusplus
 }

@@ -111,6 +111,26 @@ def test_cholesky_f32_and_solve(gpu):
     np.testing.assert_array_equal(tb[n:].cpu().numpy(), 0.0)
 
 
+def test_cholesky_f64_and_solve(gpu):
+    say("cholesky f64 n=700 lda=768")
+    L = _lib()
+    n, lda = 700, 768
+    A = _spd(n, 12)
+    tA = torch.zeros((lda, lda), dtype=torch.float64, device=gpu)
+    tA[:n, :n] = torch.tensor(A)
+    dinv = torch.empty(lda * 256, dtype=torch.float64, device=gpu)
+    info = C.c_int32(-1)
+    L.check("chol64", L.load().gk_cholesky_f64(_h(), _p(tA), n, lda, _p(dinv), C.byref(info), _st()))
+    assert info.value == 0
+    Lf = torch.tril(tA[:n, :n]).cpu().numpy()
+    np.testing.assert_allclose(Lf @ Lf.T, A, atol=1e-12 * np.abs(A).max())
+    b = np.random.default_rng(13).standard_normal(n)
+    tb = torch.zeros(lda, dtype=torch.float64, device=gpu)
+    tb[:n] = torch.tensor(b)
+    L.check("chol64_solve", L.load().gk_cholesky_solve_f64(_h(), _p(tA), _p(dinv), lda, _p(tb), _st()))
+    np.testing.assert_allclose(tb[:n].cpu().numpy(), np.linalg.solve(A, b), rtol=1e-9, atol=1e-11)
+
+
 def test_lu_f64_and_solve(gpu):
     say("lu f64 n=600 lda=768 (saddle-point like)")
     L = _lib()
@@ -239,7 +259,7 @@ def test_eval_points_and_lattice_match_oracle(gpu, precision):
         f"lattice={np.abs(S2.cpu().numpy() - So).max() / rng_s:.3e}")
 
 
-@pytest.mark.parametrize("form", ["fp64_lu", "fp32_refine"])
+@pytest.mark.parametrize("form", ["fp64_lu", "fp64_chol", "fp32_refine"])
 def test_solve_matches_oracle(gpu, form):
     say(f"solve {form}")
     from gempy_amd.engine import DeviceStack
@@ -249,7 +269,8 @@ def test_solve_matches_oracle(gpu, form):
     Ao, bo = _oracle_A(st, k)
     wo = np.linalg.solve(Ao, bo)
     ds = DeviceStack(st, k, [500, 500, 500])
-    f = L.GK_SOLVE_FP64_LU if form == "fp64_lu" else L.GK_SOLVE_FP32_REFINE
+    f = {"fp64_lu": L.GK_SOLVE_FP64_LU, "fp64_chol": L.GK_SOLVE_FP64_CHOL,
+         "fp32_refine": L.GK_SOLVE_FP32_REFINE}[form]
     w, info = ds.solve(f)
     say(f"solve {form}: {info}")
     assert info["relres"] < 1e-8
@@ -309,3 +330,36 @@ def test_compute_model_multistack_fault_matches_oracle(gpu):
         S = s_gpu.scalar_field.cpu().numpy()
         assert np.abs(S - r["scalar"]).max() <= 1e-6 * (r["scalar"].max() - r["scalar"].min())
     np.testing.assert_array_equal(sol.fault_blocks[0].cpu().numpy(), res["fault_blocks"][0])
+
+
+def test_octree_levels_consistent_with_dense_grid(gpu):
+    say("octree config2 scale=0.05 grid 32^3 levels=3")
+    from gempy_amd import GemPyEngineConfig, compute_model, synthetic
+    m = synthetic.config2(scale=0.05, resolution=(32, 32, 32), octree_levels=3)
+    sol = compute_model(m, GemPyEngineConfig(dtype="float64"))
+    octs = sol.octrees_output
+    assert [o.resolution for o in octs] == [(8, 8, 8), (16, 16, 16), (32, 32, 32)]
+    assert octs[0].voxel_index.numel() == 512
+    for a, b in zip(octs[:-1], octs[1:]):
+        assert b.voxel_index.numel() == 8 * int(a.refine.sum())
+    last = octs[-1]
+    dense = sol.lith_block.cpu().numpy()
+    vox = last.voxel_index.cpu().numpy()
+    np.testing.assert_array_equal(last.ids.cpu().numpy(), dense[vox])
+    # the octree's own representation (every unrefined voxel filled with its
+    # centre id, refined voxels by their children) reproduces the dense block
+    # except where an interface passes between the 9 probes of a coarse voxel.
+    recon = np.full((32, 32, 32), -1, dtype=np.int64)
+    for o in octs:
+        r = o.resolution[0]
+        f = 32 // r
+        v = o.voxel_index.cpu().numpy()
+        ids = o.ids.cpu().numpy()
+        keep = np.ones(len(v), dtype=bool) if o.refine is None else ~o.refine.cpu().numpy()
+        ix, iy, iz = v // (r * r), (v // r) % r, v % r
+        for k in np.nonzero(keep)[0]:
+            recon[ix[k] * f:(ix[k] + 1) * f, iy[k] * f:(iy[k] + 1) * f, iz[k] * f:(iz[k] + 1) * f] = ids[k]
+    assert (recon >= 0).all()
+    bad = np.count_nonzero(recon.ravel() != dense)
+    say(f"octree: level sizes {[o.voxel_index.numel() for o in octs]}, reconstruction mismatches {bad} / {dense.size}")
+    assert bad <= 0.005 * dense.size
