@@ -43,6 +43,8 @@ def parse():
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 direct form")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the multi-stack/octree and adjoint workloads")
     return ap.parse_args()
 
 
@@ -152,6 +154,14 @@ def main():
                             "unit": "TFLOP/s", "frac": chol_flop / (factor_ms * 1e-3) / 1e12 / PEAK["fp32_mfma"]},
     }
     out.update(res)
+    if not args.no_extras:
+        from gempy_amd import workloads
+        log("multi-stack + fault + 4 octree levels, 200^3 (FP32 form)")
+        out["multistack_octree"] = workloads.multistack_octree(steps=1, warmup=1)
+        log(f"  {out['multistack_octree']}")
+        log("adjoint of a 10,000-observation misfit (faulted stack, 128^3 grid)")
+        out["adjoint"] = workloads.adjoint_misfit(steps=1, warmup=1)
+        log(f"  {out['adjoint']}")
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(model, out_fast[0].cpu().numpy(), args.cpu_seconds, N, P)
     print(json.dumps(out), flush=True)

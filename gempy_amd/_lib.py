@@ -74,6 +74,22 @@ SIGNATURES = {
     "gk_solve": (C.c_int, [P, C.POINTER(GkSystem), C.c_int32, C.c_double, C.c_int32, P,
                            C.POINTER(GkSolveInfo), P]),
     "gk_eval": (C.c_int, [P, C.POINTER(GkSystem), P, C.POINTER(GkEvalDesc), P]),
+    "gk_solve_rhs": (C.c_int, [P, C.POINTER(GkSystem), P, P, C.POINTER(GkSolveInfo), P]),
+    "gk_adjoint": (C.c_int, [P, C.POINTER(GkSystem), P, C.c_int64, P, P, P, P, P, P, P]),
+    "gk_eval_sources": (C.c_int, [P, C.c_double, C.c_double, C.POINTER(C.c_double), C.c_int64, P, P,
+                                  C.c_int64, P, P, P, P]),
+    "gk_spai_nparams": (C.c_int, []),
+    "gk_knn": (C.c_int, [P, C.POINTER(GkSystem), P, P, P]),
+    "gk_spai_features": (C.c_int, [P, C.POINTER(GkSystem), C.c_int64, P, P, P, P, P]),
+    "gk_spai_local_blocks": (C.c_int, [P, C.POINTER(GkSystem), C.c_int64, P, P, P, P, P]),
+    "gk_mlp_forward": (C.c_int, [P, C.c_int64, P, P, P, P, P]),
+    "gk_mlp_backward": (C.c_int, [P, C.c_int64, P, P, P, P, P, P]),
+    "gk_spai_loss": (C.c_int, [P, C.POINTER(GkSystem), C.c_int64, P, P, P, P, P, P, P]),
+    "gk_adam": (C.c_int, [P, P, P, P, P, C.c_int32, C.c_float, C.c_int32, C.c_float, P]),
+    "gk_spai_apply": (C.c_int, [P, C.POINTER(GkSystem), P, P, P, P, P]),
+    "gk_diag_scale": (C.c_int, [P, C.POINTER(GkSystem), P, P]),
+    "gk_solve_spai": (C.c_int, [P, C.POINTER(GkSystem), P, P, C.c_double, C.c_int32, C.c_int32, P,
+                                C.POINTER(GkSolveInfo), P]),
     "gk_combine_lith": (C.c_int, [P, C.c_int32, P, C.c_int64, C.POINTER(C.c_int32), P, P]),
     "gk_octree_points": (C.c_int, [P, C.POINTER(C.c_double), C.POINTER(C.c_int64), P, C.c_int64, C.c_int32,
                                    P, P]),

@@ -49,6 +49,7 @@ extern "C" void gk_destroy(gk_handle h) {
   if (!h) return;
   (void)hipSetDevice(h->device);
   (void)hipDeviceSynchronize();
+  if (h->solver_state && h->solver_free) h->solver_free(h->solver_state);
   for (int i = 0; i < WS_COUNT; ++i)
     if (h->ws[i]) (void)hipFree(h->ws[i]);
   delete h;
@@ -99,7 +100,7 @@ extern "C" int gk_eval(gk_handle h, const gk_system* sys, const double* weights,
   const double* pbox = nullptr;
   if (desc && desc->kind == GK_POINTS_LATTICE && sys && sys->n_ori + sys->n_rest + sys->n_surf > 0) {
     void* ws = nullptr;
-    GK_TRY(gk_workspace(h, WS_MATRIX2, 64, &ws));
+    GK_TRY(gk_workspace(h, WS_SMALL, 256, &ws));
     hipLaunchKernelGGL(source_box_kernel, dim3(1), dim3(256), 0, st, *sys, static_cast<double*>(ws));
     GK_HIP(hipMemcpyAsync(box, ws, sizeof(box), hipMemcpyDeviceToHost, st));
     GK_HIP(hipStreamSynchronize(st));

@@ -17,20 +17,28 @@ namespace {
 
 constexpr int NB = GK_BLOCK;   // 256
 
+// K depth of one LDS stage of the GEMM (elements)
+#ifndef GK_BK_F32
+#define GK_BK_F32 32
+#endif
+#ifndef GK_BK_F64
+#define GK_BK_F64 16
+#endif
+
 typedef float f4v __attribute__((ext_vector_type(4)));
 typedef double d4v __attribute__((ext_vector_type(4)));
 
 template <typename T> struct Mfma;
 template <> struct Mfma<float> {
   using acc_t = f4v;
-  static constexpr int BK = 32;
+  static constexpr int BK = GK_BK_F32;
   static __device__ __forceinline__ acc_t mma(float a, float b, acc_t c) {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
   }
 };
 template <> struct Mfma<double> {
   using acc_t = d4v;
-  static constexpr int BK = 16;
+  static constexpr int BK = GK_BK_F64;
   static __device__ __forceinline__ acc_t mma(double a, double b, acc_t c) {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
   }

@@ -507,6 +507,42 @@ static bool needs_mask(const gk_system* sys, const double* lo, const double* hi,
   return d2 >= (1.0 - 1e-9) * sys->range * sys->range;
 }
 
+// Split the sources over blockIdx.y when the points alone cannot fill the GPU,
+// then launch the evaluation kernel for the requested variant.
+static int run_eval(gk_handle h, EvalArgs& a, bool lattice, int precision, bool grad, bool masked,
+                    hipStream_t st) {
+  a.nsplit = 1;
+  a.pts_chunk = std::max<int64_t>(a.n_pts, 1);
+  a.ori_chunk = std::max<int64_t>(a.n_ori, 1);
+  if (!lattice) {
+    const int ppt = precision == GK_F64 ? 2 : 4;
+    const int64_t blocks = gk_ceil_div(gk_ceil_div(a.n_points, ppt), TS);
+    const int64_t tiles = std::max(gk_ceil_div(a.n_pts, TS), gk_ceil_div(a.n_ori, TS));
+    int64_t want = std::min<int64_t>(std::min<int64_t>(gk_ceil_div(2048, blocks), 64), tiles);
+    if (want > 1) {
+      // chunks are at least one tile, so an empty source set (n = 0) never divides by zero
+      a.pts_chunk = std::max<int64_t>(gk_round_up(gk_ceil_div(a.n_pts, want), TS), TS);
+      a.ori_chunk = std::max<int64_t>(gk_round_up(gk_ceil_div(a.n_ori, want), TS), TS);
+      a.nsplit = (int32_t)std::max<int64_t>(
+          std::max(gk_ceil_div(a.n_pts, a.pts_chunk), gk_ceil_div(a.n_ori, a.ori_chunk)), 1);
+      if (a.nsplit > 1) {
+        void* pw = nullptr;
+        const size_t per = (size_t)a.n_points * (grad ? 4 : 1);
+        GK_TRY(gk_workspace(h, WS_EVAL_PART, per * a.nsplit * sizeof(double) + 512, &pw));
+        a.part_s = static_cast<double*>(pw);
+        a.part_g = grad ? a.part_s + (size_t)a.nsplit * a.n_points : nullptr;
+      } else {
+        a.pts_chunk = std::max<int64_t>(a.n_pts, 1);
+        a.ori_chunk = std::max<int64_t>(a.n_ori, 1);
+      }
+    }
+  }
+  if (precision == GK_F64) dispatch_eval<double>(a, lattice, grad, masked, st);
+  else dispatch_eval<float>(a, lattice, grad, masked, st);
+  GK_HIP(hipGetLastError());
+  return GK_OK;
+}
+
 extern "C" int gk_eval_impl(gk_handle h, const gk_system* sys, const double* weights,
                             gk_eval_desc* d, hipStream_t st, bool force_mask,
                             const double* src_box /*6 or null*/) {
@@ -557,41 +593,12 @@ extern "C" int gk_eval_impl(gk_handle h, const gk_system* sys, const double* wei
   void* ws = nullptr;
   GK_TRY(gk_workspace(h, WS_EVAL_SRC, sources_bytes(sys), &ws));
   GK_TRY(prepare_sources(h, sys, weights, d->precision, st, &a.pts, &a.ori, ws));
-
-  // Split the sources over blockIdx.y when the points alone cannot fill the GPU.
-  a.nsplit = 1;
-  a.pts_chunk = std::max<int64_t>(a.n_pts, 1);
-  a.ori_chunk = std::max<int64_t>(a.n_ori, 1);
-  if (d->kind == GK_POINTS_EXPLICIT) {
-    const int ppt = d->precision == GK_F64 ? 2 : 4;
-    const int64_t blocks = gk_ceil_div(gk_ceil_div(a.n_points, ppt), TS);
-    const int64_t tiles = std::max(gk_ceil_div(a.n_pts, TS), gk_ceil_div(a.n_ori, TS));
-    int64_t want = std::min<int64_t>(std::min<int64_t>(gk_ceil_div(2048, blocks), 64), tiles);
-    if (want > 1) {
-      a.pts_chunk = gk_round_up(gk_ceil_div(a.n_pts, want), TS);
-      a.ori_chunk = gk_round_up(gk_ceil_div(a.n_ori, want), TS);
-      a.nsplit = (int32_t)std::max(gk_ceil_div(a.n_pts, a.pts_chunk), gk_ceil_div(a.n_ori, a.ori_chunk));
-      if (a.nsplit > 1) {
-        void* pw = nullptr;
-        const size_t per = (size_t)a.n_points * (d->gradient ? 4 : 1);
-        GK_TRY(gk_workspace(h, WS_EVAL_PART, per * a.nsplit * sizeof(double) + 512, &pw));
-        a.part_s = static_cast<double*>(pw);
-        a.part_g = d->gradient ? a.part_s + (size_t)a.nsplit * a.n_points : nullptr;
-      } else {
-        a.pts_chunk = std::max<int64_t>(a.n_pts, 1);
-        a.ori_chunk = std::max<int64_t>(a.n_ori, 1);
-      }
-    }
-  }
-
   bool masked = true;
   if (!force_mask && d->kind == GK_POINTS_LATTICE && src_box) {
     double sl[3] = {src_box[0], src_box[1], src_box[2]}, sh[3] = {src_box[3], src_box[4], src_box[5]};
     masked = needs_mask(sys, lo, hi, sl, sh);
   }
-  const bool grad = d->gradient != nullptr;
-  if (d->precision == GK_F64) dispatch_eval<double>(a, d->kind == GK_POINTS_LATTICE, grad, masked, st);
-  else dispatch_eval<float>(a, d->kind == GK_POINTS_LATTICE, grad, masked, st);
+  GK_TRY(run_eval(h, a, d->kind == GK_POINTS_LATTICE, d->precision, d->gradient != nullptr, masked, st));
   GK_HIP(hipGetLastError());
   return GK_OK;
 }
@@ -764,7 +771,7 @@ extern "C" int gk_combine_lith(gk_handle h, int32_t n_stacks, const int32_t* ids
     off += n_surf[s];
   }
   void* ws = nullptr;
-  GK_TRY(gk_workspace(h, WS_MATRIX2, 128 * sizeof(int32_t), &ws));
+  GK_TRY(gk_workspace(h, WS_SMALL, 128 * sizeof(int32_t), &ws));
   int32_t* d = static_cast<int32_t*>(ws);
   GK_HIP(hipMemcpyAsync(d, hbuf, sizeof(hbuf), hipMemcpyHostToDevice, st));
   if (n > 0)
@@ -773,4 +780,51 @@ extern "C" int gk_combine_lith(gk_handle h, int32_t n_stacks, const int32_t* ids
   GK_HIP(hipGetLastError());
   GK_HIP(hipStreamSynchronize(st));   // hbuf is a stack buffer
   return GK_OK;
+}
+
+// ------------------------------------------------------- raw-source fields
+namespace {
+__global__ void prep_raw_kernel(int64_t n, const double* xyz, const double* w, double c0, double c1, double c2,
+                                double inv_a, double scale, V4<double>* out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  V4<double> v;
+  v.x = (xyz[3 * i] - c0) * inv_a;
+  v.y = (xyz[3 * i + 1] - c1) * inv_a;
+  v.z = (xyz[3 * i + 2] - c2) * inv_a;
+  v.w = scale * w[i];
+  out[i] = v;
+}
+}  // namespace
+
+// F(x) = sum_s w_s C(|x - s|) and grad F at explicit points (FP64), for
+// sources given explicitly (e.g. observation points weighted by dL/dS).
+extern "C" int gk_eval_sources(gk_handle h, double range, double c_o, const double* center, int64_t n_src,
+                               const double* src_xyz, const double* src_w, int64_t n_points, const double* xyz,
+                               double* scalar, double* gradient, gk_stream stream) {
+  GK_REQUIRE(h && center && range > 0 && n_src >= 0 && n_points >= 0, GK_ERR_ARG, "gk_eval_sources: bad arguments");
+  GK_REQUIRE((n_src == 0 || (src_xyz && src_w)) && (n_points == 0 || (xyz && scalar)), GK_ERR_ARG,
+             "gk_eval_sources: NULL array");
+  if (n_points == 0) return GK_OK;
+  hipStream_t st = (hipStream_t)stream;
+  void* ws = nullptr;
+  GK_TRY(gk_workspace(h, WS_EVAL_SRC, (size_t)std::max<int64_t>(n_src, 1) * 32 + 256, &ws));
+  V4<double>* src = static_cast<V4<double>*>(ws);
+  if (n_src > 0)
+    hipLaunchKernelGGL(prep_raw_kernel, dim3((unsigned)gk_ceil_div(n_src, 256)), dim3(256), 0, st, n_src, src_xyz,
+                       src_w, center[0], center[1], center[2], 1.0 / range, c_o, src);
+  EvalArgs a{};
+  a.n_points = n_points;
+  a.xyz = xyz;
+  a.cx = center[0]; a.cy = center[1]; a.cz = center[2];
+  a.inv_a = 1.0 / range;
+  a.pts = src;
+  a.n_pts = n_src;
+  a.ori = nullptr;
+  a.n_ori = 0;
+  a.uni_degree = 0;
+  a.n_fault = 0;
+  a.scalar = scalar;
+  a.grad = gradient;
+  return run_eval(h, a, false, GK_F64, gradient != nullptr, true, st);
 }

@@ -43,12 +43,16 @@ enum gk_ws_slot {
   WS_SOLVER,         // Krylov basis, preconditioner blocks
   WS_MATRIX,         // assembled matrices of gk_solve
   WS_MATRIX2,
+  WS_ADJOINT,        // vectors of the adjoint
+  WS_SMALL,          // small scratch (bounding boxes, counts)
   WS_COUNT
 };
 struct gk_context {
   int device = 0;
   void* ws[WS_COUNT] = {};
   size_t ws_bytes[WS_COUNT] = {};
+  void* solver_state = nullptr;            // factorization kept by the last gk_solve
+  void (*solver_free)(void*) = nullptr;
 };
 
 int gk_workspace(gk_handle h, int slot, size_t bytes, void** out);

@@ -1,0 +1,417 @@
+// Learned sparse approximate inverse (SPAI) of the equilibrated covariance
+// block H = D A_cc D, for the matrix-free GMRES of the largest systems.
+//
+// Pattern: the data points (orientations, then rest interface points) each own
+// a 3-dim block of unknowns (an orientation: its x, y, z gradient unknowns; an
+// interface point: its one increment unknown in slot 0, slots 1-2 dummy).  Row
+// block P of M has a 3x3 block for each of P's K nearest data points Q.  A small
+// MLP maps pair features of (P, Q) to that 3x3 block; it is trained to minimise
+// the local SPAI residual || M_P H_loc(P) - E_P ||_F^2 (H_loc = H restricted to
+// the unknowns of P's neighbourhood), whose minimiser is the neighbourhood
+// inverse row (the local cardinal-function preconditioner).
+#include <algorithm>
+#include "gk_internal.h"
+#include "gk_entry.h"
+
+using namespace gk_detail;
+
+namespace {
+
+constexpr int K = GK_SPAI_K;          // neighbours per point (including itself)
+constexpr int F = GK_SPAI_FEATURES;   // features per pair
+constexpr int H1 = GK_SPAI_HIDDEN;
+constexpr int H2 = GK_SPAI_HIDDEN;
+constexpr int O = 9;
+constexpr int NPARAM = H1 * F + H1 + H2 * H1 + H2 + O * H2 + O;
+constexpr int LD = 3 * K;             // local block dimension
+
+__device__ __forceinline__ void point_xyz(const gk_system& s, int64_t p, double* o) {
+  const double* q = p < s.n_ori ? s.ori_xyz + 3 * p : s.rest_xyz + 3 * (p - s.n_ori);
+  o[0] = q[0]; o[1] = q[1]; o[2] = q[2];
+}
+
+// unknown index of dimension d of data point p (-1 for the dummy slots)
+__device__ __forceinline__ int64_t unknown_of(const gk_system& s, int64_t p, int d) {
+  if (p < s.n_ori) return (int64_t)d * s.n_ori + p;
+  return d == 0 ? 3 * s.n_ori + (p - s.n_ori) : -1;
+}
+
+// ---------------------------------------------------------------- neighbours
+__global__ __launch_bounds__(256) void knn_kernel(gk_system sys, int64_t n, int32_t* idx, float* hk) {
+  __shared__ float sx[256], sy[256], sz[256];
+  const int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const double ia = 1.0 / sys.range;
+  float px = 0, py = 0, pz = 0;
+  if (p < n) {
+    double x[3];
+    point_xyz(sys, p, x);
+    px = (float)((x[0] - sys.center[0]) * ia);
+    py = (float)((x[1] - sys.center[1]) * ia);
+    pz = (float)((x[2] - sys.center[2]) * ia);
+  }
+  float bd[K];
+  int32_t bi[K];
+#pragma unroll
+  for (int j = 0; j < K; ++j) { bd[j] = 3.4e38f; bi[j] = -1; }
+  for (int64_t base = 0; base < n; base += 256) {
+    __syncthreads();
+    const int64_t q = base + threadIdx.x;
+    if (q < n) {
+      double x[3];
+      point_xyz(sys, q, x);
+      sx[threadIdx.x] = (float)((x[0] - sys.center[0]) * ia);
+      sy[threadIdx.x] = (float)((x[1] - sys.center[1]) * ia);
+      sz[threadIdx.x] = (float)((x[2] - sys.center[2]) * ia);
+    }
+    __syncthreads();
+    const int cnt = (int)std::min<int64_t>(256, n - base);
+    for (int j = 0; j < cnt; ++j) {
+      const float dx = sx[j] - px, dy = sy[j] - py, dz = sz[j] - pz;
+      float d = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+      if (d < bd[K - 1]) {          // insertion into the sorted list (ties keep the lower index)
+        int32_t id = (int32_t)(base + j);
+#pragma unroll
+        for (int t = 0; t < K; ++t) {
+          if (d < bd[t]) {
+            const float td = bd[t];
+            const int32_t ti = bi[t];
+            bd[t] = d; bi[t] = id;
+            d = td; id = ti;
+          }
+        }
+      }
+    }
+  }
+  if (p < n) {
+#pragma unroll
+    for (int j = 0; j < K; ++j) idx[p * K + j] = bi[j];
+    hk[p] = sqrtf(bd[K - 1]);
+  }
+}
+
+// ------------------------------------------------------------------ features
+// f = [ (q-p)/h_P (3), |q-p|/h_P, P is orientation, Q is orientation, P == Q, 10 h_P ]
+__global__ void features_kernel(gk_system sys, int64_t nq, const int32_t* pts, const int32_t* idx,
+                                const float* hk, float* feat) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nq * K) return;
+  const int64_t b = t / K;
+  const int j = (int)(t % K);
+  const int64_t p = pts ? pts[b] : b;
+  const int64_t q = idx[p * K + j];
+  double xp[3], xq[3];
+  point_xyz(sys, p, xp);
+  point_xyz(sys, q, xq);
+  const double ia = 1.0 / sys.range;
+  const double h = std::max((double)hk[p], 1e-12);
+  const double d0 = (xq[0] - xp[0]) * ia / h, d1 = (xq[1] - xp[1]) * ia / h, d2 = (xq[2] - xp[2]) * ia / h;
+  float* f = feat + t * F;
+  f[0] = (float)d0; f[1] = (float)d1; f[2] = (float)d2;
+  f[3] = (float)sqrt(d0 * d0 + d1 * d1 + d2 * d2);
+  f[4] = p < sys.n_ori ? 1.0f : 0.0f;
+  f[5] = q < sys.n_ori ? 1.0f : 0.0f;
+  f[6] = p == q ? 1.0f : 0.0f;
+  f[7] = (float)(10.0 * h);
+}
+
+// ------------------------------------------------------------- local blocks
+// H_loc of point pts[b]: (3K x 3K) equilibrated covariance among the unknowns of
+// its neighbourhood; dummy slots are identity rows/columns.
+__global__ __launch_bounds__(256) void local_blocks_kernel(Geo G, int64_t nq, const int32_t* pts, const int32_t* idx,
+                                                           const double* dscale, float* Hloc) {
+  const int64_t b = blockIdx.x;
+  const int64_t p = pts ? pts[b] : b;
+  __shared__ int64_t un[LD];
+  if (threadIdx.x < LD) {
+    const int j = threadIdx.x / 3, d = threadIdx.x % 3;
+    un[threadIdx.x] = unknown_of(G.s, idx[p * K + j], d);
+  }
+  __syncthreads();
+  float* Hb = Hloc + b * LD * LD;
+  for (int e = threadIdx.x; e < LD * LD; e += 256) {
+    const int r = e / LD, c = e % LD;
+    const int64_t ur = un[r], uc = un[c];
+    double v;
+    if (ur < 0 || uc < 0) v = (r == c) ? 1.0 : 0.0;
+    else v = entry(G, ur, uc) * dscale[ur] * dscale[uc];
+    Hb[e] = (float)v;
+  }
+}
+
+// ---------------------------------------------------------------------- MLP
+struct MlpPtrs {
+  const float *W1, *b1, *W2, *b2, *W3, *b3;
+};
+__device__ __forceinline__ MlpPtrs mlp_ptrs(const float* P) {
+  MlpPtrs m;
+  m.W1 = P; m.b1 = m.W1 + H1 * F; m.W2 = m.b1 + H1; m.b2 = m.W2 + H2 * H1; m.W3 = m.b2 + H2; m.b3 = m.W3 + O * H2;
+  return m;
+}
+
+__global__ __launch_bounds__(256) void mlp_forward_kernel(int64_t n, const float* params, const float* x, float* y,
+                                                          float* acts) {
+  __shared__ float P[NPARAM];
+  for (int i = threadIdx.x; i < NPARAM; i += 256) P[i] = params[i];
+  __syncthreads();
+  const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (t >= n) return;
+  const MlpPtrs m = mlp_ptrs(P);
+  float in[F], h1[H1], h2[H2];
+#pragma unroll
+  for (int f = 0; f < F; ++f) in[f] = x[t * F + f];
+  for (int i = 0; i < H1; ++i) {
+    float s = m.b1[i];
+#pragma unroll
+    for (int f = 0; f < F; ++f) s = fmaf(m.W1[i * F + f], in[f], s);
+    h1[i] = tanhf(s);
+  }
+  for (int i = 0; i < H2; ++i) {
+    float s = m.b2[i];
+    for (int k = 0; k < H1; ++k) s = fmaf(m.W2[i * H1 + k], h1[k], s);
+    h2[i] = tanhf(s);
+  }
+  // the self block is the identity plus the network output, so that an
+  // untrained network is the diagonal (Jacobi) preconditioner
+  const bool self = in[6] > 0.5f;
+  for (int o = 0; o < O; ++o) {
+    float s = m.b3[o];
+    for (int k = 0; k < H2; ++k) s = fmaf(m.W3[o * H2 + k], h2[k], s);
+    y[t * O + o] = s + ((self && (o % 4 == 0)) ? 1.0f : 0.0f);
+  }
+  if (acts) {
+    for (int i = 0; i < H1; ++i) acts[t * (H1 + H2) + i] = h1[i];
+    for (int i = 0; i < H2; ++i) acts[t * (H1 + H2) + H1 + i] = h2[i];
+  }
+}
+
+// Parameter gradients of sum_t dy_t . y_t; LDS accumulation per block, then one
+// partial row per block (summed in block order by mlp_reduce_kernel).
+__global__ __launch_bounds__(256) void mlp_backward_kernel(int64_t n, const float* params, const float* x,
+                                                           const float* acts, const float* dy, float* partial) {
+  __shared__ float P[NPARAM];
+  __shared__ float Gs[NPARAM];
+  for (int i = threadIdx.x; i < NPARAM; i += 256) { P[i] = params[i]; Gs[i] = 0.0f; }
+  __syncthreads();
+  const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (t < n) {
+    const MlpPtrs m = mlp_ptrs(P);
+    float* gW1 = Gs; float* gb1 = gW1 + H1 * F; float* gW2 = gb1 + H1; float* gb2 = gW2 + H2 * H1;
+    float* gW3 = gb2 + H2; float* gb3 = gW3 + O * H2;
+    float in[F], h1[H1], h2[H2], d[O], d2[H2], d1[H1];
+    for (int f = 0; f < F; ++f) in[f] = x[t * F + f];
+    for (int i = 0; i < H1; ++i) h1[i] = acts[t * (H1 + H2) + i];
+    for (int i = 0; i < H2; ++i) h2[i] = acts[t * (H1 + H2) + H1 + i];
+    for (int o = 0; o < O; ++o) d[o] = dy[t * O + o];
+    for (int k = 0; k < H2; ++k) d2[k] = 0.0f;
+    for (int o = 0; o < O; ++o) {
+      if (d[o] == 0.0f) continue;
+      atomicAdd(&gb3[o], d[o]);
+      for (int k = 0; k < H2; ++k) {
+        atomicAdd(&gW3[o * H2 + k], d[o] * h2[k]);
+        d2[k] = fmaf(m.W3[o * H2 + k], d[o], d2[k]);
+      }
+    }
+    for (int k = 0; k < H1; ++k) d1[k] = 0.0f;
+    for (int i = 0; i < H2; ++i) {
+      const float g = d2[i] * (1.0f - h2[i] * h2[i]);
+      atomicAdd(&gb2[i], g);
+      for (int k = 0; k < H1; ++k) {
+        atomicAdd(&gW2[i * H1 + k], g * h1[k]);
+        d1[k] = fmaf(m.W2[i * H1 + k], g, d1[k]);
+      }
+    }
+    for (int i = 0; i < H1; ++i) {
+      const float g = d1[i] * (1.0f - h1[i] * h1[i]);
+      atomicAdd(&gb1[i], g);
+      for (int f = 0; f < F; ++f) atomicAdd(&gW1[i * F + f], g * in[f]);
+    }
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < NPARAM; i += 256) partial[(int64_t)blockIdx.x * NPARAM + i] = Gs[i];
+}
+
+__global__ void mlp_reduce_kernel(const float* partial, int64_t nblk, float* grad) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= NPARAM) return;
+  float s = 0.0f;
+  for (int64_t b = 0; b < nblk; ++b) s += partial[b * NPARAM + i];
+  grad[i] = s;
+}
+
+// ---------------------------------------------------------------- SPAI loss
+// For point pts[b] with outputs Y[b*K + j][9] (3x3 blocks, row-major):
+// R = M_P H_loc - E_P on the valid rows of P, loss_b = ||R||^2,
+// dY = 2 R H_loc^T scattered back to the blocks (zero on the dummy rows).
+__global__ __launch_bounds__(64) void spai_loss_kernel(gk_system sys, int64_t nq, const int32_t* pts,
+                                                       const int32_t* idx, const float* Hloc, const float* Y,
+                                                       float* dY, float* loss) {
+  __shared__ float H[LD][LD + 1];
+  __shared__ float M[3][LD];
+  __shared__ float R[3][LD];
+  __shared__ float red[64];
+  const int64_t b = blockIdx.x;
+  const int64_t p = pts ? pts[b] : b;
+  const int rows = p < sys.n_ori ? 3 : 1;
+  for (int e = threadIdx.x; e < LD * LD; e += 64) H[e / LD][e % LD] = Hloc[b * LD * LD + e];
+  for (int e = threadIdx.x; e < 3 * LD; e += 64) {
+    const int r = e / LD, c = e % LD;
+    const int j = c / 3, cc = c % 3;
+    M[r][c] = Y[(b * K + j) * O + 3 * r + cc];
+  }
+  __syncthreads();
+  int self = 0;
+  for (int j = 0; j < K; ++j) if (idx[p * K + j] == p) { self = j; break; }
+  float acc = 0.0f;
+  for (int e = threadIdx.x; e < 3 * LD; e += 64) {
+    const int r = e / LD, c = e % LD;
+    float s = 0.0f;
+    for (int k = 0; k < LD; ++k) s = fmaf(M[r][k], H[k][c], s);
+    if (c == 3 * self + r) s -= 1.0f;
+    if (r >= rows) s = 0.0f;
+    R[r][c] = s;
+    acc = fmaf(s, s, acc);
+  }
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = 32; w > 0; w >>= 1) {
+    if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) loss[b] = red[0];
+  for (int e = threadIdx.x; e < 3 * LD; e += 64) {
+    const int r = e / LD, c = e % LD;
+    float s = 0.0f;
+    for (int k = 0; k < LD; ++k) s = fmaf(R[r][k], H[c][k], s);
+    const int j = c / 3, cc = c % 3;
+    dY[(b * K + j) * O + 3 * r + cc] = 2.0f * s;
+  }
+}
+
+__global__ void adam_kernel(float* params, const float* grad, float* m, float* v, int n, float lr, float b1, float b2,
+                            float eps, float c1, float c2, float scale) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float g = grad[i] * scale;
+  m[i] = b1 * m[i] + (1.0f - b1) * g;
+  v[i] = b2 * v[i] + (1.0f - b2) * g * g;
+  params[i] -= lr * (m[i] / c1) / (sqrtf(v[i] / c2) + eps);
+}
+
+// -------------------------------------------------------------- SPAI apply
+// y = M u on the covariance unknowns (u, y FP64 of length Nc; blocks FP32).
+__global__ void spai_apply_kernel(gk_system sys, int64_t n, const int32_t* idx, const float* Y, const double* u,
+                                  double* y) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  double acc[3] = {0.0, 0.0, 0.0};
+  for (int j = 0; j < K; ++j) {
+    const int64_t q = idx[p * K + j];
+    double xq[3];
+    for (int c = 0; c < 3; ++c) {
+      const int64_t uc = unknown_of(sys, q, c);
+      xq[c] = uc >= 0 ? u[uc] : 0.0;
+    }
+    const float* B = Y + (p * K + j) * O;
+    for (int r = 0; r < 3; ++r)
+      acc[r] += (double)B[3 * r] * xq[0] + (double)B[3 * r + 1] * xq[1] + (double)B[3 * r + 2] * xq[2];
+  }
+  for (int r = 0; r < 3; ++r) {
+    const int64_t ur = unknown_of(sys, p, r);
+    if (ur >= 0) y[ur] = acc[r];
+  }
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------- API
+extern "C" int gk_spai_nparams(void) { return NPARAM; }
+
+extern "C" int gk_knn(gk_handle h, const gk_system* sys, int32_t* idx, float* hk, gk_stream stream) {
+  GK_REQUIRE(h && sys && idx && hk, GK_ERR_ARG, "gk_knn: NULL argument");
+  const int64_t n = sys->n_ori + sys->n_rest;
+  GK_REQUIRE(n >= K, GK_ERR_ARG, "gk_knn: needs at least %d data points (got %lld)", K, (long long)n);
+  hipLaunchKernelGGL(knn_kernel, dim3((unsigned)gk_ceil_div(n, 256)), dim3(256), 0, (hipStream_t)stream, *sys, n,
+                     idx, hk);
+  GK_HIP(hipGetLastError());
+  return GK_OK;
+}
+
+extern "C" int gk_spai_features(gk_handle h, const gk_system* sys, int64_t nq, const int32_t* pts, const int32_t* idx,
+                                const float* hk, float* feat, gk_stream stream) {
+  GK_REQUIRE(h && sys && idx && hk && feat && nq >= 0, GK_ERR_ARG, "gk_spai_features: bad arguments");
+  if (nq == 0) return GK_OK;
+  hipLaunchKernelGGL(features_kernel, dim3((unsigned)gk_ceil_div(nq * K, 256)), dim3(256), 0, (hipStream_t)stream,
+                     *sys, nq, pts, idx, hk, feat);
+  GK_HIP(hipGetLastError());
+  return GK_OK;
+}
+
+extern "C" int gk_spai_local_blocks(gk_handle h, const gk_system* sys, int64_t nq, const int32_t* pts,
+                                    const int32_t* idx, const double* dscale, float* Hloc, gk_stream stream) {
+  GK_REQUIRE(h && sys && pts && idx && dscale && Hloc && nq >= 0, GK_ERR_ARG, "gk_spai_local_blocks: bad arguments");
+  if (nq == 0) return GK_OK;
+  hipLaunchKernelGGL(local_blocks_kernel, dim3((unsigned)nq), dim3(256), 0, (hipStream_t)stream, make_geo(sys), nq,
+                     pts, idx, dscale, Hloc);
+  GK_HIP(hipGetLastError());
+  return GK_OK;
+}
+
+extern "C" int gk_mlp_forward(gk_handle h, int64_t n, const float* params, const float* x, float* y, float* acts,
+                              gk_stream stream) {
+  GK_REQUIRE(h && params && x && y && n >= 0, GK_ERR_ARG, "gk_mlp_forward: bad arguments");
+  if (n == 0) return GK_OK;
+  hipLaunchKernelGGL(mlp_forward_kernel, dim3((unsigned)gk_ceil_div(n, 256)), dim3(256), 0, (hipStream_t)stream, n,
+                     params, x, y, acts);
+  GK_HIP(hipGetLastError());
+  return GK_OK;
+}
+
+extern "C" int gk_mlp_backward(gk_handle h, int64_t n, const float* params, const float* x, const float* acts,
+                               const float* dy, float* grad, gk_stream stream) {
+  GK_REQUIRE(h && params && x && acts && dy && grad && n > 0, GK_ERR_ARG, "gk_mlp_backward: bad arguments");
+  hipStream_t st = (hipStream_t)stream;
+  const int64_t nb = gk_ceil_div(n, 256);
+  void* ws = nullptr;
+  GK_TRY(gk_workspace(h, WS_EVAL_PART, (size_t)nb * NPARAM * sizeof(float) + 256, &ws));
+  float* partial = static_cast<float*>(ws);
+  hipLaunchKernelGGL(mlp_backward_kernel, dim3((unsigned)nb), dim3(256), 0, st, n, params, x, acts, dy, partial);
+  hipLaunchKernelGGL(mlp_reduce_kernel, dim3((unsigned)gk_ceil_div(NPARAM, 256)), dim3(256), 0, st, partial, nb, grad);
+  GK_HIP(hipGetLastError());
+  return GK_OK;
+}
+
+extern "C" int gk_spai_loss(gk_handle h, const gk_system* sys, int64_t nq, const int32_t* pts, const int32_t* idx,
+                            const float* Hloc, const float* Y, float* dY, float* loss, gk_stream stream) {
+  GK_REQUIRE(h && sys && idx && Hloc && Y && dY && loss && nq >= 0, GK_ERR_ARG, "gk_spai_loss: bad arguments");
+  if (nq == 0) return GK_OK;
+  hipLaunchKernelGGL(spai_loss_kernel, dim3((unsigned)nq), dim3(64), 0, (hipStream_t)stream, *sys, nq, pts, idx, Hloc,
+                     Y, dY, loss);
+  GK_HIP(hipGetLastError());
+  return GK_OK;
+}
+
+extern "C" int gk_adam(gk_handle h, float* params, const float* grad, float* m, float* v, int32_t n, float lr,
+                       int32_t step, float grad_scale, gk_stream stream) {
+  GK_REQUIRE(h && params && grad && m && v && n > 0 && step >= 1, GK_ERR_ARG, "gk_adam: bad arguments");
+  const float b1 = 0.9f, b2 = 0.999f;
+  const float c1 = 1.0f - powf(b1, (float)step), c2 = 1.0f - powf(b2, (float)step);
+  hipLaunchKernelGGL(adam_kernel, dim3((unsigned)gk_ceil_div(n, 256)), dim3(256), 0, (hipStream_t)stream, params, grad,
+                     m, v, n, lr, b1, b2, 1e-8f, c1, c2, grad_scale);
+  GK_HIP(hipGetLastError());
+  return GK_OK;
+}
+
+extern "C" int gk_spai_apply_impl(const gk_system* sys, const int32_t* idx, const float* Y, const double* u, double* y,
+                                  hipStream_t st) {
+  const int64_t n = sys->n_ori + sys->n_rest;
+  if (n == 0) return GK_OK;
+  hipLaunchKernelGGL(spai_apply_kernel, dim3((unsigned)gk_ceil_div(n, 256)), dim3(256), 0, st, *sys, n, idx, Y, u, y);
+  GK_HIP(hipGetLastError());
+  return GK_OK;
+}
+
+extern "C" int gk_spai_apply(gk_handle h, const gk_system* sys, const int32_t* idx, const float* Y, const double* u,
+                             double* y, gk_stream stream) {
+  GK_REQUIRE(h && sys && idx && Y && u && y, GK_ERR_ARG, "gk_spai_apply: NULL argument");
+  return gk_spai_apply_impl(sys, idx, Y, u, y, (hipStream_t)stream);
+}

@@ -112,7 +112,7 @@ extern "C" int gk_octree_refine(gk_handle h, int32_t n_fields, const int32_t* id
   hipStream_t st = (hipStream_t)stream;
   const int64_t nb = gk_ceil_div(n, 256);
   void* ws = nullptr;
-  GK_TRY(gk_workspace(h, WS_MATRIX2, (size_t)nb * (sizeof(int32_t) + sizeof(int64_t)) + 512, &ws));
+  GK_TRY(gk_workspace(h, WS_SMALL, (size_t)nb * (sizeof(int32_t) + sizeof(int64_t)) + 512, &ws));
   int32_t* counts = static_cast<int32_t*>(ws);
   int64_t* offsets = reinterpret_cast<int64_t*>(static_cast<char*>(ws) + gk_round_up(nb * 4, 256));
   hipLaunchKernelGGL(octree_mark_kernel, dim3((unsigned)nb), dim3(256), 0, st, ids, n_fields, n, refine, counts);

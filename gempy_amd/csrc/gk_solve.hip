@@ -17,6 +17,9 @@ extern "C" int gk_assemble_impl(gk_handle h, const gk_system* sys, int32_t dtype
                                 double* rhs, double* dscale, hipStream_t st, int lower_only);
 extern "C" int gk_assemble_drift_impl(const gk_system* sys, double* U, hipStream_t st);
 extern "C" int gk_rhs_impl(const gk_system* sys, double* b, hipStream_t st);
+extern "C" int gk_diag_scale_impl(const gk_system* sys, double* d, hipStream_t st);
+extern "C" int gk_spai_apply_impl(const gk_system* sys, const int32_t* idx, const float* Y, const double* u, double* y,
+                                  hipStream_t st);
 extern "C" int gk_matvec(gk_handle h, const gk_system* sys, int32_t precision, const double* x, double* y,
                          gk_stream stream);
 extern "C" int gk_cholesky_f64(gk_handle h, double* A, int64_t n, int64_t lda, double* dinv, int32_t* info,
@@ -162,11 +165,17 @@ inline unsigned nblk(int64_t n, int b = 256) { return (unsigned)std::max<int64_t
 
 struct Solver {
   gk_handle h;
+  gk_system sys_copy;
   const gk_system* sys;
   hipStream_t st;
   int64_t N, Nc, Lt, Npad, Ncpad;
-  // preconditioner (FP32 or FP64 factor of the equilibrated covariance block)
+  // preconditioner (FP32 or FP64 factor of the equilibrated covariance block,
+  // or the learned sparse approximate inverse)
   bool f64 = false;
+  bool spai = false;
+  const int32_t* spai_idx = nullptr;
+  const float* spai_Y = nullptr;
+  int32_t mv_prec = GK_F64;          // precision of the Krylov matvec
   void* H = nullptr;
   void* dinv = nullptr;
   double* dscale = nullptr;
@@ -175,6 +184,7 @@ struct Solver {
   double* Sinv = nullptr;
   void* uvec = nullptr;
   double* ytmp = nullptr;
+  double* ytmp2 = nullptr;
   double* lam = nullptr;
   double* partial = nullptr;
   // Krylov
@@ -207,6 +217,13 @@ struct Solver {
 
   // K^-1 r (covariance block) through the Cholesky factor of the equilibrated block
   int apply_K(const double* rin, double* yout) {
+    if (spai) {
+      double* u = static_cast<double*>(uvec);
+      hipLaunchKernelGGL(to_scaled_kernel<double>, dim3(nblk(Nc)), dim3(256), 0, st, rin, dscale, u, Nc, Nc);
+      GK_TRY(gk_spai_apply_impl(sys, spai_idx, spai_Y, u, ytmp2, st));
+      hipLaunchKernelGGL(from_scaled_kernel<double>, dim3(nblk(Nc)), dim3(256), 0, st, ytmp2, dscale, yout, Nc);
+      return GK_OK;
+    }
     if (f64) {
       double* u = static_cast<double*>(uvec);
       hipLaunchKernelGGL(to_scaled_kernel<double>, dim3(nblk(Ncpad)), dim3(256), 0, st, rin, dscale, u, Nc, Ncpad);
@@ -264,6 +281,11 @@ struct Solver {
     }
     *info_out = info;
     if (info != 0) return GK_ERR_NOT_SPD;
+    return setup_drift();
+  }
+
+  // Y = K^-1 U with the current apply_K and the inverse of S = U^T Y (Lt x Lt, host)
+  int setup_drift() {
     if (Lt > 0) {
       GK_TRY(gk_assemble_drift_impl(sys, U, st));
       for (int l = 0; l < Lt; ++l) {
@@ -358,7 +380,7 @@ struct Solver {
         double* vj = V + (int64_t)j * Npad;
         double* zj = Z + (int64_t)j * Npad;
         GK_TRY(apply_M(vj, zj));
-        GK_TRY(gk_matvec(h, sys, GK_F64, zj, w, st));
+        GK_TRY(gk_matvec(h, sys, mv_prec, zj, w, st));
         std::vector<double> hc(j + 1, 0.0);
         for (int pass = 0; pass < 2; ++pass) {      // classical Gram-Schmidt, two passes
           GK_TRY(host_dots(V, Npad, j + 1, w, N, hdev, hv.data()));
@@ -484,29 +506,16 @@ static int solve_lu(gk_handle h, const gk_system* sys, hipStream_t st, int64_t N
   return GK_OK;
 }
 
-extern "C" int gk_solve(gk_handle h, const gk_system* sys, int32_t form, double tol, int32_t max_iter,
-                        double* x, gk_solve_info* info, gk_stream stream) {
-  GK_REQUIRE(h && sys && x && info, GK_ERR_ARG, "gk_solve: NULL argument");
-  GK_REQUIRE(form == GK_SOLVE_FP64_LU || form == GK_SOLVE_FP32_REFINE || form == GK_SOLVE_FP64_CHOL, GK_ERR_ARG,
-             "gk_solve: bad form");
-  hipStream_t st = (hipStream_t)stream;
-  *info = gk_solve_info{};
+// Carve the solver's vectors out of the WS_SOLVER workspace and set b.
+static int init_solver(gk_handle h, const gk_system* sys, hipStream_t st, int form, Solver& S) {
   const int64_t N = gk_system_n(sys);
-  GK_REQUIRE(N > 0, GK_ERR_ARG, "gk_solve: empty system");
   const int64_t Npad = gk_round_up(N, GK_BLOCK);
   const int L = sys->uni_degree == 0 ? 0 : (sys->uni_degree == 1 ? 3 : 9);
   const int64_t Lt = L + sys->n_fault;
   const int64_t Nc = N - Lt;
-  hipEvent_t ev[4];
-  for (auto& e : ev) GK_HIP(hipEventCreate(&e));
-  struct EvGuard { hipEvent_t* e; ~EvGuard() { for (int i = 0; i < 4; ++i) (void)hipEventDestroy(e[i]); } } guard{ev};
-  if (tol <= 0) tol = 1e-10;
-  if (max_iter <= 0) max_iter = 300;
-
-  Solver S;
-  S.h = h; S.sys = sys; S.st = st;
+  S.h = h; S.sys_copy = *sys; S.sys = &S.sys_copy; S.st = st;
   S.N = N; S.Nc = Nc; S.Lt = Lt; S.Npad = Npad; S.Ncpad = gk_round_up(std::max<int64_t>(Nc, 1), GK_BLOCK);
-  const bool fast = form == GK_SOLVE_FP32_REFINE;   // needs the Krylov basis
+  const bool fast = form != GK_SOLVE_FP64_LU && form != GK_SOLVE_FP64_CHOL;   // needs the Krylov basis
   const int m = fast ? S.m : 1;
   size_t need = 0;
   auto take = [&](size_t bytes) { size_t o = need; need += gk_round_up(bytes, 256); return o; };
@@ -517,6 +526,7 @@ extern "C" int gk_solve(gk_handle h, const gk_system* sys, int32_t form, double 
   const size_t o_Si = take(32 * 32 * sizeof(double));
   const size_t o_u32 = take(S.Ncpad * sizeof(double));
   const size_t o_yt = take(Npad * sizeof(double));
+  const size_t o_yt2 = take(Npad * sizeof(double));
   const size_t o_lam = take(64 * sizeof(double));
   const size_t o_part = take((size_t)(gk_ceil_div(Npad, RED_ROWS) + 1) * 64 * sizeof(double));
   const size_t o_V = take((size_t)(m + 1) * Npad * sizeof(double));
@@ -536,6 +546,7 @@ extern "C" int gk_solve(gk_handle h, const gk_system* sys, int32_t form, double 
   S.Sinv = reinterpret_cast<double*>(base + o_Si);
   S.uvec = base + o_u32;
   S.ytmp = reinterpret_cast<double*>(base + o_yt);
+  S.ytmp2 = reinterpret_cast<double*>(base + o_yt2);
   S.lam = reinterpret_cast<double*>(base + o_lam);
   S.partial = reinterpret_cast<double*>(base + o_part);
   S.V = reinterpret_cast<double*>(base + o_V);
@@ -546,6 +557,32 @@ extern "C" int gk_solve(gk_handle h, const gk_system* sys, int32_t form, double 
   S.b = reinterpret_cast<double*>(base + o_b);
   S.xt = reinterpret_cast<double*>(base + o_xt);
   GK_TRY(gk_rhs_impl(sys, S.b, st));
+  return GK_OK;
+}
+
+extern "C" int gk_solve(gk_handle h, const gk_system* sys, int32_t form, double tol, int32_t max_iter,
+                        double* x, gk_solve_info* info, gk_stream stream) {
+  GK_REQUIRE(h && sys && x && info, GK_ERR_ARG, "gk_solve: NULL argument");
+  GK_REQUIRE(form == GK_SOLVE_FP64_LU || form == GK_SOLVE_FP32_REFINE || form == GK_SOLVE_FP64_CHOL, GK_ERR_ARG,
+             "gk_solve: bad form");
+  hipStream_t st = (hipStream_t)stream;
+  *info = gk_solve_info{};
+  const int64_t N = gk_system_n(sys);
+  GK_REQUIRE(N > 0, GK_ERR_ARG, "gk_solve: empty system");
+  const int64_t Npad = gk_round_up(N, GK_BLOCK);
+  const int L = sys->uni_degree == 0 ? 0 : (sys->uni_degree == 1 ? 3 : 9);
+  const int64_t Lt = L + sys->n_fault;
+  const int64_t Nc = N - Lt;
+  hipEvent_t ev[4];
+  for (auto& e : ev) GK_HIP(hipEventCreate(&e));
+  struct EvGuard { hipEvent_t* e; ~EvGuard() { for (int i = 0; i < 4; ++i) (void)hipEventDestroy(e[i]); } } guard{ev};
+  if (tol <= 0) tol = 1e-10;
+  if (max_iter <= 0) max_iter = 300;
+
+  if (h->solver_state && h->solver_free) h->solver_free(h->solver_state);
+  h->solver_state = nullptr;
+  Solver S;
+  GK_TRY(init_solver(h, sys, st, form, S));
 
   if (form == GK_SOLVE_FP64_LU) {
     GK_TRY(solve_lu(h, sys, st, N, Npad, x, info, ev));
@@ -568,8 +605,15 @@ extern "C" int gk_solve(gk_handle h, const gk_system* sys, int32_t form, double 
     info->form = GK_SOLVE_FP64_CHOL;
     return GK_OK;
   };
+  auto keep_state = [&]() {
+    Solver* k = new Solver(S);
+    k->sys = &k->sys_copy;
+    h->solver_state = k;
+    h->solver_free = [](void* p) { delete static_cast<Solver*>(p); };
+  };
   if (form == GK_SOLVE_FP64_CHOL) {
     GK_TRY(run_chol64());
+    keep_state();
     return GK_OK;
   }
   // FP32 factor + FP64 GMRES refinement
@@ -593,5 +637,268 @@ extern "C" int gk_solve(gk_handle h, const gk_system* sys, int32_t form, double 
     info->fallback = 1;
     GK_TRY(run_chol64());
   }
+  keep_state();
   return GK_OK;
+}
+
+// ---------------------------------------------------------------- adjoint
+namespace {
+
+__global__ void gather_all_points_kernel(gk_system sys, double* pts) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t no = sys.n_ori, R = sys.n_rest, S = sys.n_surf;
+  if (i >= no + R + S) return;
+  const double* p = i < no ? sys.ori_xyz + 3 * i
+                    : (i < no + R ? sys.rest_xyz + 3 * (i - no) : sys.ref_xyz + 3 * (i - no - R));
+  pts[3 * i] = p[0]; pts[3 * i + 1] = p[1]; pts[3 * i + 2] = p[2];
+}
+
+// c = Phi^T e on the covariance rows: gi grad E(q) and i_res (E(p_i) - E(p_ref))
+__global__ void adjoint_rhs_kernel(gk_system sys, const double* E, const double* gE, double* c) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t no = sys.n_ori, R = sys.n_rest, NG = 3 * no;
+  if (i < NG) {
+    const int64_t u = i / no, g = i % no;
+    c[i] = sys.gi_res * gE[3 * g + u];
+  } else if (i < NG + R) {
+    const int64_t r = i - NG;
+    c[i] = sys.i_res * (E[no + r] - E[no + R + sys.rest_surf[r]]);
+  }
+}
+
+// drift rows of Phi^T e: gi_res sum_o e_o f_l(x_o), one block per drift function
+__global__ void adjoint_drift_rhs_kernel(const double* obs, const double* e, int64_t n_obs, double gi, double* c) {
+  __shared__ double red[256];
+  const int l = blockIdx.x;
+  double acc = 0.0;
+  for (int64_t o = threadIdx.x; o < n_obs; o += 256) {
+    const double* p = obs + 3 * o;
+    double f;
+    if (l < 3) f = p[l];
+    else if (l < 6) f = p[l - 3] * p[l - 3];
+    else if (l == 6) f = p[0] * p[1];
+    else if (l == 7) f = p[0] * p[2];
+    else f = p[1] * p[2];
+    acc = fma(e[o], f, acc);
+  }
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) c[l] = gi * red[0];
+}
+
+__global__ void adjoint_fault_rhs_kernel(const double* F, int nf, int f, const double* e, int64_t n_obs, double gi,
+                                         double* c) {
+  __shared__ double red[256];
+  double acc = 0.0;
+  for (int64_t o = threadIdx.x; o < n_obs; o += 256) acc = fma(e[o], F[o * nf + f], acc);
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *c = gi * red[0];
+}
+
+__global__ void surface_sum2_kernel(const double* a, const double* b, const int32_t* surf, int64_t R, double* Wa,
+                                    double* Wb) {
+  __shared__ double ra[256], rb[256];
+  const int s = blockIdx.x;
+  double x = 0.0, y = 0.0;
+  for (int64_t i = threadIdx.x; i < R; i += 256)
+    if (surf[i] == s) { x += a[i]; y += b[i]; }
+  ra[threadIdx.x] = x; rb[threadIdx.x] = y;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) { ra[threadIdx.x] += ra[threadIdx.x + w]; rb[threadIdx.x] += rb[threadIdx.x + w]; }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) { Wa[s] = ra[0]; Wb[s] = rb[0]; }
+}
+
+// dL/dp_a = i_res Omega_a grad E(p_a) - Lambda_a grad S_w(p_a) - Omega_a grad S_lam(p_a)
+// with Omega (Lambda) the signed per-point weights of w (lam): w_i on a rest
+// point, -sum over the surface on its reference point.
+__global__ void adjoint_points_kernel(gk_system sys, const double* w, const double* lam, const double* Ww,
+                                      const double* Wl, const double* gE, const double* gSw, const double* gSl,
+                                      double* out) {
+  const int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t R = sys.n_rest, S = sys.n_surf, NG = 3 * sys.n_ori;
+  if (a >= R + S) return;
+  double Om, La;
+  if (a < R) { Om = w[NG + a]; La = lam[NG + a]; }
+  else { Om = -Ww[a - R]; La = -Wl[a - R]; }
+  for (int u = 0; u < 3; ++u)
+    out[3 * a + u] = sys.i_res * Om * gE[3 * a + u] - La * gSw[3 * a + u] - Om * gSl[3 * a + u];
+}
+
+__global__ void adjoint_grad_kernel(int64_t no, const double* lam, double* d_grad) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= 3 * no) return;
+  const int64_t u = i / no, g = i % no;
+  d_grad[3 * g + u] = lam[i];
+}
+
+}  // namespace
+
+extern "C" int gk_eval_sources(gk_handle h, double range, double c_o, const double* center, int64_t n_src,
+                               const double* src_xyz, const double* src_w, int64_t n_points, const double* xyz,
+                               double* scalar, double* gradient, gk_stream stream);
+extern "C" int gk_eval_impl(gk_handle h, const gk_system* sys, const double* weights, gk_eval_desc* d,
+                            hipStream_t st, bool force_mask, const double* src_box);
+
+static Solver* kept_solver(gk_handle h, const gk_system* sys) {
+  Solver* S = static_cast<Solver*>(h->solver_state);
+  if (!S || gk_system_n(sys) != S->N || sys->n_ori != S->sys_copy.n_ori || sys->n_rest != S->sys_copy.n_rest)
+    return nullptr;
+  return S;
+}
+
+extern "C" int gk_solve_rhs(gk_handle h, const gk_system* sys, const double* rhs, double* x, gk_solve_info* info,
+                            gk_stream stream) {
+  GK_REQUIRE(h && sys && rhs && x && info, GK_ERR_ARG, "gk_solve_rhs: NULL argument");
+  Solver* S = kept_solver(h, sys);
+  GK_REQUIRE(S != nullptr, GK_ERR_ARG,
+             "gk_solve_rhs: no factorization of this system is kept (call gk_solve with the FP64_CHOL or "
+             "FP32_REFINE form first)");
+  hipStream_t st = (hipStream_t)stream;
+  S->st = st;
+  *info = gk_solve_info{};
+  GK_HIP(hipMemcpyAsync(S->b, rhs, S->N * sizeof(double), hipMemcpyDeviceToDevice, st));
+  if (S->f64) {
+    int steps = 0;
+    GK_TRY(S->direct_refine(x, 3, &steps, &info->relres));
+    info->iterations = steps;
+    info->form = GK_SOLVE_FP64_CHOL;
+  } else {
+    bool ok = false;
+    GK_TRY(S->fgmres(x, 1e-10, 1e-6, 300, &info->iterations, &info->relres, &ok));
+    info->form = GK_SOLVE_FP32_REFINE;
+    GK_REQUIRE(ok, GK_ERR_NOT_CONVERGED, "gk_solve_rhs: refinement stagnated at relres %.3e", info->relres);
+  }
+  return GK_OK;
+}
+
+extern "C" int gk_adjoint(gk_handle h, const gk_system* sys, const double* w, int64_t n_obs, const double* obs_xyz,
+                          const double* obs_fault, const double* e, double* d_points, double* d_grad, double* lam_out,
+                          gk_stream stream) {
+  GK_REQUIRE(h && sys && w && d_points && d_grad && n_obs >= 0 && (n_obs == 0 || (obs_xyz && e)), GK_ERR_ARG,
+             "gk_adjoint: NULL argument");
+  GK_REQUIRE(sys->n_fault == 0 || n_obs == 0 || obs_fault, GK_ERR_ARG,
+             "gk_adjoint: obs_fault (fault-block values at the observations) required with fault columns");
+  Solver* S = kept_solver(h, sys);
+  GK_REQUIRE(S != nullptr, GK_ERR_ARG, "gk_adjoint: no factorization of this system is kept");
+  hipStream_t st = (hipStream_t)stream;
+  const int64_t no = sys->n_ori, R = sys->n_rest, Sf = sys->n_surf, N = S->N, NP = no + R + Sf;
+  const int L = sys->uni_degree == 0 ? 0 : (sys->uni_degree == 1 ? 3 : 9);
+  // buffers: pts 3NP | E NP | gE 3NP | c N | lam N | gSw 3(R+S) | gSl 3(R+S) | Ww S | Wl S
+  size_t need = 0;
+  auto take = [&](size_t n) { size_t o = need; need += gk_round_up(n * sizeof(double), 256); return o; };
+  const size_t o_pts = take(3 * NP), o_E = take(NP), o_gE = take(3 * NP), o_c = take(N), o_lam = take(N),
+               o_gSw = take(3 * (R + Sf)), o_gSl = take(3 * (R + Sf)), o_Ww = take(Sf + 1), o_Wl = take(Sf + 1),
+               o_fz = take((R + Sf) * std::max(sys->n_fault, 1));
+  void* p = nullptr;
+  GK_TRY(gk_workspace(h, WS_ADJOINT, need, &p));
+  char* base = static_cast<char*>(p);
+  double *pts = (double*)(base + o_pts), *E = (double*)(base + o_E), *gE = (double*)(base + o_gE),
+         *c = (double*)(base + o_c), *lam = (double*)(base + o_lam), *gSw = (double*)(base + o_gSw),
+         *gSl = (double*)(base + o_gSl), *Ww = (double*)(base + o_Ww), *Wl = (double*)(base + o_Wl),
+         *fz = (double*)(base + o_fz);
+  if (NP > 0)
+    hipLaunchKernelGGL(gather_all_points_kernel, dim3(nblk(NP)), dim3(256), 0, st, *sys, pts);
+  // 1. c = Phi^T e: the observation field E and its gradient at the data points
+  GK_TRY(gk_eval_sources(h, sys->range, sys->c_o, sys->center, n_obs, obs_xyz, e, NP, pts, E, gE, st));
+  GK_HIP(hipMemsetAsync(c, 0, N * sizeof(double), st));
+  hipLaunchKernelGGL(adjoint_rhs_kernel, dim3(nblk(3 * no + R)), dim3(256), 0, st, *sys, E, gE, c);
+  if (L > 0 && n_obs > 0)
+    hipLaunchKernelGGL(adjoint_drift_rhs_kernel, dim3(L), dim3(256), 0, st, obs_xyz, e, n_obs, sys->gi_res,
+                       c + 3 * no + R);
+  // fault rows: gi_res sum_o e_o F_f(x_o) (the fault block is constant in the
+  // interface coordinates almost everywhere: these columns have no other term)
+  for (int f = 0; f < sys->n_fault && n_obs > 0; ++f)
+    hipLaunchKernelGGL(adjoint_fault_rhs_kernel, dim3(1), dim3(256), 0, st, obs_fault, sys->n_fault, f, e, n_obs,
+                       sys->gi_res, c + 3 * no + R + L + f);
+  if (sys->n_fault > 0) GK_HIP(hipMemsetAsync(fz, 0, (R + Sf) * sys->n_fault * sizeof(double), st));
+  // 2. lam = A^-1 c with the kept factorization
+  gk_solve_info info{};
+  GK_TRY(gk_solve_rhs(h, sys, c, lam, &info, st));
+  // 3. grad S_w and grad S_lam at the rest + reference points
+  double* prs = pts + 3 * no;
+  for (int pass = 0; pass < 2; ++pass) {
+    gk_eval_desc d{};
+    d.kind = GK_POINTS_EXPLICIT;
+    d.precision = GK_F64;
+    d.n_points = R + Sf;
+    d.xyz = prs;
+    d.scalar = E;                 // scratch (E is no longer needed)
+    d.gradient = pass == 0 ? gSw : gSl;
+    d.fault_values = sys->n_fault > 0 ? fz : nullptr;   // no gradient from the fault block
+    GK_TRY(gk_eval_impl(h, sys, pass == 0 ? w : lam, &d, st, true, nullptr));
+  }
+  // 4. combine
+  if (Sf > 0)
+    hipLaunchKernelGGL(surface_sum2_kernel, dim3((unsigned)Sf), dim3(256), 0, st, w + 3 * no, lam + 3 * no,
+                       sys->rest_surf, R, Ww, Wl);
+  if (R + Sf > 0)
+    hipLaunchKernelGGL(adjoint_points_kernel, dim3(nblk(R + Sf)), dim3(256), 0, st, *sys, w, lam, Ww, Wl,
+                       gE + 3 * no, gSw, gSl, d_points);
+  if (no > 0)
+    hipLaunchKernelGGL(adjoint_grad_kernel, dim3(nblk(3 * no)), dim3(256), 0, st, no, lam, d_grad);
+  if (lam_out) GK_HIP(hipMemcpyAsync(lam_out, lam, N * sizeof(double), hipMemcpyDeviceToDevice, st));
+  GK_HIP(hipGetLastError());
+  return GK_OK;
+}
+
+extern "C" int gk_solve_spai(gk_handle h, const gk_system* sys, const int32_t* idx, const float* Y, double tol,
+                             int32_t max_iter, int32_t krylov_precision, double* x, gk_solve_info* info,
+                             gk_stream stream) {
+  GK_REQUIRE(h && sys && idx && Y && x && info, GK_ERR_ARG, "gk_solve_spai: NULL argument");
+  GK_REQUIRE(krylov_precision == GK_F32 || krylov_precision == GK_F64, GK_ERR_ARG, "gk_solve_spai: bad precision");
+  hipStream_t st = (hipStream_t)stream;
+  *info = gk_solve_info{};
+  if (tol <= 0) tol = 1e-10;
+  if (max_iter <= 0) max_iter = 500;
+  if (h->solver_state && h->solver_free) h->solver_free(h->solver_state);
+  h->solver_state = nullptr;
+  Solver S;
+  GK_TRY(init_solver(h, sys, st, 3, S));
+  S.spai = true;
+  S.spai_idx = idx;
+  S.spai_Y = Y;
+  S.mv_prec = krylov_precision;
+  hipEvent_t ev[3];
+  for (auto& e : ev) GK_HIP(hipEventCreate(&e));
+  struct EvGuard { hipEvent_t* e; ~EvGuard() { for (int i = 0; i < 3; ++i) (void)hipEventDestroy(e[i]); } } guard{ev};
+  GK_HIP(hipEventRecord(ev[0], st));
+  GK_TRY(gk_diag_scale_impl(sys, S.dscale, st));
+  GK_TRY(S.setup_drift());
+  GK_HIP(hipEventRecord(ev[1], st));
+  bool ok = false;
+  GK_TRY(S.fgmres(x, tol, 1e-6, max_iter, &info->iterations, &info->relres, &ok));
+  GK_HIP(hipEventRecord(ev[2], st));
+  GK_HIP(hipEventSynchronize(ev[2]));
+  info->t_factor_ms = elapsed_ms(ev[0], ev[1]);
+  info->t_refine_ms = elapsed_ms(ev[1], ev[2]);
+  info->form = 3;
+  if (!ok) {                      // stagnation: the FP64 direct solve takes over
+    info->fallback = 1;
+    S.spai = false;
+    S.mv_prec = GK_F64;
+    int32_t finfo = 0;
+    GK_TRY(S.setup_preconditioner(true, &finfo));
+    int steps = 0;
+    GK_TRY(S.direct_refine(x, 3, &steps, &info->relres));
+    info->info = finfo;
+    info->form = GK_SOLVE_FP64_CHOL;
+  }
+  return GK_OK;
+}
+
+extern "C" int gk_diag_scale(gk_handle h, const gk_system* sys, double* dscale, gk_stream stream) {
+  GK_REQUIRE(h && sys && dscale, GK_ERR_ARG, "gk_diag_scale: NULL argument");
+  return gk_diag_scale_impl(sys, dscale, (hipStream_t)stream);
 }

@@ -53,6 +53,10 @@ def split_ref_rest(counts: Sequence[int]):
 class DeviceStack:
     """One stack's co-kriging data in HBM and its gk_system descriptor."""
 
+    # incremented by every solve: the library keeps the factorization of the
+    # last solve of the handle, which the adjoint reuses
+    solve_epoch = 0
+
     def __init__(self, stack: StackInput, kopts: KernelOptions, center: Sequence[float],
                  fault_sp: Optional[np.ndarray] = None, device: int = 0):
         if any(c < 1 for c in stack.counts):
@@ -109,6 +113,7 @@ class DeviceStack:
         info = _lib.GkSolveInfo()
         _lib.check("gk_solve", lib.gk_solve(self.h, C.byref(self.sys), int(form), float(tol),
                                              int(max_iter), _ptr(x), C.byref(info), _stream()))
+        DeviceStack.solve_epoch += 1
         self.weights = x
         return x, {"form": int(info.form), "iterations": int(info.iterations),
                    "fallback": int(info.fallback), "info": int(info.info),

@@ -227,22 +227,20 @@ def config2(seed: int = 2, scale: float = 1.0, resolution=(200, 200, 200),
 
 
 def config5(seed: int = 5, scale: float = 1.0, resolution=(128, 128, 128), n_obs: int = 10_000):
-    """Differentiation form: the 'lower' stack of config 2 (fault drift frozen)
-    on a 128^3 grid, 10,000 observation points ~ U(box), and a target field from
-    the same model with every surface z0 shifted by N(0, 10 m).
-    Returns (model, perturbed_model, observation_points)."""
+    """Differentiation form: config-2 geometry on a 128^3 grid; the misfit is
+    taken on the 'lower' stack (offset by the fault, whose block is held
+    fixed) at 10,000 observation points ~ U(box) against the field of the same
+    stack with every surface z0 shifted by N(0, 10 m).
+    Returns (model, index of the differentiated stack, perturbed copy of that
+    stack, observation points)."""
     base = config2(seed=2, scale=scale, resolution=resolution)
     rng = np.random.default_rng(seed)
-    lower = base.stacks[3]
+    k = 3
+    lower = base.stacks[k]
     obs = rng.uniform(0, 1000, (n_obs, 3))
     shift = np.repeat(rng.normal(0, 10.0, len(lower.counts)), lower.counts)
     pert = lower.copy()
     pert.sp_xyz = lower.sp_xyz.copy()
     pert.sp_xyz[:, 2] += shift
-    single = StackInput(**{**lower.__dict__, "faults": [], "relation": StackRelationType.ERODE})
-    psingle = StackInput(**{**pert.__dict__, "faults": [], "relation": StackRelationType.ERODE})
-    m = GeoModelInput(stacks=[single], options=_options(), grid=RegularGrid(EXTENT, resolution),
-                      name="config5")
-    mp = GeoModelInput(stacks=[psingle], options=_options(), grid=RegularGrid(EXTENT, resolution),
-                       name="config5_target")
-    return m, mp, obs
+    base.name = "config5"
+    return base, k, pert, obs

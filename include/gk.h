@@ -42,6 +42,9 @@ enum { GK_F32 = 0, GK_F64 = 1 };
 enum { GK_POINTS_EXPLICIT = 0, GK_POINTS_LATTICE = 1 };
 enum { GK_SOLVE_FP64_LU = 0, GK_SOLVE_FP32_REFINE = 1, GK_SOLVE_FP64_CHOL = 2 };
 #define GK_MAX_THRESHOLDS 64
+#define GK_SPAI_K 16          /* neighbours per data point of the learned preconditioner */
+#define GK_SPAI_FEATURES 8    /* pair features fed to its network */
+#define GK_SPAI_HIDDEN 32     /* width of its two hidden layers */
 #define GK_BLOCK 256   /* factorization block size; lda of factored matrices is a multiple */
 
 /* One stack's co-kriging data in the engine's layout
@@ -171,9 +174,34 @@ int gk_gemm_f64(gk_handle h, int64_t m, int64_t n, int64_t k, const double* A, i
 int gk_solve(gk_handle h, const gk_system* sys, int32_t form, double tol, int32_t max_iter,
              double* x, gk_solve_info* info, gk_stream stream);
 
+/* Solve A x = rhs with the factorization kept by the last gk_solve on this
+ * handle (FP64_CHOL or FP32_REFINE forms; the same system). */
+int gk_solve_rhs(gk_handle h, const gk_system* sys, const double* rhs, double* x, gk_solve_info* info,
+                 gk_stream stream);
+
+/* ---- differentiation (the reference tapes torch autograd through the
+ * engine: compute_API.py:49,66; gradients w.r.t. sp_coords and the
+ * orientation gradients, test_ad_I.py:41-51) ---------------------------------
+ * Adjoint of  L(S(obs)) with S = field of the solution w of A w = b(G):
+ * e[n_obs] = dL/dS at the observation points.  Returns dL/dp for the rest
+ * points then the reference points (d_points [(n_rest+n_surf)*3]) and dL/dG
+ * (d_grad [n_ori*3]); lam_out (nullable, [N]) the adjoint vector.  Uses the
+ * factorization kept by the last gk_solve of `sys`.  With fault columns,
+ * obs_fault [n_obs*n_fault] holds the fault-block values at the observations
+ * (the block is piecewise constant: it enters the adjoint right-hand side
+ * only). */
+int gk_adjoint(gk_handle h, const gk_system* sys, const double* w, int64_t n_obs, const double* obs_xyz,
+               const double* obs_fault, const double* e, double* d_points, double* d_grad, double* lam_out,
+               gk_stream stream);
+
 /* ---- (c) evaluation: scalar field, gradient, ids ------------------------ */
 int gk_eval(gk_handle h, const gk_system* sys, const double* weights, gk_eval_desc* desc,
             gk_stream stream);
+/* F(x) = sum_s w_s C(|x - s|) and grad F (nullable) at explicit points in FP64,
+ * with explicitly given sources (range, c_o, frame centre as in gk_system). */
+int gk_eval_sources(gk_handle h, double range, double c_o, const double* center, int64_t n_src,
+                    const double* src_xyz, const double* src_w, int64_t n_points, const double* xyz,
+                    double* scalar, double* gradient, gk_stream stream);
 
 /* Stack combination (the engine's erosion masking; relations at
  * gempy/core/data/structural_frame.py:243-273): ids [n_stacks][n] of the
@@ -198,6 +226,46 @@ int gk_octree_points(gk_handle h, const double* extent, const int64_t* res, cons
 int gk_octree_refine(gk_handle h, int32_t n_fields, const int32_t* ids, int64_t n, const int64_t* voxels,
                      const int64_t* res, uint8_t* refine, int64_t* children, int64_t* n_children,
                      gk_stream stream);
+
+/* ---- learned preconditioner for the matrix-free solve (config 4) -------------
+ * No counterpart in the reference (its iterative solver is plain CG/KeOps,
+ * log_2024-05.md:10-50; warm start at _engine_factory.py:45-48).  Data points
+ * = orientations then rest points; each owns a 3-dim unknown block.
+ * gk_knn: the GK_SPAI_K nearest data points of every data point (itself first),
+ *   idx [n*K] int32, hk [n] distance to the K-th (range units).
+ * gk_spai_features: pair features [nq*K*GK_SPAI_FEATURES] of points pts[nq]
+ *   (NULL = all points).
+ * gk_spai_local_blocks: equilibrated covariance among the unknowns of each
+ *   neighbourhood, Hloc [nq][3K][3K] (FP32; dummy slots identity).
+ * gk_mlp_forward / gk_mlp_backward: the network (F -> H -> H -> 9, tanh) on n
+ *   samples; params [gk_spai_nparams()], acts [n*2H] saved for the backward;
+ *   grad receives d(sum dy.y)/dparams.
+ * gk_spai_loss: per point ||M_P Hloc - E_P||^2 and dY = dloss/dY.
+ * gk_adam: one Adam step (beta 0.9/0.999) with grad scaled by grad_scale.
+ * gk_spai_apply: y = M u on the covariance unknowns (FP64 vectors, FP32 blocks).
+ * gk_solve_spai: right-preconditioned restarted GMRES on the matrix-free
+ *   operator (Arnoldi matvec in `krylov_precision`, true residual in FP64)
+ *   with M (equilibrated) on the covariance block and the exact Schur
+ *   complement of the drift; falls back to GK_SOLVE_FP64_CHOL on stagnation. */
+int gk_spai_nparams(void);
+int gk_knn(gk_handle h, const gk_system* sys, int32_t* idx, float* hk, gk_stream stream);
+int gk_spai_features(gk_handle h, const gk_system* sys, int64_t nq, const int32_t* pts, const int32_t* idx,
+                     const float* hk, float* feat, gk_stream stream);
+int gk_spai_local_blocks(gk_handle h, const gk_system* sys, int64_t nq, const int32_t* pts, const int32_t* idx,
+                         const double* dscale, float* Hloc, gk_stream stream);
+int gk_mlp_forward(gk_handle h, int64_t n, const float* params, const float* x, float* y, float* acts,
+                   gk_stream stream);
+int gk_mlp_backward(gk_handle h, int64_t n, const float* params, const float* x, const float* acts,
+                    const float* dy, float* grad, gk_stream stream);
+int gk_spai_loss(gk_handle h, const gk_system* sys, int64_t nq, const int32_t* pts, const int32_t* idx,
+                 const float* Hloc, const float* Y, float* dY, float* loss, gk_stream stream);
+int gk_adam(gk_handle h, float* params, const float* grad, float* m, float* v, int32_t n, float lr, int32_t step,
+            float grad_scale, gk_stream stream);
+int gk_spai_apply(gk_handle h, const gk_system* sys, const int32_t* idx, const float* Y, const double* u, double* y,
+                  gk_stream stream);
+int gk_diag_scale(gk_handle h, const gk_system* sys, double* dscale, gk_stream stream);
+int gk_solve_spai(gk_handle h, const gk_system* sys, const int32_t* idx, const float* Y, double tol,
+                  int32_t max_iter, int32_t krylov_precision, double* x, gk_solve_info* info, gk_stream stream);
 
 #ifdef __cplusplus
 }

@@ -363,3 +363,134 @@ def test_octree_levels_consistent_with_dense_grid(gpu):
     bad = np.count_nonzero(recon.ravel() != dense)
     say(f"octree: level sizes {[o.voxel_index.numel() for o in octs]}, reconstruction mismatches {bad} / {dense.size}")
     assert bad <= 0.005 * dense.size
+
+
+def test_adjoint_matches_autograd_and_fd(gpu):
+    say("adjoint: small stack, 300 observation points")
+    from gempy_amd.autodiff import misfit_gradient
+    from oracle import torch_cokriging as tk
+    st, _ = _small_stack(21, n_surf=3, per=30, n_ori=20)
+    k = _kopts()
+    rng = np.random.default_rng(4)
+    obs = rng.uniform(0, 1000, (300, 3))
+    target = rng.normal(size=300) * 30 + 900
+    center = [500.0, 500.0, 500.0]
+    loss, d_sp, d_g = misfit_gradient(st, k, obs, target, center)
+    # torch autograd through the restated oracle (CPU, float64)
+    t = lambda x: torch.tensor(x, dtype=torch.float64)
+    sp = t(st.sp_xyz).requires_grad_(True)
+    G = t(st.ori_grad).requires_grad_(True)
+    w, rest, ref, rs = tk.build_and_solve(sp, st.counts, t(st.sp_nugget), t(st.ori_xyz), G, t(st.ori_nugget),
+                                          k.range, k.c_o)
+    S = tk.evaluate(t(obs), w, rest, ref, rs, t(st.ori_xyz), k.range, k.c_o)
+    lt = ((S - t(target)) ** 2).sum()
+    lt.backward()
+    say(f"adjoint: loss gpu={loss:.10e} torch={float(lt):.10e}")
+    assert abs(loss - float(lt)) <= 1e-8 * abs(float(lt))
+    scale_sp = np.abs(sp.grad.numpy()).max()
+    scale_g = np.abs(G.grad.numpy()).max()
+    err_sp = np.abs(d_sp - sp.grad.numpy()).max() / scale_sp
+    err_g = np.abs(d_g - G.grad.numpy()).max() / scale_g
+    say(f"adjoint vs autograd: max rel err d/dp = {err_sp:.3e}, d/dG = {err_g:.3e}")
+    assert err_sp <= 1e-6 and err_g <= 1e-6
+    # central finite differences of the GPU forward (h = 1e-3 * extent)
+    from gempy_amd.autodiff import scalar_field_at
+    o = torch.tensor(obs, dtype=torch.float64, device="cuda")
+
+    def L(spv, gv):
+        with torch.no_grad():
+            Sv = scalar_field_at(torch.tensor(spv), torch.tensor(gv), st, k, o, center)
+        return float(((Sv.cpu().numpy() - target) ** 2).sum())
+    for (i, u) in [(0, 2), (7, 0), (45, 1)]:
+        h = 1.0
+        a, b2 = st.sp_xyz.copy(), st.sp_xyz.copy()
+        a[i, u] += h; b2[i, u] -= h
+        fd = (L(a, st.ori_grad) - L(b2, st.ori_grad)) / (2 * h)
+        say(f"fd d/dp[{i},{u}] = {fd:.6e} adjoint = {d_sp[i, u]:.6e}")
+        assert abs(fd - d_sp[i, u]) <= 1e-3 * max(abs(fd), 1e-3 * scale_sp)
+
+
+# ------------------------------------------------------------ learned SPAI
+def _spai_system(seed, n):
+    from gempy_amd import learned
+    ds, m = learned.training_stack(seed, n)
+    return ds, m
+
+
+def test_spai_local_blocks_gradient_match_oracle(gpu):
+    say("spai: local blocks and MLP/loss gradient vs torch float64 reference")
+    from gempy_amd import learned
+    from oracle import spai_reference as sr
+    ds, m = _spai_system(1000, 700)
+    st = m.stacks[0]
+    k = m.options.kernel_options
+    idx, hk = learned.neighbourhoods(ds)
+    d = learned.diag_scale(ds)
+    n_o, R = ds.sys.n_ori, ds.sys.n_rest
+    pts = torch.arange(0, n_o + R, max(1, (n_o + R) // 48), dtype=torch.int32, device=gpu)[:48].contiguous()
+    net = learned.SpaiNetwork(seed=3)
+    net.params += 0.05 * torch.randn_like(net.params)     # away from the identity start
+    loss, grad, (Hloc, feat, y, dy) = learned.loss_and_grad(net, ds, idx, hk, d, pts)
+    # local blocks vs the oracle's equilibrated matrix
+    Ao, bo = ok.build_system(st.sp_xyz, st.counts, st.sp_nugget, st.ori_xyz, st.ori_grad, st.ori_nugget,
+                             k.range, k.c_o)
+    Nc = len(bo) - 3
+    dd = 1 / np.sqrt(np.diag(Ao)[:Nc])
+    Hn = dd[:, None] * Ao[:Nc, :Nc] * dd[None, :]
+    idx_h, H_h = idx.cpu().numpy(), Hloc.cpu().numpy()
+    def unk(p, c):
+        return c * n_o + p if p < n_o else (3 * n_o + p - n_o if c == 0 else -1)
+    for b, p in enumerate(pts.cpu().numpy()[:6]):
+        u = [unk(int(q), c) for q in idx_h[p] for c in range(3)]
+        for a_ in range(48):
+            for c_ in range(48):
+                ref = (1.0 if a_ == c_ else 0.0) if (u[a_] < 0 or u[c_] < 0) else Hn[u[a_], u[c_]]
+                assert abs(H_h[b, a_, c_] - ref) <= 2e-7 * max(1.0, abs(ref))
+    # loss and parameter gradient vs torch autograd (float64)
+    P = net.params.double().cpu().requires_grad_(True)
+    x = feat.double().cpu()
+    yr = sr.forward(P, x)
+    np.testing.assert_allclose(y.cpu().numpy(), yr.detach().numpy(), rtol=2e-4, atol=2e-5)
+    rows = torch.tensor([3 if int(p) < n_o else 1 for p in pts.cpu().numpy()])
+    selfp = torch.tensor([int(np.nonzero(idx_h[int(p)] == int(p))[0][0]) for p in pts.cpu().numpy()])
+    lr_ = sr.spai_loss(yr, Hloc.double().cpu(), rows, selfp, 16)
+    np.testing.assert_allclose(loss.cpu().numpy(), lr_.detach().numpy(), rtol=2e-3, atol=1e-5)
+    lr_.sum().backward()
+    g_ref = P.grad.numpy()
+    err = np.abs(grad.cpu().numpy() - g_ref).max() / np.abs(g_ref).max()
+    say(f"spai: gradient max rel err {err:.3e}")
+    assert err <= 2e-3
+
+
+def test_learned_preconditioner_training_and_gain(gpu):
+    say("spai: short training, then GMRES iterations learned vs identity")
+    from gempy_amd import learned
+    L = _lib()
+    net = learned.SpaiNetwork(seed=0)
+    val = [_spai_system(9001, 1500)[0], _spai_system(9002, 2500)[0]]
+
+    def val_loss():
+        tot = 0.0
+        for ds in val:
+            idx, hk = learned.neighbourhoods(ds)
+            d = learned.diag_scale(ds)
+            pts = torch.arange(ds.sys.n_ori + ds.sys.n_rest, dtype=torch.int32, device=gpu)
+            loss, _, _ = learned.loss_and_grad(net, ds, idx, hk, d, pts)
+            tot += float(loss.mean())
+        return tot / len(val)
+    before = val_loss()
+    hist = learned.train(net, steps=150, seeds=range(1000, 1150), size=lambda s: 1000 + (s % 7) * 250,
+                         batch=512, lr=3e-3)
+    after = val_loss()
+    say(f"spai: validation loss {before:.4e} -> {after:.4e}; train loss {hist[0]:.3e} -> {hist[-1]:.3e}")
+    assert after < before
+    its = {}
+    for name, fn in (("identity", learned.identity_blocks), ("learned", lambda ds: learned.spai_blocks(net, ds))):
+        tot = 0
+        for ds in val:
+            idx, y = fn(ds)
+            x, info = learned.solve_spai(ds, idx, y, tol=1e-7, max_iter=400, krylov_precision=L.GK_F64)
+            say(f"spai: {name} N={ds.N} -> {info}")
+            tot += info["iterations"] if not info["fallback"] else 10 ** 6
+        its[name] = tot
+    assert its["learned"] < its["identity"]
