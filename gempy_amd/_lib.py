@@ -46,7 +46,7 @@ class GkSolveInfo(C.Structure):
     _fields_ = [
         ("form", C.c_int32), ("iterations", C.c_int32), ("fallback", C.c_int32), ("info", C.c_int32),
         ("relres", C.c_double), ("t_assemble_ms", C.c_double), ("t_factor_ms", C.c_double),
-        ("t_refine_ms", C.c_double),
+        ("t_refine_ms", C.c_double), ("shift", C.c_double),
     ]
 
 

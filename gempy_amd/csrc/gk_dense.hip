@@ -22,7 +22,7 @@ constexpr int NB = GK_BLOCK;   // 256
 #define GK_BK_F32 32
 #endif
 #ifndef GK_BK_F64
-#define GK_BK_F64 16
+#define GK_BK_F64 32
 #endif
 
 typedef float f4v __attribute__((ext_vector_type(4)));
@@ -90,19 +90,21 @@ __global__ __launch_bounds__(256) void gemm_kernel(int64_t m, int64_t n, int64_t
 #pragma unroll
     for (int j = 0; j < 4; ++j) acc[i][j] = typename M::acc_t{0, 0, 0, 0};
 
-  // global -> register staging: 128 x BK tile = 128*BK/VEC vectors = 4 per thread
+  // global -> register staging: 128 x BK tile = 128*BK/VEC vectors, NV per thread
   constexpr int VPR = BK / VEC;           // vectors per row
-  vec_t ra[4], rb[4];
+  constexpr int NV = 128 * BK / VEC / 256;
+  static_assert(NV >= 1 && NV * 256 * VEC == 128 * BK, "BK must make the tile a whole number of vectors per thread");
+  vec_t ra[NV], rb[NV];
   auto load_global = [&](int64_t k0) {
 #pragma unroll
-    for (int it = 0; it < 4; ++it) {
+    for (int it = 0; it < NV; ++it) {
       const int idx = tid + 256 * it;
       const int row = idx / VPR, cv = idx % VPR;
       ra[it] = *reinterpret_cast<const vec_t*>(A + (m0 + row) * lda + k0 + cv * VEC);
     }
     if (TRANS_B) {
 #pragma unroll
-      for (int it = 0; it < 4; ++it) {
+      for (int it = 0; it < NV; ++it) {
         const int idx = tid + 256 * it;
         const int row = idx / VPR, cv = idx % VPR;
         rb[it] = *reinterpret_cast<const vec_t*>(B + (n0 + row) * ldb + k0 + cv * VEC);
@@ -110,7 +112,7 @@ __global__ __launch_bounds__(256) void gemm_kernel(int64_t m, int64_t n, int64_t
     } else {
       constexpr int VPRB = 128 / VEC;     // vectors per k-row of the B tile
 #pragma unroll
-      for (int it = 0; it < 4; ++it) {
+      for (int it = 0; it < NV; ++it) {
         const int idx = tid + 256 * it;
         const int kk = idx / VPRB, cv = idx % VPRB;
         rb[it] = *reinterpret_cast<const vec_t*>(B + (k0 + kk) * ldb + n0 + cv * VEC);
@@ -119,14 +121,14 @@ __global__ __launch_bounds__(256) void gemm_kernel(int64_t m, int64_t n, int64_t
   };
   auto store_lds = [&](int buf) {
 #pragma unroll
-    for (int it = 0; it < 4; ++it) {
+    for (int it = 0; it < NV; ++it) {
       const int idx = tid + 256 * it;
       const int row = idx / VPR, cv = idx % VPR;
       *reinterpret_cast<vec_t*>(&As[buf][row * LDS_LD + cv * VEC]) = ra[it];
     }
     if (TRANS_B) {
 #pragma unroll
-      for (int it = 0; it < 4; ++it) {
+      for (int it = 0; it < NV; ++it) {
         const int idx = tid + 256 * it;
         const int row = idx / VPR, cv = idx % VPR;
         *reinterpret_cast<vec_t*>(&Bs[buf][row * LDS_LD + cv * VEC]) = rb[it];
@@ -134,7 +136,7 @@ __global__ __launch_bounds__(256) void gemm_kernel(int64_t m, int64_t n, int64_t
     } else {
       constexpr int VPRB = 128 / VEC;
 #pragma unroll
-      for (int it = 0; it < 4; ++it) {
+      for (int it = 0; it < NV; ++it) {
         const int idx = tid + 256 * it;
         const int kk = idx / VPRB, cv = idx % VPRB;
 #pragma unroll
@@ -465,20 +467,6 @@ __global__ __launch_bounds__(256) void gemv_panel_kernel(const T* M, int64_t ld,
   if (r < r1 && sub == 0) b[r] -= s;
 }
 
-// b[c] -= sum_r M[r0 + r, c] x[r] for c in [0, c1), r < NB.
-template <typename T>
-__global__ __launch_bounds__(256) void gemv_panelT_kernel(const T* M, int64_t ld, int64_t r0, int64_t c1,
-                                                          const T* x, T* b) {
-  __shared__ T xs[NB];
-  xs[threadIdx.x] = x[threadIdx.x];
-  __syncthreads();
-  const int64_t c = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (c >= c1) return;
-  T s = T(0);
-  for (int r = 0; r < NB; ++r) s = fma(M[(r0 + r) * ld + c], xs[r], s);
-  b[c] -= s;
-}
-
 // x_k = Dinv_k b_k (TRANS: Dinv_k^T b_k), in place on b + k0.
 template <typename T, bool TRANS>
 __global__ __launch_bounds__(256) void diag_apply_kernel(const T* Dk, T* b) {
@@ -590,7 +578,7 @@ static int cholesky_impl(gk_handle h, T* A, int64_t n, int64_t lda, T* dinv, int
                        (int64_t)NB, A + (k0 + NB) * lda + k0, lda, rem, (int64_t)NB);
     GK_TRY(gemm_launch<T>(rem, rem, NB, W, NB, W, NB, true, T(-1), T(1), A + (k0 + NB) * lda + k0 + NB, lda, true,
                           st));
-    if (kb % 32 == 31) {      // stop early once a pivot has failed
+    if (kb % 8 == 7) {        // stop early once a pivot has failed
       int32_t hinfo = 0;
       GK_HIP(hipMemcpyAsync(&hinfo, dinfo, sizeof(int32_t), hipMemcpyDeviceToHost, st));
       GK_HIP(hipStreamSynchronize(st));
@@ -605,29 +593,6 @@ static int cholesky_impl(gk_handle h, T* A, int64_t n, int64_t lda, T* dinv, int
   return hinfo == 0 ? GK_OK : GK_ERR_NOT_SPD;
 }
 
-template <typename T>
-static int cholesky_solve_impl(const T* L, const T* dinv, int64_t lda, T* b, hipStream_t st) {
-  GK_REQUIRE(L && dinv && b && lda % NB == 0, GK_ERR_ARG, "gk_cholesky_solve: bad arguments");
-  const int64_t nb = lda / NB;
-  for (int64_t kb = 0; kb < nb; ++kb) {          // forward: L y = b
-    const int64_t k0 = kb * NB;
-    hipLaunchKernelGGL((diag_apply_kernel<T, false>), dim3(1), dim3(NB), 0, st, dinv + k0 * NB, b + k0);
-    const int64_t r0 = k0 + NB;
-    if (r0 < lda)
-      hipLaunchKernelGGL(gemv_panel_kernel<T>, dim3((unsigned)gk_ceil_div(lda - r0, 32)), dim3(256), 0, st, L,
-                         lda, r0, lda, k0, b + k0, b);
-  }
-  for (int64_t kb = nb - 1; kb >= 0; --kb) {     // backward: L^T x = y
-    const int64_t k0 = kb * NB;
-    hipLaunchKernelGGL((diag_apply_kernel<T, true>), dim3(1), dim3(NB), 0, st, dinv + k0 * NB, b + k0);
-    if (k0 > 0)
-      hipLaunchKernelGGL(gemv_panelT_kernel<T>, dim3((unsigned)gk_ceil_div(k0, 256)), dim3(256), 0, st, L, lda,
-                         k0, k0, b + k0, b);
-  }
-  GK_HIP(hipGetLastError());
-  return GK_OK;
-}
-
 extern "C" int gk_cholesky_f32(gk_handle h, float* A, int64_t n, int64_t lda, float* dinv, int32_t* info,
                                gk_stream stream) {
   return cholesky_impl<float>(h, A, n, lda, dinv, info, (hipStream_t)stream);
@@ -638,13 +603,11 @@ extern "C" int gk_cholesky_f64(gk_handle h, double* A, int64_t n, int64_t lda, d
 }
 extern "C" int gk_cholesky_solve_f32(gk_handle h, const float* L, const float* dinv, int64_t lda, float* b,
                                      gk_stream stream) {
-  (void)h;
-  return cholesky_solve_impl<float>(L, dinv, lda, b, (hipStream_t)stream);
+  return gk_trsv_chol_impl<float>(h, L, dinv, lda, b, (hipStream_t)stream);
 }
 extern "C" int gk_cholesky_solve_f64(gk_handle h, const double* L, const double* dinv, int64_t lda, double* b,
                                      gk_stream stream) {
-  (void)h;
-  return cholesky_solve_impl<double>(L, dinv, lda, b, (hipStream_t)stream);
+  return gk_trsv_chol_impl<double>(h, L, dinv, lda, b, (hipStream_t)stream);
 }
 
 extern "C" int gk_lu_f64(gk_handle h, double* A, int64_t n, int64_t lda, double* dinv, int32_t* info,

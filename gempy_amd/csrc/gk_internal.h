@@ -45,6 +45,7 @@ enum gk_ws_slot {
   WS_MATRIX2,
   WS_ADJOINT,        // vectors of the adjoint
   WS_SMALL,          // small scratch (bounding boxes, counts)
+  WS_TRSV,           // work tables, readiness flags and partial sums of the triangular solves
   WS_COUNT
 };
 struct gk_context {
@@ -53,9 +54,18 @@ struct gk_context {
   size_t ws_bytes[WS_COUNT] = {};
   void* solver_state = nullptr;            // factorization kept by the last gk_solve
   void (*solver_free)(void*) = nullptr;
+  // state of the sync-free triangular solves (tables in WS_TRSV)
+  int trsv_nb = 0, trsv_slots = 0, trsv_nfwd = 0, trsv_nbwd = 0;
+  unsigned trsv_epoch = 0;
+  unsigned long long trsv_tickets = 0;
 };
 
 int gk_workspace(gk_handle h, int slot, size_t bytes, void** out);
+// L^-T L^-1 b in place (one launch per direction); gk_trsv_check reports a
+// readiness wait that reached its spin limit since the last check.
+template <typename T>
+int gk_trsv_chol_impl(gk_handle h, const T* L, const T* dinv, int64_t lda, T* b, hipStream_t st);
+int gk_trsv_check(gk_handle h, hipStream_t st);
 
 static inline int64_t gk_ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
 static inline int64_t gk_round_up(int64_t a, int64_t b) { return gk_ceil_div(a, b) * b; }

@@ -176,6 +176,7 @@ struct Solver {
   const int32_t* spai_idx = nullptr;
   const float* spai_Y = nullptr;
   int32_t mv_prec = GK_F64;          // precision of the Krylov matvec
+  double shift = 0.0;                // diagonal shift of the factored block
   void* H = nullptr;
   void* dinv = nullptr;
   double* dscale = nullptr;
@@ -253,9 +254,14 @@ struct Solver {
     return GK_OK;
   }
 
-  // Assemble and factor the equilibrated covariance block in FP32 (no retry:
-  // a breakdown means the block is too ill-conditioned for an FP32 factor) or
-  // FP64 (a tiny diagonal shift is tried if the factorization breaks down).
+  // Assemble and factor the equilibrated covariance block (unit diagonal).
+  // FP32: the block is factored with a diagonal shift, H + sigma I (shifted
+  // low-precision Cholesky, Higham & Pranesh, SIAM J. Sci. Comput. 43(1),
+  // 2021): rounding H to FP32 alone perturbs it by ~u32 sqrt(n), more than its
+  // smallest eigenvalues, so the unshifted FP32 factorization breaks down.
+  // sigma starts at 4 u32 sqrt(n) and grows 4x per breakdown (4 attempts);
+  // the FP64 GMRES outer iteration removes the shift's effect.  FP64: no
+  // shift unless the factorization breaks down (then 1e-13, 1e-11).
   int setup_preconditioner(bool use_f64, int* info_out) {
     f64 = use_f64;
     const size_t esz = f64 ? sizeof(double) : sizeof(float);
@@ -263,8 +269,8 @@ struct Solver {
     GK_TRY(gk_workspace(h, WS_MATRIX, (size_t)Ncpad * Ncpad * esz, &p));
     H = p;
     int32_t info = 0;
-    double shift = 0.0;
-    for (int attempt = 0; attempt < (f64 ? 3 : 1); ++attempt) {
+    shift = f64 ? 0.0 : 4.0 * 5.9604644775390625e-08 * std::sqrt((double)Nc);
+    for (int attempt = 0; attempt < (f64 ? 3 : 4); ++attempt) {
       GK_TRY(gk_assemble_impl(h, sys, f64 ? GK_F64 : GK_F32, H, Ncpad, nullptr, dscale, st, 1));
       int rc;
       if (f64) {
@@ -273,11 +279,13 @@ struct Solver {
                              Ncpad, Nc, shift);
         rc = gk_cholesky_f64(h, static_cast<double*>(H), Nc, Ncpad, static_cast<double*>(dinv), &info, st);
       } else {
+        hipLaunchKernelGGL(add_diag_kernel<float>, dim3(nblk(Nc)), dim3(256), 0, st, static_cast<float*>(H),
+                           Ncpad, Nc, (float)shift);
         rc = gk_cholesky_f32(h, static_cast<float*>(H), Nc, Ncpad, static_cast<float*>(dinv), &info, st);
       }
       if (rc == GK_OK) break;
       if (rc != GK_ERR_NOT_SPD) return rc;
-      shift = shift == 0.0 ? 1e-13 : shift * 100.0;
+      shift = f64 ? (shift == 0.0 ? 1e-13 : shift * 100.0) : shift * 4.0;
     }
     *info_out = info;
     if (info != 0) return GK_ERR_NOT_SPD;
@@ -339,6 +347,7 @@ struct Solver {
     GK_HIP(hipMemsetAsync(x, 0, N * sizeof(double), st));
     int total = 0;
     double last_true = 1.0;
+    std::vector<double> hist;                 // true residual at each check
     std::vector<double> Hh((m + 1) * m), cs(m), sn(m), g(m + 1), hv(m + 2), yv(m);
     *ok = false;
     bool done = false;
@@ -362,6 +371,7 @@ struct Solver {
       hipLaunchKernelGGL(sub_kernel, dim3(nblk(N)), dim3(256), 0, st, b, w, r, N);
       double nr = 0.0;
       GK_TRY(norm(r, N, &nr));
+      GK_TRY(gk_trsv_check(h, st));
       *out = nr / nb;
       return GK_OK;
     };
@@ -415,8 +425,12 @@ struct Solver {
           double rel = 0.0;
           GK_TRY(true_rel(xt, &rel));
           const bool converged = rel <= tol;
-          const bool stalled = rel > 0.5 * last_true;
+          // Below `accept` a check that does not halve the residual means the
+          // FP64 floor is reached; above it, allow 3 checks for a halving.
+          const bool stalled = rel <= accept ? rel > 0.5 * last_true
+                                             : hist.size() >= 3 && rel > 0.5 * hist[hist.size() - 3];
           last_true = rel;
+          hist.push_back(rel);
           if (converged || stalled || total >= max_iter || hn == 0.0) {
             GK_HIP(hipMemcpyAsync(x, xt, N * sizeof(double), hipMemcpyDeviceToDevice, st));
             *ok = rel <= accept;
@@ -633,6 +647,7 @@ extern "C" int gk_solve(gk_handle h, const gk_system* sys, int32_t form, double 
   info->t_factor_ms = elapsed_ms(ev[0], ev[1]);
   info->t_refine_ms = elapsed_ms(ev[1], ev[2]);
   info->form = GK_SOLVE_FP32_REFINE;
+  info->shift = S.shift;
   if (!ok) {
     info->fallback = 1;
     GK_TRY(run_chol64());

@@ -1,0 +1,415 @@
+// Sync-free blocked triangular solves with a Cholesky factor: one launch for
+// L y = b and one for L^T x = y (in place on b), replacing a launch pair per
+// 256-wide block.
+//
+// L is row-major (ld = lda, a multiple of 256), its 256 x 256 diagonal blocks
+// are not read: dinv holds their inverses (block k at dinv + k0 * 256, ld 256,
+// lower triangular).  Forward, block row k:
+//     y_k = Dinv_k (b_k - sum_{j<k} L_kj y_j).
+// The sum is split over workgroups: a "finisher" for block row k applies the
+// block next to the diagonal (j = k-1), adds the published partial sums of the
+// "far" chunks (j in groups of CHUNK blocks, further from the diagonal), applies
+// Dinv_k and publishes y_k.  The backward solve mirrors it on block columns.
+//
+// Ordering and progress: every workgroup takes a ticket (atomic counter) when it
+// starts; the ticket, not blockIdx, selects its work from a host-built table in
+// which every item depends only on items with smaller tickets (a far chunk of
+// row k is placed right after the finisher that produces the last vector block
+// it reads).  A workgroup that waits therefore waits on workgroups that have
+// already started, so the grid always drains.  Readiness flags hold the launch's
+// epoch (no clearing between launches).  Each wait is bounded: past the spin
+// limit the workgroup raises an error word, publishes its flags without
+// computing, and every later wait sees the error and returns at once.
+#include <vector>
+#include <algorithm>
+#include "gk_internal.h"
+
+namespace {
+
+constexpr int NB = GK_BLOCK;            // 256
+constexpr int CHUNK = 4;                // blocks per far chunk
+constexpr unsigned SPIN_LIMIT = 1u << 21;
+
+template <typename T> struct V4;
+template <> struct V4<float> { typedef float type __attribute__((ext_vector_type(4))); };
+template <> struct V4<double> { typedef double type __attribute__((ext_vector_type(4))); };
+
+struct Item { int k, s, jlo, jhi; };    // s < 0: finisher of block row k
+
+// Relaxed polling (an acquire load would invalidate the caches on every
+// poll); the caller issues one acquire fence after the workgroup joins.
+__device__ __forceinline__ bool wait_epoch(const unsigned* flag, unsigned epoch, int* err) {
+  for (unsigned it = 0; it < SPIN_LIMIT; ++it) {
+    if (__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == epoch) return true;
+    if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return false;
+    __builtin_amdgcn_s_sleep(2);
+  }
+  __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  return false;
+}
+
+// thread 0 waits, the workgroup joins; returns false on error
+__device__ __forceinline__ bool group_wait(const unsigned* flag, unsigned epoch, int* err, int* s_ok) {
+  if (threadIdx.x == 0) *s_ok = wait_epoch(flag, epoch, err) ? 1 : 0;
+  __syncthreads();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  return *s_ok != 0;
+}
+
+// wait for flags[0..n) in parallel (one per thread, strided), then join
+__device__ __forceinline__ bool group_wait_all(const unsigned* flags, int n, unsigned epoch, int* err, int* s_ok) {
+  if (threadIdx.x == 0) *s_ok = 1;
+  __syncthreads();
+  for (int i = threadIdx.x; i < n; i += blockDim.x)
+    if (!wait_epoch(flags + i, epoch, err)) *s_ok = 0;
+  __syncthreads();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  return *s_ok != 0;
+}
+
+__device__ __forceinline__ void group_publish(unsigned* flag, unsigned epoch) {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+  __syncthreads();
+  if (threadIdx.x == 0) __hip_atomic_store(flag, epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Row dots of one wave: lane l returns sum_c M[r0 + 64 w + l, c0 + c] v[c] over
+// nblk consecutive 256-column blocks (v + 256 jj for block jj).  Rows go in
+// groups of 16: lane l loads columns 4l..4l+3 of each row (one coalesced row
+// per load), then a transpose-reduce leaves row 16 g + (l & 15) in lane l.
+template <typename T, bool MASK_UPPER>
+__device__ __forceinline__ T rows64(const T* M, int64_t ld, int64_t r0, int64_t c0, const T* v, int nblk, int w,
+                                    int l) {
+  typedef typename V4<T>::type v4;
+  T mine = T(0);
+#pragma unroll
+  for (int g = 0; g < 4; ++g) {
+    T acc[16];
+#pragma unroll
+    for (int r = 0; r < 16; ++r) acc[r] = T(0);
+    for (int jj = 0; jj < nblk; ++jj) {
+      const v4 x = *reinterpret_cast<const v4*>(v + (int64_t)jj * NB + 4 * l);
+      const T* base = M + (r0 + 64 * w + 16 * g) * ld + c0 + (int64_t)jj * NB + 4 * l;
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        v4 a = *reinterpret_cast<const v4*>(base + r * ld);
+        if (MASK_UPPER) {       // keep c <= row within the diagonal block
+          const int row = 64 * w + 16 * g + r, c = 4 * l;
+          a.x = c <= row ? a.x : T(0);
+          a.y = c + 1 <= row ? a.y : T(0);
+          a.z = c + 2 <= row ? a.z : T(0);
+          a.w = c + 3 <= row ? a.w : T(0);
+        }
+        T s = acc[r];
+        s = fma(a.x, x.x, s);
+        s = fma(a.y, x.y, s);
+        s = fma(a.z, x.z, s);
+        s = fma(a.w, x.w, s);
+        acc[r] = s;
+      }
+    }
+#pragma unroll
+    for (int width = 8; width >= 1; width >>= 1) {
+      const bool upper = (l & width) != 0;
+#pragma unroll
+      for (int i = 0; i < width; ++i) {
+        const T send = upper ? acc[i] : acc[i + width];
+        const T keep = upper ? acc[i + width] : acc[i];
+        acc[i] = keep + __shfl_xor(send, width);
+      }
+    }
+    T t = acc[0];
+    t += __shfl_xor(t, 16);
+    t += __shfl_xor(t, 32);
+    if ((l >> 4) == g) mine = t;
+  }
+  return mine;
+}
+
+// acc4 += sum_{r in wave rows} M[r0 + 64 w + r, c0 + 4l..4l+3] v[64 w + r]
+template <typename T, bool MASK_LOWER>
+__device__ __forceinline__ void coldot(const T* M, int64_t ld, int64_t r0, int64_t c0, const T* v,
+                                       typename V4<T>::type& acc, int w, int l) {
+  typedef typename V4<T>::type v4;
+  const T* base = M + (r0 + 64 * w) * ld + c0 + 4 * l;
+#pragma unroll 16
+  for (int r = 0; r < 64; ++r) {
+    v4 a = *reinterpret_cast<const v4*>(base + r * ld);
+    const T xr = v[64 * w + r];
+    if (MASK_LOWER) {           // keep row >= c within the diagonal block
+      const int row = 64 * w + r, c = 4 * l;
+      a.x = row >= c ? a.x : T(0);
+      a.y = row >= c + 1 ? a.y : T(0);
+      a.z = row >= c + 2 ? a.z : T(0);
+      a.w = row >= c + 3 ? a.w : T(0);
+    }
+    acc.x = fma(a.x, xr, acc.x);
+    acc.y = fma(a.y, xr, acc.y);
+    acc.z = fma(a.z, xr, acc.z);
+    acc.w = fma(a.w, xr, acc.w);
+  }
+}
+
+struct TrsvCtl {
+  const Item* items;
+  const int* rowbase;       // far-chunk slot of (k, s) = rowbase[k] + s
+  unsigned* yflag;          // [nb]
+  unsigned* pflag;          // [slots]
+  unsigned long long* ticket;
+  unsigned long long ticket_base;
+  unsigned epoch;
+  int n_items;
+  int* err;
+};
+
+template <typename T>
+__global__ __launch_bounds__(256) void trsv_fwd_kernel(const T* __restrict__ L, int64_t lda, const T* __restrict__ dinv,
+                                                       T* b, T* part, TrsvCtl ctl) {
+  __shared__ int s_id, s_ok, s_ok2;
+  __shared__ __attribute__((aligned(32))) T tv[NB];
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  if (threadIdx.x == 0)
+    s_id = (int)(atomicAdd(ctl.ticket, 1ull) - ctl.ticket_base);
+  __syncthreads();
+  if (s_id < 0 || s_id >= ctl.n_items) {                      // ticket counter out of step
+    if (threadIdx.x == 0) __hip_atomic_store(ctl.err, 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return;
+  }
+  const Item it = ctl.items[s_id];
+  const int64_t k0 = (int64_t)it.k * NB;
+  if (it.s >= 0) {                                            // far chunk
+    const int slot = ctl.rowbase[it.k] + it.s;
+    if (group_wait(ctl.yflag + it.jhi - 1, ctl.epoch, ctl.err, &s_ok)) {
+      const T sum = rows64<T, false>(L, lda, k0, (int64_t)it.jlo * NB, b + (int64_t)it.jlo * NB, it.jhi - it.jlo,
+                                     w, l);
+      part[(int64_t)slot * NB + 64 * w + l] = sum;
+    }
+    group_publish(ctl.pflag + slot, ctl.epoch);
+    return;
+  }
+  // finisher of block row k
+  bool ok = true;
+  T t = T(0);
+  if (it.k > 0) {
+    ok = group_wait(ctl.yflag + it.k - 1, ctl.epoch, ctl.err, &s_ok);
+    if (ok) t = rows64<T, false>(L, lda, k0, k0 - NB, b + k0 - NB, 1, w, l);
+  }
+  if (ok) {
+    t = b[k0 + 64 * w + l] - t;
+    const int f0 = ctl.rowbase[it.k], nfar = ctl.rowbase[it.k + 1] - f0;
+    ok = group_wait_all(ctl.pflag + f0, nfar, ctl.epoch, ctl.err, &s_ok2);
+    if (ok)
+      for (int s = 0; s < nfar; ++s) t -= part[(int64_t)(f0 + s) * NB + 64 * w + l];
+  }
+  if (ok) {
+    tv[64 * w + l] = t;
+    __syncthreads();
+    b[k0 + 64 * w + l] = rows64<T, true>(dinv + k0 * NB, NB, 0, 0, tv, 1, w, l);
+  }
+  group_publish(ctl.yflag + it.k, ctl.epoch);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void trsv_bwd_kernel(const T* __restrict__ L, int64_t lda, const T* __restrict__ dinv,
+                                                       T* b, T* part, TrsvCtl ctl) {
+  typedef typename V4<T>::type v4;
+  __shared__ int s_id, s_ok, s_ok2;
+  __shared__ __attribute__((aligned(32))) T tv[NB];
+  __shared__ __attribute__((aligned(32))) T red[4][NB];
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  if (threadIdx.x == 0)
+    s_id = (int)(atomicAdd(ctl.ticket, 1ull) - ctl.ticket_base);
+  __syncthreads();
+  if (s_id < 0 || s_id >= ctl.n_items) {                      // ticket counter out of step
+    if (threadIdx.x == 0) __hip_atomic_store(ctl.err, 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return;
+  }
+  const Item it = ctl.items[s_id];
+  const int64_t k0 = (int64_t)it.k * NB;
+  v4 acc = {T(0), T(0), T(0), T(0)};
+  auto reduce_waves = [&](v4 a) -> T {       // column 4l + e of wave w -> thread's column
+    *reinterpret_cast<v4*>(&red[w][4 * l]) = a;
+    __syncthreads();
+    const int c = threadIdx.x;
+    const T s = red[0][c] + red[1][c] + red[2][c] + red[3][c];
+    __syncthreads();
+    return s;
+  };
+  if (it.s >= 0) {                                            // far chunk: rows j in [jlo, jhi)
+    const int slot = ctl.rowbase[it.k + 1] + it.s;
+    if (group_wait(ctl.yflag + it.jlo, ctl.epoch, ctl.err, &s_ok)) {
+      for (int j = it.jlo; j < it.jhi; ++j)
+        coldot<T, false>(L, lda, (int64_t)j * NB, k0, b + (int64_t)j * NB, acc, w, l);
+      part[(int64_t)slot * NB + threadIdx.x] = reduce_waves(acc);
+    }
+    group_publish(ctl.pflag + slot, ctl.epoch);
+    return;
+  }
+  const int nbk = ctl.rowbase[0];        // rowbase[0] carries nb for the backward table
+  bool ok = true;
+  if (it.k + 1 < nbk) {
+    ok = group_wait(ctl.yflag + it.k + 1, ctl.epoch, ctl.err, &s_ok);
+    if (ok) coldot<T, false>(L, lda, k0 + NB, k0, b + k0 + NB, acc, w, l);
+  }
+  T t = T(0);
+  if (ok) {
+    t = b[k0 + threadIdx.x] - reduce_waves(acc);
+    const int f0 = ctl.rowbase[it.k + 1], nfar = ctl.rowbase[it.k + 2] - f0;
+    ok = group_wait_all(ctl.pflag + f0, nfar, ctl.epoch, ctl.err, &s_ok2);
+    if (ok)
+      for (int s = 0; s < nfar; ++s) t -= part[(int64_t)(f0 + s) * NB + threadIdx.x];
+  }
+  if (ok) {
+    tv[threadIdx.x] = t;
+    __syncthreads();
+    v4 a2 = {T(0), T(0), T(0), T(0)};
+    coldot<T, true>(dinv + k0 * NB, NB, 0, 0, tv, a2, w, l);
+    b[k0 + threadIdx.x] = reduce_waves(a2);
+  }
+  group_publish(ctl.yflag + it.k, ctl.epoch);
+}
+
+// Host-side work tables.  Forward: finisher(j) then the far chunks whose
+// nearest block is j (rows k = j + 2 + CHUNK s, blocks [k-1-CHUNK(s+1), k-1-CHUNK s)).
+void build_tables(int nb, std::vector<Item>& fwd, std::vector<int>& fbase, std::vector<Item>& bwd,
+                  std::vector<int>& bbase) {
+  auto nfar = [&](int k) { return k >= 2 ? (k - 1 + CHUNK - 1) / CHUNK : 0; };     // forward row k
+  fbase.assign(nb + 1, 0);
+  for (int k = 0; k < nb; ++k) fbase[k + 1] = fbase[k] + nfar(k);
+  fwd.clear();
+  for (int j = 0; j < nb; ++j) {
+    fwd.push_back({j, -1, 0, 0});
+    for (int k = j + 2; k < nb; k += CHUNK) {
+      const int s = (k - j - 2) / CHUNK;
+      const int jhi = k - 1 - CHUNK * s, jlo = std::max(0, jhi - CHUNK);
+      fwd.push_back({k, s, jlo, jhi});
+    }
+  }
+  // backward row k: far blocks j in [k+2, nb) in chunks [k+2+CHUNK s, ...)
+  auto bfar = [&](int k) { const int m = nb - (k + 2); return m > 0 ? (m + CHUNK - 1) / CHUNK : 0; };
+  bbase.assign(nb + 2, 0);
+  bbase[0] = nb;
+  for (int k = 0; k < nb; ++k) bbase[k + 2] = bbase[k + 1] + bfar(k);
+  bwd.clear();
+  for (int j = nb - 1; j >= 0; --j) {
+    bwd.push_back({j, -1, 0, 0});
+    for (int k = j - 2; k >= 0; k -= CHUNK) {
+      const int s = (j - k - 2) / CHUNK;
+      const int jlo = k + 2 + CHUNK * s, jhi = std::min(nb, jlo + CHUNK);
+      bwd.push_back({k, s, jlo, jhi});
+    }
+  }
+}
+
+}  // namespace
+
+// Workspace: [fwd items][bwd items][fwd base][bwd base][yflag][pflag][ticket][err][partials]
+struct TrsvLayout {
+  size_t o_fi, o_bi, o_fb, o_bb, o_yf, o_pf, o_tk, o_err, o_part, bytes;
+  int nfwd, nbwd, slots;
+};
+
+static TrsvLayout trsv_layout(int nb, int nfwd, int nbwd, int slots) {
+  TrsvLayout L{};
+  size_t o = 0;
+  auto take = [&](size_t b) { size_t r = o; o += gk_round_up((int64_t)b, 256); return r; };
+  L.o_fi = take(sizeof(Item) * nfwd);
+  L.o_bi = take(sizeof(Item) * nbwd);
+  L.o_fb = take(sizeof(int) * (nb + 1));
+  L.o_bb = take(sizeof(int) * (nb + 2));
+  L.o_yf = take(sizeof(unsigned) * nb);
+  L.o_pf = take(sizeof(unsigned) * std::max(slots, 1));
+  L.o_tk = take(sizeof(unsigned long long));
+  L.o_err = take(sizeof(int));
+  L.o_part = take(sizeof(double) * NB * (size_t)std::max(slots, 1));
+  L.bytes = o;
+  L.nfwd = nfwd; L.nbwd = nbwd; L.slots = slots;
+  return L;
+}
+
+static int trsv_prepare(gk_handle h, int nb, hipStream_t st, char** base, TrsvLayout* lay) {
+  std::vector<Item> fwd, bwd;
+  std::vector<int> fb, bb;
+  const bool rebuild = h->trsv_nb != nb || h->ws[WS_TRSV] == nullptr;
+  if (rebuild) build_tables(nb, fwd, fb, bwd, bb);
+  const int slots = rebuild ? std::max(fb[nb], bb[nb + 1]) : h->trsv_slots;
+  const int nfwd = rebuild ? (int)fwd.size() : h->trsv_nfwd;
+  const int nbwd = rebuild ? (int)bwd.size() : h->trsv_nbwd;
+  *lay = trsv_layout(nb, nfwd, nbwd, slots);
+  void* p = nullptr;
+  const void* before = h->ws[WS_TRSV];
+  GK_TRY(gk_workspace(h, WS_TRSV, lay->bytes, &p));
+  *base = static_cast<char*>(p);
+  if (rebuild || p != before) {
+    if (!rebuild) build_tables(nb, fwd, fb, bwd, bb);
+    GK_HIP(hipMemcpyAsync(*base + lay->o_fi, fwd.data(), sizeof(Item) * fwd.size(), hipMemcpyHostToDevice, st));
+    GK_HIP(hipMemcpyAsync(*base + lay->o_bi, bwd.data(), sizeof(Item) * bwd.size(), hipMemcpyHostToDevice, st));
+    GK_HIP(hipMemcpyAsync(*base + lay->o_fb, fb.data(), sizeof(int) * fb.size(), hipMemcpyHostToDevice, st));
+    GK_HIP(hipMemcpyAsync(*base + lay->o_bb, bb.data(), sizeof(int) * bb.size(), hipMemcpyHostToDevice, st));
+    GK_HIP(hipMemsetAsync(*base + lay->o_yf, 0, lay->o_part - lay->o_yf, st));   // flags, ticket, error
+    GK_HIP(hipStreamSynchronize(st));          // host tables go out of scope
+    h->trsv_nb = nb;
+    h->trsv_slots = slots;
+    h->trsv_nfwd = nfwd;
+    h->trsv_nbwd = nbwd;
+    h->trsv_epoch = 0;
+    h->trsv_tickets = 0;
+  }
+  return GK_OK;
+}
+
+template <typename T>
+int gk_trsv_chol_impl(gk_handle h, const T* L, const T* dinv, int64_t lda, T* b, hipStream_t st) {
+  GK_REQUIRE(h && L && dinv && b && lda > 0 && lda % NB == 0, GK_ERR_ARG, "triangular solve: bad arguments");
+  const int nb = (int)(lda / NB);
+  char* base = nullptr;
+  TrsvLayout lay;
+  GK_TRY(trsv_prepare(h, nb, st, &base, &lay));
+  TrsvCtl c;
+  c.yflag = reinterpret_cast<unsigned*>(base + lay.o_yf);
+  c.pflag = reinterpret_cast<unsigned*>(base + lay.o_pf);
+  c.ticket = reinterpret_cast<unsigned long long*>(base + lay.o_tk);
+  c.err = reinterpret_cast<int*>(base + lay.o_err);
+  T* part = reinterpret_cast<T*>(base + lay.o_part);
+  // forward
+  c.items = reinterpret_cast<const Item*>(base + lay.o_fi);
+  c.rowbase = reinterpret_cast<const int*>(base + lay.o_fb);
+  c.ticket_base = h->trsv_tickets;
+  c.epoch = ++h->trsv_epoch;
+  c.n_items = lay.nfwd;
+  hipLaunchKernelGGL(trsv_fwd_kernel<T>, dim3((unsigned)lay.nfwd), dim3(256), 0, st, L, lda, dinv, b, part, c);
+  h->trsv_tickets += (unsigned long long)lay.nfwd;
+  // backward
+  c.items = reinterpret_cast<const Item*>(base + lay.o_bi);
+  c.rowbase = reinterpret_cast<const int*>(base + lay.o_bb);
+  c.ticket_base = h->trsv_tickets;
+  c.epoch = ++h->trsv_epoch;
+  c.n_items = lay.nbwd;
+  hipLaunchKernelGGL(trsv_bwd_kernel<T>, dim3((unsigned)lay.nbwd), dim3(256), 0, st, L, lda, dinv, b, part, c);
+  h->trsv_tickets += (unsigned long long)lay.nbwd;
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) {
+    h->trsv_nb = 0;              // counters no longer in step: rebuild on the next call
+    GK_REQUIRE(false, GK_ERR_HIP, "triangular solve launch: %s", hipGetErrorString(e));
+  }
+  return GK_OK;
+}
+
+template int gk_trsv_chol_impl<float>(gk_handle, const float*, const float*, int64_t, float*, hipStream_t);
+template int gk_trsv_chol_impl<double>(gk_handle, const double*, const double*, int64_t, double*, hipStream_t);
+
+// Error word of the sync-free solves (spin limit reached); cleared by reading.
+int gk_trsv_check(gk_handle h, hipStream_t st) {
+  if (!h->ws[WS_TRSV] || h->trsv_nb == 0) return GK_OK;
+  TrsvLayout lay = trsv_layout(h->trsv_nb, h->trsv_nfwd, h->trsv_nbwd, h->trsv_slots);
+  char* base = static_cast<char*>(h->ws[WS_TRSV]);
+  int err = 0;
+  GK_HIP(hipMemcpyAsync(&err, base + lay.o_err, sizeof(int), hipMemcpyDeviceToHost, st));
+  GK_HIP(hipStreamSynchronize(st));
+  if (err) {
+    h->trsv_nb = 0;              // reset tables, flags and counters on the next call
+    GK_REQUIRE(false, GK_ERR_HIP, err == 1 ? "triangular solve: a readiness wait reached its spin limit"
+                                           : "triangular solve: work ticket out of range");
+  }
+  return GK_OK;
+}

@@ -118,7 +118,8 @@ class DeviceStack:
         return x, {"form": int(info.form), "iterations": int(info.iterations),
                    "fallback": int(info.fallback), "info": int(info.info),
                    "relres": float(info.relres), "t_assemble_ms": float(info.t_assemble_ms),
-                   "t_factor_ms": float(info.t_factor_ms), "t_refine_ms": float(info.t_refine_ms)}
+                   "t_factor_ms": float(info.t_factor_ms), "t_refine_ms": float(info.t_refine_ms),
+                   "shift": float(info.shift)}
 
     # ------------------------------------------------------------ assembly
     def assemble(self, dtype: int = _lib.GK_F64):

@@ -96,10 +96,11 @@ typedef struct {
 typedef struct {
   int32_t form;                /* GK_SOLVE_* actually used (after any fallback) */
   int32_t iterations;          /* Krylov iterations of the refinement */
-  int32_t fallback;            /* 1 if FP32_REFINE stagnated and fell back to FP64 LU */
+  int32_t fallback;            /* 1 if the iterative form fell back to the FP64 Cholesky form */
   int32_t info;                /* factorization info (0 = ok, k>0 = failed pivot) */
   double relres;               /* final ||b - A x|| / ||b|| (FP64, matrix-free) */
   double t_assemble_ms, t_factor_ms, t_refine_ms;
+  double shift;                /* diagonal shift of the FP32-factored block (unit diagonal) */
 } gk_solve_info;
 
 /* ---- library ---------------------------------------------------------- */

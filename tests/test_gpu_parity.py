@@ -111,6 +111,36 @@ def test_cholesky_f32_and_solve(gpu):
     np.testing.assert_array_equal(tb[n:].cpu().numpy(), 0.0)
 
 
+@pytest.mark.parametrize("dtype", ["f32", "f64"])
+def test_triangular_solve_many_blocks(gpu, dtype):
+    """Sync-free blocked solve L^-T L^-1 b over 23 block rows (full and partial
+    far chunks), three calls in a row on one handle, against numpy in FP64."""
+    nb = 23
+    say(f"triangular solve {dtype} nb={nb}")
+    L = _lib()
+    lda = 256 * nb
+    rng = np.random.default_rng(11)
+    Lm = np.tril(rng.standard_normal((lda, lda)) * (0.5 / np.sqrt(lda)), -1) + np.diag(1.0 + rng.random(lda))
+    dinv = np.zeros((lda, 256))
+    for k in range(nb):
+        blk = slice(256 * k, 256 * (k + 1))
+        dinv[blk] = np.linalg.inv(Lm[blk, blk])
+    tdt = torch.float32 if dtype == "f32" else torch.float64
+    fn = L.load().gk_cholesky_solve_f32 if dtype == "f32" else L.load().gk_cholesky_solve_f64
+    tL = torch.tensor(Lm, dtype=tdt, device=gpu)
+    tD = torch.tensor(dinv, dtype=tdt, device=gpu)
+    tol = 2e-5 if dtype == "f32" else 1e-12
+    for call in range(3):
+        b = rng.standard_normal(lda)
+        tb = torch.tensor(b, dtype=tdt, device=gpu)
+        L.check("trsv", fn(_h(), _p(tL), _p(tD), lda, _p(tb), _st()))
+        want = np.linalg.solve(Lm.T, np.linalg.solve(Lm, b))
+        got = tb.double().cpu().numpy()
+        err = np.abs(got - want).max() / np.abs(want).max()
+        say(f"  call {call}: max rel err {err:.2e}")
+        assert err <= tol
+
+
 def test_cholesky_f64_and_solve(gpu):
     say("cholesky f64 n=700 lda=768")
     L = _lib()
@@ -401,13 +431,19 @@ def test_adjoint_matches_autograd_and_fd(gpu):
         with torch.no_grad():
             Sv = scalar_field_at(torch.tensor(spv), torch.tensor(gv), st, k, o, center)
         return float(((Sv.cpu().numpy() - target) ** 2).sum())
-    for (i, u) in [(0, 2), (7, 0), (45, 1)]:
-        h = 1.0
+    def fd(i, u, h):
         a, b2 = st.sp_xyz.copy(), st.sp_xyz.copy()
         a[i, u] += h; b2[i, u] -= h
-        fd = (L(a, st.ori_grad) - L(b2, st.ori_grad)) / (2 * h)
-        say(f"fd d/dp[{i},{u}] = {fd:.6e} adjoint = {d_sp[i, u]:.6e}")
-        assert abs(fd - d_sp[i, u]) <= 1e-3 * max(abs(fd), 1e-3 * scale_sp)
+        return (L(a, st.ori_grad) - L(b2, st.ori_grad)) / (2 * h)
+    for (i, u) in [(0, 2), (7, 0), (45, 1)]:
+        h = 1e-3 * 1000.0
+        f1, f2 = fd(i, u, h), fd(i, u, h / 2)
+        rich = (4 * f2 - f1) / 3              # O(h^4) Richardson extrapolant
+        say(f"fd d/dp[{i},{u}] h={h}: {f1:.6e}, extrapolated {rich:.6e}, adjoint {d_sp[i, u]:.6e}")
+        # the O(h^2) truncation error of the h = 1e-3 extent difference reaches
+        # ~1% on the reference point of a surface (oracle: 312281 vs 314924)
+        assert abs(f1 - d_sp[i, u]) <= 2e-2 * max(abs(f1), 1e-3 * scale_sp)
+        assert abs(rich - d_sp[i, u]) <= 1e-4 * max(abs(rich), 1e-3 * scale_sp)
 
 
 # ------------------------------------------------------------ learned SPAI
