@@ -88,7 +88,8 @@ SIGNATURES = {
     "gk_adam": (C.c_int, [P, P, P, P, P, C.c_int32, C.c_float, C.c_int32, C.c_float, P]),
     "gk_spai_apply": (C.c_int, [P, C.POINTER(GkSystem), P, P, P, P, P]),
     "gk_diag_scale": (C.c_int, [P, C.POINTER(GkSystem), P, P]),
-    "gk_solve_spai": (C.c_int, [P, C.POINTER(GkSystem), P, P, C.c_double, C.c_int32, C.c_int32, P,
+    "gk_spai_local_inverse": (C.c_int, [P, C.POINTER(GkSystem), P, P, P, P]),
+    "gk_solve_spai": (C.c_int, [P, C.POINTER(GkSystem), P, P, C.c_double, C.c_int32, C.c_int32, C.c_int32, P,
                                 C.POINTER(GkSolveInfo), P]),
     "gk_combine_lith": (C.c_int, [P, C.c_int32, P, C.c_int64, C.POINTER(C.c_int32), P, P]),
     "gk_octree_points": (C.c_int, [P, C.POINTER(C.c_double), C.POINTER(C.c_int64), P, C.c_int64, C.c_int32,

@@ -2,14 +2,15 @@
 //
 //  * GEMM C = beta C + alpha A op(B) on v_mfma_f32_16x16x4f32 / v_mfma_f64_16x16x4f64:
 //    128x128 block tile, 4 waves of 64x64, LDS double buffer.  The K index is
-//    permuted so that each lane reads its 4 (f64) / 8 (f32) consecutive k
-//    values with 128-bit LDS loads; MFMA only needs A and B to agree on it.
-//  * Right-looking blocked Cholesky (FP32) and LU without pivoting (FP64) with
-//    256-wide panels: the diagonal block is factored and inverted as two
-//    LDS-resident 128-wide halves joined by 128^3 GEMMs, the panel is a GEMM
-//    with the inverted diagonal block, and the trailing update is the MFMA
-//    GEMM (lower tiles only for Cholesky).
-//  * Blocked triangular solves with the inverted diagonal blocks.
+//    permuted so that each lane reads its BK/4 consecutive k values with
+//    128-bit LDS loads; MFMA only needs A and B to agree on it.
+//  * Cholesky FP64: recursive on halves of 256-blocks, with a recursive TRSM,
+//    so most flops are large-K GEMMs; Cholesky FP32 and LU without pivoting
+//    (FP64): right-looking with 256-wide panels.  Each 256 diagonal block is factored
+//    and inverted as two LDS-resident 128-wide halves joined by 128^3 GEMMs;
+//    a panel solve is a GEMM with the inverted diagonal block.
+//  * Blocked triangular solves of the LU with the inverted diagonal blocks
+//    (the Cholesky solves are in gk_trsv.hip).
 #include <cmath>
 #include "gk_internal.h"
 
@@ -550,6 +551,65 @@ extern "C" int gk_gemm_f64(gk_handle h, int64_t m, int64_t n, int64_t k, const d
                              (hipStream_t)stream);
 }
 
+// Recursive Cholesky (lower, row-major, in place) of the trailing principal
+// block of nblk 256-blocks at k0:
+//   chol(A11); A21 <- A21 L11^-T (recursive TRSM); A22 -= A21 A21^T; chol(A22).
+// Almost all flops land in GEMMs whose K is half the current order, where the
+// MFMA GEMM runs far closer to its peak than with 256-deep rank updates.
+template <typename T>
+struct CholRec {
+  gk_handle h;
+  T* A;
+  int64_t lda;
+  T* dinv;
+  int32_t* dinfo;
+  T* W;            // lda x NB panel buffer
+  T* Tb;           // 2 DB x DB scratch of the diagonal-block kernel
+  hipStream_t st;
+  int64_t done = 0;      // diagonal blocks factored so far
+  bool failed = false;
+
+  // X <- X L^-T for X = A[r0:r0+m, c0:c0+nbk*NB], L = A[c0:, c0:] (nbk blocks)
+  int trsm(int64_t r0, int64_t m, int64_t c0, int64_t nbk) {
+    if (nbk == 1) {
+      GK_TRY(gemm_launch<T>(m, NB, NB, A + r0 * lda + c0, lda, dinv + c0 * NB, NB, true, T(1), T(0), W, NB, false,
+                            st));
+      hipLaunchKernelGGL(copy_block_kernel<T>, dim3((unsigned)gk_ceil_div(m * NB, 256)), dim3(256), 0, st, W,
+                         (int64_t)NB, A + r0 * lda + c0, lda, m, (int64_t)NB);
+      return GK_OK;
+    }
+    const int64_t a = nbk / 2, b = nbk - a;
+    const int64_t ca = c0, cb = c0 + a * NB;
+    GK_TRY(trsm(r0, m, ca, a));
+    // Xb -= Xa Lb^T, Lb = A[cb:cb+b NB, ca:ca+a NB]
+    GK_TRY(gemm_launch<T>(m, b * NB, a * NB, A + r0 * lda + ca, lda, A + cb * lda + ca, lda, true, T(-1), T(1),
+                          A + r0 * lda + cb, lda, false, st));
+    return trsm(r0, m, cb, b);
+  }
+
+  int chol(int64_t k0, int64_t nblk) {
+    if (failed) return GK_OK;
+    if (nblk == 1) {
+      GK_TRY(diag256_chol<T>(A, lda, k0, dinv, dinfo, Tb, Tb + DB * DB, st));
+      if (++done % 8 == 0) {          // stop early once a pivot has failed
+        int32_t hinfo = 0;
+        GK_HIP(hipMemcpyAsync(&hinfo, dinfo, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+        GK_HIP(hipStreamSynchronize(st));
+        failed = hinfo != 0;
+      }
+      return GK_OK;
+    }
+    const int64_t n1 = nblk / 2, n2 = nblk - n1;
+    const int64_t k1 = k0 + n1 * NB;
+    GK_TRY(chol(k0, n1));
+    if (failed) return GK_OK;
+    GK_TRY(trsm(k1, n2 * NB, k0, n1));
+    GK_TRY(gemm_launch<T>(n2 * NB, n2 * NB, n1 * NB, A + k1 * lda + k0, lda, A + k1 * lda + k0, lda, true, T(-1),
+                          T(1), A + k1 * lda + k1, lda, true, st));
+    return chol(k1, n2);
+  }
+};
+
 template <typename T>
 static int cholesky_impl(gk_handle h, T* A, int64_t n, int64_t lda, T* dinv, int32_t* info, hipStream_t st) {
   GK_REQUIRE(A && dinv, GK_ERR_ARG, "gk_cholesky: A and dinv required");
@@ -566,23 +626,34 @@ static int cholesky_impl(gk_handle h, T* A, int64_t n, int64_t lda, T* dinv, int
     const int64_t tot = (lda - n) * lda + n * (lda - n);
     hipLaunchKernelGGL(pad_identity_kernel<T>, dim3((unsigned)gk_ceil_div(tot, 256)), dim3(256), 0, st, A, lda, n);
   }
-  for (int64_t kb = 0; kb < nb; ++kb) {
-    const int64_t k0 = kb * NB;
-    GK_TRY(diag256_chol<T>(A, lda, k0, dinv, dinfo, Tb, Tb + DB * DB, st));
-    const int64_t rem = lda - k0 - NB;
-    if (rem <= 0) break;
-    // W = A21 Linv^T ;  A21 <- W ; A22 -= W W^T (lower)
-    GK_TRY(gemm_launch<T>(rem, NB, NB, A + (k0 + NB) * lda + k0, lda, dinv + k0 * NB, NB, true, T(1), T(0), W,
-                          NB, false, st));
-    hipLaunchKernelGGL(copy_block_kernel<T>, dim3((unsigned)gk_ceil_div(rem * NB, 256)), dim3(256), 0, st, W,
-                       (int64_t)NB, A + (k0 + NB) * lda + k0, lda, rem, (int64_t)NB);
-    GK_TRY(gemm_launch<T>(rem, rem, NB, W, NB, W, NB, true, T(-1), T(1), A + (k0 + NB) * lda + k0 + NB, lda, true,
-                          st));
-    if (kb % 8 == 7) {        // stop early once a pivot has failed
-      int32_t hinfo = 0;
-      GK_HIP(hipMemcpyAsync(&hinfo, dinfo, sizeof(int32_t), hipMemcpyDeviceToHost, st));
-      GK_HIP(hipStreamSynchronize(st));
-      if (hinfo != 0) break;
+  if (sizeof(T) == 8) {
+    if (nb > 0) {
+      CholRec<T> R{h, A, lda, dinv, dinfo, W, Tb, st};
+      GK_TRY(R.chol(0, nb));
+    }
+  } else {
+    // FP32: right-looking order.  Each rank-256 update rounds the trailing
+    // block relative to the Schur complement as it shrinks, which a one-shot
+    // large-K update does not; on the kernel matrices of this method the
+    // recursive order needs a ~60x larger diagonal shift to factor in FP32.
+    for (int64_t kb = 0; kb < nb; ++kb) {
+      const int64_t k0 = kb * NB;
+      GK_TRY(diag256_chol<T>(A, lda, k0, dinv, dinfo, Tb, Tb + DB * DB, st));
+      const int64_t rem = lda - k0 - NB;
+      if (rem <= 0) break;
+      // W = A21 Linv^T ;  A21 <- W ; A22 -= W W^T (lower)
+      GK_TRY(gemm_launch<T>(rem, NB, NB, A + (k0 + NB) * lda + k0, lda, dinv + k0 * NB, NB, true, T(1), T(0), W,
+                            NB, false, st));
+      hipLaunchKernelGGL(copy_block_kernel<T>, dim3((unsigned)gk_ceil_div(rem * NB, 256)), dim3(256), 0, st, W,
+                         (int64_t)NB, A + (k0 + NB) * lda + k0, lda, rem, (int64_t)NB);
+      GK_TRY(gemm_launch<T>(rem, rem, NB, W, NB, W, NB, true, T(-1), T(1), A + (k0 + NB) * lda + k0 + NB, lda,
+                            true, st));
+      if (kb % 8 == 7) {        // stop early once a pivot has failed
+        int32_t hinfo = 0;
+        GK_HIP(hipMemcpyAsync(&hinfo, dinfo, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+        GK_HIP(hipStreamSynchronize(st));
+        if (hinfo != 0) break;
+      }
     }
   }
   GK_HIP(hipGetLastError());

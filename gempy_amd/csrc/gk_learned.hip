@@ -138,6 +138,63 @@ __global__ __launch_bounds__(256) void local_blocks_kernel(Geo G, int64_t nq, co
   }
 }
 
+// Exact local sparse approximate inverse: for point p, the rows of
+// H_loc^-1 that belong to p's own unknowns, as K 3x3 blocks Y[p][j] (the same
+// layout the network predicts).  H_loc (FP64, equilibrated, dummy slots
+// identity) is inverted in LDS by Gauss-Jordan without pivoting (SPD).
+__global__ __launch_bounds__(256) void local_inverse_kernel(Geo G, int64_t n, const int32_t* idx,
+                                                            const double* dscale, float* Y) {
+  const int64_t p = blockIdx.x;
+  __shared__ double A[LD][LD + 1];
+  __shared__ double X[LD][LD + 1];
+  __shared__ int64_t un[LD];
+  __shared__ double piv;
+  __shared__ int self;
+  if (threadIdx.x < LD) {
+    const int j = threadIdx.x / 3, d = threadIdx.x % 3;
+    un[threadIdx.x] = unknown_of(G.s, idx[p * K + j], d);
+  }
+  if (threadIdx.x == 0) {
+    self = 0;
+    for (int j = 0; j < K; ++j)
+      if (idx[p * K + j] == p) { self = j; break; }
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < LD * LD; e += 256) {
+    const int r = e / LD, c = e % LD;
+    const int64_t ur = un[r], uc = un[c];
+    A[r][c] = (ur < 0 || uc < 0) ? (r == c ? 1.0 : 0.0) : entry(G, ur, uc) * dscale[ur] * dscale[uc];
+    X[r][c] = r == c ? 1.0 : 0.0;
+  }
+  __syncthreads();
+  for (int k = 0; k < LD; ++k) {
+    if (threadIdx.x == 0) piv = 1.0 / A[k][k];
+    __syncthreads();
+    // scale row k
+    for (int c = threadIdx.x; c < 2 * LD; c += 256) {
+      if (c < LD) A[k][c] *= piv;
+      else X[k][c - LD] *= piv;
+    }
+    __syncthreads();
+    // eliminate column k from every other row
+    for (int e = threadIdx.x; e < LD * 2 * LD; e += 256) {
+      const int r = e / (2 * LD), c = e % (2 * LD);
+      if (r == k) continue;
+      const double f = A[r][k];
+      if (c < LD) { if (c != k) A[r][c] -= f * A[k][c]; }
+      else X[r][c - LD] -= f * X[k][c - LD];
+    }
+    __syncthreads();
+    if (threadIdx.x < LD && threadIdx.x != k) A[threadIdx.x][k] = 0.0;
+    __syncthreads();
+  }
+  // Y[p][j][3 r + c] = X[3 self + r][3 j + c]
+  for (int e = threadIdx.x; e < K * 9; e += 256) {
+    const int j = e / 9, rc = e % 9, r = rc / 3, c = rc % 3;
+    Y[(p * K + j) * 9 + rc] = (float)X[3 * self + r][3 * j + c];
+  }
+}
+
 // ---------------------------------------------------------------------- MLP
 struct MlpPtrs {
   const float *W1, *b1, *W2, *b2, *W3, *b3;
@@ -352,6 +409,17 @@ extern "C" int gk_spai_local_blocks(gk_handle h, const gk_system* sys, int64_t n
   if (nq == 0) return GK_OK;
   hipLaunchKernelGGL(local_blocks_kernel, dim3((unsigned)nq), dim3(256), 0, (hipStream_t)stream, make_geo(sys), nq,
                      pts, idx, dscale, Hloc);
+  GK_HIP(hipGetLastError());
+  return GK_OK;
+}
+
+extern "C" int gk_spai_local_inverse(gk_handle h, const gk_system* sys, const int32_t* idx, const double* dscale,
+                                     float* Y, gk_stream stream) {
+  GK_REQUIRE(h && sys && idx && dscale && Y, GK_ERR_ARG, "gk_spai_local_inverse: bad arguments");
+  const int64_t n = sys->n_ori + sys->n_rest;
+  if (n == 0) return GK_OK;
+  hipLaunchKernelGGL(local_inverse_kernel, dim3((unsigned)n), dim3(256), 0, (hipStream_t)stream, make_geo(sys), n,
+                     idx, dscale, Y);
   GK_HIP(hipGetLastError());
   return GK_OK;
 }

@@ -9,6 +9,7 @@
 //   progress at the FP64 rounding floor); if it is still above `accept` the
 //   solve falls back to the FP64 LU.
 #include <cmath>
+#include <cstdlib>
 #include <vector>
 #include <algorithm>
 #include "gk_internal.h"
@@ -22,6 +23,9 @@ extern "C" int gk_spai_apply_impl(const gk_system* sys, const int32_t* idx, cons
                                   hipStream_t st);
 extern "C" int gk_matvec(gk_handle h, const gk_system* sys, int32_t precision, const double* x, double* y,
                          gk_stream stream);
+extern "C" int gk_knn(gk_handle h, const gk_system* sys, int32_t* idx, float* hk, gk_stream stream);
+extern "C" int gk_spai_local_inverse(gk_handle h, const gk_system* sys, const int32_t* idx, const double* dscale,
+                                     float* Y, gk_stream stream);
 extern "C" int gk_cholesky_f64(gk_handle h, double* A, int64_t n, int64_t lda, double* dinv, int32_t* info,
                                gk_stream stream);
 extern "C" int gk_cholesky_solve_f64(gk_handle h, const double* L, const double* dinv, int64_t lda, double* b,
@@ -161,6 +165,36 @@ __global__ void get_col_kernel(const double* U, int Lt, int l, double* dst, int6
   if (i < nc) dst[i] = U[i * Lt + l];
 }
 
+// GK_VERBOSE=1 in the environment prints the refinement's residual history
+inline bool gk_verbose() {
+  static const bool v = [] { const char* e = getenv("GK_VERBOSE"); return e && *e && *e != '0'; }();
+  return v;
+}
+
+// float copy of a scaled vector (padding rows zero)
+__global__ void to_float_kernel(const double* u, float* f, int64_t nc, int64_t npad) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < npad) f[i] = i < nc ? (float)u[i] : 0.0f;
+}
+
+// y (+)= (double) f
+__global__ void add_float_kernel(const float* f, double* y, int64_t nc, int accumulate) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nc) y[i] = accumulate ? y[i] + (double)f[i] : (double)f[i];
+}
+
+// x = [D y ; 0] (covariance unknowns scaled back, drift unknowns zero)
+__global__ void unscale_pad_kernel(const double* y, const double* d, double* x, int64_t nc, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) x[i] = i < nc ? d[i] * y[i] : 0.0;
+}
+
+// v = u - D w  (scaled residual of the covariance block)
+__global__ void scaled_residual_kernel(const double* u, const double* d, const double* w, double* v, int64_t nc) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nc) v[i] = u[i] - d[i] * w[i];
+}
+
 inline unsigned nblk(int64_t n, int b = 256) { return (unsigned)std::max<int64_t>(1, gk_ceil_div(n, b)); }
 
 struct Solver {
@@ -173,6 +207,16 @@ struct Solver {
   // or the learned sparse approximate inverse)
   bool f64 = false;
   bool spai = false;
+  bool pcp = false;                  // factor, local-inverse correction, factor (needs spai blocks and a factor)
+  double* pc_u = nullptr;            // scaled right-hand side of the covariance block
+  double* pc_y = nullptr;            // scaled iterate
+  double* pc_v = nullptr;            // scaled residual / correction
+  double* pc_x = nullptr;            // full-length operator input / output (N)
+  double* pc_w = nullptr;
+  float* pc_f = nullptr;             // FP32 right-hand side of the triangular solves
+  int32_t* own_idx = nullptr;        // neighbour lists and exact local-inverse blocks of the FP32 form
+  float* own_Y = nullptr;
+  float* own_hk = nullptr;
   const int32_t* spai_idx = nullptr;
   const float* spai_Y = nullptr;
   int32_t mv_prec = GK_F64;          // precision of the Krylov matvec
@@ -216,8 +260,40 @@ struct Solver {
     return GK_OK;
   }
 
-  // K^-1 r (covariance block) through the Cholesky factor of the equilibrated block
+  // FP32 shifted factor applied to a scaled vector: f = (L L^T)^-1 u
+  int factor_apply_f32(const double* u, double* y, bool accumulate) {
+    hipLaunchKernelGGL(to_float_kernel, dim3(nblk(Ncpad)), dim3(256), 0, st, u, pc_f, Nc, Ncpad);
+    GK_TRY(gk_trsv_chol_impl<float>(h, static_cast<float*>(H), static_cast<float*>(dinv), Ncpad, pc_f, st));
+    hipLaunchKernelGGL(add_float_kernel, dim3(nblk(Nc)), dim3(256), 0, st, pc_f, y, Nc, accumulate ? 1 : 0);
+    return GK_OK;
+  }
+
+  // v = u - H y on the equilibrated covariance block (H = D A_cc D), matrix-free
+  int scaled_residual(const double* u, const double* y, double* v) {
+    hipLaunchKernelGGL(unscale_pad_kernel, dim3(nblk(N)), dim3(256), 0, st, y, dscale, pc_x, Nc, N);
+    GK_TRY(gk_matvec(h, sys, GK_F64, pc_x, pc_w, st));
+    hipLaunchKernelGGL(scaled_residual_kernel, dim3(nblk(Nc)), dim3(256), 0, st, u, dscale, pc_w, v, Nc);
+    return GK_OK;
+  }
+
+  // K^-1 r (covariance block).  Forms: the Cholesky factor of the equilibrated
+  // block (FP32 shifted or FP64); the sparse approximate inverse M alone; or
+  // the multiplicative combination P, M, P with P the FP32 shifted factor:
+  //   y = P u;  y += M (u - H y);  y += P (u - H y)
+  // P removes the smooth (large-eigenvalue) error, M the local modes that the
+  // shift leaves (eigenvalues below the shift).
   int apply_K(const double* rin, double* yout) {
+    if (pcp && !f64) {
+      hipLaunchKernelGGL(to_scaled_kernel<double>, dim3(nblk(Nc)), dim3(256), 0, st, rin, dscale, pc_u, Nc, Nc);
+      GK_TRY(factor_apply_f32(pc_u, pc_y, false));
+      GK_TRY(scaled_residual(pc_u, pc_y, pc_v));
+      GK_TRY(gk_spai_apply_impl(sys, spai_idx, spai_Y, pc_v, ytmp2, st));
+      hipLaunchKernelGGL(axpy_kernel, dim3(nblk(Nc)), dim3(256), 0, st, ytmp2, pc_y, Nc);
+      GK_TRY(scaled_residual(pc_u, pc_y, pc_v));
+      GK_TRY(factor_apply_f32(pc_v, pc_y, true));
+      hipLaunchKernelGGL(from_scaled_kernel<double>, dim3(nblk(Nc)), dim3(256), 0, st, pc_y, dscale, yout, Nc);
+      return GK_OK;
+    }
     if (spai) {
       double* u = static_cast<double*>(uvec);
       hipLaunchKernelGGL(to_scaled_kernel<double>, dim3(nblk(Nc)), dim3(256), 0, st, rin, dscale, u, Nc, Nc);
@@ -283,6 +359,9 @@ struct Solver {
                            Ncpad, Nc, (float)shift);
         rc = gk_cholesky_f32(h, static_cast<float*>(H), Nc, Ncpad, static_cast<float*>(dinv), &info, st);
       }
+      if (gk_verbose())
+        fprintf(stderr, "[gk] %s Cholesky of the covariance block, shift %.3e: %s (info %d)\n", f64 ? "FP64" : "FP32",
+                shift, rc == GK_OK ? "ok" : "breakdown", info);
       if (rc == GK_OK) break;
       if (rc != GK_ERR_NOT_SPD) return rc;
       shift = f64 ? (shift == 0.0 ? 1e-13 : shift * 100.0) : shift * 4.0;
@@ -431,6 +510,9 @@ struct Solver {
                                              : hist.size() >= 3 && rel > 0.5 * hist[hist.size() - 3];
           last_true = rel;
           hist.push_back(rel);
+          if (gk_verbose())
+            fprintf(stderr, "[gk] gmres it %d: true relres %.3e%s%s\n", total, rel, converged ? " (converged)" : "",
+                    stalled ? " (stalled)" : "");
           if (converged || stalled || total >= max_iter || hn == 0.0) {
             GK_HIP(hipMemcpyAsync(x, xt, N * sizeof(double), hipMemcpyDeviceToDevice, st));
             *ok = rel <= accept;
@@ -550,6 +632,16 @@ static int init_solver(gk_handle h, const gk_system* sys, hipStream_t st, int fo
   const size_t o_h = take(256 * sizeof(double));
   const size_t o_b = take(Npad * sizeof(double));
   const size_t o_xt = take(Npad * sizeof(double));
+  const bool pc = form == GK_SOLVE_FP32_REFINE || form == 3;
+  const size_t o_pu = take(pc ? S.Ncpad * sizeof(double) : 0);
+  const size_t o_py = take(pc ? S.Ncpad * sizeof(double) : 0);
+  const size_t o_pv = take(pc ? S.Ncpad * sizeof(double) : 0);
+  const size_t o_px = take(pc ? Npad * sizeof(double) : 0);
+  const size_t o_pw = take(pc ? Npad * sizeof(double) : 0);
+  const size_t o_pf = take(pc ? S.Ncpad * sizeof(float) : 0);
+  const size_t o_si = take(pc ? (size_t)(sys->n_ori + sys->n_rest) * GK_SPAI_K * sizeof(int32_t) : 0);
+  const size_t o_sy = take(pc ? (size_t)(sys->n_ori + sys->n_rest) * GK_SPAI_K * 9 * sizeof(float) : 0);
+  const size_t o_hk = take(pc ? (size_t)(sys->n_ori + sys->n_rest) * sizeof(float) : 0);
   void* p = nullptr;
   GK_TRY(gk_workspace(h, WS_SOLVER, need, &p));
   char* base = static_cast<char*>(p);
@@ -570,6 +662,17 @@ static int init_solver(gk_handle h, const gk_system* sys, hipStream_t st, int fo
   S.hdev = reinterpret_cast<double*>(base + o_h);
   S.b = reinterpret_cast<double*>(base + o_b);
   S.xt = reinterpret_cast<double*>(base + o_xt);
+  if (pc) {
+    S.pc_u = reinterpret_cast<double*>(base + o_pu);
+    S.pc_y = reinterpret_cast<double*>(base + o_py);
+    S.pc_v = reinterpret_cast<double*>(base + o_pv);
+    S.pc_x = reinterpret_cast<double*>(base + o_px);
+    S.pc_w = reinterpret_cast<double*>(base + o_pw);
+    S.pc_f = reinterpret_cast<float*>(base + o_pf);
+    S.own_idx = reinterpret_cast<int32_t*>(base + o_si);
+    S.own_Y = reinterpret_cast<float*>(base + o_sy);
+    S.own_hk = reinterpret_cast<float*>(base + o_hk);
+  }
   GK_TRY(gk_rhs_impl(sys, S.b, st));
   return GK_OK;
 }
@@ -630,8 +733,14 @@ extern "C" int gk_solve(gk_handle h, const gk_system* sys, int32_t form, double 
     keep_state();
     return GK_OK;
   }
-  // FP32 factor + FP64 GMRES refinement
+  // FP32 shifted factor + exact local-inverse correction, FP64 GMRES refinement
   GK_HIP(hipEventRecord(ev[0], st));
+  GK_TRY(gk_diag_scale_impl(sys, S.dscale, st));
+  GK_TRY(gk_knn(h, sys, S.own_idx, S.own_hk, (gk_stream)st));
+  GK_TRY(gk_spai_local_inverse(h, sys, S.own_idx, S.dscale, S.own_Y, (gk_stream)st));
+  S.spai_idx = S.own_idx;
+  S.spai_Y = S.own_Y;
+  S.pcp = true;
   int32_t finfo = 0;
   int rc = S.setup_preconditioner(false, &finfo);
   info->info = finfo;
@@ -869,9 +978,10 @@ extern "C" int gk_adjoint(gk_handle h, const gk_system* sys, const double* w, in
 }
 
 extern "C" int gk_solve_spai(gk_handle h, const gk_system* sys, const int32_t* idx, const float* Y, double tol,
-                             int32_t max_iter, int32_t krylov_precision, double* x, gk_solve_info* info,
-                             gk_stream stream) {
-  GK_REQUIRE(h && sys && idx && Y && x && info, GK_ERR_ARG, "gk_solve_spai: NULL argument");
+                             int32_t max_iter, int32_t krylov_precision, int32_t with_factor, double* x,
+                             gk_solve_info* info, gk_stream stream) {
+  GK_REQUIRE(h && sys && x && info, GK_ERR_ARG, "gk_solve_spai: NULL argument");
+  GK_REQUIRE(with_factor || (idx && Y), GK_ERR_ARG, "gk_solve_spai: blocks required without the factor");
   GK_REQUIRE(krylov_precision == GK_F32 || krylov_precision == GK_F64, GK_ERR_ARG, "gk_solve_spai: bad precision");
   hipStream_t st = (hipStream_t)stream;
   *info = gk_solve_info{};
@@ -881,7 +991,6 @@ extern "C" int gk_solve_spai(gk_handle h, const gk_system* sys, const int32_t* i
   h->solver_state = nullptr;
   Solver S;
   GK_TRY(init_solver(h, sys, st, 3, S));
-  S.spai = true;
   S.spai_idx = idx;
   S.spai_Y = Y;
   S.mv_prec = krylov_precision;
@@ -890,18 +999,32 @@ extern "C" int gk_solve_spai(gk_handle h, const gk_system* sys, const int32_t* i
   struct EvGuard { hipEvent_t* e; ~EvGuard() { for (int i = 0; i < 3; ++i) (void)hipEventDestroy(e[i]); } } guard{ev};
   GK_HIP(hipEventRecord(ev[0], st));
   GK_TRY(gk_diag_scale_impl(sys, S.dscale, st));
-  GK_TRY(S.setup_drift());
+  bool have_factor = false;
+  if (with_factor) {              // shifted FP32 factor, with M as its correction when given
+    S.pcp = idx && Y;
+    int32_t finfo = 0;
+    const int rc = S.setup_preconditioner(false, &finfo);
+    if (rc != GK_OK && rc != GK_ERR_NOT_SPD) return rc;
+    have_factor = rc == GK_OK;
+    info->shift = S.shift;
+    info->info = finfo;
+  } else {
+    S.spai = true;
+    GK_TRY(S.setup_drift());
+  }
   GK_HIP(hipEventRecord(ev[1], st));
   bool ok = false;
-  GK_TRY(S.fgmres(x, tol, 1e-6, max_iter, &info->iterations, &info->relres, &ok));
+  if (!with_factor || have_factor)
+    GK_TRY(S.fgmres(x, tol, 1e-6, max_iter, &info->iterations, &info->relres, &ok));
   GK_HIP(hipEventRecord(ev[2], st));
   GK_HIP(hipEventSynchronize(ev[2]));
   info->t_factor_ms = elapsed_ms(ev[0], ev[1]);
   info->t_refine_ms = elapsed_ms(ev[1], ev[2]);
-  info->form = 3;
+  info->form = with_factor ? GK_SOLVE_FP32_REFINE : 3;
   if (!ok) {                      // stagnation: the FP64 direct solve takes over
     info->fallback = 1;
     S.spai = false;
+    S.pcp = false;
     S.mv_prec = GK_F64;
     int32_t finfo = 0;
     GK_TRY(S.setup_preconditioner(true, &finfo));

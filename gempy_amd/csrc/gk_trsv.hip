@@ -8,8 +8,12 @@
 //     y_k = Dinv_k (b_k - sum_{j<k} L_kj y_j).
 // The sum is split over workgroups: a "finisher" for block row k applies the
 // block next to the diagonal (j = k-1), adds the published partial sums of the
-// "far" chunks (j in groups of CHUNK blocks, further from the diagonal), applies
-// Dinv_k and publishes y_k.  The backward solve mirrors it on block columns.
+// "far" chunks, applies Dinv_k and publishes y_k.  Far chunks grow with their
+// distance d = k - j from the diagonal (d in [2,3), [3,5), [5,9), then 8 blocks
+// each), so a chunk's work fits in the time the diagonal chain needs to reach
+// row k after the chunk's vector blocks are ready.  The backward solve mirrors
+// it on block columns.  A workgroup is 16 waves: each wave streams 16 rows of a
+// 256 x 256 block, so a whole block is in flight at once.
 //
 // Ordering and progress: every workgroup takes a ticket (atomic counter) when it
 // starts; the ticket, not blockIdx, selects its work from a host-built table in
@@ -27,8 +31,16 @@
 namespace {
 
 constexpr int NB = GK_BLOCK;            // 256
-constexpr int CHUNK = 4;                // blocks per far chunk
+constexpr int NT = 1024;                // threads per workgroup (16 waves x 16 rows)
 constexpr unsigned SPIN_LIMIT = 1u << 21;
+
+// distance range [lo, hi) of far chunk s
+inline void chunk_range(int s, int* lo, int* hi) {
+  if (s == 0) { *lo = 2; *hi = 3; }
+  else if (s == 1) { *lo = 3; *hi = 5; }
+  else if (s == 2) { *lo = 5; *hi = 9; }
+  else { *lo = 9 + 8 * (s - 3); *hi = 17 + 8 * (s - 3); }
+}
 
 template <typename T> struct V4;
 template <> struct V4<float> { typedef float type __attribute__((ext_vector_type(4))); };
@@ -37,10 +49,14 @@ template <> struct V4<double> { typedef double type __attribute__((ext_vector_ty
 struct Item { int k, s, jlo, jhi; };    // s < 0: finisher of block row k
 
 // Relaxed polling (an acquire load would invalidate the caches on every
-// poll); the caller issues one acquire fence after the workgroup joins.
+// poll), then one acquire fence by the polling thread once the flag is seen;
+// a workgroup barrier carries the ordering to the other waves.
 __device__ __forceinline__ bool wait_epoch(const unsigned* flag, unsigned epoch, int* err) {
   for (unsigned it = 0; it < SPIN_LIMIT; ++it) {
-    if (__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == epoch) return true;
+    if (__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == epoch) {
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+      return true;
+    }
     if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return false;
     __builtin_amdgcn_s_sleep(2);
   }
@@ -52,7 +68,6 @@ __device__ __forceinline__ bool wait_epoch(const unsigned* flag, unsigned epoch,
 __device__ __forceinline__ bool group_wait(const unsigned* flag, unsigned epoch, int* err, int* s_ok) {
   if (threadIdx.x == 0) *s_ok = wait_epoch(flag, epoch, err) ? 1 : 0;
   __syncthreads();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
   return *s_ok != 0;
 }
 
@@ -63,81 +78,76 @@ __device__ __forceinline__ bool group_wait_all(const unsigned* flags, int n, uns
   for (int i = threadIdx.x; i < n; i += blockDim.x)
     if (!wait_epoch(flags + i, epoch, err)) *s_ok = 0;
   __syncthreads();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
   return *s_ok != 0;
 }
 
+// The barrier completes every wave's stores at workgroup scope; the release
+// store of thread 0 then makes them visible device-wide before the flag.
 __device__ __forceinline__ void group_publish(unsigned* flag, unsigned epoch) {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
   __syncthreads();
   if (threadIdx.x == 0) __hip_atomic_store(flag, epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Row dots of one wave: lane l returns sum_c M[r0 + 64 w + l, c0 + c] v[c] over
-// nblk consecutive 256-column blocks (v + 256 jj for block jj).  Rows go in
-// groups of 16: lane l loads columns 4l..4l+3 of each row (one coalesced row
-// per load), then a transpose-reduce leaves row 16 g + (l & 15) in lane l.
+// Row dots of one wave over one 256 x 256 block: lane l < 16 returns
+// sum_c M[r0 + l, c0 + c] v[c].  Lane l loads columns 4l..4l+3 of each of the
+// 16 rows (one coalesced row per load, all 16 in flight), then a
+// transpose-reduce leaves row l & 15 in lane l.
 template <typename T, bool MASK_UPPER>
-__device__ __forceinline__ T rows64(const T* M, int64_t ld, int64_t r0, int64_t c0, const T* v, int nblk, int w,
-                                    int l) {
+__device__ __forceinline__ T rows16(const T* M, int64_t ld, int64_t r0, int64_t c0, const T* v, int rloc0, int l) {
   typedef typename V4<T>::type v4;
-  T mine = T(0);
+  const T* base = M + r0 * ld + c0 + 4 * l;
+  v4 rowv[16];
 #pragma unroll
-  for (int g = 0; g < 4; ++g) {
-    T acc[16];
+  for (int r = 0; r < 16; ++r) rowv[r] = __builtin_nontemporal_load(reinterpret_cast<const v4*>(base + r * ld));
+  const v4 x = *reinterpret_cast<const v4*>(v + 4 * l);
+  T acc[16];
 #pragma unroll
-    for (int r = 0; r < 16; ++r) acc[r] = T(0);
-    for (int jj = 0; jj < nblk; ++jj) {
-      const v4 x = *reinterpret_cast<const v4*>(v + (int64_t)jj * NB + 4 * l);
-      const T* base = M + (r0 + 64 * w + 16 * g) * ld + c0 + (int64_t)jj * NB + 4 * l;
-#pragma unroll
-      for (int r = 0; r < 16; ++r) {
-        v4 a = *reinterpret_cast<const v4*>(base + r * ld);
-        if (MASK_UPPER) {       // keep c <= row within the diagonal block
-          const int row = 64 * w + 16 * g + r, c = 4 * l;
-          a.x = c <= row ? a.x : T(0);
-          a.y = c + 1 <= row ? a.y : T(0);
-          a.z = c + 2 <= row ? a.z : T(0);
-          a.w = c + 3 <= row ? a.w : T(0);
-        }
-        T s = acc[r];
-        s = fma(a.x, x.x, s);
-        s = fma(a.y, x.y, s);
-        s = fma(a.z, x.z, s);
-        s = fma(a.w, x.w, s);
-        acc[r] = s;
-      }
+  for (int r = 0; r < 16; ++r) {
+    v4 a = rowv[r];
+    if (MASK_UPPER) {           // keep c <= row within the diagonal block
+      const int row = rloc0 + r, c = 4 * l;
+      a.x = c <= row ? a.x : T(0);
+      a.y = c + 1 <= row ? a.y : T(0);
+      a.z = c + 2 <= row ? a.z : T(0);
+      a.w = c + 3 <= row ? a.w : T(0);
     }
-#pragma unroll
-    for (int width = 8; width >= 1; width >>= 1) {
-      const bool upper = (l & width) != 0;
-#pragma unroll
-      for (int i = 0; i < width; ++i) {
-        const T send = upper ? acc[i] : acc[i + width];
-        const T keep = upper ? acc[i + width] : acc[i];
-        acc[i] = keep + __shfl_xor(send, width);
-      }
-    }
-    T t = acc[0];
-    t += __shfl_xor(t, 16);
-    t += __shfl_xor(t, 32);
-    if ((l >> 4) == g) mine = t;
+    T s = a.x * x.x;
+    s = fma(a.y, x.y, s);
+    s = fma(a.z, x.z, s);
+    s = fma(a.w, x.w, s);
+    acc[r] = s;
   }
-  return mine;
+#pragma unroll
+  for (int width = 8; width >= 1; width >>= 1) {
+    const bool upper = (l & width) != 0;
+#pragma unroll
+    for (int i = 0; i < width; ++i) {
+      const T send = upper ? acc[i] : acc[i + width];
+      const T keep = upper ? acc[i + width] : acc[i];
+      acc[i] = keep + __shfl_xor(send, width);
+    }
+  }
+  T t = acc[0];
+  t += __shfl_xor(t, 16);
+  t += __shfl_xor(t, 32);
+  return t;
 }
 
-// acc4 += sum_{r in wave rows} M[r0 + 64 w + r, c0 + 4l..4l+3] v[64 w + r]
+// acc4 += sum_{r < 16} M[r0 + 16 w + r, c0 + 4l..4l+3] v[16 w + r]
 template <typename T, bool MASK_LOWER>
 __device__ __forceinline__ void coldot(const T* M, int64_t ld, int64_t r0, int64_t c0, const T* v,
                                        typename V4<T>::type& acc, int w, int l) {
   typedef typename V4<T>::type v4;
-  const T* base = M + (r0 + 64 * w) * ld + c0 + 4 * l;
-#pragma unroll 16
-  for (int r = 0; r < 64; ++r) {
-    v4 a = *reinterpret_cast<const v4*>(base + r * ld);
-    const T xr = v[64 * w + r];
+  const T* base = M + (r0 + 16 * w) * ld + c0 + 4 * l;
+  v4 rowv[16];                  // all 16 row loads in flight before the first use
+#pragma unroll
+  for (int r = 0; r < 16; ++r) rowv[r] = __builtin_nontemporal_load(reinterpret_cast<const v4*>(base + r * ld));
+#pragma unroll
+  for (int r = 0; r < 16; ++r) {
+    v4 a = rowv[r];
+    const T xr = v[16 * w + r];
     if (MASK_LOWER) {           // keep row >= c within the diagonal block
-      const int row = 64 * w + r, c = 4 * l;
+      const int row = 16 * w + r, c = 4 * l;
       a.x = row >= c ? a.x : T(0);
       a.y = row >= c + 1 ? a.y : T(0);
       a.z = row >= c + 2 ? a.z : T(0);
@@ -163,11 +173,12 @@ struct TrsvCtl {
 };
 
 template <typename T>
-__global__ __launch_bounds__(256) void trsv_fwd_kernel(const T* __restrict__ L, int64_t lda, const T* __restrict__ dinv,
-                                                       T* b, T* part, TrsvCtl ctl) {
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4, 4))) void trsv_fwd_kernel(const T* __restrict__ L, int64_t lda, const T* __restrict__ dinv,
+                                                      T* b, T* part, TrsvCtl ctl) {
   __shared__ int s_id, s_ok, s_ok2;
   __shared__ __attribute__((aligned(32))) T tv[NB];
   const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  const int row = 16 * w + (l & 15);              // row of the 256-block owned by lanes l < 16
   if (threadIdx.x == 0)
     s_id = (int)(atomicAdd(ctl.ticket, 1ull) - ctl.ticket_base);
   __syncthreads();
@@ -180,9 +191,10 @@ __global__ __launch_bounds__(256) void trsv_fwd_kernel(const T* __restrict__ L, 
   if (it.s >= 0) {                                            // far chunk
     const int slot = ctl.rowbase[it.k] + it.s;
     if (group_wait(ctl.yflag + it.jhi - 1, ctl.epoch, ctl.err, &s_ok)) {
-      const T sum = rows64<T, false>(L, lda, k0, (int64_t)it.jlo * NB, b + (int64_t)it.jlo * NB, it.jhi - it.jlo,
-                                     w, l);
-      part[(int64_t)slot * NB + 64 * w + l] = sum;
+      T sum = T(0);
+      for (int j = it.jlo; j < it.jhi; ++j)
+        sum += rows16<T, false>(L, lda, k0 + 16 * w, (int64_t)j * NB, b + (int64_t)j * NB, 0, l);
+      if (l < 16) part[(int64_t)slot * NB + row] = sum;
     }
     group_publish(ctl.pflag + slot, ctl.epoch);
     return;
@@ -192,30 +204,31 @@ __global__ __launch_bounds__(256) void trsv_fwd_kernel(const T* __restrict__ L, 
   T t = T(0);
   if (it.k > 0) {
     ok = group_wait(ctl.yflag + it.k - 1, ctl.epoch, ctl.err, &s_ok);
-    if (ok) t = rows64<T, false>(L, lda, k0, k0 - NB, b + k0 - NB, 1, w, l);
+    if (ok) t = rows16<T, false>(L, lda, k0 + 16 * w, k0 - NB, b + k0 - NB, 0, l);
   }
   if (ok) {
-    t = b[k0 + 64 * w + l] - t;
+    t = b[k0 + row] - t;
     const int f0 = ctl.rowbase[it.k], nfar = ctl.rowbase[it.k + 1] - f0;
     ok = group_wait_all(ctl.pflag + f0, nfar, ctl.epoch, ctl.err, &s_ok2);
     if (ok)
-      for (int s = 0; s < nfar; ++s) t -= part[(int64_t)(f0 + s) * NB + 64 * w + l];
+      for (int s = 0; s < nfar; ++s) t -= part[(int64_t)(f0 + s) * NB + row];
   }
   if (ok) {
-    tv[64 * w + l] = t;
+    if (l < 16) tv[row] = t;
     __syncthreads();
-    b[k0 + 64 * w + l] = rows64<T, true>(dinv + k0 * NB, NB, 0, 0, tv, 1, w, l);
+    const T y = rows16<T, true>(dinv + k0 * NB, NB, 16 * w, 0, tv, 16 * w, l);
+    if (l < 16) b[k0 + row] = y;
   }
   group_publish(ctl.yflag + it.k, ctl.epoch);
 }
 
 template <typename T>
-__global__ __launch_bounds__(256) void trsv_bwd_kernel(const T* __restrict__ L, int64_t lda, const T* __restrict__ dinv,
-                                                       T* b, T* part, TrsvCtl ctl) {
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4, 4))) void trsv_bwd_kernel(const T* __restrict__ L, int64_t lda, const T* __restrict__ dinv,
+                                                      T* b, T* part, TrsvCtl ctl) {
   typedef typename V4<T>::type v4;
   __shared__ int s_id, s_ok, s_ok2;
   __shared__ __attribute__((aligned(32))) T tv[NB];
-  __shared__ __attribute__((aligned(32))) T red[4][NB];
+  __shared__ __attribute__((aligned(32))) T red[NT / 64][NB];
   const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
   if (threadIdx.x == 0)
     s_id = (int)(atomicAdd(ctl.ticket, 1ull) - ctl.ticket_base);
@@ -226,21 +239,26 @@ __global__ __launch_bounds__(256) void trsv_bwd_kernel(const T* __restrict__ L, 
   }
   const Item it = ctl.items[s_id];
   const int64_t k0 = (int64_t)it.k * NB;
-  v4 acc = {T(0), T(0), T(0), T(0)};
-  auto reduce_waves = [&](v4 a) -> T {       // column 4l + e of wave w -> thread's column
+  // column c = threadIdx.x (< 256) gets the sum of the 16 waves' partials
+  auto reduce_waves = [&](v4 a) -> T {
     *reinterpret_cast<v4*>(&red[w][4 * l]) = a;
     __syncthreads();
-    const int c = threadIdx.x;
-    const T s = red[0][c] + red[1][c] + red[2][c] + red[3][c];
+    T s = T(0);
+    if (threadIdx.x < NB) {
+#pragma unroll
+      for (int q = 0; q < NT / 64; ++q) s += red[q][threadIdx.x];
+    }
     __syncthreads();
     return s;
   };
+  v4 acc = {T(0), T(0), T(0), T(0)};
   if (it.s >= 0) {                                            // far chunk: rows j in [jlo, jhi)
     const int slot = ctl.rowbase[it.k + 1] + it.s;
     if (group_wait(ctl.yflag + it.jlo, ctl.epoch, ctl.err, &s_ok)) {
       for (int j = it.jlo; j < it.jhi; ++j)
         coldot<T, false>(L, lda, (int64_t)j * NB, k0, b + (int64_t)j * NB, acc, w, l);
-      part[(int64_t)slot * NB + threadIdx.x] = reduce_waves(acc);
+      const T sum = reduce_waves(acc);
+      if (threadIdx.x < NB) part[(int64_t)slot * NB + threadIdx.x] = sum;
     }
     group_publish(ctl.pflag + slot, ctl.epoch);
     return;
@@ -253,50 +271,58 @@ __global__ __launch_bounds__(256) void trsv_bwd_kernel(const T* __restrict__ L, 
   }
   T t = T(0);
   if (ok) {
-    t = b[k0 + threadIdx.x] - reduce_waves(acc);
+    t = reduce_waves(acc);
+    if (threadIdx.x < NB) t = b[k0 + threadIdx.x] - t;
     const int f0 = ctl.rowbase[it.k + 1], nfar = ctl.rowbase[it.k + 2] - f0;
     ok = group_wait_all(ctl.pflag + f0, nfar, ctl.epoch, ctl.err, &s_ok2);
-    if (ok)
+    if (ok && threadIdx.x < NB)
       for (int s = 0; s < nfar; ++s) t -= part[(int64_t)(f0 + s) * NB + threadIdx.x];
   }
   if (ok) {
-    tv[threadIdx.x] = t;
+    if (threadIdx.x < NB) tv[threadIdx.x] = t;
     __syncthreads();
     v4 a2 = {T(0), T(0), T(0), T(0)};
     coldot<T, true>(dinv + k0 * NB, NB, 0, 0, tv, a2, w, l);
-    b[k0 + threadIdx.x] = reduce_waves(a2);
+    const T x = reduce_waves(a2);
+    if (threadIdx.x < NB) b[k0 + threadIdx.x] = x;
   }
   group_publish(ctl.yflag + it.k, ctl.epoch);
 }
 
-// Host-side work tables.  Forward: finisher(j) then the far chunks whose
-// nearest block is j (rows k = j + 2 + CHUNK s, blocks [k-1-CHUNK(s+1), k-1-CHUNK s)).
+// Host-side work tables.  Forward: finisher(j), then for each distance range s
+// the far chunk of row k = j + lo_s (blocks at distances [lo_s, hi_s) of row
+// k, nearest block j).  Backward mirrors it with k = j - lo_s.
 void build_tables(int nb, std::vector<Item>& fwd, std::vector<int>& fbase, std::vector<Item>& bwd,
                   std::vector<int>& bbase) {
-  auto nfar = [&](int k) { return k >= 2 ? (k - 1 + CHUNK - 1) / CHUNK : 0; };     // forward row k
+  auto nfar = [](int k) {        // far chunks of a row with k blocks on its far side of the diagonal
+    int s = 0, lo, hi;
+    for (;;) { chunk_range(s, &lo, &hi); if (lo > k) return s; ++s; }
+  };
   fbase.assign(nb + 1, 0);
   for (int k = 0; k < nb; ++k) fbase[k + 1] = fbase[k] + nfar(k);
   fwd.clear();
   for (int j = 0; j < nb; ++j) {
     fwd.push_back({j, -1, 0, 0});
-    for (int k = j + 2; k < nb; k += CHUNK) {
-      const int s = (k - j - 2) / CHUNK;
-      const int jhi = k - 1 - CHUNK * s, jlo = std::max(0, jhi - CHUNK);
-      fwd.push_back({k, s, jlo, jhi});
+    for (int s = 0;; ++s) {
+      int lo, hi;
+      chunk_range(s, &lo, &hi);
+      const int k = j + lo;
+      if (k >= nb) break;
+      fwd.push_back({k, s, std::max(0, k - hi + 1), k - lo + 1});
     }
   }
-  // backward row k: far blocks j in [k+2, nb) in chunks [k+2+CHUNK s, ...)
-  auto bfar = [&](int k) { const int m = nb - (k + 2); return m > 0 ? (m + CHUNK - 1) / CHUNK : 0; };
   bbase.assign(nb + 2, 0);
   bbase[0] = nb;
-  for (int k = 0; k < nb; ++k) bbase[k + 2] = bbase[k + 1] + bfar(k);
+  for (int k = 0; k < nb; ++k) bbase[k + 2] = bbase[k + 1] + nfar(nb - 1 - k);
   bwd.clear();
   for (int j = nb - 1; j >= 0; --j) {
     bwd.push_back({j, -1, 0, 0});
-    for (int k = j - 2; k >= 0; k -= CHUNK) {
-      const int s = (j - k - 2) / CHUNK;
-      const int jlo = k + 2 + CHUNK * s, jhi = std::min(nb, jlo + CHUNK);
-      bwd.push_back({k, s, jlo, jhi});
+    for (int s = 0;; ++s) {
+      int lo, hi;
+      chunk_range(s, &lo, &hi);
+      const int k = j - lo;
+      if (k < 0) break;
+      bwd.push_back({k, s, k + lo, std::min(nb, k + hi)});
     }
   }
 }
@@ -377,7 +403,7 @@ int gk_trsv_chol_impl(gk_handle h, const T* L, const T* dinv, int64_t lda, T* b,
   c.ticket_base = h->trsv_tickets;
   c.epoch = ++h->trsv_epoch;
   c.n_items = lay.nfwd;
-  hipLaunchKernelGGL(trsv_fwd_kernel<T>, dim3((unsigned)lay.nfwd), dim3(256), 0, st, L, lda, dinv, b, part, c);
+  hipLaunchKernelGGL(trsv_fwd_kernel<T>, dim3((unsigned)lay.nfwd), dim3(NT), 0, st, L, lda, dinv, b, part, c);
   h->trsv_tickets += (unsigned long long)lay.nfwd;
   // backward
   c.items = reinterpret_cast<const Item*>(base + lay.o_bi);
@@ -385,7 +411,7 @@ int gk_trsv_chol_impl(gk_handle h, const T* L, const T* dinv, int64_t lda, T* b,
   c.ticket_base = h->trsv_tickets;
   c.epoch = ++h->trsv_epoch;
   c.n_items = lay.nbwd;
-  hipLaunchKernelGGL(trsv_bwd_kernel<T>, dim3((unsigned)lay.nbwd), dim3(256), 0, st, L, lda, dinv, b, part, c);
+  hipLaunchKernelGGL(trsv_bwd_kernel<T>, dim3((unsigned)lay.nbwd), dim3(NT), 0, st, L, lda, dinv, b, part, c);
   h->trsv_tickets += (unsigned long long)lay.nbwd;
   const hipError_t e = hipGetLastError();
   if (e != hipSuccess) {

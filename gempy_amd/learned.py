@@ -8,7 +8,10 @@ M_PQ of an approximate inverse of the equilibrated covariance block H.  It is
 trained on systems generated from seeded models (``synthetic.training_system``,
 seeds 1000-2999) to minimise the local SPAI residual ||M_P H_loc - E_P||^2.
 Training and inference run in libgkhip.so (knn, features, local blocks, MLP
-forward/backward, loss, Adam); the network is used by ``gk_solve_spai``.
+forward/backward, loss, Adam); the network is used by ``gk_solve_spai``, alone
+(matrix-free GMRES) or as the local correction of the shifted FP32 Cholesky
+factor (factor, M, factor), where it removes the local error modes that the
+shift leaves.
 """
 from __future__ import annotations
 
@@ -180,15 +183,30 @@ def train(net: SpaiNetwork, steps: int, seeds=range(1000, 3000), size: Callable[
     return hist
 
 
-def solve_spai(ds: DeviceStack, idx, y, tol: float = 1e-8, max_iter: int = 500,
-               krylov_precision: int = _lib.GK_F32):
+def local_inverse_blocks(ds: DeviceStack):
+    """The exact local sparse approximate inverse (what the network is trained
+    towards): rows of each neighbourhood's inverted equilibrated block."""
+    lib = _lib.load()
+    idx, _ = neighbourhoods(ds)
+    y = torch.empty((idx.shape[0] * K, 9), dtype=torch.float32, device="cuda")
+    d = diag_scale(ds)
+    _lib.check("gk_spai_local_inverse", lib.gk_spai_local_inverse(_handle(), C.byref(ds.sys), _ptr(idx), _ptr(d),
+                                                                  _ptr(y), _stream()))
+    return idx, y
+
+
+def solve_spai(ds: DeviceStack, idx=None, y=None, tol: float = 1e-8, max_iter: int = 500,
+               krylov_precision: int = _lib.GK_F32, with_factor: bool = False):
+    """GMRES preconditioned by the blocks (idx, y) alone (with_factor False),
+    or by the shifted FP32 factor combined with them (with_factor True; the
+    factor alone when idx/y are None)."""
     lib = _lib.load()
     x = torch.empty(ds.N, dtype=torch.float64, device="cuda")
     info = _lib.GkSolveInfo()
     _lib.check("gk_solve_spai", lib.gk_solve_spai(ds.h, C.byref(ds.sys), _ptr(idx), _ptr(y), float(tol),
-                                                  int(max_iter), int(krylov_precision), _ptr(x), C.byref(info),
-                                                  _stream()))
+                                                  int(max_iter), int(krylov_precision), int(bool(with_factor)),
+                                                  _ptr(x), C.byref(info), _stream()))
     DeviceStack.solve_epoch += 1
     return x, {"form": int(info.form), "iterations": int(info.iterations), "fallback": int(info.fallback),
-               "relres": float(info.relres), "t_setup_ms": float(info.t_factor_ms),
+               "relres": float(info.relres), "shift": float(info.shift), "t_setup_ms": float(info.t_factor_ms),
                "t_krylov_ms": float(info.t_refine_ms)}

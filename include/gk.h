@@ -168,9 +168,12 @@ int gk_gemm_f64(gk_handle h, int64_t m, int64_t n, int64_t k, const double* A, i
  *                        dense direct solve, numpy.linalg.solve class).
  *  GK_SOLVE_FP64_CHOL  : FP64 equilibrated Cholesky of the SPD covariance block +
  *                        Schur complement on the drift block + FP64 refinement steps.
- *  GK_SOLVE_FP32_REFINE: FP32 equilibrated Cholesky used as the preconditioner of
- *                        FP64 right-preconditioned GMRES on the matrix-free FP64
- *                        operator; if the FP32 factor breaks down or the iteration
+ *  GK_SOLVE_FP32_REFINE: FP32 Cholesky of the equilibrated covariance block with a
+ *                        diagonal shift (4 u32 sqrt(n), x4 per breakdown), combined
+ *                        with the exact local sparse approximate inverse (factor,
+ *                        local inverse, factor) as the preconditioner of FP64
+ *                        right-preconditioned flexible GMRES on the matrix-free FP64
+ *                        operator; if the factor breaks down 4 times or the iteration
  *                        stagnates above 1e-6, falls back to GK_SOLVE_FP64_CHOL. */
 int gk_solve(gk_handle h, const gk_system* sys, int32_t form, double tol, int32_t max_iter,
              double* x, gk_solve_info* info, gk_stream stream);
@@ -243,11 +246,17 @@ int gk_octree_refine(gk_handle h, int32_t n_fields, const int32_t* ids, int64_t 
  *   grad receives d(sum dy.y)/dparams.
  * gk_spai_loss: per point ||M_P Hloc - E_P||^2 and dY = dloss/dY.
  * gk_adam: one Adam step (beta 0.9/0.999) with grad scaled by grad_scale.
+ * gk_spai_local_inverse: the exact local sparse approximate inverse, Y[p][j] =
+ *   the rows of H_loc(p)^-1 belonging to p's unknowns (FP64 inversion), in the
+ *   layout the network predicts (what the network is trained towards).
  * gk_spai_apply: y = M u on the covariance unknowns (FP64 vectors, FP32 blocks).
- * gk_solve_spai: right-preconditioned restarted GMRES on the matrix-free
- *   operator (Arnoldi matvec in `krylov_precision`, true residual in FP64)
- *   with M (equilibrated) on the covariance block and the exact Schur
- *   complement of the drift; falls back to GK_SOLVE_FP64_CHOL on stagnation. */
+ * gk_solve_spai: right-preconditioned restarted flexible GMRES on the
+ *   matrix-free operator (Arnoldi matvec in `krylov_precision`, true residual
+ *   in FP64) and the exact Schur complement of the drift.  with_factor = 0:
+ *   the covariance block is preconditioned by M alone (idx, Y required);
+ *   with_factor = 1: by the shifted FP32 Cholesky factor P, and, when idx and
+ *   Y are given, by the combination P, M, P (M corrects the local modes the
+ *   shift leaves).  Falls back to GK_SOLVE_FP64_CHOL on stagnation. */
 int gk_spai_nparams(void);
 int gk_knn(gk_handle h, const gk_system* sys, int32_t* idx, float* hk, gk_stream stream);
 int gk_spai_features(gk_handle h, const gk_system* sys, int64_t nq, const int32_t* pts, const int32_t* idx,
@@ -265,8 +274,11 @@ int gk_adam(gk_handle h, float* params, const float* grad, float* m, float* v, i
 int gk_spai_apply(gk_handle h, const gk_system* sys, const int32_t* idx, const float* Y, const double* u, double* y,
                   gk_stream stream);
 int gk_diag_scale(gk_handle h, const gk_system* sys, double* dscale, gk_stream stream);
+int gk_spai_local_inverse(gk_handle h, const gk_system* sys, const int32_t* idx, const double* dscale, float* Y,
+                          gk_stream stream);
 int gk_solve_spai(gk_handle h, const gk_system* sys, const int32_t* idx, const float* Y, double tol,
-                  int32_t max_iter, int32_t krylov_precision, double* x, gk_solve_info* info, gk_stream stream);
+                  int32_t max_iter, int32_t krylov_precision, int32_t with_factor, double* x, gk_solve_info* info,
+                  gk_stream stream);
 
 #ifdef __cplusplus
 }

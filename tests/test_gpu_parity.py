@@ -498,8 +498,41 @@ def test_spai_local_blocks_gradient_match_oracle(gpu):
     assert err <= 2e-3
 
 
+def test_spai_local_inverse_matches_numpy(gpu):
+    """Exact local sparse approximate inverse rows vs numpy on the oracle matrix."""
+    say("spai: exact local inverse blocks vs numpy")
+    from gempy_amd import learned
+    ds, m = _spai_system(1001, 900)
+    st, k = m.stacks[0], m.options.kernel_options
+    idx, y = learned.local_inverse_blocks(ds)
+    Ao, bo = ok.build_system(st.sp_xyz, st.counts, st.sp_nugget, st.ori_xyz, st.ori_grad, st.ori_nugget,
+                             k.range, k.c_o)
+    Nc = len(bo) - 3
+    dd = 1 / np.sqrt(np.diag(Ao)[:Nc])
+    Hn = dd[:, None] * Ao[:Nc, :Nc] * dd[None, :]
+    n_o = ds.sys.n_ori
+    idx_h, y_h = idx.cpu().numpy(), y.cpu().numpy().reshape(-1, 16, 3, 3)
+
+    def unk(p, c):
+        return c * n_o + p if p < n_o else (3 * n_o + p - n_o if c == 0 else -1)
+    for p in [0, 1, n_o - 1, n_o, n_o + 7, idx_h.shape[0] - 1]:
+        u = [unk(int(q), c) for q in idx_h[p] for c in range(3)]
+        Hl = np.array([[(1.0 if a_ == c_ else 0.0) if (u[a_] < 0 or u[c_] < 0) else Hn[u[a_], u[c_]]
+                        for c_ in range(48)] for a_ in range(48)])
+        inv = np.linalg.inv(Hl)
+        js = int(np.nonzero(idx_h[p] == p)[0][0])
+        want = inv[3 * js:3 * js + 3].reshape(3, 16, 3).transpose(1, 0, 2)
+        nrows = 3 if p < n_o else 1
+        err = np.abs(y_h[p, :, :nrows] - want[:, :nrows]).max() / np.abs(want[:, :nrows]).max()
+        say(f"  point {p}: max rel err {err:.2e}")
+        assert err <= 1e-5
+
+
 def test_learned_preconditioner_training_and_gain(gpu):
-    say("spai: short training, then GMRES iterations learned vs identity")
+    """The trained network as the local correction of the shifted FP32 factor
+    (factor, M, factor) against the factor alone (the method without it), on
+    held-out systems; FP64 GMRES to 1e-9 in both, no fallback."""
+    say("spai: short training, then GMRES iterations factor+learned vs factor alone")
     from gempy_amd import learned
     L = _lib()
     net = learned.SpaiNetwork(seed=0)
@@ -521,12 +554,18 @@ def test_learned_preconditioner_training_and_gain(gpu):
     say(f"spai: validation loss {before:.4e} -> {after:.4e}; train loss {hist[0]:.3e} -> {hist[-1]:.3e}")
     assert after < before
     its = {}
-    for name, fn in (("identity", learned.identity_blocks), ("learned", lambda ds: learned.spai_blocks(net, ds))):
+    variants = (("factor alone", lambda ds: (None, None)),
+                ("factor + learned", lambda ds: learned.spai_blocks(net, ds)),
+                ("factor + exact local inverse", learned.local_inverse_blocks))
+    for name, fn in variants:
         tot = 0
         for ds in val:
             idx, y = fn(ds)
-            x, info = learned.solve_spai(ds, idx, y, tol=1e-7, max_iter=400, krylov_precision=L.GK_F64)
+            x, info = learned.solve_spai(ds, idx, y, tol=1e-9, max_iter=400, krylov_precision=L.GK_F64,
+                                         with_factor=True)
             say(f"spai: {name} N={ds.N} -> {info}")
-            tot += info["iterations"] if not info["fallback"] else 10 ** 6
+            assert info["fallback"] == 0 and info["relres"] <= 1e-8
+            tot += info["iterations"]
         its[name] = tot
-    assert its["learned"] < its["identity"]
+    say(f"spai: GMRES iterations {its}")
+    assert its["factor + learned"] < its["factor alone"]
