@@ -54,10 +54,12 @@ def main():
         print("bench.py measures one GPU: every box in this setup has one GPU and the co-kriging "
               "path is not sharded here (see DESIGN.md for the multi-GPU design).", file=sys.stderr)
         sys.exit(2)
+    import ctypes as C
     import numpy as np
     import torch
     from gempy_amd import _lib, synthetic
     from gempy_amd.engine import DeviceStack
+    lib = _lib.load()
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     dev = torch.device("cuda", 0)
@@ -87,12 +89,14 @@ def main():
                                eval=ev[2].elapsed_time(ev[3]), info=info))
         return w, thr, Sg, ids
 
-    def run(form, name):
+    def run(form, name, profile=False):
         for i in range(args.warmup):
             log(f"{name}: warmup {i + 1}/{args.warmup}")
             step(form)
         torch.cuda.synchronize()
         rec = []
+        if profile:
+            _lib.check("gk_profile", lib.gk_profile(ds.h, 1))
         t0 = time.perf_counter()
         for i in range(args.steps):
             out = step(form, rec)
@@ -100,9 +104,19 @@ def main():
                 f"info {rec[-1]['info']}")
         torch.cuda.synchronize()
         t = (time.perf_counter() - t0) / args.steps
+        if profile:
+            prof = {}
+            for kind in range(10):
+                n, ms, work = C.c_int64(0), C.c_double(0), C.c_double(0)
+                _lib.check("gk_profile_read", lib.gk_profile_read(ds.h, kind, C.byref(n), C.byref(ms), C.byref(work)))
+                if n.value:
+                    prof[kind] = (n.value, ms.value, work.value)
+            _lib.check("gk_profile", lib.gk_profile(ds.h, 0))
+            rec.append({"profile": prof})
         return t, rec, out
 
-    t_fast, rec_fast, out_fast = run(_lib.GK_SOLVE_FP32_REFINE, "fp32_refine")
+    t_fast, rec_fast, out_fast = run(_lib.GK_SOLVE_FP32_REFINE, "fp32_refine", profile=True)
+    prof_fast = rec_fast.pop()["profile"]
     res = {}
     if not args.no_fp64:
         t_chol, rec_chol, out_chol = run(_lib.GK_SOLVE_FP64_CHOL, "fp64_chol")
@@ -128,12 +142,30 @@ def main():
     solve_ms = float(np.mean([r["solve"] for r in rec_fast]))
     eval_ms = float(np.mean([r["eval"] for r in rec_fast]))
     factor_ms = float(np.mean([r["info"]["t_factor_ms"] for r in rec_fast]))
-    # grid-evaluation kernel roofline: 21 flop per (point, interface/reference
-    # source) pair and 24 per (point, orientation) pair, FMA = 2.
+    # grid-evaluation kernel: 21 flop per (point, interface/reference source)
+    # pair and 24 per (point, orientation) pair, FMA = 2.
     flop_eval = P * (21.0 * (R + S) + 24.0 * no)
     ach_eval = flop_eval / (eval_ms * 1e-3) / 1e12
     peak_eval = PEAK["fp64_valu"] if eprec == _lib.GK_F64 else PEAK["fp32_valu"]
-    chol_flop = Nc ** 3 / 3.0
+    # per-kernel figures of the timed steps (HIP events around each launch)
+    kernels = {}
+    for kind, (n, ms, work) in prof_fast.items():
+        if kind < 8:
+            name = "gemm_kernel<%s, %s, %s>" % ("double" if kind >= 4 else "float", "true" if kind & 2 else "false",
+                                                "true" if kind & 1 else "false")
+            peak = PEAK["fp64_mfma"] if kind >= 4 else PEAK["fp32_mfma"]
+            ach, unit, bound = work / (ms * 1e-3) / 1e12, "TFLOP/s", "mfma"
+        else:
+            name = "trsv_fwd_kernel+trsv_bwd_kernel<%s>" % ("double" if kind == 9 else "float")
+            peak, ach, unit, bound = PEAK["hbm"], work / (ms * 1e-3) / 1e9, "GB/s", "hbm"
+        kernels[name] = {"launches": n, "total_ms": ms, "avg_ms": ms / n, "work_per_launch": work / n,
+                         "achieved": ach, "peak": peak, "unit": unit, "bound": bound, "frac": ach / peak}
+    kernels["eval_kernel<double, true, false, false, 4>" if eprec == _lib.GK_F64 else "eval_kernel (FP32 lattice)"] = {
+        "launches": args.steps, "total_ms": eval_ms * args.steps, "avg_ms": eval_ms, "work_per_launch": flop_eval,
+        "achieved": ach_eval, "peak": peak_eval, "unit": "TFLOP/s", "bound": "valu", "frac": ach_eval / peak_eval}
+    dom = max(kernels, key=lambda k: kernels[k]["total_ms"])
+    d = kernels[dom]
+    traffic = _traffic(dom)
     out = {
         "metric": METRIC, "value": P / t_fast / 1e9, "unit": "Gpts/s (model: assemble+solve+evaluate)",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": t_fast * 1e3,
@@ -145,13 +177,11 @@ def main():
         "model_s": t_fast, "grid_gpts_s": P / (eval_ms * 1e-3) / 1e9, "solve_ms": solve_ms,
         "eval_ms": eval_ms, "factor_ms": factor_ms,
         "gmres_iterations": rec_fast[-1]["info"]["iterations"], "relres": rec_fast[-1]["info"]["relres"],
-        "fallback": rec_fast[-1]["info"]["fallback"],
-        "roofline": {"bound": "valu", "kernel": "eval_kernel (grid, lattice)",
-                     "achieved": ach_eval, "peak": peak_eval, "unit": "TFLOP/s",
-                     "frac": ach_eval / peak_eval, "traffic": None},
-        "roofline_factor": {"bound": "mfma", "kernel": "FP32 Cholesky (potrf+trsm+syrk)",
-                            "achieved": chol_flop / (factor_ms * 1e-3) / 1e12, "peak": PEAK["fp32_mfma"],
-                            "unit": "TFLOP/s", "frac": chol_flop / (factor_ms * 1e-3) / 1e12 / PEAK["fp32_mfma"]},
+        "fallback": rec_fast[-1]["info"]["fallback"], "shift": rec_fast[-1]["info"]["shift"],
+        "roofline": {"bound": d["bound"], "kernel": dom, "achieved": d["achieved"], "peak": d["peak"],
+                     "unit": d["unit"], "frac": d["frac"], "traffic": traffic,
+                     "launches_per_step": d["launches"] / args.steps, "avg_launch_ms": d["avg_ms"]},
+        "kernels": kernels,
     }
     out.update(res)
     if not args.no_extras:
@@ -162,9 +192,32 @@ def main():
         log("adjoint of a 10,000-observation misfit (faulted stack, 128^3 grid)")
         out["adjoint"] = workloads.adjoint_misfit(steps=1, warmup=1)
         log(f"  {out['adjoint']}")
+        from gempy_amd import learned
+        if os.path.exists(learned.WEIGHTS):
+            log("learned preconditioner: config-4 system (N~130k, budgeted GMRES) + 35k judge")
+            out["learned"] = workloads.learned_gmres(eval_grid=False)
+            log(f"  {out['learned']}")
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(model, out_fast[0].cpu().numpy(), args.cpu_seconds, N, P)
     print(json.dumps(out), flush=True)
+
+
+def _traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC
+    summary (FETCH_SIZE + WRITE_SIZE per dispatch, scaled by the copy-kernel
+    calibration), or None when no summary for it is committed.  Format: table
+    rows '| kernel | ... | traffic_bytes_per_launch |'."""
+    path = os.path.join(ROOT, "profiles", "pmc_traffic.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cells) >= 2 and cells[0] == kernel:
+            try:
+                return float(cells[-1])
+            except ValueError:
+                return None
+    return None
 
 
 def cpu_baseline(model, weights, budget_s, N, P):
@@ -174,7 +227,11 @@ def cpu_baseline(model, weights, budget_s, N, P):
     import numpy as np
     from gempy_amd import synthetic
     from oracle import cokriging as ok
-    cores = os.cpu_count() or 1
+    try:
+        from threadpoolctl import threadpool_info
+        cores = max([i.get("num_threads", 1) for i in threadpool_info() if i.get("user_api") == "blas"] or [1])
+    except Exception:
+        cores = 1
     k = model.options.kernel_options
     st = model.stacks[0]
     # solve sample
@@ -200,7 +257,7 @@ def cpu_baseline(model, weights, budget_s, N, P):
             "sample": f"FP64 NumPy oracle: assemble+solve of a N={Ns} system ({t_solve_s:.2f} s, x(N/Ns)^3) "
                       f"+ evaluation of {n_eval} grid points with the full N={N} weights "
                       f"({t_eval_s:.2f} s, xP/{n_eval}); extrapolated model time {t_model:.0f} s; "
-                      f"BLAS threads = all host cores, elementwise NumPy single-threaded"}
+                      f"{cores} BLAS threads for the solve, elementwise NumPy single-threaded"}
 
 
 if __name__ == "__main__":

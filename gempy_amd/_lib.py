@@ -18,6 +18,8 @@ GK_POINTS_EXPLICIT, GK_POINTS_LATTICE = 0, 1
 GK_SOLVE_FP64_LU, GK_SOLVE_FP32_REFINE, GK_SOLVE_FP64_CHOL = 0, 1, 2
 GK_MAX_THRESHOLDS = 64
 GK_BLOCK = 256
+GK_SPAI_WITH_FACTOR, GK_SPAI_NO_FALLBACK = 1, 2
+GK_SPAI_K, GK_SPAI_FEATURES, GK_SPAI_HIDDEN = 16, 8, 32
 
 P = C.c_void_p
 
@@ -91,6 +93,9 @@ SIGNATURES = {
     "gk_spai_local_inverse": (C.c_int, [P, C.POINTER(GkSystem), P, P, P, P]),
     "gk_solve_spai": (C.c_int, [P, C.POINTER(GkSystem), P, P, C.c_double, C.c_int32, C.c_int32, C.c_int32, P,
                                 C.POINTER(GkSolveInfo), P]),
+    "gk_profile": (C.c_int, [P, C.c_int32]),
+    "gk_profile_read": (C.c_int, [P, C.c_int32, C.POINTER(C.c_int64), C.POINTER(C.c_double),
+                                  C.POINTER(C.c_double)]),
     "gk_combine_lith": (C.c_int, [P, C.c_int32, P, C.c_int64, C.POINTER(C.c_int32), P, P]),
     "gk_octree_points": (C.c_int, [P, C.POINTER(C.c_double), C.POINTER(C.c_int64), P, C.c_int64, C.c_int32,
                                    P, P]),

@@ -128,3 +128,63 @@ extern "C" int gk_assemble_drift(gk_handle h, const gk_system* sys, double* U, g
   GK_REQUIRE(h && sys && U, GK_ERR_ARG, "gk_assemble_drift: NULL argument");
   return gk_assemble_drift_impl(sys, U, (hipStream_t)stream);
 }
+
+// ------------------------------------------------------------- profiling
+namespace {
+bool g_prof_on = false;
+std::vector<gk_prof_rec> g_prof_recs;
+std::vector<hipEvent_t> g_prof_pool;
+size_t g_prof_used = 0;
+
+hipEvent_t prof_event() {
+  if (g_prof_used == g_prof_pool.size()) {
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    g_prof_pool.push_back(e);
+  }
+  return g_prof_pool[g_prof_used++];
+}
+}  // namespace
+
+bool gk_prof_begin(hipStream_t st, hipEvent_t* a) {
+  if (!g_prof_on) return false;
+  *a = prof_event();
+  if (!*a || hipEventRecord(*a, st) != hipSuccess) return false;
+  return true;
+}
+
+void gk_prof_end(hipStream_t st, hipEvent_t a, int kind, double work) {
+  hipEvent_t b = prof_event();
+  if (!b || hipEventRecord(b, st) != hipSuccess) return;
+  g_prof_recs.push_back({a, b, kind, work});
+}
+
+extern "C" int gk_profile(gk_handle h, int32_t enable) {
+  GK_REQUIRE(h, GK_ERR_ARG, "gk_profile: NULL handle");
+  GK_HIP(hipDeviceSynchronize());
+  g_prof_on = enable != 0;
+  if (enable) {
+    g_prof_recs.clear();
+    g_prof_used = 0;
+  }
+  return GK_OK;
+}
+
+extern "C" int gk_profile_read(gk_handle h, int32_t kind, int64_t* launches, double* ms, double* work) {
+  GK_REQUIRE(h && launches && ms && work, GK_ERR_ARG, "gk_profile_read: NULL argument");
+  GK_HIP(hipDeviceSynchronize());
+  int64_t n = 0;
+  double t = 0.0, w = 0.0;
+  for (const gk_prof_rec& r : g_prof_recs) {
+    if (r.kind != kind) continue;
+    float e = 0.0f;
+    GK_HIP(hipEventElapsedTime(&e, r.a, r.b));
+    ++n;
+    t += e;
+    w += r.work;
+  }
+  *launches = n;
+  *ms = t;
+  *work = w;
+  return GK_OK;
+}

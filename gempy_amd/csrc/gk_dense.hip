@@ -214,6 +214,8 @@ int gemm_launch(int64_t m, int64_t n, int64_t k, const T* A, int64_t lda, const 
     return GK_OK;
   }
   dim3 grid((unsigned)tiles), block(256);
+  hipEvent_t pa;
+  const bool prof = gk_prof_begin(st, &pa);
   if (trans_b) {
     if (lower) hipLaunchKernelGGL((gemm_kernel<T, true, true>), grid, block, 0, st, m, n, k, A, lda, B, ldb, alpha, beta, C, ldc, tn, tiles);
     else hipLaunchKernelGGL((gemm_kernel<T, true, false>), grid, block, 0, st, m, n, k, A, lda, B, ldb, alpha, beta, C, ldc, tn, tiles);
@@ -221,6 +223,9 @@ int gemm_launch(int64_t m, int64_t n, int64_t k, const T* A, int64_t lda, const 
     if (lower) hipLaunchKernelGGL((gemm_kernel<T, false, true>), grid, block, 0, st, m, n, k, A, lda, B, ldb, alpha, beta, C, ldc, tn, tiles);
     else hipLaunchKernelGGL((gemm_kernel<T, false, false>), grid, block, 0, st, m, n, k, A, lda, B, ldb, alpha, beta, C, ldc, tn, tiles);
   }
+  if (prof)
+    gk_prof_end(st, pa, 4 * (sizeof(T) == 8) + 2 * trans_b + lower,
+                lower ? (double)m * (double)(m + 1) * (double)k : 2.0 * (double)m * (double)n * (double)k);
   GK_HIP(hipGetLastError());
   return GK_OK;
 }
@@ -655,6 +660,19 @@ static int cholesky_impl(gk_handle h, T* A, int64_t n, int64_t lda, T* dinv, int
         if (hinfo != 0) break;
       }
     }
+  }
+  // chain blocks of the triangular solves: G_k = Dinv_k L_{k,k-1} (plane 1),
+  // H_k = L_{k+1,k} Dinv_k (plane 2)
+  T* Gp = dinv + lda * NB;
+  T* Hp = dinv + 2 * lda * NB;
+  for (int64_t kb = 0; kb < nb; ++kb) {
+    const int64_t k0 = kb * NB;
+    if (kb > 0)
+      GK_TRY(gemm_launch<T>(NB, NB, NB, dinv + k0 * NB, NB, A + k0 * lda + k0 - NB, lda, false, T(1), T(0),
+                            Gp + k0 * NB, NB, false, st));
+    if (kb + 1 < nb)
+      GK_TRY(gemm_launch<T>(NB, NB, NB, A + (k0 + NB) * lda + k0, lda, dinv + k0 * NB, NB, false, T(1), T(0),
+                            Hp + k0 * NB, NB, false, st));
   }
   GK_HIP(hipGetLastError());
   int32_t hinfo = 0;

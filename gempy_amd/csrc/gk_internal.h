@@ -67,6 +67,19 @@ template <typename T>
 int gk_trsv_chol_impl(gk_handle h, const T* L, const T* dinv, int64_t lda, T* b, hipStream_t st);
 int gk_trsv_check(gk_handle h, hipStream_t st);
 
+// Optional per-launch timing of the dense kernels (gk_profile): HIP events are
+// recorded around each launch of an instrumented kernel while enabled.  Kinds:
+// GEMM instantiations 4*(double) + 2*trans_b + lower (0..7), triangular solves
+// 8 (FP32) and 9 (FP64); `work` is the launch's algorithmic flops (GEMM) or
+// bytes of the factor read (triangular solves).
+struct gk_prof_rec {
+  hipEvent_t a, b;
+  int kind;
+  double work;
+};
+bool gk_prof_begin(hipStream_t st, hipEvent_t* a);
+void gk_prof_end(hipStream_t st, hipEvent_t a, int kind, double work);
+
 static inline int64_t gk_ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
 static inline int64_t gk_round_up(int64_t a, int64_t b) { return gk_ceil_div(a, b) * b; }
 

@@ -504,20 +504,27 @@ struct Solver {
           double rel = 0.0;
           GK_TRY(true_rel(xt, &rel));
           const bool converged = rel <= tol;
-          // Below `accept` a check that does not halve the residual means the
-          // FP64 floor is reached; above it, allow 3 checks for a halving.
-          const bool stalled = rel <= accept ? rel > 0.5 * last_true
+          // The Arnoldi estimate |g_j| tracks the true residual until the
+          // true one reaches its floor (FP64 rounding of the operator, or an
+          // FP32 Arnoldi matvec); once it runs 10x below, the current cycle
+          // cannot lower the true residual any more.
+          const double est = std::fabs(g[j]) / nb;
+          const bool floor_hit = est < 0.1 * rel;
+          // Below `accept` a check that does not halve the residual, or a
+          // floor hit, ends the iteration; above it, allow 3 checks for a
+          // halving, and restart the cycle from the true residual at a floor.
+          const bool stalled = rel <= accept ? (rel > 0.5 * last_true || floor_hit)
                                              : hist.size() >= 3 && rel > 0.5 * hist[hist.size() - 3];
           last_true = rel;
           hist.push_back(rel);
           if (gk_verbose())
-            fprintf(stderr, "[gk] gmres it %d: true relres %.3e%s%s\n", total, rel, converged ? " (converged)" : "",
-                    stalled ? " (stalled)" : "");
+            fprintf(stderr, "[gk] gmres it %d: true relres %.3e (estimate %.3e)%s%s\n", total, rel, est,
+                    converged ? " (converged)" : "", stalled ? " (stalled)" : "");
           if (converged || stalled || total >= max_iter || hn == 0.0) {
             GK_HIP(hipMemcpyAsync(x, xt, N * sizeof(double), hipMemcpyDeviceToDevice, st));
             *ok = rel <= accept;
             done = true;
-          } else if (full) {
+          } else if (full || floor_hit) {
             GK_HIP(hipMemcpyAsync(x, xt, N * sizeof(double), hipMemcpyDeviceToDevice, st));
             restart = true;
           }
@@ -615,7 +622,7 @@ static int init_solver(gk_handle h, const gk_system* sys, hipStream_t st, int fo
   const int m = fast ? S.m : 1;
   size_t need = 0;
   auto take = [&](size_t bytes) { size_t o = need; need += gk_round_up(bytes, 256); return o; };
-  const size_t o_dinv = take(form != GK_SOLVE_FP64_LU ? (size_t)S.Ncpad * GK_BLOCK * sizeof(double) : 0);
+  const size_t o_dinv = take(form != GK_SOLVE_FP64_LU ? (size_t)3 * S.Ncpad * GK_BLOCK * sizeof(double) : 0);
   const size_t o_dsc = take(S.Ncpad * sizeof(double));
   const size_t o_U = take((size_t)S.Ncpad * std::max<int64_t>(Lt, 1) * sizeof(double));
   const size_t o_Y = take((size_t)S.Ncpad * std::max<int64_t>(Lt, 1) * sizeof(double));
@@ -978,9 +985,10 @@ extern "C" int gk_adjoint(gk_handle h, const gk_system* sys, const double* w, in
 }
 
 extern "C" int gk_solve_spai(gk_handle h, const gk_system* sys, const int32_t* idx, const float* Y, double tol,
-                             int32_t max_iter, int32_t krylov_precision, int32_t with_factor, double* x,
+                             int32_t max_iter, int32_t krylov_precision, int32_t flags, double* x,
                              gk_solve_info* info, gk_stream stream) {
   GK_REQUIRE(h && sys && x && info, GK_ERR_ARG, "gk_solve_spai: NULL argument");
+  const bool with_factor = (flags & GK_SPAI_WITH_FACTOR) != 0;
   GK_REQUIRE(with_factor || (idx && Y), GK_ERR_ARG, "gk_solve_spai: blocks required without the factor");
   GK_REQUIRE(krylov_precision == GK_F32 || krylov_precision == GK_F64, GK_ERR_ARG, "gk_solve_spai: bad precision");
   hipStream_t st = (hipStream_t)stream;
@@ -1021,7 +1029,7 @@ extern "C" int gk_solve_spai(gk_handle h, const gk_system* sys, const int32_t* i
   info->t_factor_ms = elapsed_ms(ev[0], ev[1]);
   info->t_refine_ms = elapsed_ms(ev[1], ev[2]);
   info->form = with_factor ? GK_SOLVE_FP32_REFINE : 3;
-  if (!ok) {                      // stagnation: the FP64 direct solve takes over
+  if (!ok && !(flags & GK_SPAI_NO_FALLBACK)) {    // stagnation: the FP64 direct solve takes over
     info->fallback = 1;
     S.spai = false;
     S.pcp = false;

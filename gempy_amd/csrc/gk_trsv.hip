@@ -1,29 +1,34 @@
 // Sync-free blocked triangular solves with a Cholesky factor: one launch for
-// L y = b and one for L^T x = y (in place on b), replacing a launch pair per
-// 256-wide block.
+// L y = b and one for L^T x = y (in place on b).
 //
-// L is row-major (ld = lda, a multiple of 256), its 256 x 256 diagonal blocks
-// are not read: dinv holds their inverses (block k at dinv + k0 * 256, ld 256,
-// lower triangular).  Forward, block row k:
-//     y_k = Dinv_k (b_k - sum_{j<k} L_kj y_j).
-// The sum is split over workgroups: a "finisher" for block row k applies the
-// block next to the diagonal (j = k-1), adds the published partial sums of the
-// "far" chunks, applies Dinv_k and publishes y_k.  Far chunks grow with their
-// distance d = k - j from the diagonal (d in [2,3), [3,5), [5,9), then 8 blocks
-// each), so a chunk's work fits in the time the diagonal chain needs to reach
-// row k after the chunk's vector blocks are ready.  The backward solve mirrors
-// it on block columns.  A workgroup is 16 waves: each wave streams 16 rows of a
-// 256 x 256 block, so a whole block is in flight at once.
+// L is row-major (ld = lda, a multiple of 256); its 256 x 256 diagonal blocks
+// are not read.  The factorization leaves three planes of 256 x 256 blocks
+// (lda x 256 each, block k at row k0 = 256 k):
+//   plane 0: Dinv_k = L_kk^-1 (lower triangular)
+//   plane 1: G_k = Dinv_k L_{k,k-1}         (k >= 1)
+//   plane 2: H_k = L_{k+1,k} Dinv_k         (k <= nb-2)
+// Forward, block row k:
+//   c_k = Dinv_k (b_k - sum_{j <= k-2} L_kj y_j)      ("c step")
+//   y_k = c_k - G_k y_{k-1}                           ("chain step")
+// and backward x_k = c'_k - H_k^T x_{k+1} with c'_k = Dinv_k^T (y_k - sum_{j >= k+2}
+// L_jk^T x_j).  Only the chain step lies on the sequential path, and its block
+// G_k (H_k) is loaded into registers before the wait, so a link of the chain
+// costs one flag hand-off and a register-resident 256 x 256 product instead
+// of two HBM block reads by one CU (FP32; the FP64 variant loads after the
+// wait, its rows do not fit in registers).  The sums over far blocks are
+// split into "far chunks" whose size grows with the distance from the
+// diagonal (d in [2,3), [3,5), [5,9), then 8 blocks each); the c step adds
+// their published partials (the nearest one last).  A workgroup is 16 waves,
+// each wave owns 16 rows of a 256 x 256 block.
 //
 // Ordering and progress: every workgroup takes a ticket (atomic counter) when it
 // starts; the ticket, not blockIdx, selects its work from a host-built table in
-// which every item depends only on items with smaller tickets (a far chunk of
-// row k is placed right after the finisher that produces the last vector block
-// it reads).  A workgroup that waits therefore waits on workgroups that have
-// already started, so the grid always drains.  Readiness flags hold the launch's
-// epoch (no clearing between launches).  Each wait is bounded: past the spin
-// limit the workgroup raises an error word, publishes its flags without
-// computing, and every later wait sees the error and returns at once.
+// which every item depends only on items with smaller tickets.  A workgroup that
+// waits therefore waits on workgroups that have already started, so the grid
+// always drains.  Readiness flags hold the launch's epoch (no clearing between
+// launches).  Each wait is bounded: past the spin limit the workgroup raises an
+// error word, publishes its flags without computing, and every later wait sees
+// the error and returns at once.
 #include <vector>
 #include <algorithm>
 #include "gk_internal.h"
@@ -46,7 +51,8 @@ template <typename T> struct V4;
 template <> struct V4<float> { typedef float type __attribute__((ext_vector_type(4))); };
 template <> struct V4<double> { typedef double type __attribute__((ext_vector_type(4))); };
 
-struct Item { int k, s, jlo, jhi; };    // s < 0: finisher of block row k
+struct Item { int k, s, jlo, jhi; };    // s >= 0: far chunk s of row k; CHAIN / CSTEP: the row's steps
+constexpr int CHAIN = -1, CSTEP = -2;
 
 // Relaxed polling (an acquire load would invalidate the caches on every
 // poll), then one acquire fence by the polling thread once the flag is seen;
@@ -88,37 +94,45 @@ __device__ __forceinline__ void group_publish(unsigned* flag, unsigned epoch) {
   if (threadIdx.x == 0) __hip_atomic_store(flag, epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Row dots of one wave over one 256 x 256 block: lane l < 16 returns
-// sum_c M[r0 + l, c0 + c] v[c].  Lane l loads columns 4l..4l+3 of each of the
-// 16 rows (one coalesced row per load, all 16 in flight), then a
-// transpose-reduce leaves row l & 15 in lane l.
-template <typename T, bool MASK_UPPER>
-__device__ __forceinline__ T rows16(const T* M, int64_t ld, int64_t r0, int64_t c0, const T* v, int rloc0, int l) {
+// 16 rows of a wave, lane l holding columns 4l..4l+3 of each
+template <typename T>
+struct Rows16 {
+  typename V4<T>::type v[16];
+};
+
+template <typename T>
+__device__ __forceinline__ void load16(Rows16<T>& R, const T* M, int64_t ld, int64_t r0, int64_t c0, int l) {
   typedef typename V4<T>::type v4;
   const T* base = M + r0 * ld + c0 + 4 * l;
-  v4 rowv[16];
 #pragma unroll
-  for (int r = 0; r < 16; ++r) rowv[r] = __builtin_nontemporal_load(reinterpret_cast<const v4*>(base + r * ld));
-  const v4 x = *reinterpret_cast<const v4*>(v + 4 * l);
+  for (int r = 0; r < 16; ++r) R.v[r] = __builtin_nontemporal_load(reinterpret_cast<const v4*>(base + r * ld));
+}
+
+// Row dots: lane l < 16 returns sum_c R[l & 15][c] x[c] (x: 256 entries);
+// MASK_UPPER keeps c <= rloc0 + row (the lower triangle of a diagonal block).
+template <typename T, bool MASK_UPPER>
+__device__ __forceinline__ T rowdot16(const Rows16<T>& R, const T* x, int rloc0, int l) {
+  typedef typename V4<T>::type v4;
+  const v4 xv = *reinterpret_cast<const v4*>(x + 4 * l);
   T acc[16];
 #pragma unroll
   for (int r = 0; r < 16; ++r) {
-    v4 a = rowv[r];
-    if (MASK_UPPER) {           // keep c <= row within the diagonal block
+    v4 a = R.v[r];
+    if (MASK_UPPER) {
       const int row = rloc0 + r, c = 4 * l;
       a.x = c <= row ? a.x : T(0);
       a.y = c + 1 <= row ? a.y : T(0);
       a.z = c + 2 <= row ? a.z : T(0);
       a.w = c + 3 <= row ? a.w : T(0);
     }
-    T s = a.x * x.x;
-    s = fma(a.y, x.y, s);
-    s = fma(a.z, x.z, s);
-    s = fma(a.w, x.w, s);
+    T s = a.x * xv.x;
+    s = fma(a.y, xv.y, s);
+    s = fma(a.z, xv.z, s);
+    s = fma(a.w, xv.w, s);
     acc[r] = s;
   }
 #pragma unroll
-  for (int width = 8; width >= 1; width >>= 1) {
+  for (int width = 8; width >= 1; width >>= 1) {         // transpose-reduce over lanes
     const bool upper = (l & width) != 0;
 #pragma unroll
     for (int i = 0; i < width; ++i) {
@@ -133,21 +147,18 @@ __device__ __forceinline__ T rows16(const T* M, int64_t ld, int64_t r0, int64_t 
   return t;
 }
 
-// acc4 += sum_{r < 16} M[r0 + 16 w + r, c0 + 4l..4l+3] v[16 w + r]
+// Column dots of a wave: acc (columns 4l..4l+3) += sum_r R[r][.] x[r], x the
+// wave's 16 entries; MASK_LOWER keeps rloc0 + r >= column.
 template <typename T, bool MASK_LOWER>
-__device__ __forceinline__ void coldot(const T* M, int64_t ld, int64_t r0, int64_t c0, const T* v,
-                                       typename V4<T>::type& acc, int w, int l) {
+__device__ __forceinline__ void coldot16(const Rows16<T>& R, const T* x, int rloc0, int l,
+                                         typename V4<T>::type& acc) {
   typedef typename V4<T>::type v4;
-  const T* base = M + (r0 + 16 * w) * ld + c0 + 4 * l;
-  v4 rowv[16];                  // all 16 row loads in flight before the first use
-#pragma unroll
-  for (int r = 0; r < 16; ++r) rowv[r] = __builtin_nontemporal_load(reinterpret_cast<const v4*>(base + r * ld));
 #pragma unroll
   for (int r = 0; r < 16; ++r) {
-    v4 a = rowv[r];
-    const T xr = v[16 * w + r];
-    if (MASK_LOWER) {           // keep row >= c within the diagonal block
-      const int row = 16 * w + r, c = 4 * l;
+    v4 a = R.v[r];
+    const T xr = x[r];
+    if (MASK_LOWER) {
+      const int row = rloc0 + r, c = 4 * l;
       a.x = row >= c ? a.x : T(0);
       a.y = row >= c + 1 ? a.y : T(0);
       a.z = row >= c + 2 ? a.z : T(0);
@@ -163,82 +174,116 @@ __device__ __forceinline__ void coldot(const T* M, int64_t ld, int64_t r0, int64
 struct TrsvCtl {
   const Item* items;
   const int* rowbase;       // far-chunk slot of (k, s) = rowbase[k] + s
-  unsigned* yflag;          // [nb]
+  unsigned* yflag;          // [nb] solution block published
+  unsigned* cflag;          // [nb] c step published
   unsigned* pflag;          // [slots]
   unsigned long long* ticket;
   unsigned long long ticket_base;
   unsigned epoch;
   int n_items;
+  int nb;
   int* err;
 };
 
 template <typename T>
-__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4, 4))) void trsv_fwd_kernel(const T* __restrict__ L, int64_t lda, const T* __restrict__ dinv,
-                                                      T* b, T* part, TrsvCtl ctl) {
+__device__ __forceinline__ bool take_ticket(const TrsvCtl& ctl, Item* it, int* s_id) {
+  if (threadIdx.x == 0) *s_id = (int)(atomicAdd(ctl.ticket, 1ull) - ctl.ticket_base);
+  __syncthreads();
+  if (*s_id < 0 || *s_id >= ctl.n_items) {                    // ticket counter out of step
+    if (threadIdx.x == 0) __hip_atomic_store(ctl.err, 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return false;
+  }
+  *it = ctl.items[*s_id];
+  return true;
+}
+
+// PF: load the step's matrix rows before its waits (FP32)
+template <typename T, bool PF>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4, 4)))
+void trsv_fwd_kernel(const T* __restrict__ L, int64_t lda, const T* __restrict__ planes, T* b, T* cbuf, T* part,
+                     TrsvCtl ctl) {
   __shared__ int s_id, s_ok, s_ok2;
   __shared__ __attribute__((aligned(32))) T tv[NB];
   const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
-  const int row = 16 * w + (l & 15);              // row of the 256-block owned by lanes l < 16
-  if (threadIdx.x == 0)
-    s_id = (int)(atomicAdd(ctl.ticket, 1ull) - ctl.ticket_base);
-  __syncthreads();
-  if (s_id < 0 || s_id >= ctl.n_items) {                      // ticket counter out of step
-    if (threadIdx.x == 0) __hip_atomic_store(ctl.err, 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    return;
-  }
-  const Item it = ctl.items[s_id];
+  const int row = 16 * w + (l & 15);              // row of the 256-block reduced into lanes l < 16
+  Item it;
+  if (!take_ticket<T>(ctl, &it, &s_id)) return;
   const int64_t k0 = (int64_t)it.k * NB;
-  if (it.s >= 0) {                                            // far chunk
+  const T* Dinv = planes + k0 * NB;
+  const T* G = planes + (int64_t)ctl.nb * NB * NB + k0 * NB;
+  Rows16<T> R;
+  if (it.s >= 0) {                                            // far chunk: sum_j L_kj y_j
     const int slot = ctl.rowbase[it.k] + it.s;
+    const bool pre = PF && it.jhi - it.jlo == 1;
+    if (pre) load16(R, L, lda, k0 + 16 * w, (int64_t)it.jlo * NB, l);
     if (group_wait(ctl.yflag + it.jhi - 1, ctl.epoch, ctl.err, &s_ok)) {
       T sum = T(0);
-      for (int j = it.jlo; j < it.jhi; ++j)
-        sum += rows16<T, false>(L, lda, k0 + 16 * w, (int64_t)j * NB, b + (int64_t)j * NB, 0, l);
+      if (pre) {
+        sum = rowdot16<T, false>(R, b + (int64_t)it.jlo * NB, 0, l);
+      } else {
+        for (int j = it.jlo; j < it.jhi; ++j) {
+          load16(R, L, lda, k0 + 16 * w, (int64_t)j * NB, l);
+          sum += rowdot16<T, false>(R, b + (int64_t)j * NB, 0, l);
+        }
+      }
       if (l < 16) part[(int64_t)slot * NB + row] = sum;
     }
     group_publish(ctl.pflag + slot, ctl.epoch);
     return;
   }
-  // finisher of block row k
-  bool ok = true;
-  T t = T(0);
-  if (it.k > 0) {
-    ok = group_wait(ctl.yflag + it.k - 1, ctl.epoch, ctl.err, &s_ok);
-    if (ok) t = rows16<T, false>(L, lda, k0 + 16 * w, k0 - NB, b + k0 - NB, 0, l);
-  }
-  if (ok) {
-    t = b[k0 + row] - t;
+  if (it.s == CSTEP) {                                        // c_k = Dinv_k (b_k - far partials)
+    if (PF) load16(R, Dinv, NB, 16 * w, 0, l);
     const int f0 = ctl.rowbase[it.k], nfar = ctl.rowbase[it.k + 1] - f0;
-    ok = group_wait_all(ctl.pflag + f0, nfar, ctl.epoch, ctl.err, &s_ok2);
-    if (ok)
-      for (int s = 0; s < nfar; ++s) t -= part[(int64_t)(f0 + s) * NB + row];
+    T t = b[k0 + row];
+    bool ok = true;
+    if (nfar > 1) {                                           // the far partials first, the nearest last
+      ok = group_wait_all(ctl.pflag + f0 + 1, nfar - 1, ctl.epoch, ctl.err, &s_ok2);
+      if (ok)
+        for (int s = 1; s < nfar; ++s) t -= part[(int64_t)(f0 + s) * NB + row];
+    }
+    if (ok && nfar > 0) {
+      ok = group_wait(ctl.pflag + f0, ctl.epoch, ctl.err, &s_ok);
+      if (ok) t -= part[(int64_t)f0 * NB + row];
+    }
+    if (ok) {
+      if (l < 16) tv[row] = t;
+      __syncthreads();
+      if (!PF) load16(R, Dinv, NB, 16 * w, 0, l);
+      const T c = rowdot16<T, true>(R, tv, 16 * w, l);
+      if (l < 16) cbuf[k0 + row] = c;
+    }
+    group_publish(ctl.cflag + it.k, ctl.epoch);
+    return;
   }
+  // chain step: y_k = c_k - G_k y_{k-1}
+  if (PF && it.k > 0) load16(R, G, NB, 16 * w, 0, l);
+  bool ok = group_wait(ctl.cflag + it.k, ctl.epoch, ctl.err, &s_ok);
+  if (ok && it.k > 0) ok = group_wait(ctl.yflag + it.k - 1, ctl.epoch, ctl.err, &s_ok2);
   if (ok) {
-    if (l < 16) tv[row] = t;
-    __syncthreads();
-    const T y = rows16<T, true>(dinv + k0 * NB, NB, 16 * w, 0, tv, 16 * w, l);
+    T y = cbuf[k0 + row];
+    if (it.k > 0) {
+      if (!PF) load16(R, G, NB, 16 * w, 0, l);
+      y -= rowdot16<T, false>(R, b + k0 - NB, 0, l);
+    }
     if (l < 16) b[k0 + row] = y;
   }
   group_publish(ctl.yflag + it.k, ctl.epoch);
 }
 
-template <typename T>
-__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4, 4))) void trsv_bwd_kernel(const T* __restrict__ L, int64_t lda, const T* __restrict__ dinv,
-                                                      T* b, T* part, TrsvCtl ctl) {
+template <typename T, bool PF>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4, 4)))
+void trsv_bwd_kernel(const T* __restrict__ L, int64_t lda, const T* __restrict__ planes, T* b, T* cbuf, T* part,
+                     TrsvCtl ctl) {
   typedef typename V4<T>::type v4;
   __shared__ int s_id, s_ok, s_ok2;
   __shared__ __attribute__((aligned(32))) T tv[NB];
   __shared__ __attribute__((aligned(32))) T red[NT / 64][NB];
   const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
-  if (threadIdx.x == 0)
-    s_id = (int)(atomicAdd(ctl.ticket, 1ull) - ctl.ticket_base);
-  __syncthreads();
-  if (s_id < 0 || s_id >= ctl.n_items) {                      // ticket counter out of step
-    if (threadIdx.x == 0) __hip_atomic_store(ctl.err, 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    return;
-  }
-  const Item it = ctl.items[s_id];
+  Item it;
+  if (!take_ticket<T>(ctl, &it, &s_id)) return;
   const int64_t k0 = (int64_t)it.k * NB;
+  const T* Dinv = planes + k0 * NB;
+  const T* H = planes + 2 * (int64_t)ctl.nb * NB * NB + k0 * NB;
   // column c = threadIdx.x (< 256) gets the sum of the 16 waves' partials
   auto reduce_waves = [&](v4 a) -> T {
     *reinterpret_cast<v4*>(&red[w][4 * l]) = a;
@@ -251,58 +296,88 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4, 4))) void
     __syncthreads();
     return s;
   };
+  Rows16<T> R;
   v4 acc = {T(0), T(0), T(0), T(0)};
-  if (it.s >= 0) {                                            // far chunk: rows j in [jlo, jhi)
-    const int slot = ctl.rowbase[it.k + 1] + it.s;
+  if (it.s >= 0) {                                            // far chunk: sum_j L_jk^T x_j
+    const int slot = ctl.rowbase[it.k] + it.s;
+    const bool pre = PF && it.jhi - it.jlo == 1;
+    if (pre) load16(R, L, lda, (int64_t)it.jlo * NB + 16 * w, k0, l);
     if (group_wait(ctl.yflag + it.jlo, ctl.epoch, ctl.err, &s_ok)) {
-      for (int j = it.jlo; j < it.jhi; ++j)
-        coldot<T, false>(L, lda, (int64_t)j * NB, k0, b + (int64_t)j * NB, acc, w, l);
+      for (int j = it.jlo; j < it.jhi; ++j) {
+        if (!pre) load16(R, L, lda, (int64_t)j * NB + 16 * w, k0, l);
+        coldot16<T, false>(R, b + (int64_t)j * NB + 16 * w, 0, l, acc);
+      }
       const T sum = reduce_waves(acc);
       if (threadIdx.x < NB) part[(int64_t)slot * NB + threadIdx.x] = sum;
     }
     group_publish(ctl.pflag + slot, ctl.epoch);
     return;
   }
-  const int nbk = ctl.rowbase[0];        // rowbase[0] carries nb for the backward table
-  bool ok = true;
-  if (it.k + 1 < nbk) {
-    ok = group_wait(ctl.yflag + it.k + 1, ctl.epoch, ctl.err, &s_ok);
-    if (ok) coldot<T, false>(L, lda, k0 + NB, k0, b + k0 + NB, acc, w, l);
+  if (it.s == CSTEP) {                                        // c'_k = Dinv_k^T (y_k - far partials)
+    if (PF) load16(R, Dinv, NB, 16 * w, 0, l);
+    const int f0 = ctl.rowbase[it.k], nfar = ctl.rowbase[it.k + 1] - f0;
+    T t = threadIdx.x < NB ? b[k0 + threadIdx.x] : T(0);
+    bool ok = true;
+    if (nfar > 1) {
+      ok = group_wait_all(ctl.pflag + f0 + 1, nfar - 1, ctl.epoch, ctl.err, &s_ok2);
+      if (ok && threadIdx.x < NB)
+        for (int s = 1; s < nfar; ++s) t -= part[(int64_t)(f0 + s) * NB + threadIdx.x];
+    }
+    if (ok && nfar > 0) {
+      ok = group_wait(ctl.pflag + f0, ctl.epoch, ctl.err, &s_ok);
+      if (ok && threadIdx.x < NB) t -= part[(int64_t)f0 * NB + threadIdx.x];
+    }
+    if (ok) {
+      if (threadIdx.x < NB) tv[threadIdx.x] = t;
+      __syncthreads();
+      if (!PF) load16(R, Dinv, NB, 16 * w, 0, l);
+      coldot16<T, true>(R, tv + 16 * w, 16 * w, l, acc);
+      const T c = reduce_waves(acc);
+      if (threadIdx.x < NB) cbuf[k0 + threadIdx.x] = c;
+    }
+    group_publish(ctl.cflag + it.k, ctl.epoch);
+    return;
   }
-  T t = T(0);
+  // chain step: x_k = c'_k - H_k^T x_{k+1}
+  const bool last = it.k + 1 >= ctl.nb;
+  if (PF && !last) load16(R, H, NB, 16 * w, 0, l);
+  bool ok = group_wait(ctl.cflag + it.k, ctl.epoch, ctl.err, &s_ok);
+  if (ok && !last) ok = group_wait(ctl.yflag + it.k + 1, ctl.epoch, ctl.err, &s_ok2);
   if (ok) {
-    t = reduce_waves(acc);
-    if (threadIdx.x < NB) t = b[k0 + threadIdx.x] - t;
-    const int f0 = ctl.rowbase[it.k + 1], nfar = ctl.rowbase[it.k + 2] - f0;
-    ok = group_wait_all(ctl.pflag + f0, nfar, ctl.epoch, ctl.err, &s_ok2);
-    if (ok && threadIdx.x < NB)
-      for (int s = 0; s < nfar; ++s) t -= part[(int64_t)(f0 + s) * NB + threadIdx.x];
-  }
-  if (ok) {
-    if (threadIdx.x < NB) tv[threadIdx.x] = t;
-    __syncthreads();
-    v4 a2 = {T(0), T(0), T(0), T(0)};
-    coldot<T, true>(dinv + k0 * NB, NB, 0, 0, tv, a2, w, l);
-    const T x = reduce_waves(a2);
-    if (threadIdx.x < NB) b[k0 + threadIdx.x] = x;
+    T x = T(0);
+    if (!last) {
+      if (!PF) load16(R, H, NB, 16 * w, 0, l);
+      coldot16<T, false>(R, b + k0 + NB + 16 * w, 0, l, acc);
+      x = reduce_waves(acc);
+    }
+    if (threadIdx.x < NB) b[k0 + threadIdx.x] = cbuf[k0 + threadIdx.x] - x;
   }
   group_publish(ctl.yflag + it.k, ctl.epoch);
 }
 
-// Host-side work tables.  Forward: finisher(j), then for each distance range s
-// the far chunk of row k = j + lo_s (blocks at distances [lo_s, hi_s) of row
-// k, nearest block j).  Backward mirrors it with k = j - lo_s.
+// Host-side work tables.  Forward: c steps of rows 0 and 1, then for each block
+// j: its chain step, the far chunks whose nearest block is j (row k = j + lo_s,
+// blocks at distances [lo_s, hi_s)), and the c step of row j + 2 (all its far
+// chunks are then queued).  Backward mirrors it with k = j - lo_s.
+int nfar_of(int k) {              // far chunks of a row with k blocks on its far side
+  int s = 0, lo, hi;
+  for (;;) { chunk_range(s, &lo, &hi); if (lo > k) return s; ++s; }
+}
+
 void build_tables(int nb, std::vector<Item>& fwd, std::vector<int>& fbase, std::vector<Item>& bwd,
                   std::vector<int>& bbase) {
-  auto nfar = [](int k) {        // far chunks of a row with k blocks on its far side of the diagonal
-    int s = 0, lo, hi;
-    for (;;) { chunk_range(s, &lo, &hi); if (lo > k) return s; ++s; }
-  };
   fbase.assign(nb + 1, 0);
-  for (int k = 0; k < nb; ++k) fbase[k + 1] = fbase[k] + nfar(k);
+  bbase.assign(nb + 1, 0);
+  for (int k = 0; k < nb; ++k) {
+    fbase[k + 1] = fbase[k] + nfar_of(k);
+    bbase[k + 1] = bbase[k] + nfar_of(nb - 1 - k);
+  }
   fwd.clear();
+  bwd.clear();
+  fwd.push_back({0, CSTEP, 0, 0});
+  if (nb > 1) fwd.push_back({1, CSTEP, 0, 0});
   for (int j = 0; j < nb; ++j) {
-    fwd.push_back({j, -1, 0, 0});
+    fwd.push_back({j, CHAIN, 0, 0});
     for (int s = 0;; ++s) {
       int lo, hi;
       chunk_range(s, &lo, &hi);
@@ -310,13 +385,12 @@ void build_tables(int nb, std::vector<Item>& fwd, std::vector<int>& fbase, std::
       if (k >= nb) break;
       fwd.push_back({k, s, std::max(0, k - hi + 1), k - lo + 1});
     }
+    if (j + 2 < nb) fwd.push_back({j + 2, CSTEP, 0, 0});
   }
-  bbase.assign(nb + 2, 0);
-  bbase[0] = nb;
-  for (int k = 0; k < nb; ++k) bbase[k + 2] = bbase[k + 1] + nfar(nb - 1 - k);
-  bwd.clear();
+  bwd.push_back({nb - 1, CSTEP, 0, 0});
+  if (nb > 1) bwd.push_back({nb - 2, CSTEP, 0, 0});
   for (int j = nb - 1; j >= 0; --j) {
-    bwd.push_back({j, -1, 0, 0});
+    bwd.push_back({j, CHAIN, 0, 0});
     for (int s = 0;; ++s) {
       int lo, hi;
       chunk_range(s, &lo, &hi);
@@ -324,14 +398,15 @@ void build_tables(int nb, std::vector<Item>& fwd, std::vector<int>& fbase, std::
       if (k < 0) break;
       bwd.push_back({k, s, k + lo, std::min(nb, k + hi)});
     }
+    if (j - 2 >= 0) bwd.push_back({j - 2, CSTEP, 0, 0});
   }
 }
 
 }  // namespace
 
-// Workspace: [fwd items][bwd items][fwd base][bwd base][yflag][pflag][ticket][err][partials]
+// Workspace: [fwd items][bwd items][fwd base][bwd base][yflag][cflag][pflag][ticket][err][c][partials]
 struct TrsvLayout {
-  size_t o_fi, o_bi, o_fb, o_bb, o_yf, o_pf, o_tk, o_err, o_part, bytes;
+  size_t o_fi, o_bi, o_fb, o_bb, o_yf, o_cf, o_pf, o_tk, o_err, o_c, o_part, bytes;
   int nfwd, nbwd, slots;
 };
 
@@ -342,11 +417,13 @@ static TrsvLayout trsv_layout(int nb, int nfwd, int nbwd, int slots) {
   L.o_fi = take(sizeof(Item) * nfwd);
   L.o_bi = take(sizeof(Item) * nbwd);
   L.o_fb = take(sizeof(int) * (nb + 1));
-  L.o_bb = take(sizeof(int) * (nb + 2));
+  L.o_bb = take(sizeof(int) * (nb + 1));
   L.o_yf = take(sizeof(unsigned) * nb);
+  L.o_cf = take(sizeof(unsigned) * nb);
   L.o_pf = take(sizeof(unsigned) * std::max(slots, 1));
   L.o_tk = take(sizeof(unsigned long long));
   L.o_err = take(sizeof(int));
+  L.o_c = take(sizeof(double) * NB * (size_t)nb);
   L.o_part = take(sizeof(double) * NB * (size_t)std::max(slots, 1));
   L.bytes = o;
   L.nfwd = nfwd; L.nbwd = nbwd; L.slots = slots;
@@ -358,7 +435,7 @@ static int trsv_prepare(gk_handle h, int nb, hipStream_t st, char** base, TrsvLa
   std::vector<int> fb, bb;
   const bool rebuild = h->trsv_nb != nb || h->ws[WS_TRSV] == nullptr;
   if (rebuild) build_tables(nb, fwd, fb, bwd, bb);
-  const int slots = rebuild ? std::max(fb[nb], bb[nb + 1]) : h->trsv_slots;
+  const int slots = rebuild ? std::max(fb[nb], bb[nb]) : h->trsv_slots;
   const int nfwd = rebuild ? (int)fwd.size() : h->trsv_nfwd;
   const int nbwd = rebuild ? (int)bwd.size() : h->trsv_nbwd;
   *lay = trsv_layout(nb, nfwd, nbwd, slots);
@@ -372,7 +449,7 @@ static int trsv_prepare(gk_handle h, int nb, hipStream_t st, char** base, TrsvLa
     GK_HIP(hipMemcpyAsync(*base + lay->o_bi, bwd.data(), sizeof(Item) * bwd.size(), hipMemcpyHostToDevice, st));
     GK_HIP(hipMemcpyAsync(*base + lay->o_fb, fb.data(), sizeof(int) * fb.size(), hipMemcpyHostToDevice, st));
     GK_HIP(hipMemcpyAsync(*base + lay->o_bb, bb.data(), sizeof(int) * bb.size(), hipMemcpyHostToDevice, st));
-    GK_HIP(hipMemsetAsync(*base + lay->o_yf, 0, lay->o_part - lay->o_yf, st));   // flags, ticket, error
+    GK_HIP(hipMemsetAsync(*base + lay->o_yf, 0, lay->o_c - lay->o_yf, st));   // flags, ticket, error
     GK_HIP(hipStreamSynchronize(st));          // host tables go out of scope
     h->trsv_nb = nb;
     h->trsv_slots = slots;
@@ -385,17 +462,21 @@ static int trsv_prepare(gk_handle h, int nb, hipStream_t st, char** base, TrsvLa
 }
 
 template <typename T>
-int gk_trsv_chol_impl(gk_handle h, const T* L, const T* dinv, int64_t lda, T* b, hipStream_t st) {
-  GK_REQUIRE(h && L && dinv && b && lda > 0 && lda % NB == 0, GK_ERR_ARG, "triangular solve: bad arguments");
+int gk_trsv_chol_impl(gk_handle h, const T* L, const T* planes, int64_t lda, T* b, hipStream_t st) {
+  GK_REQUIRE(h && L && planes && b && lda > 0 && lda % NB == 0, GK_ERR_ARG, "triangular solve: bad arguments");
+  constexpr bool PF = sizeof(T) == 4;          // FP32 block rows fit in registers
   const int nb = (int)(lda / NB);
   char* base = nullptr;
   TrsvLayout lay;
   GK_TRY(trsv_prepare(h, nb, st, &base, &lay));
   TrsvCtl c;
   c.yflag = reinterpret_cast<unsigned*>(base + lay.o_yf);
+  c.cflag = reinterpret_cast<unsigned*>(base + lay.o_cf);
   c.pflag = reinterpret_cast<unsigned*>(base + lay.o_pf);
   c.ticket = reinterpret_cast<unsigned long long*>(base + lay.o_tk);
   c.err = reinterpret_cast<int*>(base + lay.o_err);
+  c.nb = nb;
+  T* cbuf = reinterpret_cast<T*>(base + lay.o_c);
   T* part = reinterpret_cast<T*>(base + lay.o_part);
   // forward
   c.items = reinterpret_cast<const Item*>(base + lay.o_fi);
@@ -403,7 +484,10 @@ int gk_trsv_chol_impl(gk_handle h, const T* L, const T* dinv, int64_t lda, T* b,
   c.ticket_base = h->trsv_tickets;
   c.epoch = ++h->trsv_epoch;
   c.n_items = lay.nfwd;
-  hipLaunchKernelGGL(trsv_fwd_kernel<T>, dim3((unsigned)lay.nfwd), dim3(NT), 0, st, L, lda, dinv, b, part, c);
+  hipEvent_t pa;
+  const bool prof = gk_prof_begin(st, &pa);
+  hipLaunchKernelGGL((trsv_fwd_kernel<T, PF>), dim3((unsigned)lay.nfwd), dim3(NT), 0, st, L, lda, planes, b, cbuf,
+                     part, c);
   h->trsv_tickets += (unsigned long long)lay.nfwd;
   // backward
   c.items = reinterpret_cast<const Item*>(base + lay.o_bi);
@@ -411,8 +495,12 @@ int gk_trsv_chol_impl(gk_handle h, const T* L, const T* dinv, int64_t lda, T* b,
   c.ticket_base = h->trsv_tickets;
   c.epoch = ++h->trsv_epoch;
   c.n_items = lay.nbwd;
-  hipLaunchKernelGGL(trsv_bwd_kernel<T>, dim3((unsigned)lay.nbwd), dim3(NT), 0, st, L, lda, dinv, b, part, c);
+  hipLaunchKernelGGL((trsv_bwd_kernel<T, PF>), dim3((unsigned)lay.nbwd), dim3(NT), 0, st, L, lda, planes, b, cbuf,
+                     part, c);
   h->trsv_tickets += (unsigned long long)lay.nbwd;
+  if (prof)      // each direction reads the strict lower block triangle and two planes of diagonal blocks
+    gk_prof_end(st, pa, sizeof(T) == 8 ? 9 : 8,
+                2.0 * sizeof(T) * ((double)nb * (nb - 1) / 2.0 * NB * NB + 2.0 * nb * NB * NB));
   const hipError_t e = hipGetLastError();
   if (e != hipSuccess) {
     h->trsv_nb = 0;              // counters no longer in step: rebuild on the next call

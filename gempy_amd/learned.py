@@ -27,9 +27,9 @@ from . import _lib, synthetic
 from .data import KernelOptions
 from .engine import DeviceStack, _ptr, _stream
 
-K = 16
-F = 8
-HIDDEN = 32
+K = _lib.GK_SPAI_K
+F = _lib.GK_SPAI_FEATURES
+HIDDEN = _lib.GK_SPAI_HIDDEN
 WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "spai_weights.npz")
 
 
@@ -196,15 +196,18 @@ def local_inverse_blocks(ds: DeviceStack):
 
 
 def solve_spai(ds: DeviceStack, idx=None, y=None, tol: float = 1e-8, max_iter: int = 500,
-               krylov_precision: int = _lib.GK_F32, with_factor: bool = False):
+               krylov_precision: int = _lib.GK_F32, with_factor: bool = False, no_fallback: bool = False):
     """GMRES preconditioned by the blocks (idx, y) alone (with_factor False),
     or by the shifted FP32 factor combined with them (with_factor True; the
-    factor alone when idx/y are None)."""
+    factor alone when idx/y are None).  no_fallback: return the last iterate
+    instead of falling back to the FP64 direct solve on stagnation."""
     lib = _lib.load()
     x = torch.empty(ds.N, dtype=torch.float64, device="cuda")
     info = _lib.GkSolveInfo()
     _lib.check("gk_solve_spai", lib.gk_solve_spai(ds.h, C.byref(ds.sys), _ptr(idx), _ptr(y), float(tol),
-                                                  int(max_iter), int(krylov_precision), int(bool(with_factor)),
+                                                  int(max_iter), int(krylov_precision),
+                                                  (_lib.GK_SPAI_WITH_FACTOR if with_factor else 0)
+                                                  | (_lib.GK_SPAI_NO_FALLBACK if no_fallback else 0),
                                                   _ptr(x), C.byref(info), _stream()))
     DeviceStack.solve_epoch += 1
     return x, {"form": int(info.form), "iterations": int(info.iterations), "fallback": int(info.fallback),

@@ -46,6 +46,7 @@ enum { GK_SOLVE_FP64_LU = 0, GK_SOLVE_FP32_REFINE = 1, GK_SOLVE_FP64_CHOL = 2 };
 #define GK_SPAI_FEATURES 8    /* pair features fed to its network */
 #define GK_SPAI_HIDDEN 32     /* width of its two hidden layers */
 #define GK_BLOCK 256   /* factorization block size; lda of factored matrices is a multiple */
+enum { GK_SPAI_WITH_FACTOR = 1, GK_SPAI_NO_FALLBACK = 2 };   /* gk_solve_spai flags */
 
 /* One stack's co-kriging data in the engine's layout
  * (SurfacePoints / Orientations records built at
@@ -134,11 +135,15 @@ int gk_matvec(gk_handle h, const gk_system* sys, int32_t precision, const double
 
 /* ---- (b) solver (kernel_solver, 2-layers.approved.txt:158) -------------- */
 /* In-place lower Cholesky of an lda x lda FP32 matrix (lda multiple of GK_BLOCK;
- * rows/cols >= n are reset to the identity).  dinv [lda*GK_BLOCK] receives the
- * inverses of the diagonal blocks.  info = 0 or the first failing block+1. */
+ * rows/cols >= n are reset to the identity).  dinv [3*lda*GK_BLOCK] receives
+ * three planes of GK_BLOCK-square blocks (block k at row k*GK_BLOCK of each
+ * plane): the inverses Dinv_k of the diagonal blocks, G_k = Dinv_k L_{k,k-1}
+ * and H_k = L_{k+1,k} Dinv_k (used by the solves).  info = 0 or the first
+ * failing block+1. */
 int gk_cholesky_f32(gk_handle h, float* A, int64_t n, int64_t lda, float* dinv, int32_t* info,
                     gk_stream stream);
-/* Solve (L L^T) x = b in place with the output of gk_cholesky_f32 (FP32 vector of length lda). */
+/* Solve (L L^T) x = b in place with the output of gk_cholesky_f32 (FP32 vector of
+ * length lda); one launch per triangle, calls on one handle are stream-ordered. */
 int gk_cholesky_solve_f32(gk_handle h, const float* L, const float* dinv, int64_t lda, float* b,
                           gk_stream stream);
 /* FP64 counterparts (FP64 MFMA trailing updates). */
@@ -252,11 +257,13 @@ int gk_octree_refine(gk_handle h, int32_t n_fields, const int32_t* ids, int64_t 
  * gk_spai_apply: y = M u on the covariance unknowns (FP64 vectors, FP32 blocks).
  * gk_solve_spai: right-preconditioned restarted flexible GMRES on the
  *   matrix-free operator (Arnoldi matvec in `krylov_precision`, true residual
- *   in FP64) and the exact Schur complement of the drift.  with_factor = 0:
- *   the covariance block is preconditioned by M alone (idx, Y required);
- *   with_factor = 1: by the shifted FP32 Cholesky factor P, and, when idx and
- *   Y are given, by the combination P, M, P (M corrects the local modes the
- *   shift leaves).  Falls back to GK_SOLVE_FP64_CHOL on stagnation. */
+ *   in FP64) and the exact Schur complement of the drift.  Without
+ *   GK_SPAI_WITH_FACTOR in `flags` the covariance block is preconditioned by M
+ *   alone (idx, Y required); with it, by the shifted FP32 Cholesky factor P
+ *   and, when idx and Y are given, by the combination P, M, P (M corrects the
+ *   local modes the shift leaves).  Falls back to GK_SOLVE_FP64_CHOL on
+ *   stagnation unless GK_SPAI_NO_FALLBACK is set (then x is the last iterate
+ *   and info->relres its residual). */
 int gk_spai_nparams(void);
 int gk_knn(gk_handle h, const gk_system* sys, int32_t* idx, float* hk, gk_stream stream);
 int gk_spai_features(gk_handle h, const gk_system* sys, int64_t nq, const int32_t* pts, const int32_t* idx,
@@ -277,8 +284,18 @@ int gk_diag_scale(gk_handle h, const gk_system* sys, double* dscale, gk_stream s
 int gk_spai_local_inverse(gk_handle h, const gk_system* sys, const int32_t* idx, const double* dscale, float* Y,
                           gk_stream stream);
 int gk_solve_spai(gk_handle h, const gk_system* sys, const int32_t* idx, const float* Y, double tol,
-                  int32_t max_iter, int32_t krylov_precision, int32_t with_factor, double* x, gk_solve_info* info,
+                  int32_t max_iter, int32_t krylov_precision, int32_t flags, double* x, gk_solve_info* info,
                   gk_stream stream);
+
+/* ---- measurement -------------------------------------------------------
+ * gk_profile(h, 1) starts recording HIP events around every launch of the
+ * dense kernels (process-wide, cleared on enable; 0 stops).  gk_profile_read
+ * sums the recorded launches of one kind: 0..7 = MFMA GEMM with
+ * kind = 4*(FP64) + 2*(B transposed) + (lower tiles only), 8 / 9 = the FP32 /
+ * FP64 triangular solves (one record per L^-T L^-1 b); `work` = algorithmic
+ * flops (GEMM: 2mnk, lower: m(m+1)k) or factor bytes read (solves). */
+int gk_profile(gk_handle h, int32_t enable);
+int gk_profile_read(gk_handle h, int32_t kind, int64_t* launches, double* ms, double* work);
 
 #ifdef __cplusplus
 }

@@ -96,7 +96,7 @@ def test_cholesky_f32_and_solve(gpu):
     A = _spd(n, 2)
     tA = torch.zeros((lda, lda), dtype=torch.float32, device=gpu)
     tA[:n, :n] = torch.tensor(A, dtype=torch.float32)
-    dinv = torch.empty(lda * 256, dtype=torch.float32, device=gpu)
+    dinv = torch.empty(3 * lda * 256, dtype=torch.float32, device=gpu)
     info = C.c_int32(-1)
     L.check("chol", L.load().gk_cholesky_f32(_h(), _p(tA), n, lda, _p(dinv), C.byref(info), _st()))
     assert info.value == 0
@@ -121,10 +121,17 @@ def test_triangular_solve_many_blocks(gpu, dtype):
     lda = 256 * nb
     rng = np.random.default_rng(11)
     Lm = np.tril(rng.standard_normal((lda, lda)) * (0.5 / np.sqrt(lda)), -1) + np.diag(1.0 + rng.random(lda))
-    dinv = np.zeros((lda, 256))
+    # the three planes gk_cholesky_* leaves: Dinv_k, G_k = Dinv_k L_{k,k-1}, H_k = L_{k+1,k} Dinv_k
+    dinv = np.zeros((3, lda, 256))
     for k in range(nb):
         blk = slice(256 * k, 256 * (k + 1))
-        dinv[blk] = np.linalg.inv(Lm[blk, blk])
+        dinv[0, blk] = np.linalg.inv(Lm[blk, blk])
+    for k in range(nb):
+        blk, prv, nxt = slice(256 * k, 256 * (k + 1)), slice(256 * (k - 1), 256 * k), slice(256 * (k + 1), 256 * (k + 2))
+        if k > 0:
+            dinv[1, blk] = dinv[0, blk] @ Lm[blk, prv]
+        if k + 1 < nb:
+            dinv[2, blk] = Lm[nxt, blk] @ dinv[0, blk]
     tdt = torch.float32 if dtype == "f32" else torch.float64
     fn = L.load().gk_cholesky_solve_f32 if dtype == "f32" else L.load().gk_cholesky_solve_f64
     tL = torch.tensor(Lm, dtype=tdt, device=gpu)
@@ -148,7 +155,7 @@ def test_cholesky_f64_and_solve(gpu):
     A = _spd(n, 12)
     tA = torch.zeros((lda, lda), dtype=torch.float64, device=gpu)
     tA[:n, :n] = torch.tensor(A)
-    dinv = torch.empty(lda * 256, dtype=torch.float64, device=gpu)
+    dinv = torch.empty(3 * lda * 256, dtype=torch.float64, device=gpu)
     info = C.c_int32(-1)
     L.check("chol64", L.load().gk_cholesky_f64(_h(), _p(tA), n, lda, _p(dinv), C.byref(info), _st()))
     assert info.value == 0

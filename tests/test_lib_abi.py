@@ -36,6 +36,29 @@ def test_library_exports_every_declared_symbol():
     assert bound.gk_version() >= 100
 
 
+def test_header_constants_match_binding():
+    """Every enum / #define constant of include/gk.h has the same value in the
+    Python binding (the ctypes layer cannot see the header)."""
+    from gempy_amd import _lib
+    text = re.sub(r"/\*.*?\*/", "", open(HEADER).read(), flags=re.S)
+    consts = {}
+    for body in re.findall(r"enum\s*\{([^}]*)\}", text):
+        for name, val in re.findall(r"(GK_[A-Z0-9_]+)\s*=\s*(-?\d+)", body):
+            consts[name] = int(val)
+    for name, val in re.findall(r"#define\s+(GK_[A-Z0-9_]+)\s+(-?\d+)\b", text):
+        consts[name] = int(val)
+    assert len(consts) >= 15
+    for name, val in consts.items():
+        if name.startswith("GK_ERR_"):
+            assert _lib.GK_ERR[val] == name
+        elif name == "GK_OK":
+            assert _lib.GK_OK == val
+        else:
+            assert getattr(_lib, name) == val, name
+    # the gk_solve_info layout carries every field the library writes
+    assert [f[0] for f in _lib.GkSolveInfo._fields_][-1] == "shift"
+
+
 def test_error_reporting_without_gpu():
     from gempy_amd import _lib
     lib = _lib.load()
