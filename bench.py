@@ -45,6 +45,9 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the multi-stack/octree and adjoint workloads")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run a plain 1 GiB device copy (3 times), the reference kernel for "
+                         "scaling rocprofv3 FETCH_SIZE / WRITE_SIZE counters to bytes")
     return ap.parse_args()
 
 
@@ -57,6 +60,15 @@ def main():
     import ctypes as C
     import numpy as np
     import torch
+    if args.calibrate_copy:
+        src = torch.ones(1 << 28, dtype=torch.float32, device="cuda")      # 1 GiB
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibration": "device copy", "bytes_read_per_launch": 1 << 30,
+                          "bytes_written_per_launch": 1 << 30, "launches": 3}), flush=True)
+        return
     from gempy_amd import _lib, synthetic
     from gempy_amd.engine import DeviceStack
     lib = _lib.load()

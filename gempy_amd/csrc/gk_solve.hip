@@ -207,7 +207,8 @@ struct Solver {
   // or the learned sparse approximate inverse)
   bool f64 = false;
   bool spai = false;
-  bool pcp = false;                  // factor, local-inverse correction, factor (needs spai blocks and a factor)
+  bool pcp = false;                  // local-inverse correction with the FP32 factor (needs spai blocks and a factor)
+  bool pcp_sandwich = false;         // factor, correction, factor (for an approximate correction)
   double* pc_u = nullptr;            // scaled right-hand side of the covariance block
   double* pc_y = nullptr;            // scaled iterate
   double* pc_v = nullptr;            // scaled residual / correction
@@ -278,17 +279,24 @@ struct Solver {
 
   // K^-1 r (covariance block).  Forms: the Cholesky factor of the equilibrated
   // block (FP32 shifted or FP64); the sparse approximate inverse M alone; or
-  // the multiplicative combination P, M, P with P the FP32 shifted factor:
-  //   y = P u;  y += M (u - H y);  y += P (u - H y)
-  // P removes the smooth (large-eigenvalue) error, M the local modes that the
-  // shift leaves (eigenvalues below the shift).
+  // the multiplicative combination of M and the FP32 shifted factor P:
+  //   y = M u;  y += P (u - H y)
+  // M removes the local modes that the shift leaves (eigenvalues below the
+  // shift), P the smooth (large-eigenvalue) error.  One factor application
+  // and one operator product per call.  The sandwich P, M, P costs two of
+  // each; it needs a few iterations fewer and tolerates an inexact M (the
+  // last factor step removes the smooth error M introduces).
   int apply_K(const double* rin, double* yout) {
     if (pcp && !f64) {
       hipLaunchKernelGGL(to_scaled_kernel<double>, dim3(nblk(Nc)), dim3(256), 0, st, rin, dscale, pc_u, Nc, Nc);
-      GK_TRY(factor_apply_f32(pc_u, pc_y, false));
-      GK_TRY(scaled_residual(pc_u, pc_y, pc_v));
-      GK_TRY(gk_spai_apply_impl(sys, spai_idx, spai_Y, pc_v, ytmp2, st));
-      hipLaunchKernelGGL(axpy_kernel, dim3(nblk(Nc)), dim3(256), 0, st, ytmp2, pc_y, Nc);
+      if (pcp_sandwich) {         // y = P u;  y += M (u - H y);  y += P (u - H y)
+        GK_TRY(factor_apply_f32(pc_u, pc_y, false));
+        GK_TRY(scaled_residual(pc_u, pc_y, pc_v));
+        GK_TRY(gk_spai_apply_impl(sys, spai_idx, spai_Y, pc_v, ytmp2, st));
+        hipLaunchKernelGGL(axpy_kernel, dim3(nblk(Nc)), dim3(256), 0, st, ytmp2, pc_y, Nc);
+      } else {
+        GK_TRY(gk_spai_apply_impl(sys, spai_idx, spai_Y, pc_u, pc_y, st));
+      }
       GK_TRY(scaled_residual(pc_u, pc_y, pc_v));
       GK_TRY(factor_apply_f32(pc_v, pc_y, true));
       hipLaunchKernelGGL(from_scaled_kernel<double>, dim3(nblk(Nc)), dim3(256), 0, st, pc_y, dscale, yout, Nc);
@@ -1010,6 +1018,7 @@ extern "C" int gk_solve_spai(gk_handle h, const gk_system* sys, const int32_t* i
   bool have_factor = false;
   if (with_factor) {              // shifted FP32 factor, with M as its correction when given
     S.pcp = idx && Y;
+    S.pcp_sandwich = (flags & GK_SPAI_SANDWICH) != 0;
     int32_t finfo = 0;
     const int rc = S.setup_preconditioner(false, &finfo);
     if (rc != GK_OK && rc != GK_ERR_NOT_SPD) return rc;

@@ -18,8 +18,9 @@
 // wait, its rows do not fit in registers).  The sums over far blocks are
 // split into "far chunks" whose size grows with the distance from the
 // diagonal (d in [2,3), [3,5), [5,9), then 8 blocks each); the c step adds
-// their published partials (the nearest one last).  A workgroup is 16 waves,
-// each wave owns 16 rows of a 256 x 256 block.
+// their published partials (the nearest one last).  A workgroup covers a
+// 256 x 256 block: FP32 8 waves of 32 rows (two waves per SIMD, so a wave can
+// keep its 32 prefetched rows in registers), FP64 16 waves of 16 rows.
 //
 // Ordering and progress: every workgroup takes a ticket (atomic counter) when it
 // starts; the ticket, not blockIdx, selects its work from a host-built table in
@@ -36,8 +37,13 @@
 namespace {
 
 constexpr int NB = GK_BLOCK;            // 256
-constexpr int NT = 1024;                // threads per workgroup (16 waves x 16 rows)
 constexpr unsigned SPIN_LIMIT = 1u << 21;
+
+// Rows of a 256-block per wave: FP32 32 (8 waves, 2 per SIMD, so a wave may
+// hold 32 prefetched rows = 128 VGPRs), FP64 16 (16 waves).
+template <typename T> struct Shape;
+template <> struct Shape<float> { static constexpr int RPW = 32, NW = 8, NT = 512, WPE = 2; };
+template <> struct Shape<double> { static constexpr int RPW = 16, NW = 16, NT = 1024, WPE = 4; };
 
 // distance range [lo, hi) of far chunk s
 inline void chunk_range(int s, int* lo, int* hi) {
@@ -94,32 +100,50 @@ __device__ __forceinline__ void group_publish(unsigned* flag, unsigned epoch) {
   if (threadIdx.x == 0) __hip_atomic_store(flag, epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// 16 rows of a wave, lane l holding columns 4l..4l+3 of each
+// RPW rows of a wave, lane l holding columns 4l..4l+3 of each
 template <typename T>
-struct Rows16 {
-  typename V4<T>::type v[16];
+struct Rows {
+  static constexpr int R = Shape<T>::RPW;
+  typename V4<T>::type v[R];
 };
 
 template <typename T>
-__device__ __forceinline__ void load16(Rows16<T>& R, const T* M, int64_t ld, int64_t r0, int64_t c0, int l) {
+__device__ __forceinline__ void loadr(Rows<T>& R, const T* M, int64_t ld, int64_t r0, int64_t c0, int l) {
   typedef typename V4<T>::type v4;
   const T* base = M + r0 * ld + c0 + 4 * l;
 #pragma unroll
-  for (int r = 0; r < 16; ++r) R.v[r] = __builtin_nontemporal_load(reinterpret_cast<const v4*>(base + r * ld));
+  for (int r = 0; r < Rows<T>::R; ++r)
+    R.v[r] = __builtin_nontemporal_load(reinterpret_cast<const v4*>(base + r * ld));
 }
 
-// Row dots: lane l < 16 returns sum_c R[l & 15][c] x[c] (x: 256 entries);
-// MASK_UPPER keeps c <= rloc0 + row (the lower triangle of a diagonal block).
+// Completes the outstanding loads into R before the caller's flag wait: the
+// registers are asm operands, so the loads cannot be sunk past the wait.
+template <typename T>
+__device__ __forceinline__ void pin(Rows<T>& R) {
+#pragma unroll
+  for (int q = 0; q < Rows<T>::R; q += 8)
+    asm volatile("s_waitcnt vmcnt(0)"
+                 : "+v"(R.v[q]), "+v"(R.v[q + 1]), "+v"(R.v[q + 2]), "+v"(R.v[q + 3]), "+v"(R.v[q + 4]),
+                   "+v"(R.v[q + 5]), "+v"(R.v[q + 6]), "+v"(R.v[q + 7])
+                 :
+                 : "memory");
+}
+
+template <typename T>
+__device__ __forceinline__ void pin1(T& v) {
+  asm volatile("s_waitcnt vmcnt(0)" : "+v"(v) : : "memory");
+}
+
+// Row dots of 16 rows R.v[q..q+15]: lane l gets row q + (l & 15).
 template <typename T, bool MASK_UPPER>
-__device__ __forceinline__ T rowdot16(const Rows16<T>& R, const T* x, int rloc0, int l) {
+__device__ __forceinline__ T rowdot16(const Rows<T>& R, int q, typename V4<T>::type xv, int rloc0, int l) {
   typedef typename V4<T>::type v4;
-  const v4 xv = *reinterpret_cast<const v4*>(x + 4 * l);
   T acc[16];
 #pragma unroll
   for (int r = 0; r < 16; ++r) {
-    v4 a = R.v[r];
+    v4 a = R.v[q + r];
     if (MASK_UPPER) {
-      const int row = rloc0 + r, c = 4 * l;
+      const int row = rloc0 + q + r, c = 4 * l;
       a.x = c <= row ? a.x : T(0);
       a.y = c + 1 <= row ? a.y : T(0);
       a.z = c + 2 <= row ? a.z : T(0);
@@ -132,7 +156,7 @@ __device__ __forceinline__ T rowdot16(const Rows16<T>& R, const T* x, int rloc0,
     acc[r] = s;
   }
 #pragma unroll
-  for (int width = 8; width >= 1; width >>= 1) {         // transpose-reduce over lanes
+  for (int width = 8; width >= 1; width >>= 1) {        // transpose-reduce over lanes
     const bool upper = (l & width) != 0;
 #pragma unroll
     for (int i = 0; i < width; ++i) {
@@ -147,14 +171,28 @@ __device__ __forceinline__ T rowdot16(const Rows16<T>& R, const T* x, int rloc0,
   return t;
 }
 
+// Row dots: lane l < RPW returns sum_c R[l][c] x[c] (x: 256 entries);
+// MASK_UPPER keeps c <= rloc0 + row (the lower triangle of a diagonal block).
+template <typename T, bool MASK_UPPER>
+__device__ __forceinline__ T rowdot(const Rows<T>& R, const T* x, int rloc0, int l) {
+  typedef typename V4<T>::type v4;
+  const v4 xv = *reinterpret_cast<const v4*>(x + 4 * l);
+  T t = rowdot16<T, MASK_UPPER>(R, 0, xv, rloc0, l);
+#pragma unroll
+  for (int q = 16; q < Rows<T>::R; q += 16) {
+    const T u = rowdot16<T, MASK_UPPER>(R, q, xv, rloc0, l);
+    if ((l >> 4) == q / 16) t = u;
+  }
+  return t;
+}
+
 // Column dots of a wave: acc (columns 4l..4l+3) += sum_r R[r][.] x[r], x the
-// wave's 16 entries; MASK_LOWER keeps rloc0 + r >= column.
+// wave's RPW entries; MASK_LOWER keeps rloc0 + r >= column.
 template <typename T, bool MASK_LOWER>
-__device__ __forceinline__ void coldot16(const Rows16<T>& R, const T* x, int rloc0, int l,
-                                         typename V4<T>::type& acc) {
+__device__ __forceinline__ void coldot(const Rows<T>& R, const T* x, int rloc0, int l, typename V4<T>::type& acc) {
   typedef typename V4<T>::type v4;
 #pragma unroll
-  for (int r = 0; r < 16; ++r) {
+  for (int r = 0; r < Rows<T>::R; ++r) {
     v4 a = R.v[r];
     const T xr = x[r];
     if (MASK_LOWER) {
@@ -199,40 +237,44 @@ __device__ __forceinline__ bool take_ticket(const TrsvCtl& ctl, Item* it, int* s
 
 // PF: load the step's matrix rows before its waits (FP32)
 template <typename T, bool PF>
-__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4, 4)))
+__global__ __launch_bounds__(Shape<T>::NT) __attribute__((amdgpu_waves_per_eu(Shape<T>::WPE, Shape<T>::WPE)))
 void trsv_fwd_kernel(const T* __restrict__ L, int64_t lda, const T* __restrict__ planes, T* b, T* cbuf, T* part,
                      TrsvCtl ctl) {
   __shared__ int s_id, s_ok, s_ok2;
   __shared__ __attribute__((aligned(32))) T tv[NB];
   const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
-  const int row = 16 * w + (l & 15);              // row of the 256-block reduced into lanes l < 16
+  constexpr int RPW = Shape<T>::RPW, NW = Shape<T>::NW, NT = Shape<T>::NT;
+  const int row = RPW * w + (l & (RPW - 1));     // row of the 256-block reduced into lanes l < RPW
   Item it;
   if (!take_ticket<T>(ctl, &it, &s_id)) return;
   const int64_t k0 = (int64_t)it.k * NB;
   const T* Dinv = planes + k0 * NB;
   const T* G = planes + (int64_t)ctl.nb * NB * NB + k0 * NB;
-  Rows16<T> R;
+  Rows<T> R;
   if (it.s >= 0) {                                            // far chunk: sum_j L_kj y_j
     const int slot = ctl.rowbase[it.k] + it.s;
     const bool pre = PF && it.jhi - it.jlo == 1;
-    if (pre) load16(R, L, lda, k0 + 16 * w, (int64_t)it.jlo * NB, l);
+    if constexpr (PF) {
+      if (pre) { loadr(R, L, lda, k0 + RPW * w, (int64_t)it.jlo * NB, l); pin(R); }
+    }
     if (group_wait(ctl.yflag + it.jhi - 1, ctl.epoch, ctl.err, &s_ok)) {
       T sum = T(0);
       if (pre) {
-        sum = rowdot16<T, false>(R, b + (int64_t)it.jlo * NB, 0, l);
+        sum = rowdot<T, false>(R, b + (int64_t)it.jlo * NB, 0, l);
       } else {
+        #pragma unroll 1
         for (int j = it.jlo; j < it.jhi; ++j) {
-          load16(R, L, lda, k0 + 16 * w, (int64_t)j * NB, l);
-          sum += rowdot16<T, false>(R, b + (int64_t)j * NB, 0, l);
+          loadr(R, L, lda, k0 + RPW * w, (int64_t)j * NB, l);
+          sum += rowdot<T, false>(R, b + (int64_t)j * NB, 0, l);
         }
       }
-      if (l < 16) part[(int64_t)slot * NB + row] = sum;
+      if (l < RPW) part[(int64_t)slot * NB + row] = sum;
     }
     group_publish(ctl.pflag + slot, ctl.epoch);
     return;
   }
   if (it.s == CSTEP) {                                        // c_k = Dinv_k (b_k - far partials)
-    if (PF) load16(R, Dinv, NB, 16 * w, 0, l);
+    if constexpr (PF) { loadr(R, Dinv, NB, RPW * w, 0, l); pin(R); }
     const int f0 = ctl.rowbase[it.k], nfar = ctl.rowbase[it.k + 1] - f0;
     T t = b[k0 + row];
     bool ok = true;
@@ -246,38 +288,42 @@ void trsv_fwd_kernel(const T* __restrict__ L, int64_t lda, const T* __restrict__
       if (ok) t -= part[(int64_t)f0 * NB + row];
     }
     if (ok) {
-      if (l < 16) tv[row] = t;
+      if (l < RPW) tv[row] = t;
       __syncthreads();
-      if (!PF) load16(R, Dinv, NB, 16 * w, 0, l);
-      const T c = rowdot16<T, true>(R, tv, 16 * w, l);
-      if (l < 16) cbuf[k0 + row] = c;
+      if (!PF) loadr(R, Dinv, NB, RPW * w, 0, l);
+      const T c = rowdot<T, true>(R, tv, RPW * w, l);
+      if (l < RPW) cbuf[k0 + row] = c;
     }
     group_publish(ctl.cflag + it.k, ctl.epoch);
     return;
   }
   // chain step: y_k = c_k - G_k y_{k-1}
-  if (PF && it.k > 0) load16(R, G, NB, 16 * w, 0, l);
+  if constexpr (PF) {
+    if (it.k > 0) { loadr(R, G, NB, RPW * w, 0, l); pin(R); }
+  }
   bool ok = group_wait(ctl.cflag + it.k, ctl.epoch, ctl.err, &s_ok);
+  T y = T(0);
+  if (ok) { y = cbuf[k0 + row]; pin1(y); }         // c_k lands before the wait for y_{k-1}
   if (ok && it.k > 0) ok = group_wait(ctl.yflag + it.k - 1, ctl.epoch, ctl.err, &s_ok2);
   if (ok) {
-    T y = cbuf[k0 + row];
     if (it.k > 0) {
-      if (!PF) load16(R, G, NB, 16 * w, 0, l);
-      y -= rowdot16<T, false>(R, b + k0 - NB, 0, l);
+      if (!PF) loadr(R, G, NB, RPW * w, 0, l);
+      y -= rowdot<T, false>(R, b + k0 - NB, 0, l);
     }
-    if (l < 16) b[k0 + row] = y;
+    if (l < RPW) b[k0 + row] = y;
   }
   group_publish(ctl.yflag + it.k, ctl.epoch);
 }
 
 template <typename T, bool PF>
-__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4, 4)))
+__global__ __launch_bounds__(Shape<T>::NT) __attribute__((amdgpu_waves_per_eu(Shape<T>::WPE, Shape<T>::WPE)))
 void trsv_bwd_kernel(const T* __restrict__ L, int64_t lda, const T* __restrict__ planes, T* b, T* cbuf, T* part,
                      TrsvCtl ctl) {
   typedef typename V4<T>::type v4;
   __shared__ int s_id, s_ok, s_ok2;
   __shared__ __attribute__((aligned(32))) T tv[NB];
-  __shared__ __attribute__((aligned(32))) T red[NT / 64][NB];
+  constexpr int RPW = Shape<T>::RPW, NW = Shape<T>::NW, NT = Shape<T>::NT;
+  __shared__ __attribute__((aligned(32))) T red[NW][NB];
   const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
   Item it;
   if (!take_ticket<T>(ctl, &it, &s_id)) return;
@@ -291,21 +337,24 @@ void trsv_bwd_kernel(const T* __restrict__ L, int64_t lda, const T* __restrict__
     T s = T(0);
     if (threadIdx.x < NB) {
 #pragma unroll
-      for (int q = 0; q < NT / 64; ++q) s += red[q][threadIdx.x];
+      for (int q = 0; q < NW; ++q) s += red[q][threadIdx.x];
     }
     __syncthreads();
     return s;
   };
-  Rows16<T> R;
+  Rows<T> R;
   v4 acc = {T(0), T(0), T(0), T(0)};
   if (it.s >= 0) {                                            // far chunk: sum_j L_jk^T x_j
     const int slot = ctl.rowbase[it.k] + it.s;
     const bool pre = PF && it.jhi - it.jlo == 1;
-    if (pre) load16(R, L, lda, (int64_t)it.jlo * NB + 16 * w, k0, l);
+    if constexpr (PF) {
+      if (pre) { loadr(R, L, lda, (int64_t)it.jlo * NB + RPW * w, k0, l); pin(R); }
+    }
     if (group_wait(ctl.yflag + it.jlo, ctl.epoch, ctl.err, &s_ok)) {
+      #pragma unroll 1
       for (int j = it.jlo; j < it.jhi; ++j) {
-        if (!pre) load16(R, L, lda, (int64_t)j * NB + 16 * w, k0, l);
-        coldot16<T, false>(R, b + (int64_t)j * NB + 16 * w, 0, l, acc);
+        if (!pre) loadr(R, L, lda, (int64_t)j * NB + RPW * w, k0, l);
+        coldot<T, false>(R, b + (int64_t)j * NB + RPW * w, 0, l, acc);
       }
       const T sum = reduce_waves(acc);
       if (threadIdx.x < NB) part[(int64_t)slot * NB + threadIdx.x] = sum;
@@ -314,7 +363,7 @@ void trsv_bwd_kernel(const T* __restrict__ L, int64_t lda, const T* __restrict__
     return;
   }
   if (it.s == CSTEP) {                                        // c'_k = Dinv_k^T (y_k - far partials)
-    if (PF) load16(R, Dinv, NB, 16 * w, 0, l);
+    if constexpr (PF) { loadr(R, Dinv, NB, RPW * w, 0, l); pin(R); }
     const int f0 = ctl.rowbase[it.k], nfar = ctl.rowbase[it.k + 1] - f0;
     T t = threadIdx.x < NB ? b[k0 + threadIdx.x] : T(0);
     bool ok = true;
@@ -330,8 +379,8 @@ void trsv_bwd_kernel(const T* __restrict__ L, int64_t lda, const T* __restrict__
     if (ok) {
       if (threadIdx.x < NB) tv[threadIdx.x] = t;
       __syncthreads();
-      if (!PF) load16(R, Dinv, NB, 16 * w, 0, l);
-      coldot16<T, true>(R, tv + 16 * w, 16 * w, l, acc);
+      if (!PF) loadr(R, Dinv, NB, RPW * w, 0, l);
+      coldot<T, true>(R, tv + RPW * w, RPW * w, l, acc);
       const T c = reduce_waves(acc);
       if (threadIdx.x < NB) cbuf[k0 + threadIdx.x] = c;
     }
@@ -340,17 +389,21 @@ void trsv_bwd_kernel(const T* __restrict__ L, int64_t lda, const T* __restrict__
   }
   // chain step: x_k = c'_k - H_k^T x_{k+1}
   const bool last = it.k + 1 >= ctl.nb;
-  if (PF && !last) load16(R, H, NB, 16 * w, 0, l);
+  if constexpr (PF) {
+    if (!last) { loadr(R, H, NB, RPW * w, 0, l); pin(R); }
+  }
   bool ok = group_wait(ctl.cflag + it.k, ctl.epoch, ctl.err, &s_ok);
+  T cv = T(0);
+  if (ok && threadIdx.x < NB) { cv = cbuf[k0 + threadIdx.x]; pin1(cv); }
   if (ok && !last) ok = group_wait(ctl.yflag + it.k + 1, ctl.epoch, ctl.err, &s_ok2);
   if (ok) {
     T x = T(0);
     if (!last) {
-      if (!PF) load16(R, H, NB, 16 * w, 0, l);
-      coldot16<T, false>(R, b + k0 + NB + 16 * w, 0, l, acc);
+      if (!PF) loadr(R, H, NB, RPW * w, 0, l);
+      coldot<T, false>(R, b + k0 + NB + RPW * w, 0, l, acc);
       x = reduce_waves(acc);
     }
-    if (threadIdx.x < NB) b[k0 + threadIdx.x] = cbuf[k0 + threadIdx.x] - x;
+    if (threadIdx.x < NB) b[k0 + threadIdx.x] = cv - x;
   }
   group_publish(ctl.yflag + it.k, ctl.epoch);
 }
@@ -486,7 +539,7 @@ int gk_trsv_chol_impl(gk_handle h, const T* L, const T* planes, int64_t lda, T* 
   c.n_items = lay.nfwd;
   hipEvent_t pa;
   const bool prof = gk_prof_begin(st, &pa);
-  hipLaunchKernelGGL((trsv_fwd_kernel<T, PF>), dim3((unsigned)lay.nfwd), dim3(NT), 0, st, L, lda, planes, b, cbuf,
+  hipLaunchKernelGGL((trsv_fwd_kernel<T, PF>), dim3((unsigned)lay.nfwd), dim3(Shape<T>::NT), 0, st, L, lda, planes, b, cbuf,
                      part, c);
   h->trsv_tickets += (unsigned long long)lay.nfwd;
   // backward
@@ -495,7 +548,7 @@ int gk_trsv_chol_impl(gk_handle h, const T* L, const T* planes, int64_t lda, T* 
   c.ticket_base = h->trsv_tickets;
   c.epoch = ++h->trsv_epoch;
   c.n_items = lay.nbwd;
-  hipLaunchKernelGGL((trsv_bwd_kernel<T, PF>), dim3((unsigned)lay.nbwd), dim3(NT), 0, st, L, lda, planes, b, cbuf,
+  hipLaunchKernelGGL((trsv_bwd_kernel<T, PF>), dim3((unsigned)lay.nbwd), dim3(Shape<T>::NT), 0, st, L, lda, planes, b, cbuf,
                      part, c);
   h->trsv_tickets += (unsigned long long)lay.nbwd;
   if (prof)      // each direction reads the strict lower block triangle and two planes of diagonal blocks

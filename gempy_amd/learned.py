@@ -10,7 +10,7 @@ seeds 1000-2999) to minimise the local SPAI residual ||M_P H_loc - E_P||^2.
 Training and inference run in libgkhip.so (knn, features, local blocks, MLP
 forward/backward, loss, Adam); the network is used by ``gk_solve_spai``, alone
 (matrix-free GMRES) or as the local correction of the shifted FP32 Cholesky
-factor (factor, M, factor), where it removes the local error modes that the
+factor (y = P u; y += M (u - H y); y += P (u - H y)), where it removes the local error modes that the
 shift leaves.
 """
 from __future__ import annotations
@@ -196,18 +196,22 @@ def local_inverse_blocks(ds: DeviceStack):
 
 
 def solve_spai(ds: DeviceStack, idx=None, y=None, tol: float = 1e-8, max_iter: int = 500,
-               krylov_precision: int = _lib.GK_F32, with_factor: bool = False, no_fallback: bool = False):
+               krylov_precision: int = _lib.GK_F32, with_factor: bool = False, no_fallback: bool = False,
+               sandwich: bool = True):
     """GMRES preconditioned by the blocks (idx, y) alone (with_factor False),
     or by the shifted FP32 factor combined with them (with_factor True; the
-    factor alone when idx/y are None).  no_fallback: return the last iterate
-    instead of falling back to the FP64 direct solve on stagnation."""
+    factor alone when idx/y are None).  sandwich: factor, blocks, factor (the
+    form for the network's inexact blocks) instead of blocks then factor.
+    no_fallback: return the last iterate instead of falling back to the FP64
+    direct solve on stagnation."""
     lib = _lib.load()
     x = torch.empty(ds.N, dtype=torch.float64, device="cuda")
     info = _lib.GkSolveInfo()
     _lib.check("gk_solve_spai", lib.gk_solve_spai(ds.h, C.byref(ds.sys), _ptr(idx), _ptr(y), float(tol),
                                                   int(max_iter), int(krylov_precision),
                                                   (_lib.GK_SPAI_WITH_FACTOR if with_factor else 0)
-                                                  | (_lib.GK_SPAI_NO_FALLBACK if no_fallback else 0),
+                                                  | (_lib.GK_SPAI_NO_FALLBACK if no_fallback else 0)
+                                                  | (_lib.GK_SPAI_SANDWICH if sandwich else 0),
                                                   _ptr(x), C.byref(info), _stream()))
     DeviceStack.solve_epoch += 1
     return x, {"form": int(info.form), "iterations": int(info.iterations), "fallback": int(info.fallback),

@@ -46,7 +46,7 @@ enum { GK_SOLVE_FP64_LU = 0, GK_SOLVE_FP32_REFINE = 1, GK_SOLVE_FP64_CHOL = 2 };
 #define GK_SPAI_FEATURES 8    /* pair features fed to its network */
 #define GK_SPAI_HIDDEN 32     /* width of its two hidden layers */
 #define GK_BLOCK 256   /* factorization block size; lda of factored matrices is a multiple */
-enum { GK_SPAI_WITH_FACTOR = 1, GK_SPAI_NO_FALLBACK = 2 };   /* gk_solve_spai flags */
+enum { GK_SPAI_WITH_FACTOR = 1, GK_SPAI_NO_FALLBACK = 2, GK_SPAI_SANDWICH = 4 };   /* gk_solve_spai flags */
 
 /* One stack's co-kriging data in the engine's layout
  * (SurfacePoints / Orientations records built at
@@ -175,8 +175,8 @@ int gk_gemm_f64(gk_handle h, int64_t m, int64_t n, int64_t k, const double* A, i
  *                        Schur complement on the drift block + FP64 refinement steps.
  *  GK_SOLVE_FP32_REFINE: FP32 Cholesky of the equilibrated covariance block with a
  *                        diagonal shift (4 u32 sqrt(n), x4 per breakdown), combined
- *                        with the exact local sparse approximate inverse (factor,
- *                        local inverse, factor) as the preconditioner of FP64
+ *                        with the exact local sparse approximate inverse M
+ *                        (y = M u; y += P (u - H y)) as the preconditioner of FP64
  *                        right-preconditioned flexible GMRES on the matrix-free FP64
  *                        operator; if the factor breaks down 4 times or the iteration
  *                        stagnates above 1e-6, falls back to GK_SOLVE_FP64_CHOL. */
@@ -260,8 +260,9 @@ int gk_octree_refine(gk_handle h, int32_t n_fields, const int32_t* ids, int64_t 
  *   in FP64) and the exact Schur complement of the drift.  Without
  *   GK_SPAI_WITH_FACTOR in `flags` the covariance block is preconditioned by M
  *   alone (idx, Y required); with it, by the shifted FP32 Cholesky factor P
- *   and, when idx and Y are given, by the combination P, M, P (M corrects the
- *   local modes the shift leaves).  Falls back to GK_SOLVE_FP64_CHOL on
+ *   and, when idx and Y are given, by the combination y = M u; y += P (u - H y)
+ *   (M corrects the local modes the shift leaves), or with GK_SPAI_SANDWICH
+ *   by y = P u; y += M (u - H y); y += P (u - H y) (suits an inexact M).  Falls back to GK_SOLVE_FP64_CHOL on
  *   stagnation unless GK_SPAI_NO_FALLBACK is set (then x is the last iterate
  *   and info->relres its residual). */
 int gk_spai_nparams(void);

@@ -537,7 +537,7 @@ def test_spai_local_inverse_matches_numpy(gpu):
 
 def test_learned_preconditioner_training_and_gain(gpu):
     """The trained network as the local correction of the shifted FP32 factor
-    (factor, M, factor) against the factor alone (the method without it), on
+    (y = P u; y += M(u - Hy); y += P(u - Hy)) against the factor alone (the method without it), on
     held-out systems; FP64 GMRES to 1e-9 in both, no fallback."""
     say("spai: short training, then GMRES iterations factor+learned vs factor alone")
     from gempy_amd import learned
