@@ -20,10 +20,29 @@ constexpr int NB = GK_BLOCK;   // 256
 
 // K depth of one LDS stage of the GEMM (elements)
 #ifndef GK_BK_F32
-#define GK_BK_F32 32
+#define GK_BK_F32 16
 #endif
 #ifndef GK_BK_F64
-#define GK_BK_F64 32
+#define GK_BK_F64 16
+#endif
+// GEMM register stages of the global -> LDS copy (1 or 2; FP32 only) and
+// whether fragments are read from LDS one 16-byte vector at a time.  Measured
+// on MI355X (K = 256 rank updates): BK 16 with chunked fragments at 3 (FP32)
+// and 2 (FP64) waves per SIMD runs 82-87 / 42 TF/s against 66-73 / 29 TF/s
+// for BK 32 with whole-stage fragments (2 / 1 waves per SIMD); a second
+// register stage does not fit FP32's register budget at that occupancy.
+#ifndef GK_GEMM_RS_F32
+#define GK_GEMM_RS_F32 1
+#endif
+#ifndef GK_GEMM_CHUNKED
+#define GK_GEMM_CHUNKED 1
+#endif
+// minimum waves per SIMD the GEMM is compiled for (0 = compiler's choice)
+#ifndef GK_GEMM_WPE_F32
+#define GK_GEMM_WPE_F32 3
+#endif
+#ifndef GK_GEMM_WPE_F64
+#define GK_GEMM_WPE_F64 2
 #endif
 
 typedef float f4v __attribute__((ext_vector_type(4)));
@@ -33,6 +52,7 @@ template <typename T> struct Mfma;
 template <> struct Mfma<float> {
   using acc_t = f4v;
   static constexpr int BK = GK_BK_F32;
+  static constexpr int WPE = GK_GEMM_WPE_F32;
   static __device__ __forceinline__ acc_t mma(float a, float b, acc_t c) {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
   }
@@ -40,6 +60,7 @@ template <> struct Mfma<float> {
 template <> struct Mfma<double> {
   using acc_t = d4v;
   static constexpr int BK = GK_BK_F64;
+  static constexpr int WPE = GK_GEMM_WPE_F64;
   static __device__ __forceinline__ acc_t mma(double a, double b, acc_t c) {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
   }
@@ -49,7 +70,7 @@ template <> struct Mfma<double> {
 // Requirements (checked on the host): m, n multiples of 128, k multiple of BK,
 // leading dimensions multiples of 4 (16-byte rows), 16-byte aligned bases.
 template <typename T, bool TRANS_B, bool LOWER>
-__global__ __launch_bounds__(256) void gemm_kernel(int64_t m, int64_t n, int64_t k, const T* __restrict__ A,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(Mfma<T>::WPE, 0))) void gemm_kernel(int64_t m, int64_t n, int64_t k, const T* __restrict__ A,
                                                    int64_t lda, const T* __restrict__ B, int64_t ldb,
                                                    T alpha, T beta, T* C, int64_t ldc, int64_t tiles_n,
                                                    int64_t n_tiles) {
@@ -95,20 +116,21 @@ __global__ __launch_bounds__(256) void gemm_kernel(int64_t m, int64_t n, int64_t
   constexpr int VPR = BK / VEC;           // vectors per row
   constexpr int NV = 128 * BK / VEC / 256;
   static_assert(NV >= 1 && NV * 256 * VEC == 128 * BK, "BK must make the tile a whole number of vectors per thread");
-  vec_t ra[NV], rb[NV];
-  auto load_global = [&](int64_t k0) {
+  constexpr int RS = sizeof(T) == 4 ? GK_GEMM_RS_F32 : 1;
+  vec_t ra[RS][NV], rb[RS][NV];
+  auto load_global = [&](int64_t k0, vec_t (&xa)[NV], vec_t (&xb)[NV]) {
 #pragma unroll
     for (int it = 0; it < NV; ++it) {
       const int idx = tid + 256 * it;
       const int row = idx / VPR, cv = idx % VPR;
-      ra[it] = *reinterpret_cast<const vec_t*>(A + (m0 + row) * lda + k0 + cv * VEC);
+      xa[it] = *reinterpret_cast<const vec_t*>(A + (m0 + row) * lda + k0 + cv * VEC);
     }
     if (TRANS_B) {
 #pragma unroll
       for (int it = 0; it < NV; ++it) {
         const int idx = tid + 256 * it;
         const int row = idx / VPR, cv = idx % VPR;
-        rb[it] = *reinterpret_cast<const vec_t*>(B + (n0 + row) * ldb + k0 + cv * VEC);
+        xb[it] = *reinterpret_cast<const vec_t*>(B + (n0 + row) * ldb + k0 + cv * VEC);
       }
     } else {
       constexpr int VPRB = 128 / VEC;     // vectors per k-row of the B tile
@@ -116,23 +138,23 @@ __global__ __launch_bounds__(256) void gemm_kernel(int64_t m, int64_t n, int64_t
       for (int it = 0; it < NV; ++it) {
         const int idx = tid + 256 * it;
         const int kk = idx / VPRB, cv = idx % VPRB;
-        rb[it] = *reinterpret_cast<const vec_t*>(B + (k0 + kk) * ldb + n0 + cv * VEC);
+        xb[it] = *reinterpret_cast<const vec_t*>(B + (k0 + kk) * ldb + n0 + cv * VEC);
       }
     }
   };
-  auto store_lds = [&](int buf) {
+  auto store_lds = [&](int buf, const vec_t (&xa)[NV], const vec_t (&xb)[NV]) {
 #pragma unroll
     for (int it = 0; it < NV; ++it) {
       const int idx = tid + 256 * it;
       const int row = idx / VPR, cv = idx % VPR;
-      *reinterpret_cast<vec_t*>(&As[buf][row * LDS_LD + cv * VEC]) = ra[it];
+      *reinterpret_cast<vec_t*>(&As[buf][row * LDS_LD + cv * VEC]) = xa[it];
     }
     if (TRANS_B) {
 #pragma unroll
       for (int it = 0; it < NV; ++it) {
         const int idx = tid + 256 * it;
         const int row = idx / VPR, cv = idx % VPR;
-        *reinterpret_cast<vec_t*>(&Bs[buf][row * LDS_LD + cv * VEC]) = rb[it];
+        *reinterpret_cast<vec_t*>(&Bs[buf][row * LDS_LD + cv * VEC]) = xb[it];
       }
     } else {
       constexpr int VPRB = 128 / VEC;
@@ -141,42 +163,80 @@ __global__ __launch_bounds__(256) void gemm_kernel(int64_t m, int64_t n, int64_t
         const int idx = tid + 256 * it;
         const int kk = idx / VPRB, cv = idx % VPRB;
 #pragma unroll
-        for (int e = 0; e < VEC; ++e) Bs[buf][(cv * VEC + e) * LDS_LD + kk] = rb[it][e];
+        for (int e = 0; e < VEC; ++e) Bs[buf][(cv * VEC + e) * LDS_LD + kk] = xb[it][e];
       }
+    }
+  };
+  auto multiply = [&](int buf) {
+    if (GK_GEMM_CHUNKED) {        // fragments one 16-byte vector of k at a time
+#pragma unroll
+      for (int v = 0; v < KS / VEC; ++v) {
+        vec_t xa[4], xb[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          xa[i] = *reinterpret_cast<const vec_t*>(&As[buf][(wm * 64 + i * 16 + lrow) * LDS_LD + lk * KS + v * VEC]);
+          xb[i] = *reinterpret_cast<const vec_t*>(&Bs[buf][(wn * 64 + i * 16 + lrow) * LDS_LD + lk * KS + v * VEC]);
+        }
+#pragma unroll
+        for (int e = 0; e < VEC; ++e)
+#pragma unroll
+          for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) acc[i][j] = M::mma(xa[i][e], xb[j][e], acc[i][j]);
+      }
+    } else {                      // all KS k values of the stage at once
+      T af[4][KS], bf[4][KS];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const T* pa = &As[buf][(wm * 64 + i * 16 + lrow) * LDS_LD + lk * KS];
+        const T* pb = &Bs[buf][(wn * 64 + i * 16 + lrow) * LDS_LD + lk * KS];
+#pragma unroll
+        for (int v = 0; v < KS / VEC; ++v) {
+          vec_t x = *reinterpret_cast<const vec_t*>(pa + v * VEC);
+          vec_t y = *reinterpret_cast<const vec_t*>(pb + v * VEC);
+#pragma unroll
+          for (int e = 0; e < VEC; ++e) { af[i][v * VEC + e] = x[e]; bf[i][v * VEC + e] = y[e]; }
+        }
+      }
+#pragma unroll
+      for (int s = 0; s < KS; ++s)
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+          for (int j = 0; j < 4; ++j) acc[i][j] = M::mma(af[i][s], bf[j][s], acc[i][j]);
     }
   };
 
   const int64_t nk = k / BK;
-  load_global(0);
-  store_lds(0);
-  __syncthreads();
-  for (int64_t kt = 0; kt < nk; ++kt) {
-    const int buf = kt & 1;
-    if (kt + 1 < nk) load_global((kt + 1) * BK);
-    // fragments: lane reads its KS consecutive k values (permuted k order)
-    T af[4][KS], bf[4][KS];
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      const T* pa = &As[buf][(wm * 64 + i * 16 + lrow) * LDS_LD + lk * KS];
-      const T* pb = &Bs[buf][(wn * 64 + i * 16 + lrow) * LDS_LD + lk * KS];
-#pragma unroll
-      for (int v = 0; v < KS / VEC; ++v) {
-        vec_t x = *reinterpret_cast<const vec_t*>(pa + v * VEC);
-        vec_t y = *reinterpret_cast<const vec_t*>(pb + v * VEC);
-#pragma unroll
-        for (int e = 0; e < VEC; ++e) { af[i][v * VEC + e] = x[e]; bf[i][v * VEC + e] = y[e]; }
-      }
-    }
-#pragma unroll
-    for (int s = 0; s < KS; ++s)
-#pragma unroll
-      for (int i = 0; i < 4; ++i)
-#pragma unroll
-        for (int j = 0; j < 4; ++j) acc[i][j] = M::mma(af[i][s], bf[j][s], acc[i][j]);
-    if (kt + 1 < nk) {
-      store_lds(buf ^ 1);
-    }
+  if (RS == 2) {
+    // stage kt lives in LDS buffer kt & 1 and, before that, in register set kt & 1:
+    // the loads of stage kt+2 are in flight while stage kt is multiplied
+    load_global(0, ra[0], rb[0]);
+    if (nk > 1) load_global(BK, ra[RS - 1], rb[RS - 1]);
+    store_lds(0, ra[0], rb[0]);
     __syncthreads();
+    for (int64_t kt = 0; kt < nk; kt += 2) {
+      if (kt + 2 < nk) load_global((kt + 2) * BK, ra[0], rb[0]);
+      multiply(0);
+      if (kt + 1 < nk) store_lds(1, ra[RS - 1], rb[RS - 1]);
+      __syncthreads();
+      if (kt + 1 >= nk) break;
+      if (kt + 3 < nk) load_global((kt + 3) * BK, ra[RS - 1], rb[RS - 1]);
+      multiply(1);
+      if (kt + 2 < nk) store_lds(0, ra[0], rb[0]);
+      __syncthreads();
+    }
+  } else {
+    load_global(0, ra[0], rb[0]);
+    store_lds(0, ra[0], rb[0]);
+    __syncthreads();
+    for (int64_t kt = 0; kt < nk; ++kt) {
+      const int buf = kt & 1;
+      if (kt + 1 < nk) load_global((kt + 1) * BK, ra[0], rb[0]);
+      multiply(buf);
+      if (kt + 1 < nk) store_lds(buf ^ 1, ra[0], rb[0]);
+      __syncthreads();
+    }
   }
 
   // epilogue: lane holds column lane%16 of each 16x16 tile; its rows are
