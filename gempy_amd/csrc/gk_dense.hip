@@ -467,15 +467,154 @@ __global__ __launch_bounds__(1024) void potrf128_f64_kernel(double* A, int64_t l
   }
 }
 
+// Blocked Cholesky + inverse of a DB x DB block (one workgroup of 1024):
+// 8-column panels are factored by wave 0 in registers (cross-lane
+// broadcasts, no workgroup barrier), the rank-8 trailing update by all
+// threads; the inverse X = L^-1 goes by 8-row blocks (the update from earlier
+// rows by all threads, the 8 rows themselves by wave 0 in registers).  Two
+// barriers per 8 columns instead of three per column.  Storage: FP32 square
+// LDS arrays; FP64 packed lower triangles (the square ones do not fit).
+#ifndef GK_POTRF_PB_F32
+#define GK_POTRF_PB_F32 8
+#endif
+#ifndef GK_POTRF_PB_F64
+#define GK_POTRF_PB_F64 4
+#endif
+template <typename T> struct BlkStore;
+template <> struct BlkStore<float> {
+  float (*Ls)[DB + 1];
+  float (*Xs)[DB + 1];
+  __device__ float& L(int i, int c) { return Ls[i][c]; }
+  __device__ float& X(int i, int c) { return Xs[i][c]; }
+};
+template <> struct BlkStore<double> {
+  double* S;
+  __device__ static int pl(int i, int c) { return i * (i + 1) / 2 + c; }
+  __device__ double& L(int i, int c) { return S[pl(i, c)]; }
+  __device__ double& X(int i, int c) { return S[DB * (DB + 1) / 2 + pl(i, c)]; }
+};
+
+template <typename T, int PB>
+__device__ void potrf_blocked(BlkStore<T> B, T* A, int64_t lda, T* X, int64_t ldx, int32_t* info, int64_t gbase) {
+  const int tid = threadIdx.x, tx = tid & 31, ty = tid >> 5, lane = tid & 63, wave = tid >> 6;
+  for (int e = tid; e < DB * DB; e += 1024) {
+    const int i = e / DB, c = e % DB;
+    if (c <= i) { B.L(i, c) = A[i * lda + c]; B.X(i, c) = T(0); }
+  }
+  __syncthreads();
+  // ---- factorization
+  for (int jb = 0; jb < DB; jb += PB) {
+    if (wave == 0) {
+      const int ra = jb + lane, rb = jb + 64 + lane;
+      T a[PB], b[PB];
+#pragma unroll
+      for (int c = 0; c < PB; ++c) {
+        a[c] = (ra < DB && jb + c <= ra) ? B.L(ra, jb + c) : T(0);
+        b[c] = (rb < DB) ? B.L(rb, jb + c) : T(0);
+      }
+#pragma unroll
+      for (int j = 0; j < PB; ++j) {
+        T d = __shfl(a[j], j);                    // L[jb+j][jb+j], held by lane j
+        const bool bad = !(d > T(0));
+        if (bad) d = T(1);
+        const T piv = sqrt(d), ip = T(1) / piv;
+        if (bad && lane == 0) atomicCAS(info, 0, (int32_t)(gbase + jb + j + 1));
+        a[j] = lane == j ? piv : (lane > j ? a[j] * ip : a[j]);
+        b[j] *= ip;
+#pragma unroll
+        for (int c = j + 1; c < PB; ++c) {
+          const T lcj = __shfl(a[j], c);          // L[jb+c][jb+j]
+          if (lane > j) a[c] -= a[j] * lcj;
+          b[c] -= b[j] * lcj;
+        }
+      }
+#pragma unroll
+      for (int c = 0; c < PB; ++c) {
+        if (ra < DB && jb + c <= ra) B.L(ra, jb + c) = a[c];
+        if (rb < DB) B.L(rb, jb + c) = b[c];
+      }
+    }
+    __syncthreads();
+    const int c0 = jb + PB;
+    for (int i = c0 + ty; i < DB; i += 32) {
+      T li[PB];
+#pragma unroll
+      for (int t = 0; t < PB; ++t) li[t] = B.L(i, jb + t);
+      for (int c = c0 + tx; c <= i; c += 32) {
+        T sacc = T(0);
+#pragma unroll
+        for (int t = 0; t < PB; ++t) sacc = fma(li[t], B.L(c, jb + t), sacc);
+        B.L(i, c) -= sacc;
+      }
+    }
+    __syncthreads();
+  }
+  // ---- inverse by 8-row blocks: X[i][c] = (delta_ic - sum_{c<=k<i} L[i][k] X[k][c]) / L[i][i]
+  for (int ib = 0; ib < DB; ib += PB) {
+    // contributions of rows k < ib: X[i][c] := -sum_{c<=k<ib} L[i][k] X[k][c], c < ib
+    for (int e = tid; e < PB * ib; e += 1024) {
+      const int i = ib + e / ib, c = e % ib;
+      T sacc = T(0);
+      for (int k = c; k < ib; ++k) sacc = fma(B.L(i, k), B.X(k, c), sacc);
+      B.X(i, c) = -sacc;
+    }
+    __syncthreads();
+    if (wave == 0) {                          // the 8 rows, lane owns columns lane and lane + 64
+      T xa[PB], xb[PB];
+      const int ca = lane, cb = lane + 64;
+#pragma unroll
+      for (int r = 0; r < PB; ++r) {
+        const int i = ib + r;
+        T va = ca < ib ? B.X(i, ca) : (ca == i ? T(1) : T(0));
+        T vb = cb < ib ? B.X(i, cb) : (cb == i ? T(1) : T(0));
+#pragma unroll
+        for (int k = 0; k < r; ++k) {
+          const T lik = B.L(i, ib + k);
+          va -= lik * xa[k];
+          vb -= lik * xb[k];
+        }
+        const T ip = T(1) / B.L(i, i);
+        xa[r] = va * ip;
+        xb[r] = vb * ip;
+      }
+#pragma unroll
+      for (int r = 0; r < PB; ++r) {
+        const int i = ib + r;
+        if (ca <= i) B.X(i, ca) = xa[r];
+        if (cb <= i) B.X(i, cb) = xb[r];
+      }
+    }
+    __syncthreads();
+  }
+  for (int e = tid; e < DB * DB; e += 1024) {
+    const int i = e / DB, c = e % DB;
+    if (c <= i) A[i * lda + c] = B.L(i, c);
+    X[i * ldx + c] = (c <= i) ? B.X(i, c) : T(0);
+  }
+}
+
+__global__ __launch_bounds__(1024) void potrf128b_f32_kernel(float* A, int64_t lda, float* X, int64_t ldx,
+                                                             int32_t* info, int64_t gbase) {
+  __shared__ float Ls[DB][DB + 1];
+  __shared__ float Xs[DB][DB + 1];
+  potrf_blocked<float, GK_POTRF_PB_F32>(BlkStore<float>{Ls, Xs}, A, lda, X, ldx, info, gbase);
+}
+
+__global__ __launch_bounds__(1024) void potrf128b_f64_kernel(double* A, int64_t lda, double* X, int64_t ldx,
+                                                             int32_t* info, int64_t gbase) {
+  __shared__ double S[DB * (DB + 1)];
+  potrf_blocked<double, GK_POTRF_PB_F64>(BlkStore<double>{S}, A, lda, X, ldx, info, gbase);
+}
+
 template <typename T> struct Potrf128;
 template <> struct Potrf128<float> {
   static void launch(float* A, int64_t lda, float* X, int64_t ldx, int32_t* info, int64_t g, hipStream_t st) {
-    hipLaunchKernelGGL(potrf128_kernel, dim3(1), dim3(1024), 0, st, A, lda, X, ldx, info, g);
+    hipLaunchKernelGGL(potrf128b_f32_kernel, dim3(1), dim3(1024), 0, st, A, lda, X, ldx, info, g);
   }
 };
 template <> struct Potrf128<double> {
   static void launch(double* A, int64_t lda, double* X, int64_t ldx, int32_t* info, int64_t g, hipStream_t st) {
-    hipLaunchKernelGGL(potrf128_f64_kernel, dim3(1), dim3(1024), 0, st, A, lda, X, ldx, info, g);
+    hipLaunchKernelGGL(potrf128b_f64_kernel, dim3(1), dim3(1024), 0, st, A, lda, X, ldx, info, g);
   }
 };
 
