@@ -4,7 +4,8 @@
 // L is row-major (ld = lda, a multiple of 256); its 256 x 256 diagonal blocks
 // are not read.  The factorization leaves three planes of 256 x 256 blocks
 // (lda x 256 each, block k at row k0 = 256 k):
-//   plane 0: Dinv_k = L_kk^-1 (lower triangular)
+//   plane 0: Dinv_k = L_kk^-1 (lower triangular, zeros stored above the
+//            diagonal: the solves read whole blocks)
 //   plane 1: G_k = Dinv_k L_{k,k-1}         (k >= 1)
 //   plane 2: H_k = L_{k+1,k} Dinv_k         (k <= nb-2)
 // Forward, block row k:
@@ -14,13 +15,17 @@
 // L_jk^T x_j).  Only the chain step lies on the sequential path, and its block
 // G_k (H_k) is loaded into registers before the wait, so a link of the chain
 // costs one flag hand-off and a register-resident 256 x 256 product instead
-// of two HBM block reads by one CU (FP32; the FP64 variant loads after the
-// wait, its rows do not fit in registers).  The sums over far blocks are
+// of two HBM block reads by one CU (FP64: the first 128-column half).  The sums over far blocks are
 // split into "far chunks" whose size grows with the distance from the
 // diagonal (d in [2,3), [3,5), [5,9), then 8 blocks each); the c step adds
 // their published partials (the nearest one last).  A workgroup covers a
-// 256 x 256 block: FP32 8 waves of 32 rows (two waves per SIMD, so a wave can
-// keep its 32 prefetched rows in registers), FP64 16 waves of 16 rows.
+// 256 x 256 block with 8 waves of 32 rows (two waves per SIMD, so a wave can
+// keep its 32 prefetched rows in registers; FP64 takes a block in two
+// 128-column halves and prefetches the first).  Every item is one row-block (forward) or
+// column-block (backward) product over 1..8 blocks followed by a short
+// epilogue, so all item types share one loop: a CU runs only a few items per
+// launch, and a separate unrolled copy per item type would run from a cold
+// instruction cache on the sequential path.
 //
 // Ordering and progress: every workgroup takes a ticket (atomic counter) when it
 // starts; the ticket, not blockIdx, selects its work from a host-built table in
@@ -39,11 +44,14 @@ namespace {
 constexpr int NB = GK_BLOCK;            // 256
 constexpr unsigned SPIN_LIMIT = 1u << 21;
 
-// Rows of a 256-block per wave: FP32 32 (8 waves, 2 per SIMD, so a wave may
-// hold 32 prefetched rows = 128 VGPRs), FP64 16 (16 waves).
+// Rows of a 256-block per wave (RPW), waves per workgroup (NW), columns per
+// lane and pass (VW; a pass covers 64 VW columns): 8 waves of 32 rows, 2
+// waves per SIMD, so a wave may hold one pass of its rows (128 VGPRs) loaded
+// ahead of its waits: FP32 4 columns per lane (one pass per block), FP64 2
+// (two passes).
 template <typename T> struct Shape;
-template <> struct Shape<float> { static constexpr int RPW = 32, NW = 8, NT = 512, WPE = 2; };
-template <> struct Shape<double> { static constexpr int RPW = 16, NW = 16, NT = 1024, WPE = 4; };
+template <> struct Shape<float> { static constexpr int RPW = 32, NW = 8, NT = 512, WPE = 2, VW = 4; };
+template <> struct Shape<double> { static constexpr int RPW = 32, NW = 8, NT = 512, WPE = 2, VW = 2; };
 
 // distance range [lo, hi) of far chunk s
 inline void chunk_range(int s, int* lo, int* hi) {
@@ -53,9 +61,7 @@ inline void chunk_range(int s, int* lo, int* hi) {
   else { *lo = 9 + 8 * (s - 3); *hi = 17 + 8 * (s - 3); }
 }
 
-template <typename T> struct V4;
-template <> struct V4<float> { typedef float type __attribute__((ext_vector_type(4))); };
-template <> struct V4<double> { typedef double type __attribute__((ext_vector_type(4))); };
+template <typename T> struct Vec { typedef T type __attribute__((ext_vector_type(Shape<T>::VW))); };
 
 struct Item { int k, s, jlo, jhi; };    // s >= 0: far chunk s of row k; CHAIN / CSTEP: the row's steps
 constexpr int CHAIN = -1, CSTEP = -2;
@@ -100,20 +106,25 @@ __device__ __forceinline__ void group_publish(unsigned* flag, unsigned epoch) {
   if (threadIdx.x == 0) __hip_atomic_store(flag, epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// RPW rows of a wave, lane l holding columns 4l..4l+3 of each
+// RPW rows of a wave, lane l holding columns VW l .. VW l + VW - 1 of a pass
 template <typename T>
 struct Rows {
   static constexpr int R = Shape<T>::RPW;
-  typename V4<T>::type v[R];
+  typename Vec<T>::type v[R];
 };
 
 template <typename T>
-__device__ __forceinline__ void loadr(Rows<T>& R, const T* M, int64_t ld, int64_t r0, int64_t c0, int l) {
-  typedef typename V4<T>::type v4;
-  const T* base = M + r0 * ld + c0 + 4 * l;
+__device__ __forceinline__ void loadr(Rows<T>& R, const T* M, int64_t ld, int64_t r0, int l) {
+  typedef typename Vec<T>::type vec;
+  const T* ptr = M + r0 * ld + Shape<T>::VW * l;
 #pragma unroll
-  for (int r = 0; r < Rows<T>::R; ++r)
-    R.v[r] = __builtin_nontemporal_load(reinterpret_cast<const v4*>(base + r * ld));
+  for (int r = 0; r < Rows<T>::R; ++r) {
+    R.v[r] = __builtin_nontemporal_load(reinterpret_cast<const vec*>(ptr));
+    ptr += ld;
+    // FP64: each row address from the previous one (the 32 row addresses
+    // would otherwise be kept live beside the rows and the row partials)
+    if constexpr (sizeof(T) == 8) asm volatile("" : "+v"(ptr));
+  }
 }
 
 // Completes the outstanding loads into R before the caller's flag wait: the
@@ -134,78 +145,72 @@ __device__ __forceinline__ void pin1(T& v) {
   asm volatile("s_waitcnt vmcnt(0)" : "+v"(v) : : "memory");
 }
 
-// Row dots of 16 rows R.v[q..q+15]: lane l gets row q + (l & 15).
-template <typename T, bool MASK_UPPER>
-__device__ __forceinline__ T rowdot16(const Rows<T>& R, int q, typename V4<T>::type xv, int rloc0, int l) {
-  typedef typename V4<T>::type v4;
-  T acc[16];
-#pragma unroll
-  for (int r = 0; r < 16; ++r) {
-    v4 a = R.v[q + r];
-    if (MASK_UPPER) {
-      const int row = rloc0 + q + r, c = 4 * l;
-      a.x = c <= row ? a.x : T(0);
-      a.y = c + 1 <= row ? a.y : T(0);
-      a.z = c + 2 <= row ? a.z : T(0);
-      a.w = c + 3 <= row ? a.w : T(0);
-    }
-    T s = a.x * xv.x;
-    s = fma(a.y, xv.y, s);
-    s = fma(a.z, xv.z, s);
-    s = fma(a.w, xv.w, s);
-    acc[r] = s;
-  }
-#pragma unroll
-  for (int width = 8; width >= 1; width >>= 1) {        // transpose-reduce over lanes
-    const bool upper = (l & width) != 0;
-#pragma unroll
-    for (int i = 0; i < width; ++i) {
-      const T send = upper ? acc[i] : acc[i + width];
-      const T keep = upper ? acc[i + width] : acc[i];
-      acc[i] = keep + __shfl_xor(send, width);
-    }
-  }
-  T t = acc[0];
-  t += __shfl_xor(t, 16);
-  t += __shfl_xor(t, 32);
-  return t;
+// Sum over the 16 lanes of a DPP row, in every lane of the row: xor 1 and
+// xor 2 (quad permutes), then the mirrored quad within each half row, then
+// the mirrored half row.
+template <int CTRL>
+__device__ __forceinline__ float dpp_mov(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, true));
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp_mov(double v) {
+  const long long u = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(u & 0xffffffffll), CTRL, 0xF, 0xF, true);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(u >> 32), CTRL, 0xF, 0xF, true);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+template <typename T>
+__device__ __forceinline__ T row16_sum(T v) {
+  v += dpp_mov<0xB1>(v);      // quad_perm [1,0,3,2]
+  v += dpp_mov<0x4E>(v);      // quad_perm [2,3,0,1]
+  v += dpp_mov<0x141>(v);     // row_half_mirror
+  v += dpp_mov<0x140>(v);     // row_mirror
+  return v;
 }
 
-// Row dots: lane l < RPW returns sum_c R[l][c] x[c] (x: 256 entries);
-// MASK_UPPER keeps c <= rloc0 + row (the lower triangle of a diagonal block).
-template <typename T, bool MASK_UPPER>
-__device__ __forceinline__ T rowdot(const Rows<T>& R, const T* x, int rloc0, int l) {
-  typedef typename V4<T>::type v4;
-  const v4 xv = *reinterpret_cast<const v4*>(x + 4 * l);
-  T t = rowdot16<T, MASK_UPPER>(R, 0, xv, rloc0, l);
+// Lane partials p[r] (row r of the wave) summed over the 64 lanes: lane
+// l < RPW returns the sum of row l.
+template <typename T>
+__device__ __forceinline__ T lane_sums(const T (&p)[Shape<T>::RPW], int l) {
+  constexpr int RPW = Shape<T>::RPW;
+  T v1 = T(0), v2 = T(0);
+  const int i = l & 15;
 #pragma unroll
-  for (int q = 16; q < Rows<T>::R; q += 16) {
-    const T u = rowdot16<T, MASK_UPPER>(R, q, xv, rloc0, l);
-    if ((l >> 4) == q / 16) t = u;
+  for (int r = 0; r < RPW; ++r) {
+    const T s = row16_sum(p[r]);
+    if (r < 16) v1 = i == r ? s : v1;
+    else v2 = i == r - 16 ? s : v2;
   }
-  return t;
+  v1 += __shfl_xor(v1, 16);
+  v1 += __shfl_xor(v1, 32);
+  if constexpr (RPW > 16) {
+    v2 += __shfl_xor(v2, 16);
+    v2 += __shfl_xor(v2, 32);
+    return l < 16 ? v1 : v2;
+  }
+  return v1;
 }
 
-// Column dots of a wave: acc (columns 4l..4l+3) += sum_r R[r][.] x[r], x the
-// wave's RPW entries; MASK_LOWER keeps rloc0 + r >= column.
-template <typename T, bool MASK_LOWER>
-__device__ __forceinline__ void coldot(const Rows<T>& R, const T* x, int rloc0, int l, typename V4<T>::type& acc) {
-  typedef typename V4<T>::type v4;
+// p[r] += R[r][.] . x (the lane's VW columns)
+template <typename T>
+__device__ __forceinline__ void rowacc(const Rows<T>& R, typename Vec<T>::type xv, T (&p)[Shape<T>::RPW]) {
 #pragma unroll
   for (int r = 0; r < Rows<T>::R; ++r) {
-    v4 a = R.v[r];
+    T s = p[r];
+#pragma unroll
+    for (int e = 0; e < Shape<T>::VW; ++e) s = fma(R.v[r][e], xv[e], s);
+    p[r] = s;
+  }
+}
+
+// acc (the lane's VW columns) += sum_r R[r][.] x[r], x the wave's RPW entries
+template <typename T>
+__device__ __forceinline__ void colacc(const Rows<T>& R, const T* x, typename Vec<T>::type& acc) {
+#pragma unroll
+  for (int r = 0; r < Rows<T>::R; ++r) {
     const T xr = x[r];
-    if (MASK_LOWER) {
-      const int row = rloc0 + r, c = 4 * l;
-      a.x = row >= c ? a.x : T(0);
-      a.y = row >= c + 1 ? a.y : T(0);
-      a.z = row >= c + 2 ? a.z : T(0);
-      a.w = row >= c + 3 ? a.w : T(0);
-    }
-    acc.x = fma(a.x, xr, acc.x);
-    acc.y = fma(a.y, xr, acc.y);
-    acc.z = fma(a.z, xr, acc.z);
-    acc.w = fma(a.w, xr, acc.w);
+#pragma unroll
+    for (int e = 0; e < Shape<T>::VW; ++e) acc[e] = fma(R.v[r][e], xr, acc[e]);
   }
 }
 
@@ -235,50 +240,47 @@ __device__ __forceinline__ bool take_ticket(const TrsvCtl& ctl, Item* it, int* s
   return true;
 }
 
-// PF: load the step's matrix rows before its waits (FP32)
+// Forward items: row-block products sum_j M_j x_j; the lanes keep per-row
+// partials over all blocks and passes and reduce once.  PF: the first pass
+// of the first block is loaded into registers before the item's waits.
 template <typename T, bool PF>
 __global__ __launch_bounds__(Shape<T>::NT) __attribute__((amdgpu_waves_per_eu(Shape<T>::WPE, Shape<T>::WPE)))
 void trsv_fwd_kernel(const T* __restrict__ L, int64_t lda, const T* __restrict__ planes, T* b, T* cbuf, T* part,
                      TrsvCtl ctl) {
   __shared__ int s_id, s_ok, s_ok2;
-  __shared__ __attribute__((aligned(32))) T tv[NB];
+  typedef typename Vec<T>::type vec;
+  __shared__ __attribute__((aligned(32))) T xs[NB];
   const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
-  constexpr int RPW = Shape<T>::RPW, NW = Shape<T>::NW, NT = Shape<T>::NT;
+  constexpr int RPW = Shape<T>::RPW, NT = Shape<T>::NT, PW = 64 * Shape<T>::VW;   // PW: columns per pass
   const int row = RPW * w + (l & (RPW - 1));     // row of the 256-block reduced into lanes l < RPW
   Item it;
   if (!take_ticket<T>(ctl, &it, &s_id)) return;
   const int64_t k0 = (int64_t)it.k * NB;
-  const T* Dinv = planes + k0 * NB;
-  const T* G = planes + (int64_t)ctl.nb * NB * NB + k0 * NB;
-  Rows<T> R;
-  if (it.s >= 0) {                                            // far chunk: sum_j L_kj y_j
-    const int slot = ctl.rowbase[it.k] + it.s;
-    const bool pre = PF && it.jhi - it.jlo == 1;
-    if constexpr (PF) {
-      if (pre) { loadr(R, L, lda, k0 + RPW * w, (int64_t)it.jlo * NB, l); pin(R); }
-    }
-    if (group_wait(ctl.yflag + it.jhi - 1, ctl.epoch, ctl.err, &s_ok)) {
-      T sum = T(0);
-      if (pre) {
-        sum = rowdot<T, false>(R, b + (int64_t)it.jlo * NB, 0, l);
-      } else {
-        #pragma unroll 1
-        for (int j = it.jlo; j < it.jhi; ++j) {
-          loadr(R, L, lda, k0 + RPW * w, (int64_t)j * NB, l);
-          sum += rowdot<T, false>(R, b + (int64_t)j * NB, 0, l);
-        }
-      }
-      if (l < RPW) part[(int64_t)slot * NB + row] = sum;
-    }
-    group_publish(ctl.pflag + slot, ctl.epoch);
-    return;
+  // far chunk: sum_{j in [jlo, jhi)} L_kj y_j;  c step: Dinv_k t (t staged in LDS);
+  // chain: G_k y_{k-1}
+  const T* M;
+  int64_t ld;
+  int nblk;
+  const T* x = nullptr;
+  if (it.s >= 0) {
+    M = L + k0 * lda + (int64_t)it.jlo * NB; ld = lda; nblk = it.jhi - it.jlo; x = b + (int64_t)it.jlo * NB;
+  } else if (it.s == CSTEP) {
+    M = planes + k0 * NB; ld = NB; nblk = 1;
+  } else {
+    M = planes + (int64_t)ctl.nb * NB * NB + k0 * NB; ld = NB; nblk = it.k > 0 ? 1 : 0; x = b + k0 - NB;
   }
-  if (it.s == CSTEP) {                                        // c_k = Dinv_k (b_k - far partials)
-    if constexpr (PF) { loadr(R, Dinv, NB, RPW * w, 0, l); pin(R); }
+  Rows<T> R;
+  if constexpr (PF) {
+    if (nblk > 0) { loadr(R, M, ld, RPW * w, l); pin(R); }
+  }
+  bool ok = true;
+  T y = T(0);
+  if (it.s >= 0) {
+    ok = group_wait(ctl.yflag + it.jhi - 1, ctl.epoch, ctl.err, &s_ok);
+  } else if (it.s == CSTEP) {                                 // t = b_k - far partials (the nearest last)
     const int f0 = ctl.rowbase[it.k], nfar = ctl.rowbase[it.k + 1] - f0;
     T t = b[k0 + row];
-    bool ok = true;
-    if (nfar > 1) {                                           // the far partials first, the nearest last
+    if (nfar > 1) {
       ok = group_wait_all(ctl.pflag + f0 + 1, nfar - 1, ctl.epoch, ctl.err, &s_ok2);
       if (ok)
         for (int s = 1; s < nfar; ++s) t -= part[(int64_t)(f0 + s) * NB + row];
@@ -287,86 +289,87 @@ void trsv_fwd_kernel(const T* __restrict__ L, int64_t lda, const T* __restrict__
       ok = group_wait(ctl.pflag + f0, ctl.epoch, ctl.err, &s_ok);
       if (ok) t -= part[(int64_t)f0 * NB + row];
     }
-    if (ok) {
-      if (l < RPW) tv[row] = t;
-      __syncthreads();
-      if (!PF) loadr(R, Dinv, NB, RPW * w, 0, l);
-      const T c = rowdot<T, true>(R, tv, RPW * w, l);
-      if (l < RPW) cbuf[k0 + row] = c;
-    }
-    group_publish(ctl.cflag + it.k, ctl.epoch);
-    return;
+    if (l < RPW) xs[row] = t;
+  } else {
+    ok = group_wait(ctl.cflag + it.k, ctl.epoch, ctl.err, &s_ok);
+    if (ok) { y = cbuf[k0 + row]; pin1(y); }           // c_k lands before the wait for y_{k-1}
+    if (ok && it.k > 0) ok = group_wait(ctl.yflag + it.k - 1, ctl.epoch, ctl.err, &s_ok2);
   }
-  // chain step: y_k = c_k - G_k y_{k-1}
-  if constexpr (PF) {
-    if (it.k > 0) { loadr(R, G, NB, RPW * w, 0, l); pin(R); }
-  }
-  bool ok = group_wait(ctl.cflag + it.k, ctl.epoch, ctl.err, &s_ok);
-  T y = T(0);
-  if (ok) { y = cbuf[k0 + row]; pin1(y); }         // c_k lands before the wait for y_{k-1}
-  if (ok && it.k > 0) ok = group_wait(ctl.yflag + it.k - 1, ctl.epoch, ctl.err, &s_ok2);
+  T sum = T(0);
   if (ok) {
-    if (it.k > 0) {
-      if (!PF) loadr(R, G, NB, RPW * w, 0, l);
-      y -= rowdot<T, false>(R, b + k0 - NB, 0, l);
+    T p[RPW];
+#pragma unroll
+    for (int r = 0; r < RPW; ++r) p[r] = T(0);
+#pragma unroll 1
+    for (int j = 0; j < nblk; ++j) {
+      if (!PF || j > 0) loadr(R, M + (int64_t)j * NB, ld, RPW * w, l);
+      if (x) {
+        for (int e = threadIdx.x; e < NB; e += NT) xs[e] = x[(int64_t)j * NB + e];
+      }
+      __syncthreads();
+      rowacc<T>(R, *reinterpret_cast<const vec*>(xs + Shape<T>::VW * l), p);
+#pragma unroll
+      for (int h = 1; h < NB / PW; ++h) {
+        asm volatile("" ::: "memory");           // one pass of rows live at a time
+        loadr(R, M + (int64_t)j * NB + h * PW, ld, RPW * w, l);
+        rowacc<T>(R, *reinterpret_cast<const vec*>(xs + h * PW + Shape<T>::VW * l), p);
+      }
+      __syncthreads();
     }
-    if (l < RPW) b[k0 + row] = y;
+    sum = lane_sums<T>(p, l);
   }
-  group_publish(ctl.yflag + it.k, ctl.epoch);
+  if (it.s >= 0) {
+    if (ok && l < RPW) part[(int64_t)(ctl.rowbase[it.k] + it.s) * NB + row] = sum;
+    group_publish(ctl.pflag + ctl.rowbase[it.k] + it.s, ctl.epoch);
+  } else if (it.s == CSTEP) {
+    if (ok && l < RPW) cbuf[k0 + row] = sum;
+    group_publish(ctl.cflag + it.k, ctl.epoch);
+  } else {
+    if (ok && l < RPW) b[k0 + row] = y - sum;
+    group_publish(ctl.yflag + it.k, ctl.epoch);
+  }
 }
 
+// Backward items are column-block products sum_j M_j^T x_j sharing one loop,
+// as in the forward kernel.
 template <typename T, bool PF>
 __global__ __launch_bounds__(Shape<T>::NT) __attribute__((amdgpu_waves_per_eu(Shape<T>::WPE, Shape<T>::WPE)))
 void trsv_bwd_kernel(const T* __restrict__ L, int64_t lda, const T* __restrict__ planes, T* b, T* cbuf, T* part,
                      TrsvCtl ctl) {
-  typedef typename V4<T>::type v4;
+  typedef typename Vec<T>::type vec;
   __shared__ int s_id, s_ok, s_ok2;
-  __shared__ __attribute__((aligned(32))) T tv[NB];
-  constexpr int RPW = Shape<T>::RPW, NW = Shape<T>::NW, NT = Shape<T>::NT;
+  constexpr int RPW = Shape<T>::RPW, NW = Shape<T>::NW, NT = Shape<T>::NT, VW = Shape<T>::VW, PW = 64 * VW;
+  __shared__ __attribute__((aligned(32))) T xs[NB];
   __shared__ __attribute__((aligned(32))) T red[NW][NB];
   const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
   Item it;
   if (!take_ticket<T>(ctl, &it, &s_id)) return;
   const int64_t k0 = (int64_t)it.k * NB;
-  const T* Dinv = planes + k0 * NB;
-  const T* H = planes + 2 * (int64_t)ctl.nb * NB * NB + k0 * NB;
-  // column c = threadIdx.x (< 256) gets the sum of the 16 waves' partials
-  auto reduce_waves = [&](v4 a) -> T {
-    *reinterpret_cast<v4*>(&red[w][4 * l]) = a;
-    __syncthreads();
-    T s = T(0);
-    if (threadIdx.x < NB) {
-#pragma unroll
-      for (int q = 0; q < NW; ++q) s += red[q][threadIdx.x];
-    }
-    __syncthreads();
-    return s;
-  };
-  Rows<T> R;
-  v4 acc = {T(0), T(0), T(0), T(0)};
-  if (it.s >= 0) {                                            // far chunk: sum_j L_jk^T x_j
-    const int slot = ctl.rowbase[it.k] + it.s;
-    const bool pre = PF && it.jhi - it.jlo == 1;
-    if constexpr (PF) {
-      if (pre) { loadr(R, L, lda, (int64_t)it.jlo * NB + RPW * w, k0, l); pin(R); }
-    }
-    if (group_wait(ctl.yflag + it.jlo, ctl.epoch, ctl.err, &s_ok)) {
-      #pragma unroll 1
-      for (int j = it.jlo; j < it.jhi; ++j) {
-        if (!pre) loadr(R, L, lda, (int64_t)j * NB + RPW * w, k0, l);
-        coldot<T, false>(R, b + (int64_t)j * NB + RPW * w, 0, l, acc);
-      }
-      const T sum = reduce_waves(acc);
-      if (threadIdx.x < NB) part[(int64_t)slot * NB + threadIdx.x] = sum;
-    }
-    group_publish(ctl.pflag + slot, ctl.epoch);
-    return;
+  // far chunk: sum_{j in [jlo, jhi)} L_jk^T x_j;  c step: Dinv_k^T t (t staged in LDS);
+  // chain: H_k^T x_{k+1}
+  const T* M;
+  int64_t ld;
+  int nblk;
+  const T* x = nullptr;
+  const bool last = it.k + 1 >= ctl.nb;
+  if (it.s >= 0) {
+    M = L + (int64_t)it.jlo * NB * lda + k0; ld = lda; nblk = it.jhi - it.jlo; x = b + (int64_t)it.jlo * NB;
+  } else if (it.s == CSTEP) {
+    M = planes + k0 * NB; ld = NB; nblk = 1;
+  } else {
+    M = planes + 2 * (int64_t)ctl.nb * NB * NB + k0 * NB; ld = NB; nblk = last ? 0 : 1; x = b + k0 + NB;
   }
-  if (it.s == CSTEP) {                                        // c'_k = Dinv_k^T (y_k - far partials)
-    if constexpr (PF) { loadr(R, Dinv, NB, RPW * w, 0, l); pin(R); }
+  Rows<T> R;
+  if constexpr (PF) {
+    if (nblk > 0) { loadr(R, M, ld, RPW * w, l); pin(R); }
+  }
+  bool ok = true;
+  T cv = T(0);
+  if (it.s >= 0) {
+    ok = group_wait(ctl.yflag + it.jlo, ctl.epoch, ctl.err, &s_ok);
+  } else if (it.s == CSTEP) {                                 // t = y_k - far partials (the nearest last)
     const int f0 = ctl.rowbase[it.k], nfar = ctl.rowbase[it.k + 1] - f0;
     T t = threadIdx.x < NB ? b[k0 + threadIdx.x] : T(0);
-    bool ok = true;
     if (nfar > 1) {
       ok = group_wait_all(ctl.pflag + f0 + 1, nfar - 1, ctl.epoch, ctl.err, &s_ok2);
       if (ok && threadIdx.x < NB)
@@ -376,36 +379,54 @@ void trsv_bwd_kernel(const T* __restrict__ L, int64_t lda, const T* __restrict__
       ok = group_wait(ctl.pflag + f0, ctl.epoch, ctl.err, &s_ok);
       if (ok && threadIdx.x < NB) t -= part[(int64_t)f0 * NB + threadIdx.x];
     }
-    if (ok) {
-      if (threadIdx.x < NB) tv[threadIdx.x] = t;
-      __syncthreads();
-      if (!PF) loadr(R, Dinv, NB, RPW * w, 0, l);
-      coldot<T, true>(R, tv + RPW * w, RPW * w, l, acc);
-      const T c = reduce_waves(acc);
-      if (threadIdx.x < NB) cbuf[k0 + threadIdx.x] = c;
-    }
-    group_publish(ctl.cflag + it.k, ctl.epoch);
-    return;
+    if (threadIdx.x < NB) xs[threadIdx.x] = t;
+  } else {
+    ok = group_wait(ctl.cflag + it.k, ctl.epoch, ctl.err, &s_ok);
+    if (ok && threadIdx.x < NB) { cv = cbuf[k0 + threadIdx.x]; pin1(cv); }
+    if (ok && !last) ok = group_wait(ctl.yflag + it.k + 1, ctl.epoch, ctl.err, &s_ok2);
   }
-  // chain step: x_k = c'_k - H_k^T x_{k+1}
-  const bool last = it.k + 1 >= ctl.nb;
-  if constexpr (PF) {
-    if (!last) { loadr(R, H, NB, RPW * w, 0, l); pin(R); }
-  }
-  bool ok = group_wait(ctl.cflag + it.k, ctl.epoch, ctl.err, &s_ok);
-  T cv = T(0);
-  if (ok && threadIdx.x < NB) { cv = cbuf[k0 + threadIdx.x]; pin1(cv); }
-  if (ok && !last) ok = group_wait(ctl.yflag + it.k + 1, ctl.epoch, ctl.err, &s_ok2);
+  vec acc[NB / PW];
+#pragma unroll
+  for (int h = 0; h < NB / PW; ++h)
+#pragma unroll
+    for (int e = 0; e < VW; ++e) acc[h][e] = T(0);
+  T sum = T(0);
   if (ok) {
-    T x = T(0);
-    if (!last) {
-      if (!PF) loadr(R, H, NB, RPW * w, 0, l);
-      coldot<T, false>(R, b + k0 + NB + RPW * w, 0, l, acc);
-      x = reduce_waves(acc);
+#pragma unroll 1
+    for (int j = 0; j < nblk; ++j) {
+      if (!PF || j > 0) loadr(R, M + (int64_t)j * NB * ld, ld, RPW * w, l);
+      if (x) {
+        for (int e = threadIdx.x; e < NB; e += NT) xs[e] = x[(int64_t)j * NB + e];
+      }
+      __syncthreads();
+      colacc<T>(R, xs + RPW * w, acc[0]);
+#pragma unroll
+      for (int h = 1; h < NB / PW; ++h) {
+        asm volatile("" ::: "memory");           // one pass of rows live at a time
+        loadr(R, M + (int64_t)j * NB * ld + h * PW, ld, RPW * w, l);
+        colacc<T>(R, xs + RPW * w, acc[h]);
+      }
+      __syncthreads();
     }
-    if (threadIdx.x < NB) b[k0 + threadIdx.x] = cv - x;
+    // column c = threadIdx.x (< 256) gets the sum of the waves' partials
+#pragma unroll
+    for (int h = 0; h < NB / PW; ++h) *reinterpret_cast<vec*>(&red[w][h * PW + VW * l]) = acc[h];
+    __syncthreads();
+    if (threadIdx.x < NB) {
+#pragma unroll
+      for (int q = 0; q < NW; ++q) sum += red[q][threadIdx.x];
+    }
   }
-  group_publish(ctl.yflag + it.k, ctl.epoch);
+  if (it.s >= 0) {
+    if (ok && threadIdx.x < NB) part[(int64_t)(ctl.rowbase[it.k] + it.s) * NB + threadIdx.x] = sum;
+    group_publish(ctl.pflag + ctl.rowbase[it.k] + it.s, ctl.epoch);
+  } else if (it.s == CSTEP) {
+    if (ok && threadIdx.x < NB) cbuf[k0 + threadIdx.x] = sum;
+    group_publish(ctl.cflag + it.k, ctl.epoch);
+  } else {
+    if (ok && threadIdx.x < NB) b[k0 + threadIdx.x] = cv - sum;
+    group_publish(ctl.yflag + it.k, ctl.epoch);
+  }
 }
 
 // Host-side work tables.  Forward: c steps of rows 0 and 1, then for each block
@@ -517,7 +538,7 @@ static int trsv_prepare(gk_handle h, int nb, hipStream_t st, char** base, TrsvLa
 template <typename T>
 int gk_trsv_chol_impl(gk_handle h, const T* L, const T* planes, int64_t lda, T* b, hipStream_t st) {
   GK_REQUIRE(h && L && planes && b && lda > 0 && lda % NB == 0, GK_ERR_ARG, "triangular solve: bad arguments");
-  constexpr bool PF = sizeof(T) == 4;          // FP32 block rows fit in registers
+  constexpr bool PF = true;                     // first pass of the first block before the waits
   const int nb = (int)(lda / NB);
   char* base = nullptr;
   TrsvLayout lay;

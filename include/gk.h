@@ -137,8 +137,9 @@ int gk_matvec(gk_handle h, const gk_system* sys, int32_t precision, const double
 /* In-place lower Cholesky of an lda x lda FP32 matrix (lda multiple of GK_BLOCK;
  * rows/cols >= n are reset to the identity).  dinv [3*lda*GK_BLOCK] receives
  * three planes of GK_BLOCK-square blocks (block k at row k*GK_BLOCK of each
- * plane): the inverses Dinv_k of the diagonal blocks, G_k = Dinv_k L_{k,k-1}
- * and H_k = L_{k+1,k} Dinv_k (used by the solves).  info = 0 or the first
+ * plane): the inverses Dinv_k of the diagonal blocks (lower triangular, with
+ * zeros stored above the diagonal: the solves read whole blocks), G_k =
+ * Dinv_k L_{k,k-1} and H_k = L_{k+1,k} Dinv_k (used by the solves).  info = 0 or the first
  * failing block+1. */
 int gk_cholesky_f32(gk_handle h, float* A, int64_t n, int64_t lda, float* dinv, int32_t* info,
                     gk_stream stream);

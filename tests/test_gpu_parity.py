@@ -125,7 +125,7 @@ def test_triangular_solve_many_blocks(gpu, dtype):
     dinv = np.zeros((3, lda, 256))
     for k in range(nb):
         blk = slice(256 * k, 256 * (k + 1))
-        dinv[0, blk] = np.linalg.inv(Lm[blk, blk])
+        dinv[0, blk] = np.tril(np.linalg.inv(Lm[blk, blk]))     # zeros above the diagonal, as stored
     for k in range(nb):
         blk, prv, nxt = slice(256 * k, 256 * (k + 1)), slice(256 * (k - 1), 256 * k), slice(256 * (k + 1), 256 * (k + 2))
         if k > 0:
