@@ -67,22 +67,36 @@ template <> struct Mfma<double> {
 };
 
 // ------------------------------------------------------------------ GEMM
-// Requirements (checked on the host): m, n multiples of 128, k multiple of BK,
+// Block tile TM x TM, WM x WN waves, each wave FM x FN MFMA tiles of 16 x 16;
+// the instantiated shape is 128 x 128 with 4 waves of 64 x 64.  Measured on
+// MI355X for FP32 rank-256 updates (m = 34816): 83.5 TF/s; 256 x 256 tiles
+// with 8 waves of 128 x 64 or 64 x 128 ran 71-72 TF/s and with 4 waves of
+// 128 x 128 (accumulators in AGPRs) 49-53 TF/s -- with one workgroup per CU
+// the stage barriers are not covered by other workgroups' MFMAs.
+template <int TM_, int WM_, int WN_>
+struct Tile {
+  static constexpr int TM = TM_, WM = WM_, WN = WN_, NT = 64 * WM * WN;
+  static constexpr int FM = TM / WM / 16, FN = TM / WN / 16;
+};
+typedef Tile<128, 2, 2> SmallTile;
+
+// Requirements (checked on the host): m, n multiples of TM, k multiple of BK,
 // leading dimensions multiples of 4 (16-byte rows), 16-byte aligned bases.
-template <typename T, bool TRANS_B, bool LOWER>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(Mfma<T>::WPE, 0))) void gemm_kernel(int64_t m, int64_t n, int64_t k, const T* __restrict__ A,
+template <typename T, bool TRANS_B, bool LOWER, typename TL, int WPE>
+__global__ __launch_bounds__(TL::NT) __attribute__((amdgpu_waves_per_eu(WPE, 0))) void gemm_kernel(int64_t m, int64_t n, int64_t k, const T* __restrict__ A,
                                                    int64_t lda, const T* __restrict__ B, int64_t ldb,
                                                    T alpha, T beta, T* C, int64_t ldc, int64_t tiles_n,
                                                    int64_t n_tiles) {
   using M = Mfma<T>;
+  constexpr int TM = TL::TM, NT = TL::NT, FM = TL::FM, FN = TL::FN;
   constexpr int BK = M::BK;
   constexpr int KS = BK / 4;              // k values per lane per stage
   constexpr int PAD = 16 / sizeof(T);     // keep rows 16-byte aligned
   constexpr int LDS_LD = BK + PAD;
   constexpr int VEC = 16 / sizeof(T);     // elements per 16-byte vector
   typedef T vec_t __attribute__((ext_vector_type(VEC)));
-  __shared__ __attribute__((aligned(16))) T As[2][128 * LDS_LD];
-  __shared__ __attribute__((aligned(16))) T Bs[2][128 * LDS_LD];
+  __shared__ __attribute__((aligned(16))) T As[2][TM * LDS_LD];
+  __shared__ __attribute__((aligned(16))) T Bs[2][TM * LDS_LD];
 
   // XCD-aware tile order: consecutive hardware block ids go to different
   // XCDs, so give each XCD a contiguous range of logical tiles.
@@ -100,43 +114,43 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(Mfma<T>::WP
     tn = bid % tiles_n;
   }
   (void)n_tiles;
-  const int64_t m0 = tm * 128, n0 = tn * 128;
+  const int64_t m0 = tm * TM, n0 = tn * TM;
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
-  const int wm = wave >> 1, wn = wave & 1;
+  const int wm = wave / TL::WN, wn = wave % TL::WN;
   const int lrow = lane & 15, lk = lane >> 4;
 
-  typename M::acc_t acc[4][4];
+  typename M::acc_t acc[FM][FN];
 #pragma unroll
-  for (int i = 0; i < 4; ++i)
+  for (int i = 0; i < FM; ++i)
 #pragma unroll
-    for (int j = 0; j < 4; ++j) acc[i][j] = typename M::acc_t{0, 0, 0, 0};
+    for (int j = 0; j < FN; ++j) acc[i][j] = typename M::acc_t{0, 0, 0, 0};
 
-  // global -> register staging: 128 x BK tile = 128*BK/VEC vectors, NV per thread
+  // global -> register staging: TM x BK tile = TM*BK/VEC vectors, NV per thread
   constexpr int VPR = BK / VEC;           // vectors per row
-  constexpr int NV = 128 * BK / VEC / 256;
-  static_assert(NV >= 1 && NV * 256 * VEC == 128 * BK, "BK must make the tile a whole number of vectors per thread");
+  constexpr int NV = TM * BK / VEC / NT;
+  static_assert(NV >= 1 && NV * NT * VEC == TM * BK, "BK must make the tile a whole number of vectors per thread");
   constexpr int RS = sizeof(T) == 4 ? GK_GEMM_RS_F32 : 1;
   vec_t ra[RS][NV], rb[RS][NV];
   auto load_global = [&](int64_t k0, vec_t (&xa)[NV], vec_t (&xb)[NV]) {
 #pragma unroll
     for (int it = 0; it < NV; ++it) {
-      const int idx = tid + 256 * it;
+      const int idx = tid + NT * it;
       const int row = idx / VPR, cv = idx % VPR;
       xa[it] = *reinterpret_cast<const vec_t*>(A + (m0 + row) * lda + k0 + cv * VEC);
     }
     if (TRANS_B) {
 #pragma unroll
       for (int it = 0; it < NV; ++it) {
-        const int idx = tid + 256 * it;
+        const int idx = tid + NT * it;
         const int row = idx / VPR, cv = idx % VPR;
         xb[it] = *reinterpret_cast<const vec_t*>(B + (n0 + row) * ldb + k0 + cv * VEC);
       }
     } else {
-      constexpr int VPRB = 128 / VEC;     // vectors per k-row of the B tile
+      constexpr int VPRB = TM / VEC;      // vectors per k-row of the B tile
 #pragma unroll
       for (int it = 0; it < NV; ++it) {
-        const int idx = tid + 256 * it;
+        const int idx = tid + NT * it;
         const int kk = idx / VPRB, cv = idx % VPRB;
         xb[it] = *reinterpret_cast<const vec_t*>(B + (k0 + kk) * ldb + n0 + cv * VEC);
       }
@@ -145,22 +159,22 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(Mfma<T>::WP
   auto store_lds = [&](int buf, const vec_t (&xa)[NV], const vec_t (&xb)[NV]) {
 #pragma unroll
     for (int it = 0; it < NV; ++it) {
-      const int idx = tid + 256 * it;
+      const int idx = tid + NT * it;
       const int row = idx / VPR, cv = idx % VPR;
       *reinterpret_cast<vec_t*>(&As[buf][row * LDS_LD + cv * VEC]) = xa[it];
     }
     if (TRANS_B) {
 #pragma unroll
       for (int it = 0; it < NV; ++it) {
-        const int idx = tid + 256 * it;
+        const int idx = tid + NT * it;
         const int row = idx / VPR, cv = idx % VPR;
         *reinterpret_cast<vec_t*>(&Bs[buf][row * LDS_LD + cv * VEC]) = xb[it];
       }
     } else {
-      constexpr int VPRB = 128 / VEC;
+      constexpr int VPRB = TM / VEC;
 #pragma unroll
       for (int it = 0; it < NV; ++it) {
-        const int idx = tid + 256 * it;
+        const int idx = tid + NT * it;
         const int kk = idx / VPRB, cv = idx % VPRB;
 #pragma unroll
         for (int e = 0; e < VEC; ++e) Bs[buf][(cv * VEC + e) * LDS_LD + kk] = xb[it][e];
@@ -171,39 +185,48 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(Mfma<T>::WP
     if (GK_GEMM_CHUNKED) {        // fragments one 16-byte vector of k at a time
 #pragma unroll
       for (int v = 0; v < KS / VEC; ++v) {
-        vec_t xa[4], xb[4];
+        vec_t xa[FM], xb[FN];
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-          xa[i] = *reinterpret_cast<const vec_t*>(&As[buf][(wm * 64 + i * 16 + lrow) * LDS_LD + lk * KS + v * VEC]);
-          xb[i] = *reinterpret_cast<const vec_t*>(&Bs[buf][(wn * 64 + i * 16 + lrow) * LDS_LD + lk * KS + v * VEC]);
-        }
+        for (int i = 0; i < FM; ++i)
+          xa[i] = *reinterpret_cast<const vec_t*>(&As[buf][(wm * FM * 16 + i * 16 + lrow) * LDS_LD + lk * KS + v * VEC]);
+#pragma unroll
+        for (int j = 0; j < FN; ++j)
+          xb[j] = *reinterpret_cast<const vec_t*>(&Bs[buf][(wn * FN * 16 + j * 16 + lrow) * LDS_LD + lk * KS + v * VEC]);
 #pragma unroll
         for (int e = 0; e < VEC; ++e)
 #pragma unroll
-          for (int i = 0; i < 4; ++i)
+          for (int i = 0; i < FM; ++i)
 #pragma unroll
-            for (int j = 0; j < 4; ++j) acc[i][j] = M::mma(xa[i][e], xb[j][e], acc[i][j]);
+            for (int j = 0; j < FN; ++j) acc[i][j] = M::mma(xa[i][e], xb[j][e], acc[i][j]);
       }
     } else {                      // all KS k values of the stage at once
-      T af[4][KS], bf[4][KS];
+      T af[FM][KS], bf[FN][KS];
 #pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        const T* pa = &As[buf][(wm * 64 + i * 16 + lrow) * LDS_LD + lk * KS];
-        const T* pb = &Bs[buf][(wn * 64 + i * 16 + lrow) * LDS_LD + lk * KS];
+      for (int i = 0; i < FM; ++i) {
+        const T* pa = &As[buf][(wm * FM * 16 + i * 16 + lrow) * LDS_LD + lk * KS];
 #pragma unroll
         for (int v = 0; v < KS / VEC; ++v) {
           vec_t x = *reinterpret_cast<const vec_t*>(pa + v * VEC);
+#pragma unroll
+          for (int e = 0; e < VEC; ++e) af[i][v * VEC + e] = x[e];
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < FN; ++j) {
+        const T* pb = &Bs[buf][(wn * FN * 16 + j * 16 + lrow) * LDS_LD + lk * KS];
+#pragma unroll
+        for (int v = 0; v < KS / VEC; ++v) {
           vec_t y = *reinterpret_cast<const vec_t*>(pb + v * VEC);
 #pragma unroll
-          for (int e = 0; e < VEC; ++e) { af[i][v * VEC + e] = x[e]; bf[i][v * VEC + e] = y[e]; }
+          for (int e = 0; e < VEC; ++e) bf[j][v * VEC + e] = y[e];
         }
       }
 #pragma unroll
       for (int s = 0; s < KS; ++s)
 #pragma unroll
-        for (int i = 0; i < 4; ++i)
+        for (int i = 0; i < FM; ++i)
 #pragma unroll
-          for (int j = 0; j < 4; ++j) acc[i][j] = M::mma(af[i][s], bf[j][s], acc[i][j]);
+          for (int j = 0; j < FN; ++j) acc[i][j] = M::mma(af[i][s], bf[j][s], acc[i][j]);
     }
   };
 
@@ -243,18 +266,32 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(Mfma<T>::WP
   // 4*(lane/16)+r for the f32 instruction and lane/16+4*r for the f64 one
   // (both checked with an in-bounds lane-layout probe on gfx950).
 #pragma unroll
-  for (int i = 0; i < 4; ++i)
+  for (int i = 0; i < FM; ++i)
 #pragma unroll
-    for (int j = 0; j < 4; ++j)
+    for (int j = 0; j < FN; ++j)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int rr = sizeof(T) == 8 ? lk + 4 * r : lk * 4 + r;
-        const int64_t row = m0 + wm * 64 + i * 16 + rr;
-        const int64_t col = n0 + wn * 64 + j * 16 + lrow;
+        const int64_t row = m0 + wm * FM * 16 + i * 16 + rr;
+        const int64_t col = n0 + wn * FN * 16 + j * 16 + lrow;
         T* pc = C + row * ldc + col;
         const T prod = acc[i][j][r];
         *pc = (beta == T(0)) ? alpha * prod : fma(alpha, prod, beta * *pc);
       }
+}
+
+template <typename T, typename TL, int WPE>
+void gemm_dispatch(dim3 grid, bool trans_b, bool lower, hipStream_t st, int64_t m, int64_t n, int64_t k, const T* A,
+                   int64_t lda, const T* B, int64_t ldb, T alpha, T beta, T* C, int64_t ldc, int64_t tn,
+                   int64_t tiles) {
+  dim3 block(TL::NT);
+  if (trans_b) {
+    if (lower) hipLaunchKernelGGL((gemm_kernel<T, true, true, TL, WPE>), grid, block, 0, st, m, n, k, A, lda, B, ldb, alpha, beta, C, ldc, tn, tiles);
+    else hipLaunchKernelGGL((gemm_kernel<T, true, false, TL, WPE>), grid, block, 0, st, m, n, k, A, lda, B, ldb, alpha, beta, C, ldc, tn, tiles);
+  } else {
+    if (lower) hipLaunchKernelGGL((gemm_kernel<T, false, true, TL, WPE>), grid, block, 0, st, m, n, k, A, lda, B, ldb, alpha, beta, C, ldc, tn, tiles);
+    else hipLaunchKernelGGL((gemm_kernel<T, false, false, TL, WPE>), grid, block, 0, st, m, n, k, A, lda, B, ldb, alpha, beta, C, ldc, tn, tiles);
+  }
 }
 
 template <typename T>
@@ -267,22 +304,16 @@ int gemm_launch(int64_t m, int64_t n, int64_t k, const T* A, int64_t lda, const 
   GK_REQUIRE(lda % 4 == 0 && ldb % 4 == 0 && ldc >= n, GK_ERR_ARG, "gk_gemm: bad leading dimensions");
   GK_REQUIRE(!lower || m == n, GK_ERR_ARG, "gk_gemm: lower_only needs m == n");
   if (m == 0 || n == 0) return GK_OK;
-  const int64_t tm = m / 128, tn = n / 128;
-  const int64_t tiles = lower ? tm * (tm + 1) / 2 : tm * tn;
   if (k == 0) {
     GK_REQUIRE(beta == T(1), GK_ERR_ARG, "gk_gemm: k == 0 only with beta == 1");
     return GK_OK;
   }
-  dim3 grid((unsigned)tiles), block(256);
   hipEvent_t pa;
   const bool prof = gk_prof_begin(st, &pa);
-  if (trans_b) {
-    if (lower) hipLaunchKernelGGL((gemm_kernel<T, true, true>), grid, block, 0, st, m, n, k, A, lda, B, ldb, alpha, beta, C, ldc, tn, tiles);
-    else hipLaunchKernelGGL((gemm_kernel<T, true, false>), grid, block, 0, st, m, n, k, A, lda, B, ldb, alpha, beta, C, ldc, tn, tiles);
-  } else {
-    if (lower) hipLaunchKernelGGL((gemm_kernel<T, false, true>), grid, block, 0, st, m, n, k, A, lda, B, ldb, alpha, beta, C, ldc, tn, tiles);
-    else hipLaunchKernelGGL((gemm_kernel<T, false, false>), grid, block, 0, st, m, n, k, A, lda, B, ldb, alpha, beta, C, ldc, tn, tiles);
-  }
+  const int64_t tm = m / 128, tn = n / 128;
+  const int64_t tiles = lower ? tm * (tm + 1) / 2 : tm * tn;
+  gemm_dispatch<T, SmallTile, Mfma<T>::WPE>(dim3((unsigned)tiles), trans_b, lower, st, m, n, k, A, lda, B, ldb, alpha,
+                                            beta, C, ldc, tn, tiles);
   if (prof)
     gk_prof_end(st, pa, 4 * (sizeof(T) == 8) + 2 * trans_b + lower,
                 lower ? (double)m * (double)(m + 1) * (double)k : 2.0 * (double)m * (double)n * (double)k);

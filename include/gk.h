@@ -160,8 +160,10 @@ int gk_lu_f64(gk_handle h, double* A, int64_t n, int64_t lda, double* dinv, int3
 int gk_lu_solve_f64(gk_handle h, const double* LU, const double* dinv, int64_t lda, double* b,
                     gk_stream stream);
 /* C = beta C - A op(B): MFMA GEMM building block (exported for testing).
- * trans_b = 1: B is n x k (C -= A B^T); 0: B is k x n.  lower_only: update
- * only 256-blocks on or below the diagonal (SYRK). */
+ * trans_b = 1: B is n x k (C -= A B^T); 0: B is k x n.  lower_only (m == n,
+ * SYRK): update the lower triangle and leave the 256-blocks above the diagonal
+ * untouched; entries above the diagonal inside a diagonal 256-block may be
+ * updated too (callers read the lower triangle only). */
 int gk_gemm_f32(gk_handle h, int64_t m, int64_t n, int64_t k, const float* A, int64_t lda,
                 const float* B, int64_t ldb, int32_t trans_b, float beta, float* C, int64_t ldc,
                 int32_t lower_only, gk_stream stream);

@@ -83,6 +83,41 @@ def test_gemm_lower_only(gpu):
                 np.testing.assert_array_equal(out[blk], Cm[blk].astype(np.float32))
 
 
+@pytest.mark.parametrize("case", ["nt", "nn", "lower"])
+def test_gemm_f32_large(gpu, case):
+    """Trailing-update sizes (thousands of tiles, XCD-remapped tile order),
+    against an FP64 product; lower_only leaves the 256-blocks above the
+    diagonal untouched."""
+    m = 6144 if case == "lower" else 4096
+    k = 256
+    say(f"gemm f32 large {case} m=n={m} k={k}")
+    L = _lib()
+    g = torch.Generator(device=gpu).manual_seed(5)
+    A = torch.randn((m, k), device=gpu, generator=g)
+    B = A if case == "lower" else (torch.randn((m, k), device=gpu, generator=g) if case == "nt"
+                                   else torch.randn((k, m), device=gpu, generator=g))
+    C0 = torch.randn((m, m), device=gpu, generator=g)
+    tC = C0.clone()
+    trans_b = 0 if case == "nn" else 1
+    L.check("gemm", L.load().gk_gemm_f32(_h(), m, m, k, _p(A), k, _p(B), B.shape[1], trans_b, 1.0, _p(tC), m,
+                                         1 if case == "lower" else 0, _st()))
+    torch.cuda.synchronize()
+    Bd = B.double()
+    ref = C0.double() - A.double() @ (Bd.T.contiguous() if trans_b else Bd)
+    err = (tC.double() - ref).abs()
+    tol = 2e-5 * np.sqrt(k) * 4
+    if case == "lower":
+        nb = m // 256
+        blk = torch.arange(m, device=gpu) // 256
+        below = blk[:, None] > blk[None, :]
+        diag_lower = (blk[:, None] == blk[None, :]) & (torch.arange(m, device=gpu)[:, None] >= torch.arange(m, device=gpu)[None, :])
+        above = blk[:, None] < blk[None, :]
+        assert err[below | diag_lower].max().item() <= tol
+        assert torch.equal(tC[above], C0[above])
+    else:
+        assert err.max().item() <= tol
+
+
 def _spd(n, seed, cond_scale=1.0):
     rng = np.random.default_rng(seed)
     M = rng.standard_normal((n, n))
