@@ -85,6 +85,24 @@ __device__ __forceinline__ int32_t segment_id(const EvalArgs& a, double s) {
   return id;
 }
 
+// Lattice scalar path (no gradient, no range mask), the grid evaluation:
+// rho^2 never reaches 0 because a tiny positive offset rides on the
+// per-source xy part (so no clamp per pair); rho = sqrt(t) by one Newton step
+// on rho itself; the source weight is folded into the polynomial
+// coefficients, w C = w + t (w c0 + rho (w c1 + t (w c2 + w c3 t))), and the
+// w are added once per tile.
+template <typename T> struct Tiny;
+template <> struct Tiny<double> { static constexpr double v = 1e-280; };
+template <> struct Tiny<float> { static constexpr float v = 1e-30f; };
+
+__device__ __forceinline__ double gk_rho(double t) {
+  const double y = __builtin_amdgcn_rsq(t);
+  const double r0 = t * y;
+  const double h = fma(-r0, y, 1.0);          // 1 - r0 y: the relative error of r0, doubled
+  return fma(0.5 * r0, h, r0);
+}
+__device__ __forceinline__ float gk_rho(float t) { return t * __builtin_amdgcn_rsqf(t); }
+
 template <typename T, bool LATTICE, bool GRAD, bool MASKED, int PPT>
 __global__ __launch_bounds__(TS) void eval_kernel(EvalArgs a) {
   __shared__ V8<T> s_src[TS];
@@ -155,6 +173,30 @@ __global__ __launch_bounds__(TS) void eval_kernel(EvalArgs a) {
       s4[tid] = v;
     }
     __syncthreads();
+    if constexpr (LATTICE && !GRAD && !MASKED) {
+      T wsum = T(0);
+#pragma unroll 2
+      for (int k = 0; k < TS; ++k) {
+        const V4<T> s = s4[k];
+        const T dx = x0 - s.x, dy = y0 - s.y;
+        const T dxy = fma(dx, dx, fma(dy, dy, Tiny<T>::v));
+        const T c3 = T(0.75) * s.w, c2 = T(-3.5) * s.w, c1 = T(8.75) * s.w, c0 = T(-7.0) * s.w;
+        const T e0 = z0 - s.z;
+        wsum += s.w;
+#pragma unroll
+        for (int q = 0; q < PPT; ++q) {
+          const T e = fma(T(q), hz, e0);
+          const T t = fma(e, e, dxy);
+          const T rho = gk_rho(t);
+          T u = fma(c3, t, c2);
+          u = fma(u, t, c1);
+          u = fma(u, rho, c0);
+          part[q] = fma(u, t, part[q]);
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < PPT; ++q) part[q] += wsum;
+    } else {
 #pragma unroll 2
     for (int k = 0; k < TS; ++k) {
       const V4<T> s = s4[k];
@@ -201,6 +243,7 @@ __global__ __launch_bounds__(TS) void eval_kernel(EvalArgs a) {
         }
       }
     }
+    }
 #pragma unroll
     for (int q = 0; q < PPT; ++q) {
       acc[q] += (double)part[q];
@@ -223,6 +266,29 @@ __global__ __launch_bounds__(TS) void eval_kernel(EvalArgs a) {
       s_src[tid] = v;
     }
     __syncthreads();
+    if constexpr (LATTICE && !GRAD && !MASKED) {
+      // the derivative polynomial's constants held in registers (as literals
+      // the compiler would rebuild them inside the loop)
+      T d3 = T(5.25), d2 = T(-17.5), d1 = T(26.25), d0 = T(-14.0);
+      asm volatile("" : "+v"(d3), "+v"(d2), "+v"(d1), "+v"(d0));
+#pragma unroll 2
+      for (int k = 0; k < TS; ++k) {
+        const V8<T> s = s_src[k];
+        const T dx = s.x - x0, dy = s.y - y0;          // q - x
+        const T dxy = fma(dx, dx, fma(dy, dy, Tiny<T>::v));
+        const T base_dot = fma(s.wx, dx, s.wy * dy);
+        const T e0 = z0 - s.z;                         // x - q (z)
+#pragma unroll
+        for (int q = 0; q < PPT; ++q) {
+          const T e = fma(T(q), hz, e0);
+          const T t = fma(e, e, dxy);
+          const T rho = gk_rho(t);
+          const T p = fma(fma(d3, t, d2), t, d1) * rho + d0;
+          const T dot = fma(-s.wz, e, base_dot);
+          part[q] = fma(dot, p, part[q]);
+        }
+      }
+    } else {
 #pragma unroll 2
     for (int k = 0; k < TS; ++k) {
       const V8<T> s = s_src[k];
@@ -273,6 +339,7 @@ __global__ __launch_bounds__(TS) void eval_kernel(EvalArgs a) {
           }
         }
       }
+    }
     }
 #pragma unroll
     for (int q = 0; q < PPT; ++q) {
@@ -403,7 +470,7 @@ int n_drift_of(int deg) { return deg == 0 ? 0 : (deg == 1 ? 3 : 9); }
 template <typename T, bool LATTICE, bool GRAD, bool MASKED>
 void launch_eval(const EvalArgs& a, int ppt_runs_div, hipStream_t st) {
   (void)ppt_runs_div;
-  constexpr int PPT = LATTICE ? (sizeof(T) == 4 ? 8 : 4) : (sizeof(T) == 4 ? 4 : 2);
+  constexpr int PPT = LATTICE ? ((sizeof(T) == 4 || (!GRAD && !MASKED)) ? 8 : 4) : (sizeof(T) == 4 ? 4 : 2);
   EvalArgs b = a;
   if (LATTICE) {
     b.nzr = gk_ceil_div(a.nz, PPT);
