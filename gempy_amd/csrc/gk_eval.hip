@@ -87,21 +87,18 @@ __device__ __forceinline__ int32_t segment_id(const EvalArgs& a, double s) {
 
 // Lattice scalar path (no gradient, no range mask), the grid evaluation:
 // rho^2 never reaches 0 because a tiny positive offset rides on the
-// per-source xy part (so no clamp per pair); rho = sqrt(t) by one Newton step
-// on rho itself; the source weight is folded into the polynomial
-// coefficients, w C = w + t (w c0 + rho (w c1 + t (w c2 + w c3 t))), and the
-// w are added once per tile.
+// per-source xy part (so no clamp per pair).  rho = sqrt(t) enters only as
+// rho * Q(t), and one Newton step on r = t rsq(t) gives
+// rho Q = r (1 + (1 - r y)/2) Q = (r Q/2) (3 - r y), so the polynomial is
+// evaluated with halved coefficients and the step costs one FMA.  The source
+// weight is folded into the coefficients, w C = w + t (w c0 + rho (w c1 +
+// t (w c2 + w c3 t))), and the w are added once per tile.
 template <typename T> struct Tiny;
 template <> struct Tiny<double> { static constexpr double v = 1e-280; };
 template <> struct Tiny<float> { static constexpr float v = 1e-30f; };
 
-__device__ __forceinline__ double gk_rho(double t) {
-  const double y = __builtin_amdgcn_rsq(t);
-  const double r0 = t * y;
-  const double h = fma(-r0, y, 1.0);          // 1 - r0 y: the relative error of r0, doubled
-  return fma(0.5 * r0, h, r0);
-}
-__device__ __forceinline__ float gk_rho(float t) { return t * __builtin_amdgcn_rsqf(t); }
+__device__ __forceinline__ double gk_rsq_raw(double t) { return __builtin_amdgcn_rsq(t); }
+__device__ __forceinline__ float gk_rsq_raw(float t) { return __builtin_amdgcn_rsqf(t); }
 
 template <typename T, bool LATTICE, bool GRAD, bool MASKED, int PPT>
 __global__ __launch_bounds__(TS) void eval_kernel(EvalArgs a) {
@@ -180,18 +177,20 @@ __global__ __launch_bounds__(TS) void eval_kernel(EvalArgs a) {
         const V4<T> s = s4[k];
         const T dx = x0 - s.x, dy = y0 - s.y;
         const T dxy = fma(dx, dx, fma(dy, dy, Tiny<T>::v));
-        const T c3 = T(0.75) * s.w, c2 = T(-3.5) * s.w, c1 = T(8.75) * s.w, c0 = T(-7.0) * s.w;
+        const T c3 = T(0.375) * s.w, c2 = T(-1.75) * s.w, c1 = T(4.375) * s.w, c0 = T(-7.0) * s.w;
         const T e0 = z0 - s.z;
         wsum += s.w;
 #pragma unroll
         for (int q = 0; q < PPT; ++q) {
           const T e = fma(T(q), hz, e0);
           const T t = fma(e, e, dxy);
-          const T rho = gk_rho(t);
+          const T y = gk_rsq_raw(t);
+          const T r = t * y;
+          const T h3 = fma(-r, y, T(3));
           T u = fma(c3, t, c2);
           u = fma(u, t, c1);
-          u = fma(u, rho, c0);
-          part[q] = fma(u, t, part[q]);
+          const T v = fma(r * u, h3, c0);
+          part[q] = fma(v, t, part[q]);
         }
       }
 #pragma unroll
@@ -269,7 +268,7 @@ __global__ __launch_bounds__(TS) void eval_kernel(EvalArgs a) {
     if constexpr (LATTICE && !GRAD && !MASKED) {
       // the derivative polynomial's constants held in registers (as literals
       // the compiler would rebuild them inside the loop)
-      T d3 = T(5.25), d2 = T(-17.5), d1 = T(26.25), d0 = T(-14.0);
+      T d3 = T(2.625), d2 = T(-8.75), d1 = T(13.125), d0 = T(-14.0);     // d1..d3 halved
       asm volatile("" : "+v"(d3), "+v"(d2), "+v"(d1), "+v"(d0));
 #pragma unroll 2
       for (int k = 0; k < TS; ++k) {
@@ -282,8 +281,10 @@ __global__ __launch_bounds__(TS) void eval_kernel(EvalArgs a) {
         for (int q = 0; q < PPT; ++q) {
           const T e = fma(T(q), hz, e0);
           const T t = fma(e, e, dxy);
-          const T rho = gk_rho(t);
-          const T p = fma(fma(d3, t, d2), t, d1) * rho + d0;
+          const T y = gk_rsq_raw(t);
+          const T r = t * y;
+          const T h3 = fma(-r, y, T(3));
+          const T p = fma(r * fma(fma(d3, t, d2), t, d1), h3, d0);
           const T dot = fma(-s.wz, e, base_dot);
           part[q] = fma(dot, p, part[q]);
         }
