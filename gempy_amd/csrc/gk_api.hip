@@ -52,7 +52,20 @@ extern "C" void gk_destroy(gk_handle h) {
   if (h->solver_state && h->solver_free) h->solver_free(h->solver_state);
   for (int i = 0; i < WS_COUNT; ++i)
     if (h->ws[i]) (void)hipFree(h->ws[i]);
+  for (hipEvent_t& e : h->side_ev)
+    if (e) (void)hipEventDestroy(e);
+  if (h->side) (void)hipStreamDestroy(h->side);
   delete h;
+}
+
+int gk_side_stream(gk_handle h, hipStream_t* s2, hipEvent_t** ev) {
+  if (!h->side) {
+    GK_HIP(hipStreamCreateWithFlags(&h->side, hipStreamNonBlocking));
+    for (hipEvent_t& e : h->side_ev) GK_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  }
+  *s2 = h->side;
+  *ev = h->side_ev;
+  return GK_OK;
 }
 
 // implemented in the other translation units

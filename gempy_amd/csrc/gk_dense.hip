@@ -511,24 +511,21 @@ __global__ __launch_bounds__(1024) void potrf128_f64_kernel(double* A, int64_t l
 #ifndef GK_POTRF_PB_F64
 #define GK_POTRF_PB_F64 4
 #endif
-template <typename T> struct BlkStore;
-template <> struct BlkStore<float> {
-  float (*Ls)[DB + 1];
-  float (*Xs)[DB + 1];
-  __device__ float& L(int i, int c) { return Ls[i][c]; }
-  __device__ float& X(int i, int c) { return Xs[i][c]; }
-};
-template <> struct BlkStore<double> {
-  double* S;
+// L and X = L^-1 as packed lower triangles in one LDS array (FP32 66 KB,
+// FP64 132 KB).
+template <typename T>
+struct BlkStore {
+  T* S;
   __device__ static int pl(int i, int c) { return i * (i + 1) / 2 + c; }
-  __device__ double& L(int i, int c) { return S[pl(i, c)]; }
-  __device__ double& X(int i, int c) { return S[DB * (DB + 1) / 2 + pl(i, c)]; }
+  __device__ T& L(int i, int c) { return S[pl(i, c)]; }
+  __device__ T& X(int i, int c) { return S[DB * (DB + 1) / 2 + pl(i, c)]; }
 };
 
-template <typename T, int PB>
+template <typename T, int PB, int NT>
 __device__ void potrf_blocked(BlkStore<T> B, T* A, int64_t lda, T* X, int64_t ldx, int32_t* info, int64_t gbase) {
+  constexpr int RS = NT / 32;                 // rows of the trailing update in flight
   const int tid = threadIdx.x, tx = tid & 31, ty = tid >> 5, lane = tid & 63, wave = tid >> 6;
-  for (int e = tid; e < DB * DB; e += 1024) {
+  for (int e = tid; e < DB * DB; e += NT) {
     const int i = e / DB, c = e % DB;
     if (c <= i) { B.L(i, c) = A[i * lda + c]; B.X(i, c) = T(0); }
   }
@@ -567,7 +564,7 @@ __device__ void potrf_blocked(BlkStore<T> B, T* A, int64_t lda, T* X, int64_t ld
     }
     __syncthreads();
     const int c0 = jb + PB;
-    for (int i = c0 + ty; i < DB; i += 32) {
+    for (int i = c0 + ty; i < DB; i += RS) {
       T li[PB];
 #pragma unroll
       for (int t = 0; t < PB; ++t) li[t] = B.L(i, jb + t);
@@ -583,7 +580,7 @@ __device__ void potrf_blocked(BlkStore<T> B, T* A, int64_t lda, T* X, int64_t ld
   // ---- inverse by 8-row blocks: X[i][c] = (delta_ic - sum_{c<=k<i} L[i][k] X[k][c]) / L[i][i]
   for (int ib = 0; ib < DB; ib += PB) {
     // contributions of rows k < ib: X[i][c] := -sum_{c<=k<ib} L[i][k] X[k][c], c < ib
-    for (int e = tid; e < PB * ib; e += 1024) {
+    for (int e = tid; e < PB * ib; e += NT) {
       const int i = ib + e / ib, c = e % ib;
       T sacc = T(0);
       for (int k = c; k < ib; ++k) sacc = fma(B.L(i, k), B.X(k, c), sacc);
@@ -617,30 +614,32 @@ __device__ void potrf_blocked(BlkStore<T> B, T* A, int64_t lda, T* X, int64_t ld
     }
     __syncthreads();
   }
-  for (int e = tid; e < DB * DB; e += 1024) {
+  for (int e = tid; e < DB * DB; e += NT) {
     const int i = e / DB, c = e % DB;
     if (c <= i) A[i * lda + c] = B.L(i, c);
     X[i * ldx + c] = (c <= i) ? B.X(i, c) : T(0);
   }
 }
 
-__global__ __launch_bounds__(1024) void potrf128b_f32_kernel(float* A, int64_t lda, float* X, int64_t ldx,
-                                                             int32_t* info, int64_t gbase) {
-  __shared__ float Ls[DB][DB + 1];
-  __shared__ float Xs[DB][DB + 1];
-  potrf_blocked<float, GK_POTRF_PB_F32>(BlkStore<float>{Ls, Xs}, A, lda, X, ldx, info, gbase);
+// FP32: 512 threads and 66 KB of LDS, so the kernel fits on a CU beside two
+// workgroups of the concurrent trailing update (the look-ahead factorization
+// runs it on a second stream).
+__global__ __launch_bounds__(512) void potrf128b_f32_kernel(float* A, int64_t lda, float* X, int64_t ldx,
+                                                            int32_t* info, int64_t gbase) {
+  __shared__ float S[DB * (DB + 1)];
+  potrf_blocked<float, GK_POTRF_PB_F32, 512>(BlkStore<float>{S}, A, lda, X, ldx, info, gbase);
 }
 
 __global__ __launch_bounds__(1024) void potrf128b_f64_kernel(double* A, int64_t lda, double* X, int64_t ldx,
                                                              int32_t* info, int64_t gbase) {
   __shared__ double S[DB * (DB + 1)];
-  potrf_blocked<double, GK_POTRF_PB_F64>(BlkStore<double>{S}, A, lda, X, ldx, info, gbase);
+  potrf_blocked<double, GK_POTRF_PB_F64, 1024>(BlkStore<double>{S}, A, lda, X, ldx, info, gbase);
 }
 
 template <typename T> struct Potrf128;
 template <> struct Potrf128<float> {
   static void launch(float* A, int64_t lda, float* X, int64_t ldx, int32_t* info, int64_t g, hipStream_t st) {
-    hipLaunchKernelGGL(potrf128b_f32_kernel, dim3(1), dim3(1024), 0, st, A, lda, X, ldx, info, g);
+    hipLaunchKernelGGL(potrf128b_f32_kernel, dim3(1), dim3(512), 0, st, A, lda, X, ldx, info, g);
   }
 };
 template <> struct Potrf128<double> {
@@ -845,6 +844,38 @@ struct CholRec {
   }
 };
 
+// One FP32 panel of the look-ahead factorization, on the side stream.  The
+// part the next trailing update waits for: factor and invert diagonal block kb
+// and solve the panel, W = A21 Dinv_kb^T.
+static int chol32_panel(float* A, int64_t lda, int64_t kb, float* dinv, int32_t* dinfo, float* W, float* Tb,
+                        hipStream_t s) {
+  const int64_t k0 = kb * NB, rem = lda - k0 - NB;
+  GK_TRY(diag256_chol<float>(A, lda, k0, dinv, dinfo, Tb, Tb + DB * DB, s));
+  if (rem > 0)
+    GK_TRY(gemm_launch<float>(rem, NB, NB, A + (k0 + NB) * lda + k0, lda, dinv + k0 * NB, NB, true, 1.0f, 0.0f, W,
+                              NB, false, s));
+  return GK_OK;
+}
+
+// The rest of the panel, off the sequential path: W back into A21 (columns no
+// other kernel touches until the end), chain planes G_kb = Dinv_kb L_{kb,kb-1}
+// and H_kb = L_{kb+1,kb} Dinv_kb.
+static int chol32_panel_post(float* A, int64_t lda, int64_t kb, float* dinv, const float* W, hipStream_t s) {
+  const int64_t k0 = kb * NB, rem = lda - k0 - NB;
+  float* Gp = dinv + lda * NB;
+  float* Hp = dinv + 2 * lda * NB;
+  if (kb > 0)
+    GK_TRY(gemm_launch<float>(NB, NB, NB, dinv + k0 * NB, NB, A + k0 * lda + k0 - NB, lda, false, 1.0f, 0.0f,
+                              Gp + k0 * NB, NB, false, s));
+  if (rem > 0) {
+    hipLaunchKernelGGL(copy_block_kernel<float>, dim3((unsigned)gk_ceil_div(rem * NB, 256)), dim3(256), 0, s, W,
+                       (int64_t)NB, A + (k0 + NB) * lda + k0, lda, rem, (int64_t)NB);
+    GK_TRY(gemm_launch<float>(NB, NB, NB, W, NB, dinv + k0 * NB, NB, false, 1.0f, 0.0f, Hp + k0 * NB, NB, false,
+                              s));
+  }
+  return GK_OK;
+}
+
 template <typename T>
 static int cholesky_impl(gk_handle h, T* A, int64_t n, int64_t lda, T* dinv, int32_t* info, hipStream_t st) {
   GK_REQUIRE(A && dinv, GK_ERR_ARG, "gk_cholesky: A and dinv required");
@@ -852,10 +883,10 @@ static int cholesky_impl(gk_handle h, T* A, int64_t n, int64_t lda, T* dinv, int
              "gk_cholesky: lda must be a multiple of %d and >= n", NB);
   const int64_t nb = lda / NB;
   void* ws = nullptr;
-  GK_TRY(gk_workspace(h, WS_FACTOR, 256 + ((size_t)lda * NB + 2 * DB * DB) * sizeof(T), &ws));
+  GK_TRY(gk_workspace(h, WS_FACTOR, 256 + (2 * (size_t)lda * NB + 2 * DB * DB) * sizeof(T), &ws));
   int32_t* dinfo = static_cast<int32_t*>(ws);
-  T* W = reinterpret_cast<T*>(static_cast<char*>(ws) + 256);
-  T* Tb = W + (size_t)lda * NB;
+  T* W = reinterpret_cast<T*>(static_cast<char*>(ws) + 256);       // two panel buffers (FP32 look-ahead)
+  T* Tb = W + 2 * (size_t)lda * NB;
   GK_HIP(hipMemsetAsync(dinfo, 0, sizeof(int32_t), st));
   if (n < lda) {
     const int64_t tot = (lda - n) * lda + n * (lda - n);
@@ -871,38 +902,62 @@ static int cholesky_impl(gk_handle h, T* A, int64_t n, int64_t lda, T* dinv, int
     // block relative to the Schur complement as it shrinks, which a one-shot
     // large-K update does not; on the kernel matrices of this method the
     // recursive order needs a ~60x larger diagonal shift to factor in FP32.
-    for (int64_t kb = 0; kb < nb; ++kb) {
-      const int64_t k0 = kb * NB;
-      GK_TRY(diag256_chol<T>(A, lda, k0, dinv, dinfo, Tb, Tb + DB * DB, st));
-      const int64_t rem = lda - k0 - NB;
-      if (rem <= 0) break;
-      // W = A21 Linv^T ;  A21 <- W ; A22 -= W W^T (lower)
-      GK_TRY(gemm_launch<T>(rem, NB, NB, A + (k0 + NB) * lda + k0, lda, dinv + k0 * NB, NB, true, T(1), T(0), W,
-                            NB, false, st));
-      hipLaunchKernelGGL(copy_block_kernel<T>, dim3((unsigned)gk_ceil_div(rem * NB, 256)), dim3(256), 0, st, W,
-                         (int64_t)NB, A + (k0 + NB) * lda + k0, lda, rem, (int64_t)NB);
-      GK_TRY(gemm_launch<T>(rem, rem, NB, W, NB, W, NB, true, T(-1), T(1), A + (k0 + NB) * lda + k0 + NB, lda,
-                            true, st));
-      if (kb % 8 == 7) {        // stop early once a pivot has failed
-        int32_t hinfo = 0;
-        GK_HIP(hipMemcpyAsync(&hinfo, dinfo, sizeof(int32_t), hipMemcpyDeviceToHost, st));
-        GK_HIP(hipStreamSynchronize(st));
-        if (hinfo != 0) break;
+    // Look-ahead: the caller's stream first updates the next panel's columns
+    // and signals; the side stream factors that panel (diagonal block, panel
+    // solve) and signals, then finishes it (copy back, chain planes) while the
+    // caller's stream applies the rest of the trailing update.  W alternates
+    // between two buffers: the side stream writes panel kb+2 only after the
+    // caller's stream has passed the update that last read it.
+    if constexpr (sizeof(T) == 4) {
+      hipStream_t s2;
+      hipEvent_t* ev;
+      GK_TRY(gk_side_stream(h, &s2, &ev));
+      T* Wb[2] = {W, W + (size_t)lda * NB};
+      GK_HIP(hipEventRecord(ev[0], st));
+      GK_HIP(hipStreamWaitEvent(s2, ev[0], 0));
+      if (nb > 0) GK_TRY(chol32_panel(A, lda, 0, dinv, dinfo, Wb[0], Tb, s2));
+      GK_HIP(hipEventRecord(ev[1], s2));                    // panel 0 factored
+      if (nb > 0) GK_TRY(chol32_panel_post(A, lda, 0, dinv, Wb[0], s2));
+      for (int64_t kb = 0; kb + 1 < nb; ++kb) {
+        const int64_t k0 = kb * NB, rem = lda - k0 - NB;
+        T* Wk = Wb[kb & 1];
+        GK_HIP(hipStreamWaitEvent(st, ev[1], 0));           // panel kb factored, Wk = its rows below
+        // next panel's columns: A[k0+NB:, k0+NB : k0+2NB] -= Wk Wk[0:NB]^T
+        GK_TRY(gemm_launch<T>(rem, NB, NB, Wk, NB, Wk, NB, true, T(-1), T(1), A + (k0 + NB) * lda + k0 + NB, lda,
+                              false, st));
+        GK_HIP(hipEventRecord(ev[0], st));
+        GK_HIP(hipStreamWaitEvent(s2, ev[0], 0));
+        GK_TRY(chol32_panel(A, lda, kb + 1, dinv, dinfo, Wb[(kb + 1) & 1], Tb, s2));
+        GK_HIP(hipEventRecord(ev[1], s2));
+        GK_TRY(chol32_panel_post(A, lda, kb + 1, dinv, Wb[(kb + 1) & 1], s2));
+        if (rem > NB)                                       // the rest of the trailing update (lower)
+          GK_TRY(gemm_launch<T>(rem - NB, rem - NB, NB, Wk + NB * NB, NB, Wk + NB * NB, NB, true, T(-1), T(1),
+                                A + (k0 + 2 * NB) * lda + k0 + 2 * NB, lda, true, st));
+        if (kb % 16 == 15) {                                // stop early once a pivot has failed
+          int32_t hinfo = 0;
+          GK_HIP(hipMemcpyAsync(&hinfo, dinfo, sizeof(int32_t), hipMemcpyDeviceToHost, s2));
+          GK_HIP(hipStreamSynchronize(s2));
+          if (hinfo != 0) break;
+        }
       }
+      GK_HIP(hipEventRecord(ev[2], s2));
+      GK_HIP(hipStreamWaitEvent(st, ev[2], 0));
     }
   }
-  // chain blocks of the triangular solves: G_k = Dinv_k L_{k,k-1} (plane 1),
-  // H_k = L_{k+1,k} Dinv_k (plane 2)
-  T* Gp = dinv + lda * NB;
-  T* Hp = dinv + 2 * lda * NB;
-  for (int64_t kb = 0; kb < nb; ++kb) {
-    const int64_t k0 = kb * NB;
-    if (kb > 0)
-      GK_TRY(gemm_launch<T>(NB, NB, NB, dinv + k0 * NB, NB, A + k0 * lda + k0 - NB, lda, false, T(1), T(0),
-                            Gp + k0 * NB, NB, false, st));
-    if (kb + 1 < nb)
-      GK_TRY(gemm_launch<T>(NB, NB, NB, A + (k0 + NB) * lda + k0, lda, dinv + k0 * NB, NB, false, T(1), T(0),
-                            Hp + k0 * NB, NB, false, st));
+  if (sizeof(T) == 8) {
+    // chain blocks of the triangular solves: G_k = Dinv_k L_{k,k-1} (plane 1),
+    // H_k = L_{k+1,k} Dinv_k (plane 2)
+    T* Gp = dinv + lda * NB;
+    T* Hp = dinv + 2 * lda * NB;
+    for (int64_t kb = 0; kb < nb; ++kb) {
+      const int64_t k0 = kb * NB;
+      if (kb > 0)
+        GK_TRY(gemm_launch<T>(NB, NB, NB, dinv + k0 * NB, NB, A + k0 * lda + k0 - NB, lda, false, T(1), T(0),
+                              Gp + k0 * NB, NB, false, st));
+      if (kb + 1 < nb)
+        GK_TRY(gemm_launch<T>(NB, NB, NB, A + (k0 + NB) * lda + k0, lda, dinv + k0 * NB, NB, false, T(1), T(0),
+                              Hp + k0 * NB, NB, false, st));
+    }
   }
   GK_HIP(hipGetLastError());
   int32_t hinfo = 0;

@@ -58,7 +58,13 @@ struct gk_context {
   int trsv_nb = 0, trsv_slots = 0, trsv_nfwd = 0, trsv_nbwd = 0;
   unsigned trsv_epoch = 0;
   unsigned long long trsv_tickets = 0;
+  // side stream (default priority) and events of the look-ahead factorization
+  hipStream_t side = nullptr;
+  hipEvent_t side_ev[3] = {};
 };
+
+// The handle's side stream and its three events, created on first use.
+int gk_side_stream(gk_handle h, hipStream_t* s2, hipEvent_t** ev);
 
 int gk_workspace(gk_handle h, int slot, size_t bytes, void** out);
 // L^-T L^-1 b in place (one launch per direction); gk_trsv_check reports a
