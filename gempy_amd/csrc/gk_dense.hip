@@ -511,18 +511,27 @@ __global__ __launch_bounds__(1024) void potrf128_f64_kernel(double* A, int64_t l
 #ifndef GK_POTRF_PB_F64
 #define GK_POTRF_PB_F64 4
 #endif
-// L and X = L^-1 as packed lower triangles in one LDS array (FP32 66 KB,
-// FP64 132 KB).
-template <typename T>
+// L and X = L^-1 in LDS: packed lower triangles in one array (FP32 66 KB,
+// FP64 132 KB), or (SQ) L square with a padded row and X packed (FP32 99 KB).
+template <typename T, bool SQ = false>
 struct BlkStore {
   T* S;
   __device__ static int pl(int i, int c) { return i * (i + 1) / 2 + c; }
-  __device__ T& L(int i, int c) { return S[pl(i, c)]; }
-  __device__ T& X(int i, int c) { return S[DB * (DB + 1) / 2 + pl(i, c)]; }
+  __device__ T& L(int i, int c) { return SQ ? S[i * (DB + 1) + c] : S[pl(i, c)]; }
+  __device__ T& X(int i, int c) { return S[(SQ ? DB * (DB + 1) : DB * (DB + 1) / 2) + pl(i, c)]; }
 };
+// FP32 diagonal-block kernel: threads and layout.  Measured in the look-ahead
+// factorization at N = 35072 (factor ms): 1024 packed 197.5, 512 packed
+// 199.7, 512 square-L 204, 1024 square-L 219.
+#ifndef GK_POTRF_NT_F32
+#define GK_POTRF_NT_F32 1024
+#endif
+#ifndef GK_POTRF_SQ_F32
+#define GK_POTRF_SQ_F32 0
+#endif
 
-template <typename T, int PB, int NT>
-__device__ void potrf_blocked(BlkStore<T> B, T* A, int64_t lda, T* X, int64_t ldx, int32_t* info, int64_t gbase) {
+template <typename T, int PB, int NT, bool SQ>
+__device__ void potrf_blocked(BlkStore<T, SQ> B, T* A, int64_t lda, T* X, int64_t ldx, int32_t* info, int64_t gbase) {
   constexpr int RS = NT / 32;                 // rows of the trailing update in flight
   const int tid = threadIdx.x, tx = tid & 31, ty = tid >> 5, lane = tid & 63, wave = tid >> 6;
   for (int e = tid; e < DB * DB; e += NT) {
@@ -621,25 +630,26 @@ __device__ void potrf_blocked(BlkStore<T> B, T* A, int64_t lda, T* X, int64_t ld
   }
 }
 
-// FP32: 512 threads and 66 KB of LDS, so the kernel fits on a CU beside two
-// workgroups of the concurrent trailing update (the look-ahead factorization
-// runs it on a second stream).
-__global__ __launch_bounds__(512) void potrf128b_f32_kernel(float* A, int64_t lda, float* X, int64_t ldx,
-                                                            int32_t* info, int64_t gbase) {
-  __shared__ float S[DB * (DB + 1)];
-  potrf_blocked<float, GK_POTRF_PB_F32, 512>(BlkStore<float>{S}, A, lda, X, ldx, info, gbase);
+// FP32: packed 66 KB of LDS, so the kernel fits on a CU beside workgroups of
+// the concurrent trailing update (the look-ahead factorization runs it on a
+// second stream).
+__global__ __launch_bounds__(GK_POTRF_NT_F32) void potrf128b_f32_kernel(float* A, int64_t lda, float* X, int64_t ldx,
+                                                                        int32_t* info, int64_t gbase) {
+  __shared__ float S[(GK_POTRF_SQ_F32 ? DB * (DB + 1) : 0) + DB * (DB + 1) / 2 * (GK_POTRF_SQ_F32 ? 1 : 2)];
+  potrf_blocked<float, GK_POTRF_PB_F32, GK_POTRF_NT_F32, GK_POTRF_SQ_F32>(BlkStore<float, GK_POTRF_SQ_F32>{S}, A,
+                                                                          lda, X, ldx, info, gbase);
 }
 
 __global__ __launch_bounds__(1024) void potrf128b_f64_kernel(double* A, int64_t lda, double* X, int64_t ldx,
                                                              int32_t* info, int64_t gbase) {
   __shared__ double S[DB * (DB + 1)];
-  potrf_blocked<double, GK_POTRF_PB_F64, 1024>(BlkStore<double>{S}, A, lda, X, ldx, info, gbase);
+  potrf_blocked<double, GK_POTRF_PB_F64, 1024, false>(BlkStore<double>{S}, A, lda, X, ldx, info, gbase);
 }
 
 template <typename T> struct Potrf128;
 template <> struct Potrf128<float> {
   static void launch(float* A, int64_t lda, float* X, int64_t ldx, int32_t* info, int64_t g, hipStream_t st) {
-    hipLaunchKernelGGL(potrf128b_f32_kernel, dim3(1), dim3(512), 0, st, A, lda, X, ldx, info, g);
+    hipLaunchKernelGGL(potrf128b_f32_kernel, dim3(1), dim3(GK_POTRF_NT_F32), 0, st, A, lda, X, ldx, info, g);
   }
 };
 template <> struct Potrf128<double> {
