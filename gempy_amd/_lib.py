@@ -18,6 +18,7 @@ GK_POINTS_EXPLICIT, GK_POINTS_LATTICE = 0, 1
 GK_SOLVE_FP64_LU, GK_SOLVE_FP32_REFINE, GK_SOLVE_FP64_CHOL = 0, 1, 2
 GK_MAX_THRESHOLDS = 64
 GK_BLOCK = 256
+GK_CHOL_PLANES = 5
 GK_SPAI_WITH_FACTOR, GK_SPAI_NO_FALLBACK, GK_SPAI_SANDWICH = 1, 2, 4
 GK_SPAI_K, GK_SPAI_FEATURES, GK_SPAI_HIDDEN = 16, 8, 32
 

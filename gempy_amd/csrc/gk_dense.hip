@@ -867,23 +867,35 @@ static int chol32_panel(float* A, int64_t lda, int64_t kb, float* dinv, int32_t*
   return GK_OK;
 }
 
-// The rest of the panel, off the sequential path: W back into A21 (columns no
-// other kernel touches until the end), chain planes G_kb = Dinv_kb L_{kb,kb-1}
-// and H_kb = L_{kb+1,kb} Dinv_kb.
-static int chol32_panel_post(float* A, int64_t lda, int64_t kb, float* dinv, const float* W, hipStream_t s) {
-  const int64_t k0 = kb * NB, rem = lda - k0 - NB;
-  float* Gp = dinv + lda * NB;
-  float* Hp = dinv + 2 * lda * NB;
-  if (kb > 0)
-    GK_TRY(gemm_launch<float>(NB, NB, NB, dinv + k0 * NB, NB, A + k0 * lda + k0 - NB, lda, false, 1.0f, 0.0f,
-                              Gp + k0 * NB, NB, false, s));
-  if (rem > 0) {
-    hipLaunchKernelGGL(copy_block_kernel<float>, dim3((unsigned)gk_ceil_div(rem * NB, 256)), dim3(256), 0, s, W,
-                       (int64_t)NB, A + (k0 + NB) * lda + k0, lda, rem, (int64_t)NB);
-    GK_TRY(gemm_launch<float>(NB, NB, NB, W, NB, dinv + k0 * NB, NB, false, 1.0f, 0.0f, Hp + k0 * NB, NB, false,
-                              s));
+// Chain planes of the triangular solves for block row kb (planes of
+// lda x 256, D = GK_CHAIN_DEPTH): plane D+1-d: G_d = Dinv_kb L_{kb,kb-d};
+// plane 2D+1-d: H_d = L_{kb+d,kb} Dinv_kb (d = 1..D).  Lb = L_{kb+1:,kb}
+// (ld ldb): the finished panel below the diagonal block.
+template <typename T>
+static int chain_planes(const T* A, int64_t lda, int64_t kb, T* dinv, const T* Lb, int64_t ldb, hipStream_t s) {
+  constexpr int D = GK_CHAIN_DEPTH;
+  const int64_t k0 = kb * NB, nb = lda / NB;
+  const T* Dk = dinv + k0 * NB;
+  for (int d = 1; d <= D; ++d) {
+    T* Gd = dinv + (D + 1 - d) * lda * NB + k0 * NB;
+    T* Hd = dinv + (2 * D + 1 - d) * lda * NB + k0 * NB;
+    if (kb >= d)
+      GK_TRY(gemm_launch<T>(NB, NB, NB, Dk, NB, A + k0 * lda + k0 - d * NB, lda, false, T(1), T(0), Gd, NB, false,
+                            s));
+    if (kb + d < nb)
+      GK_TRY(gemm_launch<T>(NB, NB, NB, Lb + (d - 1) * NB * ldb, ldb, Dk, NB, false, T(1), T(0), Hd, NB, false, s));
   }
   return GK_OK;
+}
+
+// The rest of the panel, off the sequential path: W back into A21 (columns no
+// other kernel touches until the end) and the chain planes.
+static int chol32_panel_post(float* A, int64_t lda, int64_t kb, float* dinv, const float* W, hipStream_t s) {
+  const int64_t k0 = kb * NB, rem = lda - k0 - NB;
+  if (rem > 0)
+    hipLaunchKernelGGL(copy_block_kernel<float>, dim3((unsigned)gk_ceil_div(rem * NB, 256)), dim3(256), 0, s, W,
+                       (int64_t)NB, A + (k0 + NB) * lda + k0, lda, rem, (int64_t)NB);
+  return chain_planes<float>(A, lda, kb, dinv, W, NB, s);
 }
 
 template <typename T>
@@ -954,21 +966,9 @@ static int cholesky_impl(gk_handle h, T* A, int64_t n, int64_t lda, T* dinv, int
       GK_HIP(hipStreamWaitEvent(st, ev[2], 0));
     }
   }
-  if (sizeof(T) == 8) {
-    // chain blocks of the triangular solves: G_k = Dinv_k L_{k,k-1} (plane 1),
-    // H_k = L_{k+1,k} Dinv_k (plane 2)
-    T* Gp = dinv + lda * NB;
-    T* Hp = dinv + 2 * lda * NB;
-    for (int64_t kb = 0; kb < nb; ++kb) {
-      const int64_t k0 = kb * NB;
-      if (kb > 0)
-        GK_TRY(gemm_launch<T>(NB, NB, NB, dinv + k0 * NB, NB, A + k0 * lda + k0 - NB, lda, false, T(1), T(0),
-                              Gp + k0 * NB, NB, false, st));
-      if (kb + 1 < nb)
-        GK_TRY(gemm_launch<T>(NB, NB, NB, A + (k0 + NB) * lda + k0, lda, dinv + k0 * NB, NB, false, T(1), T(0),
-                              Hp + k0 * NB, NB, false, st));
-    }
-  }
+  if (sizeof(T) == 8)                     // chain planes of the triangular solves
+    for (int64_t kb = 0; kb < nb; ++kb)
+      GK_TRY(chain_planes<T>(A, lda, kb, dinv, A + (kb + 1) * NB * lda + kb * NB, lda, st));
   GK_HIP(hipGetLastError());
   int32_t hinfo = 0;
   GK_HIP(hipMemcpyAsync(&hinfo, dinfo, sizeof(int32_t), hipMemcpyDeviceToHost, st));

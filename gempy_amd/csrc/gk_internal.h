@@ -63,6 +63,12 @@ struct gk_context {
   hipEvent_t side_ev[3] = {};
 };
 
+// Depth of the chain step of the triangular solves: it multiplies the planes
+// Dinv_k L_{k,k-d} (forward) and L_{k+d,k} Dinv_k (backward) for d = 1..depth;
+// the factorization leaves 1 + 2 depth planes (GK_CHOL_PLANES).
+constexpr int GK_CHAIN_DEPTH = 2;
+static_assert(1 + 2 * GK_CHAIN_DEPTH == GK_CHOL_PLANES, "plane count of the Cholesky output");
+
 // The handle's side stream and its three events, created on first use.
 int gk_side_stream(gk_handle h, hipStream_t* s2, hipEvent_t** ev);
 

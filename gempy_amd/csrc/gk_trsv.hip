@@ -2,29 +2,32 @@
 // L y = b and one for L^T x = y (in place on b).
 //
 // L is row-major (ld = lda, a multiple of 256); its 256 x 256 diagonal blocks
-// are not read.  The factorization leaves three planes of 256 x 256 blocks
-// (lda x 256 each, block k at row k0 = 256 k):
+// are not read.  The factorization leaves 1 + 2 D planes of 256 x 256 blocks
+// (lda x 256 each, block k at row k0 = 256 k; D = GK_CHAIN_DEPTH = 2):
 //   plane 0: Dinv_k = L_kk^-1 (lower triangular, zeros stored above the
 //            diagonal: the solves read whole blocks)
-//   plane 1: G_k = Dinv_k L_{k,k-1}         (k >= 1)
-//   plane 2: H_k = L_{k+1,k} Dinv_k         (k <= nb-2)
+//   planes D+1-d (d = 1..D):  G_{k,d} = Dinv_k L_{k,k-d}
+//   planes 2D+1-d (d = 1..D): H_{k,d} = L_{k+d,k} Dinv_k
 // Forward, block row k:
-//   c_k = Dinv_k (b_k - sum_{j <= k-2} L_kj y_j)      ("c step")
-//   y_k = c_k - G_k y_{k-1}                           ("chain step")
-// and backward x_k = c'_k - H_k^T x_{k+1} with c'_k = Dinv_k^T (y_k - sum_{j >= k+2}
-// L_jk^T x_j).  Only the chain step lies on the sequential path, and its block
-// G_k (H_k) is loaded into registers before the wait, so a link of the chain
-// costs one flag hand-off and a register-resident 256 x 256 product instead
-// of two HBM block reads by one CU (FP64: the first 128-column half).  The sums over far blocks are
-// split into "far chunks" whose size grows with the distance from the
-// diagonal (d in [2,3), [3,5), [5,9), then 8 blocks each); the c step adds
-// their published partials (the nearest one last).  A workgroup covers a
+//   c_k = Dinv_k (b_k - sum_{j <= k-D-1} L_kj y_j)                ("c step")
+//   y_k = c_k - sum_{d=1..D} G_{k,d} y_{k-d}                      ("chain step")
+// and backward x_k = c'_k - sum_d H_{k,d}^T x_{k+d} with c'_k =
+// Dinv_k^T (y_k - sum_{j >= k+D+1} L_jk^T x_j).  Only the chain step lies on
+// the sequential path: it multiplies G_{k,D} (loaded into registers before
+// its waits) as soon as y_{k-D} is published, loads each next block while the
+// next y is pending, and takes c_k, computed ahead.  The sums over far blocks
+// are split into "far chunks" whose size grows with the distance from the
+// diagonal (d in [3,5), [5,9), then 8 blocks each); the c step adds their
+// published partials (the nearest one last).  Measured at N = 35k: the solves
+// are bound by the rate at which the far chunks stream L (each published y_j
+// releases up to 33 MB of block reads, ~2.7 TB/s achieved), not by the chain;
+// a depth of 3 with 4-block chunks ran slower.  A workgroup covers a
 // 256 x 256 block with 8 waves of 32 rows (two waves per SIMD, so a wave can
 // keep its 32 prefetched rows in registers; FP64 takes a block in two
-// 128-column halves and prefetches the first).  Every item is one row-block (forward) or
-// column-block (backward) product over 1..8 blocks followed by a short
-// epilogue, so all item types share one loop: a CU runs only a few items per
-// launch, and a separate unrolled copy per item type would run from a cold
+// 128-column halves and prefetches the first).  Every item is one row-block
+// (forward) or column-block (backward) product over 1..8 blocks followed by a
+// short epilogue, so all item types share one loop: a CU runs only a few items
+// per launch, and a separate unrolled copy per item type would run from a cold
 // instruction cache on the sequential path.
 //
 // Ordering and progress: every workgroup takes a ticket (atomic counter) when it
@@ -54,11 +57,12 @@ template <> struct Shape<float> { static constexpr int RPW = 32, NW = 8, NT = 51
 template <> struct Shape<double> { static constexpr int RPW = 32, NW = 8, NT = 512, WPE = 2, VW = 2; };
 
 // distance range [lo, hi) of far chunk s
+constexpr int NEAR = GK_CHAIN_DEPTH + 1;     // the chain step covers distances 1 .. NEAR-1
 inline void chunk_range(int s, int* lo, int* hi) {
-  if (s == 0) { *lo = 2; *hi = 3; }
-  else if (s == 1) { *lo = 3; *hi = 5; }
-  else if (s == 2) { *lo = 5; *hi = 9; }
-  else { *lo = 9 + 8 * (s - 3); *hi = 17 + 8 * (s - 3); }
+  static_assert(NEAR == 3, "chunk ranges start at the first distance past the chain");
+  if (s == 0) { *lo = 3; *hi = 5; }
+  else if (s == 1) { *lo = 5; *hi = 9; }
+  else { *lo = 9 + 8 * (s - 2); *hi = 17 + 8 * (s - 2); }
 }
 
 template <typename T> struct Vec { typedef T type __attribute__((ext_vector_type(Shape<T>::VW))); };
@@ -99,9 +103,12 @@ __device__ __forceinline__ bool group_wait_all(const unsigned* flags, int n, uns
   return *s_ok != 0;
 }
 
-// The barrier completes every wave's stores at workgroup scope; the release
-// store of thread 0 then makes them visible device-wide before the flag.
+// Every wave first waits for its own global stores to complete (the
+// workgroup barrier alone does not: it orders LDS, and the release of thread 0
+// covers only its own wave's outstanding stores); after the barrier the
+// release store of thread 0 writes back the L2 and publishes the flag.
 __device__ __forceinline__ void group_publish(unsigned* flag, unsigned epoch) {
+  asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
   __syncthreads();
   if (threadIdx.x == 0) __hip_atomic_store(flag, epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
 }
@@ -257,17 +264,22 @@ void trsv_fwd_kernel(const T* __restrict__ L, int64_t lda, const T* __restrict__
   if (!take_ticket<T>(ctl, &it, &s_id)) return;
   const int64_t k0 = (int64_t)it.k * NB;
   // far chunk: sum_{j in [jlo, jhi)} L_kj y_j;  c step: Dinv_k t (t staged in LDS);
-  // chain: G_k y_{k-1}
+  // chain: sum_d G_{k,d} y_{k-d} (the blocks that exist), one wait per block
+  const int64_t plane = (int64_t)ctl.nb * NB * NB;
   const T* M;
-  int64_t ld;
+  int64_t ld, bstride = 0;
   int nblk;
   const T* x = nullptr;
+  const unsigned* wflag = nullptr;
   if (it.s >= 0) {
-    M = L + k0 * lda + (int64_t)it.jlo * NB; ld = lda; nblk = it.jhi - it.jlo; x = b + (int64_t)it.jlo * NB;
+    M = L + k0 * lda + (int64_t)it.jlo * NB; ld = lda; bstride = NB; nblk = it.jhi - it.jlo;
+    x = b + (int64_t)it.jlo * NB;
   } else if (it.s == CSTEP) {
     M = planes + k0 * NB; ld = NB; nblk = 1;
   } else {
-    M = planes + (int64_t)ctl.nb * NB * NB + k0 * NB; ld = NB; nblk = it.k > 0 ? 1 : 0; x = b + k0 - NB;
+    nblk = min(it.k, NEAR - 1);
+    M = planes + (NEAR - nblk) * plane + k0 * NB; ld = NB; bstride = plane;
+    x = b + k0 - (int64_t)nblk * NB; wflag = ctl.yflag + it.k - nblk;
   }
   Rows<T> R;
   if constexpr (PF) {
@@ -290,10 +302,9 @@ void trsv_fwd_kernel(const T* __restrict__ L, int64_t lda, const T* __restrict__
       if (ok) t -= part[(int64_t)f0 * NB + row];
     }
     if (l < RPW) xs[row] = t;
-  } else {
+  } else if (nblk == 0) {
     ok = group_wait(ctl.cflag + it.k, ctl.epoch, ctl.err, &s_ok);
-    if (ok) { y = cbuf[k0 + row]; pin1(y); }           // c_k lands before the wait for y_{k-1}
-    if (ok && it.k > 0) ok = group_wait(ctl.yflag + it.k - 1, ctl.epoch, ctl.err, &s_ok2);
+    if (ok) y = cbuf[k0 + row];
   }
   T sum = T(0);
   if (ok) {
@@ -302,7 +313,15 @@ void trsv_fwd_kernel(const T* __restrict__ L, int64_t lda, const T* __restrict__
     for (int r = 0; r < RPW; ++r) p[r] = T(0);
 #pragma unroll 1
     for (int j = 0; j < nblk; ++j) {
-      if (!PF || j > 0) loadr(R, M + (int64_t)j * NB, ld, RPW * w, l);
+      if (!PF || j > 0) loadr(R, M + j * bstride, ld, RPW * w, l);
+      if (wflag) {           // chain: c_k (ready links ahead) before the last block's wait
+        if (j == nblk - 1) {
+          ok = group_wait(ctl.cflag + it.k, ctl.epoch, ctl.err, &s_ok);
+          if (ok) { y = cbuf[k0 + row]; pin1(y); }
+        }
+        if (ok) ok = group_wait(wflag + j, ctl.epoch, ctl.err, &s_ok2);
+        if (!ok) break;
+      }
       if (x) {
         for (int e = threadIdx.x; e < NB; e += NT) xs[e] = x[(int64_t)j * NB + e];
       }
@@ -311,7 +330,7 @@ void trsv_fwd_kernel(const T* __restrict__ L, int64_t lda, const T* __restrict__
 #pragma unroll
       for (int h = 1; h < NB / PW; ++h) {
         asm volatile("" ::: "memory");           // one pass of rows live at a time
-        loadr(R, M + (int64_t)j * NB + h * PW, ld, RPW * w, l);
+        loadr(R, M + j * bstride + h * PW, ld, RPW * w, l);
         rowacc<T>(R, *reinterpret_cast<const vec*>(xs + h * PW + Shape<T>::VW * l), p);
       }
       __syncthreads();
@@ -346,18 +365,22 @@ void trsv_bwd_kernel(const T* __restrict__ L, int64_t lda, const T* __restrict__
   if (!take_ticket<T>(ctl, &it, &s_id)) return;
   const int64_t k0 = (int64_t)it.k * NB;
   // far chunk: sum_{j in [jlo, jhi)} L_jk^T x_j;  c step: Dinv_k^T t (t staged in LDS);
-  // chain: H_k^T x_{k+1}
+  // chain: sum_d H_{k,d}^T x_{k+d} (the blocks that exist), one wait per block
+  const int64_t plane = (int64_t)ctl.nb * NB * NB;
   const T* M;
-  int64_t ld;
+  int64_t ld, bstride = 0, xstride = NB;
   int nblk;
   const T* x = nullptr;
-  const bool last = it.k + 1 >= ctl.nb;
+  const unsigned* wflag = nullptr;
   if (it.s >= 0) {
-    M = L + (int64_t)it.jlo * NB * lda + k0; ld = lda; nblk = it.jhi - it.jlo; x = b + (int64_t)it.jlo * NB;
+    M = L + (int64_t)it.jlo * NB * lda + k0; ld = lda; bstride = (int64_t)NB * lda; nblk = it.jhi - it.jlo;
+    x = b + (int64_t)it.jlo * NB;
   } else if (it.s == CSTEP) {
     M = planes + k0 * NB; ld = NB; nblk = 1;
   } else {
-    M = planes + 2 * (int64_t)ctl.nb * NB * NB + k0 * NB; ld = NB; nblk = last ? 0 : 1; x = b + k0 + NB;
+    nblk = min(ctl.nb - 1 - it.k, NEAR - 1);
+    M = planes + (2 * NEAR - 1 - nblk) * plane + k0 * NB; ld = NB; bstride = plane;
+    x = b + k0 + (int64_t)nblk * NB; xstride = -NB; wflag = ctl.yflag + it.k + nblk;
   }
   Rows<T> R;
   if constexpr (PF) {
@@ -380,10 +403,9 @@ void trsv_bwd_kernel(const T* __restrict__ L, int64_t lda, const T* __restrict__
       if (ok && threadIdx.x < NB) t -= part[(int64_t)f0 * NB + threadIdx.x];
     }
     if (threadIdx.x < NB) xs[threadIdx.x] = t;
-  } else {
+  } else if (nblk == 0) {
     ok = group_wait(ctl.cflag + it.k, ctl.epoch, ctl.err, &s_ok);
-    if (ok && threadIdx.x < NB) { cv = cbuf[k0 + threadIdx.x]; pin1(cv); }
-    if (ok && !last) ok = group_wait(ctl.yflag + it.k + 1, ctl.epoch, ctl.err, &s_ok2);
+    if (ok && threadIdx.x < NB) cv = cbuf[k0 + threadIdx.x];
   }
   vec acc[NB / PW];
 #pragma unroll
@@ -394,16 +416,24 @@ void trsv_bwd_kernel(const T* __restrict__ L, int64_t lda, const T* __restrict__
   if (ok) {
 #pragma unroll 1
     for (int j = 0; j < nblk; ++j) {
-      if (!PF || j > 0) loadr(R, M + (int64_t)j * NB * ld, ld, RPW * w, l);
+      if (!PF || j > 0) loadr(R, M + j * bstride, ld, RPW * w, l);
+      if (wflag) {           // chain: c'_k (ready links ahead) before the last block's wait
+        if (j == nblk - 1) {
+          ok = group_wait(ctl.cflag + it.k, ctl.epoch, ctl.err, &s_ok);
+          if (ok && threadIdx.x < NB) { cv = cbuf[k0 + threadIdx.x]; pin1(cv); }
+        }
+        if (ok) ok = group_wait(wflag - j, ctl.epoch, ctl.err, &s_ok2);
+        if (!ok) break;
+      }
       if (x) {
-        for (int e = threadIdx.x; e < NB; e += NT) xs[e] = x[(int64_t)j * NB + e];
+        for (int e = threadIdx.x; e < NB; e += NT) xs[e] = x[(int64_t)j * xstride + e];
       }
       __syncthreads();
       colacc<T>(R, xs + RPW * w, acc[0]);
 #pragma unroll
       for (int h = 1; h < NB / PW; ++h) {
         asm volatile("" ::: "memory");           // one pass of rows live at a time
-        loadr(R, M + (int64_t)j * NB * ld + h * PW, ld, RPW * w, l);
+        loadr(R, M + j * bstride + h * PW, ld, RPW * w, l);
         colacc<T>(R, xs + RPW * w, acc[h]);
       }
       __syncthreads();
@@ -429,10 +459,11 @@ void trsv_bwd_kernel(const T* __restrict__ L, int64_t lda, const T* __restrict__
   }
 }
 
-// Host-side work tables.  Forward: c steps of rows 0 and 1, then for each block
-// j: its chain step, the far chunks whose nearest block is j (row k = j + lo_s,
-// blocks at distances [lo_s, hi_s)), and the c step of row j + 2 (all its far
-// chunks are then queued).  Backward mirrors it with k = j - lo_s.
+// Host-side work tables.  Forward: the c steps of rows 0..NEAR-1 (no far
+// chunks), then for each block j: its chain step, the far chunks whose nearest
+// block is j (row k = j + lo_s, blocks at distances [lo_s, hi_s)), and the c
+// step of row j + NEAR (all its far chunks are then queued).  Backward mirrors it with
+// k = j - lo_s.  Every wait of an item refers to an item queued before it.
 int nfar_of(int k) {              // far chunks of a row with k blocks on its far side
   int s = 0, lo, hi;
   for (;;) { chunk_range(s, &lo, &hi); if (lo > k) return s; ++s; }
@@ -448,8 +479,7 @@ void build_tables(int nb, std::vector<Item>& fwd, std::vector<int>& fbase, std::
   }
   fwd.clear();
   bwd.clear();
-  fwd.push_back({0, CSTEP, 0, 0});
-  if (nb > 1) fwd.push_back({1, CSTEP, 0, 0});
+  for (int k = 0; k < std::min(NEAR, nb); ++k) fwd.push_back({k, CSTEP, 0, 0});
   for (int j = 0; j < nb; ++j) {
     fwd.push_back({j, CHAIN, 0, 0});
     for (int s = 0;; ++s) {
@@ -459,10 +489,9 @@ void build_tables(int nb, std::vector<Item>& fwd, std::vector<int>& fbase, std::
       if (k >= nb) break;
       fwd.push_back({k, s, std::max(0, k - hi + 1), k - lo + 1});
     }
-    if (j + 2 < nb) fwd.push_back({j + 2, CSTEP, 0, 0});
+    if (j + NEAR < nb) fwd.push_back({j + NEAR, CSTEP, 0, 0});
   }
-  bwd.push_back({nb - 1, CSTEP, 0, 0});
-  if (nb > 1) bwd.push_back({nb - 2, CSTEP, 0, 0});
+  for (int k = nb - 1; k >= std::max(nb - NEAR, 0); --k) bwd.push_back({k, CSTEP, 0, 0});
   for (int j = nb - 1; j >= 0; --j) {
     bwd.push_back({j, CHAIN, 0, 0});
     for (int s = 0;; ++s) {
@@ -472,7 +501,7 @@ void build_tables(int nb, std::vector<Item>& fwd, std::vector<int>& fbase, std::
       if (k < 0) break;
       bwd.push_back({k, s, k + lo, std::min(nb, k + hi)});
     }
-    if (j - 2 >= 0) bwd.push_back({j - 2, CSTEP, 0, 0});
+    if (j - NEAR >= 0) bwd.push_back({j - NEAR, CSTEP, 0, 0});
   }
 }
 
@@ -572,9 +601,12 @@ int gk_trsv_chol_impl(gk_handle h, const T* L, const T* planes, int64_t lda, T* 
   hipLaunchKernelGGL((trsv_bwd_kernel<T, PF>), dim3((unsigned)lay.nbwd), dim3(Shape<T>::NT), 0, st, L, lda, planes, b, cbuf,
                      part, c);
   h->trsv_tickets += (unsigned long long)lay.nbwd;
-  if (prof)      // each direction reads the strict lower block triangle and two planes of diagonal blocks
-    gk_prof_end(st, pa, sizeof(T) == 8 ? 9 : 8,
-                2.0 * sizeof(T) * ((double)nb * (nb - 1) / 2.0 * NB * NB + 2.0 * nb * NB * NB));
+  if (prof) {    // each direction reads the blocks at distance >= NEAR, Dinv and the chain planes
+    const double far = nb > NEAR ? (double)(nb - NEAR) * (nb - NEAR + 1) / 2.0 : 0.0;
+    double nearb = nb;
+    for (int d = 1; d < NEAR; ++d) nearb += std::max(nb - d, 0);
+    gk_prof_end(st, pa, sizeof(T) == 8 ? 9 : 8, 2.0 * sizeof(T) * (far + nearb) * NB * NB);
+  }
   const hipError_t e = hipGetLastError();
   if (e != hipSuccess) {
     h->trsv_nb = 0;              // counters no longer in step: rebuild on the next call

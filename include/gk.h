@@ -46,6 +46,7 @@ enum { GK_SOLVE_FP64_LU = 0, GK_SOLVE_FP32_REFINE = 1, GK_SOLVE_FP64_CHOL = 2 };
 #define GK_SPAI_FEATURES 8    /* pair features fed to its network */
 #define GK_SPAI_HIDDEN 32     /* width of its two hidden layers */
 #define GK_BLOCK 256   /* factorization block size; lda of factored matrices is a multiple */
+#define GK_CHOL_PLANES 5     /* planes of GK_BLOCK-wide blocks left by gk_cholesky_* */
 enum { GK_SPAI_WITH_FACTOR = 1, GK_SPAI_NO_FALLBACK = 2, GK_SPAI_SANDWICH = 4 };   /* gk_solve_spai flags */
 
 /* One stack's co-kriging data in the engine's layout
@@ -135,11 +136,12 @@ int gk_matvec(gk_handle h, const gk_system* sys, int32_t precision, const double
 
 /* ---- (b) solver (kernel_solver, 2-layers.approved.txt:158) -------------- */
 /* In-place lower Cholesky of an lda x lda FP32 matrix (lda multiple of GK_BLOCK;
- * rows/cols >= n are reset to the identity).  dinv [3*lda*GK_BLOCK] receives
- * three planes of GK_BLOCK-square blocks (block k at row k*GK_BLOCK of each
- * plane): the inverses Dinv_k of the diagonal blocks (lower triangular, with
- * zeros stored above the diagonal: the solves read whole blocks), G_k =
- * Dinv_k L_{k,k-1} and H_k = L_{k+1,k} Dinv_k (used by the solves).  info = 0 or the first
+ * rows/cols >= n are reset to the identity).  dinv [GK_CHOL_PLANES*lda*GK_BLOCK]
+ * receives five planes of GK_BLOCK-square blocks (block k at row k*GK_BLOCK
+ * of each plane): 0: the inverses Dinv_k of the diagonal blocks (lower
+ * triangular, with zeros stored above the diagonal: the solves read whole
+ * blocks); 1, 2: Dinv_k L_{k,k-d} for d = 2, 1; 3, 4: L_{k+d,k} Dinv_k for
+ * d = 2, 1 (used by the solves).  info = 0 or the first
  * failing block+1. */
 int gk_cholesky_f32(gk_handle h, float* A, int64_t n, int64_t lda, float* dinv, int32_t* info,
                     gk_stream stream);

@@ -131,7 +131,7 @@ def test_cholesky_f32_and_solve(gpu):
     A = _spd(n, 2)
     tA = torch.zeros((lda, lda), dtype=torch.float32, device=gpu)
     tA[:n, :n] = torch.tensor(A, dtype=torch.float32)
-    dinv = torch.empty(3 * lda * 256, dtype=torch.float32, device=gpu)
+    dinv = torch.empty(L.GK_CHOL_PLANES * lda * 256, dtype=torch.float32, device=gpu)
     info = C.c_int32(-1)
     L.check("chol", L.load().gk_cholesky_f32(_h(), _p(tA), n, lda, _p(dinv), C.byref(info), _st()))
     assert info.value == 0
@@ -146,6 +146,23 @@ def test_cholesky_f32_and_solve(gpu):
     np.testing.assert_array_equal(tb[n:].cpu().numpy(), 0.0)
 
 
+def _chain_planes(Lm, nb, depth=2):
+    """The planes gk_cholesky_* leaves (block k at rows 256k..): Dinv_k (zeros
+    above the diagonal, as stored); plane depth+1-d: Dinv_k L_{k,k-d};
+    plane 2 depth+1-d: L_{k+d,k} Dinv_k (d = 1..depth)."""
+    B = lambda k: slice(256 * k, 256 * (k + 1))
+    P = np.zeros((1 + 2 * depth, 256 * nb, 256))
+    for k in range(nb):
+        P[0, B(k)] = np.tril(np.linalg.inv(Lm[B(k), B(k)]))
+    for k in range(nb):
+        for d in range(1, depth + 1):
+            if k >= d:
+                P[depth + 1 - d, B(k)] = P[0, B(k)] @ Lm[B(k), B(k - d)]
+            if k + d < nb:
+                P[2 * depth + 1 - d, B(k)] = Lm[B(k + d), B(k)] @ P[0, B(k)]
+    return P
+
+
 @pytest.mark.parametrize("dtype", ["f32", "f64"])
 def test_triangular_solve_many_blocks(gpu, dtype):
     """Sync-free blocked solve L^-T L^-1 b over 23 block rows (full and partial
@@ -156,17 +173,7 @@ def test_triangular_solve_many_blocks(gpu, dtype):
     lda = 256 * nb
     rng = np.random.default_rng(11)
     Lm = np.tril(rng.standard_normal((lda, lda)) * (0.5 / np.sqrt(lda)), -1) + np.diag(1.0 + rng.random(lda))
-    # the three planes gk_cholesky_* leaves: Dinv_k, G_k = Dinv_k L_{k,k-1}, H_k = L_{k+1,k} Dinv_k
-    dinv = np.zeros((3, lda, 256))
-    for k in range(nb):
-        blk = slice(256 * k, 256 * (k + 1))
-        dinv[0, blk] = np.tril(np.linalg.inv(Lm[blk, blk]))     # zeros above the diagonal, as stored
-    for k in range(nb):
-        blk, prv, nxt = slice(256 * k, 256 * (k + 1)), slice(256 * (k - 1), 256 * k), slice(256 * (k + 1), 256 * (k + 2))
-        if k > 0:
-            dinv[1, blk] = dinv[0, blk] @ Lm[blk, prv]
-        if k + 1 < nb:
-            dinv[2, blk] = Lm[nxt, blk] @ dinv[0, blk]
+    dinv = _chain_planes(Lm, nb)
     tdt = torch.float32 if dtype == "f32" else torch.float64
     fn = L.load().gk_cholesky_solve_f32 if dtype == "f32" else L.load().gk_cholesky_solve_f64
     tL = torch.tensor(Lm, dtype=tdt, device=gpu)
@@ -190,7 +197,7 @@ def test_cholesky_f64_and_solve(gpu):
     A = _spd(n, 12)
     tA = torch.zeros((lda, lda), dtype=torch.float64, device=gpu)
     tA[:n, :n] = torch.tensor(A)
-    dinv = torch.empty(3 * lda * 256, dtype=torch.float64, device=gpu)
+    dinv = torch.empty(L.GK_CHOL_PLANES * lda * 256, dtype=torch.float64, device=gpu)
     info = C.c_int32(-1)
     L.check("chol64", L.load().gk_cholesky_f64(_h(), _p(tA), n, lda, _p(dinv), C.byref(info), _st()))
     assert info.value == 0
