@@ -55,7 +55,20 @@ extern "C" void gk_destroy(gk_handle h) {
   for (hipEvent_t& e : h->side_ev)
     if (e) (void)hipEventDestroy(e);
   if (h->side) (void)hipStreamDestroy(h->side);
+  for (hipEvent_t& e : h->aux_ev)
+    if (e) (void)hipEventDestroy(e);
+  if (h->aux) (void)hipStreamDestroy(h->aux);
   delete h;
+}
+
+int gk_aux_stream(gk_handle h, hipStream_t* s3, hipEvent_t** ev) {
+  if (!h->aux) {
+    GK_HIP(hipStreamCreateWithFlags(&h->aux, hipStreamNonBlocking));
+    for (hipEvent_t& e : h->aux_ev) GK_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  }
+  *s3 = h->aux;
+  *ev = h->aux_ev;
+  return GK_OK;
 }
 
 int gk_side_stream(gk_handle h, hipStream_t* s2, hipEvent_t** ev) {

@@ -61,6 +61,9 @@ struct gk_context {
   // side stream (default priority) and events of the look-ahead factorization
   hipStream_t side = nullptr;
   hipEvent_t side_ev[3] = {};
+  // auxiliary stream (default priority) for set-up work that overlaps a factorization
+  hipStream_t aux = nullptr;
+  hipEvent_t aux_ev[2] = {};
 };
 
 // Depth of the chain step of the triangular solves: it multiplies the planes
@@ -71,6 +74,8 @@ static_assert(1 + 2 * GK_CHAIN_DEPTH == GK_CHOL_PLANES, "plane count of the Chol
 
 // The handle's side stream and its three events, created on first use.
 int gk_side_stream(gk_handle h, hipStream_t* s2, hipEvent_t** ev);
+// The handle's auxiliary stream and its two events, created on first use.
+int gk_aux_stream(gk_handle h, hipStream_t* s3, hipEvent_t** ev);
 
 int gk_workspace(gk_handle h, int slot, size_t bytes, void** out);
 // L^-T L^-1 b in place (one launch per direction); gk_trsv_check reports a

@@ -209,6 +209,7 @@ struct Solver {
   bool spai = false;
   bool pcp = false;                  // local-inverse correction with the FP32 factor (needs spai blocks and a factor)
   bool pcp_sandwich = false;         // factor, correction, factor (for an approximate correction)
+  hipEvent_t pc_ready = nullptr;     // recorded when spai_idx / spai_Y are built on another stream
   double* pc_u = nullptr;            // scaled right-hand side of the covariance block
   double* pc_y = nullptr;            // scaled iterate
   double* pc_v = nullptr;            // scaled residual / correction
@@ -376,6 +377,7 @@ struct Solver {
     }
     *info_out = info;
     if (info != 0) return GK_ERR_NOT_SPD;
+    if (pc_ready) GK_HIP(hipStreamWaitEvent(st, pc_ready, 0));    // local preconditioner built beside the factor
     return setup_drift();
   }
 
@@ -630,7 +632,7 @@ static int init_solver(gk_handle h, const gk_system* sys, hipStream_t st, int fo
   const int m = fast ? S.m : 1;
   size_t need = 0;
   auto take = [&](size_t bytes) { size_t o = need; need += gk_round_up(bytes, 256); return o; };
-  const size_t o_dinv = take(form != GK_SOLVE_FP64_LU ? (size_t)3 * S.Ncpad * GK_BLOCK * sizeof(double) : 0);
+  const size_t o_dinv = take(form != GK_SOLVE_FP64_LU ? (size_t)GK_CHOL_PLANES * S.Ncpad * GK_BLOCK * sizeof(double) : 0);
   const size_t o_dsc = take(S.Ncpad * sizeof(double));
   const size_t o_U = take((size_t)S.Ncpad * std::max<int64_t>(Lt, 1) * sizeof(double));
   const size_t o_Y = take((size_t)S.Ncpad * std::max<int64_t>(Lt, 1) * sizeof(double));
@@ -751,8 +753,17 @@ extern "C" int gk_solve(gk_handle h, const gk_system* sys, int32_t form, double 
   // FP32 shifted factor + exact local-inverse correction, FP64 GMRES refinement
   GK_HIP(hipEventRecord(ev[0], st));
   GK_TRY(gk_diag_scale_impl(sys, S.dscale, st));
-  GK_TRY(gk_knn(h, sys, S.own_idx, S.own_hk, (gk_stream)st));
-  GK_TRY(gk_spai_local_inverse(h, sys, S.own_idx, S.dscale, S.own_Y, (gk_stream)st));
+  {                          // neighbourhoods and local inverses overlap the factorization
+    hipStream_t s3;
+    hipEvent_t* aev;
+    GK_TRY(gk_aux_stream(h, &s3, &aev));
+    GK_HIP(hipEventRecord(aev[0], st));
+    GK_HIP(hipStreamWaitEvent(s3, aev[0], 0));
+    GK_TRY(gk_knn(h, sys, S.own_idx, S.own_hk, (gk_stream)s3));
+    GK_TRY(gk_spai_local_inverse(h, sys, S.own_idx, S.dscale, S.own_Y, (gk_stream)s3));
+    GK_HIP(hipEventRecord(aev[1], s3));
+    S.pc_ready = aev[1];
+  }
   S.spai_idx = S.own_idx;
   S.spai_Y = S.own_Y;
   S.pcp = true;
