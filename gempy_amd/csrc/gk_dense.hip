@@ -27,10 +27,9 @@ constexpr int NB = GK_BLOCK;   // 256
 #endif
 // GEMM register stages of the global -> LDS copy (1 or 2; FP32 only) and
 // whether fragments are read from LDS one 16-byte vector at a time.  Measured
-// on MI355X (K = 256 rank updates): BK 16 with chunked fragments at 3 (FP32)
-// and 2 (FP64) waves per SIMD runs 82-87 / 42 TF/s against 66-73 / 29 TF/s
-// for BK 32 with whole-stage fragments (2 / 1 waves per SIMD); a second
-// register stage does not fit FP32's register budget at that occupancy.
+// on MI355X (K = 256 rank updates): BK 16 with chunked fragments beats BK 32
+// with whole-stage fragments; a second register stage ran slower with 8 waves
+// (91 against 99 TF/s) and does not fit the 4-wave tile's register budget.
 #ifndef GK_GEMM_RS_F32
 #define GK_GEMM_RS_F32 1
 #endif
@@ -67,18 +66,36 @@ template <> struct Mfma<double> {
 };
 
 // ------------------------------------------------------------------ GEMM
-// Block tile TM x TM, WM x WN waves, each wave FM x FN MFMA tiles of 16 x 16;
-// the instantiated shape is 128 x 128 with 4 waves of 64 x 64.  Measured on
-// MI355X for FP32 rank-256 updates (m = 34816): 83.5 TF/s; 256 x 256 tiles
-// with 8 waves of 128 x 64 or 64 x 128 ran 71-72 TF/s and with 4 waves of
-// 128 x 128 (accumulators in AGPRs) 49-53 TF/s -- with one workgroup per CU
-// the stage barriers are not covered by other workgroups' MFMAs.
+// Block tile TM x TM, WM x WN waves, each wave FM x FN MFMA tiles of 16 x 16.
+// Measured on MI355X for FP32 rank-256 updates (m = 34816, TF/s): 128 x 128
+// with 8 waves of 64 x 32 (2 x 4), 3 waves per SIMD and tiles grouped by 16
+// rows 103.6; the same without grouping 98.8; 4 waves of 64 x 64 82.7;
+// 256 x 256 tiles with 8 waves 71-72, with 4 waves (accumulators in AGPRs)
+// 49-53 (one workgroup per CU leaves the stage barriers uncovered); a second
+// register stage or BK 32 were slower.
 template <int TM_, int WM_, int WN_>
 struct Tile {
   static constexpr int TM = TM_, WM = WM_, WN = WN_, NT = 64 * WM * WN;
   static constexpr int FM = TM / WM / 16, FN = TM / WN / 16;
 };
-typedef Tile<128, 2, 2> SmallTile;
+#ifndef GK_GEMM_GROUP
+#define GK_GEMM_GROUP 16
+#endif
+#ifndef GK_GEMM_WM_F32
+#define GK_GEMM_WM_F32 2
+#endif
+#ifndef GK_GEMM_WN_F32
+#define GK_GEMM_WN_F32 4
+#endif
+#ifndef GK_GEMM_WM_F64
+#define GK_GEMM_WM_F64 2
+#endif
+#ifndef GK_GEMM_WN_F64
+#define GK_GEMM_WN_F64 2
+#endif
+template <typename T> struct GemmTile;
+template <> struct GemmTile<float> { typedef Tile<128, GK_GEMM_WM_F32, GK_GEMM_WN_F32> type; };
+template <> struct GemmTile<double> { typedef Tile<128, GK_GEMM_WM_F64, GK_GEMM_WN_F64> type; };
 
 // Requirements (checked on the host): m, n multiples of TM, k multiple of BK,
 // leading dimensions multiples of 4 (16-byte rows), 16-byte aligned bases.
@@ -103,17 +120,38 @@ __global__ __launch_bounds__(TL::NT) __attribute__((amdgpu_waves_per_eu(WPE, 0))
   int64_t bid = blockIdx.x;
   const int64_t nblk = gridDim.x;
   if (nblk % 8 == 0) bid = (bid % 8) * (nblk / 8) + bid / 8;
+  // With GK_GEMM_GROUP = G > 0 the tiles go column by column within groups of
+  // G tile rows, so a B tile is reused by the G rows while their A rows stay
+  // in L2.
   int64_t tm, tn;
   if (LOWER) {
     tm = (int64_t)((sqrt(8.0 * (double)bid + 1.0) - 1.0) * 0.5);
     while ((tm + 1) * (tm + 2) / 2 <= bid) ++tm;
     while (tm * (tm + 1) / 2 > bid) --tm;
     tn = bid - tm * (tm + 1) / 2;
+    if (GK_GEMM_GROUP > 0) {            // group of rows [r0, r0 + S): a rectangle, then a triangle
+      const int64_t G = GK_GEMM_GROUP, r0 = (tm / G) * G, S = min(G, tiles_n - r0);
+      const int64_t local = bid - r0 * (r0 + 1) / 2;
+      if (local < r0 * S) {
+        tn = local / S;
+        tm = r0 + local % S;
+      } else {
+        int64_t l2 = local - r0 * S, j = 0, acc = 0;
+        while (acc + (S - j) <= l2) { acc += S - j; ++j; }
+        tn = r0 + j;
+        tm = tn + (l2 - acc);
+      }
+    }
+  } else if (GK_GEMM_GROUP > 0) {
+    const int64_t G = GK_GEMM_GROUP, tiles_m = n_tiles / tiles_n;
+    const int64_t first = bid / (G * tiles_n) * G, S = min(tiles_m - first, G);
+    const int64_t loc = bid - first * tiles_n;
+    tm = first + loc % S;
+    tn = loc / S;
   } else {
     tm = bid / tiles_n;
     tn = bid % tiles_n;
   }
-  (void)n_tiles;
   const int64_t m0 = tm * TM, n0 = tn * TM;
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
@@ -312,8 +350,8 @@ int gemm_launch(int64_t m, int64_t n, int64_t k, const T* A, int64_t lda, const 
   const bool prof = gk_prof_begin(st, &pa);
   const int64_t tm = m / 128, tn = n / 128;
   const int64_t tiles = lower ? tm * (tm + 1) / 2 : tm * tn;
-  gemm_dispatch<T, SmallTile, Mfma<T>::WPE>(dim3((unsigned)tiles), trans_b, lower, st, m, n, k, A, lda, B, ldb, alpha,
-                                            beta, C, ldc, tn, tiles);
+  gemm_dispatch<T, typename GemmTile<T>::type, Mfma<T>::WPE>(dim3((unsigned)tiles), trans_b, lower, st, m, n, k, A,
+                                                             lda, B, ldb, alpha, beta, C, ldc, tn, tiles);
   if (prof)
     gk_prof_end(st, pa, 4 * (sizeof(T) == 8) + 2 * trans_b + lower,
                 lower ? (double)m * (double)(m + 1) * (double)k : 2.0 * (double)m * (double)n * (double)k);
