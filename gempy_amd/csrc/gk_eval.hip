@@ -153,6 +153,11 @@ __global__ __launch_bounds__(TS) void eval_kernel(EvalArgs a) {
     gpart[q][0] = gpart[q][1] = gpart[q][2] = T(0);
   }
 
+  // halved polynomial constants for the explicit-point loops, held in
+  // registers: C: 3/4, -7/2, 35/4; C'/r: 21/4, -35/2, 105/4 (all halved)
+  T kc[6] = {T(0.375), T(-1.75), T(4.375), T(2.625), T(-8.75), T(13.125)};
+  if constexpr (!LATTICE) asm volatile("" : "+v"(kc[0]), "+v"(kc[1]), "+v"(kc[2]), "+v"(kc[3]), "+v"(kc[4]), "+v"(kc[5]));
+
   // ---- interface + reference sources: weight * C(|x - p|) -----------------
   const V4<T>* pts = reinterpret_cast<const V4<T>*>(a.pts);
   V4<T>* s4 = reinterpret_cast<V4<T>*>(s_src);
@@ -221,18 +226,19 @@ __global__ __launch_bounds__(TS) void eval_kernel(EvalArgs a) {
           }
           e += hz;
         }
-      } else {
+      } else {           // explicit points: no clamp per pair, Newton step folded as on the lattice
 #pragma unroll
         for (int q = 0; q < PPT; ++q) {
           const T dx = px[q] - s.x, dy = py[q] - s.y, dz = pz[q] - s.z;
-          const T t = fma(dz, dz, fma(dy, dy, dx * dx));
-          const T y = gk_rsqrt(t);
-          const T rho = t * y;
-          T c = gk_cov_poly(t, rho);
+          const T t = fma(dz, dz, fma(dy, dy, fma(dx, dx, Tiny<T>::v)));
+          const T y = gk_rsq_raw(t);
+          const T r = t * y;
+          const T h3 = fma(-r, y, T(3));
+          T c = fma(fma(r * fma(fma(kc[0], t, kc[1]), t, kc[2]), h3, T(-7)), t, T(1));
           if (MASKED) c = t < T(1) ? c : T(0);
           part[q] = fma(s.w, c, part[q]);
           if (GRAD) {
-            T dp = gk_dcov_poly(t, rho);
+            T dp = fma(r * fma(fma(kc[3], t, kc[4]), t, kc[5]), h3, T(-14));
             if (MASKED) dp = t < T(1) ? dp : T(0);
             const T wd = s.w * dp;
             gpart[q][0] = fma(wd, dx, gpart[q][0]);
@@ -318,20 +324,21 @@ __global__ __launch_bounds__(TS) void eval_kernel(EvalArgs a) {
           }
           e += hz;
         }
-      } else {
+      } else {           // explicit points (see the interface-source loop)
 #pragma unroll
         for (int q = 0; q < PPT; ++q) {
           const T dx = s.x - px[q], dy = s.y - py[q], dz = s.z - pz[q];   // q - x
-          const T t = fma(dz, dz, fma(dy, dy, dx * dx));
-          const T y = gk_rsqrt(t);
-          const T rho = t * y;
-          T p = gk_dcov_poly(t, rho);
+          const T t = fma(dz, dz, fma(dy, dy, fma(dx, dx, Tiny<T>::v)));
+          const T y = gk_rsq_raw(t);
+          const T r = t * y;
+          const T h3 = fma(-r, y, T(3));
+          T p = fma(r * fma(fma(kc[3], t, kc[4]), t, kc[5]), h3, T(-14));
           if (MASKED) p = t < T(1) ? p : T(0);
           const T dot = fma(s.wz, dz, fma(s.wy, dy, s.wx * dx));
           part[q] = fma(dot, p, part[q]);
           if (GRAD) {
             const T om = T(1) - t;
-            T dd = T(26.25) * om * om * y;
+            T dd = T(13.125) * om * om * (y * h3);     // 26.25 (1-t)^2 / rho, 1/rho = y (3 - r y) / 2
             if (MASKED) dd = t < T(1) ? dd : T(0);
             const T hw = dot * dd;
             gpart[q][0] -= fma(s.wx, p, hw * dx);
