@@ -123,14 +123,15 @@ struct Rows {
 template <typename T>
 __device__ __forceinline__ void loadr(Rows<T>& R, const T* M, int64_t ld, int64_t r0, int l) {
   typedef typename Vec<T>::type vec;
-  const T* ptr = M + r0 * ld + Shape<T>::VW * l;
+  const T* base = M + r0 * ld + Shape<T>::VW * l;
+  int64_t off = 0;
 #pragma unroll
   for (int r = 0; r < Rows<T>::R; ++r) {
-    R.v[r] = __builtin_nontemporal_load(reinterpret_cast<const vec*>(ptr));
-    ptr += ld;
-    // FP64: each row address from the previous one (the 32 row addresses
+    R.v[r] = __builtin_nontemporal_load(reinterpret_cast<const vec*>(base + off));
+    off += ld;
+    // FP64: each row offset from the previous one (the 32 row addresses
     // would otherwise be kept live beside the rows and the row partials)
-    if constexpr (sizeof(T) == 8) asm volatile("" : "+v"(ptr));
+    if constexpr (sizeof(T) == 8) asm volatile("" : "+v"(off));
   }
 }
 
@@ -221,6 +222,70 @@ __device__ __forceinline__ void colacc(const Rows<T>& R, const T* x, typename Ve
   }
 }
 
+// Half of a wave's rows (16), for the software-pipelined far chunks: one
+// half is multiplied while the next half's loads are in flight.
+template <typename T>
+struct Half {
+  typename Vec<T>::type v[16];
+};
+
+template <typename T>
+__device__ __forceinline__ void loadh(Half<T>& H, const T* base, int64_t ld) {
+  typedef typename Vec<T>::type vec;
+  int64_t off = 0;
+#pragma unroll
+  for (int r = 0; r < 16; ++r) {
+    H.v[r] = __builtin_nontemporal_load(reinterpret_cast<const vec*>(base + off));
+    off += ld;
+    asm volatile("" : "+v"(off));       // each row offset from the previous one (the base stays global)
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ void pinh(Half<T>& H) {
+#pragma unroll
+  for (int q = 0; q < 16; q += 8)
+    asm volatile("s_waitcnt vmcnt(0)"
+                 : "+v"(H.v[q]), "+v"(H.v[q + 1]), "+v"(H.v[q + 2]), "+v"(H.v[q + 3]), "+v"(H.v[q + 4]),
+                   "+v"(H.v[q + 5]), "+v"(H.v[q + 6]), "+v"(H.v[q + 7])
+                 :
+                 : "memory");
+}
+
+// Completes the products into p[0..15] before the next loads are issued (the
+// loads overwrite the half just consumed; hoisting them above the products
+// would need a second copy of it).
+template <typename T>
+__device__ __forceinline__ void fence16(T* p) {
+#pragma unroll
+  for (int q = 0; q < 16; q += 8)
+    asm volatile("" : "+v"(p[q]), "+v"(p[q + 1]), "+v"(p[q + 2]), "+v"(p[q + 3]), "+v"(p[q + 4]), "+v"(p[q + 5]),
+                 "+v"(p[q + 6]), "+v"(p[q + 7]) : : "memory");
+}
+
+// p[r] += H[r][.] . x for the 16 rows of a half
+template <typename T>
+__device__ __forceinline__ void rowacc_h(const Half<T>& H, typename Vec<T>::type xv, T* p) {
+#pragma unroll
+  for (int r = 0; r < 16; ++r) {
+    T s = p[r];
+#pragma unroll
+    for (int e = 0; e < Shape<T>::VW; ++e) s = fma(H.v[r][e], xv[e], s);
+    p[r] = s;
+  }
+}
+
+// acc (the lane's columns) += sum_r H[r][.] x[r], x the half's 16 entries
+template <typename T>
+__device__ __forceinline__ void colacc_h(const Half<T>& H, const T* x, typename Vec<T>::type& acc) {
+#pragma unroll
+  for (int r = 0; r < 16; ++r) {
+    const T xr = x[r];
+#pragma unroll
+    for (int e = 0; e < Shape<T>::VW; ++e) acc[e] = fma(H.v[r][e], xr, acc[e]);
+  }
+}
+
 struct TrsvCtl {
   const Item* items;
   const int* rowbase;       // far-chunk slot of (k, s) = rowbase[k] + s
@@ -247,6 +312,119 @@ __device__ __forceinline__ bool take_ticket(const TrsvCtl& ctl, Item* it, int* s
   return true;
 }
 
+// Far chunks stream their blocks: all x blocks of the chunk are staged in
+// LDS once (no barrier inside the loop), and the wave's rows are taken in
+// halves of 16, the loads of the next half in flight while the current one is
+// multiplied.  The first block's first pass is loaded before the wait.
+constexpr int MAXCHUNK = 8;              // blocks per far chunk at most
+
+template <typename T>
+__device__ __forceinline__ void far_fwd(const Item& it, const T* __restrict__ L, int64_t lda, T* b, T* part,
+                                        const TrsvCtl& ctl, T* xall, int* s_ok) {
+  typedef typename Vec<T>::type vec;
+  constexpr int RPW = Shape<T>::RPW, NT = Shape<T>::NT, VW = Shape<T>::VW, PW = 64 * VW, PS = NB / PW;
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  const int row = RPW * w + (l & (RPW - 1));
+  const int64_t k0 = (int64_t)it.k * NB;
+  const T* M = L + k0 * lda + (int64_t)it.jlo * NB + VW * l;
+  const int nblk = it.jhi - it.jlo, U = nblk * PS * 2;
+  auto base = [&](int u) -> const T* {          // unit u: block j, pass h, half q
+    const int j = u / (2 * PS), h = (u >> 1) % PS, q = u & 1;
+    return M + (int64_t)j * NB + h * PW + (int64_t)(RPW * w + 16 * q) * lda;
+  };
+  Half<T> A, B;
+  loadh(A, base(0), lda);
+  loadh(B, base(1), lda);
+  pinh(A);
+  pinh(B);
+  const bool ok = group_wait(ctl.yflag + it.jhi - 1, ctl.epoch, ctl.err, s_ok);
+  T sum = T(0);
+  if (ok) {
+    const T* x = b + (int64_t)it.jlo * NB;
+    for (int e = threadIdx.x; e < nblk * NB; e += NT) xall[e] = x[e];
+    __syncthreads();
+    T p[RPW];
+#pragma unroll
+    for (int r = 0; r < RPW; ++r) p[r] = T(0);
+#pragma unroll 1
+    for (int u = 0; u < U; u += 2) {
+      const int j = u / (2 * PS), h = (u >> 1) % PS;
+      const vec xv = *reinterpret_cast<const vec*>(xall + j * NB + h * PW + VW * l);
+      rowacc_h<T>(A, xv, p);
+      fence16(p);
+      if (u + 2 < U) loadh(A, base(u + 2), lda);
+      rowacc_h<T>(B, xv, p + 16);
+      fence16(p + 16);
+      if (u + 3 < U) loadh(B, base(u + 3), lda);
+    }
+    sum = lane_sums<T>(p, l);
+  }
+  if (ok && l < RPW) part[(int64_t)(ctl.rowbase[it.k] + it.s) * NB + row] = sum;
+  group_publish(ctl.pflag + ctl.rowbase[it.k] + it.s, ctl.epoch);
+}
+
+template <typename T>
+__device__ __forceinline__ void far_bwd(const Item& it, const T* __restrict__ L, int64_t lda, T* b, T* part,
+                                        const TrsvCtl& ctl, T* xall, T (*red)[NB], int* s_ok) {
+  typedef typename Vec<T>::type vec;
+  constexpr int RPW = Shape<T>::RPW, NW = Shape<T>::NW, NT = Shape<T>::NT, VW = Shape<T>::VW, PW = 64 * VW,
+                PS = NB / PW;
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  const int64_t k0 = (int64_t)it.k * NB;
+  const T* M = L + (int64_t)it.jlo * NB * lda + k0 + VW * l;
+  const int nblk = it.jhi - it.jlo, U = nblk * PS * 2;
+  auto base = [&](int u) -> const T* {          // unit u: block j, pass h, half q
+    const int j = u / (2 * PS), h = (u >> 1) % PS, q = u & 1;
+    return M + ((int64_t)j * NB + RPW * w + 16 * q) * lda + h * PW;
+  };
+  Half<T> A, B;
+  loadh(A, base(0), lda);
+  loadh(B, base(1), lda);
+  pinh(A);
+  pinh(B);
+  const bool ok = group_wait(ctl.yflag + it.jlo, ctl.epoch, ctl.err, s_ok);
+  T sum = T(0);
+  if (ok) {
+    const T* x = b + (int64_t)it.jlo * NB;
+    for (int e = threadIdx.x; e < nblk * NB; e += NT) xall[e] = x[e];
+    __syncthreads();
+    vec acc[PS];
+#pragma unroll
+    for (int h = 0; h < PS; ++h)
+#pragma unroll
+      for (int e = 0; e < VW; ++e) acc[h][e] = T(0);
+#pragma unroll 1
+    for (int u = 0; u < U; u += 2) {
+      const int j = u / (2 * PS), h = (u >> 1) % PS;
+      const T* xw = xall + j * NB + RPW * w;
+      vec a = {};
+#pragma unroll
+      for (int e = 0; e < VW; ++e) a[e] = T(0);
+      colacc_h<T>(A, xw, a);
+      asm volatile("" : "+v"(a) : : "memory");
+      if (u + 2 < U) loadh(A, base(u + 2), lda);
+      colacc_h<T>(B, xw + 16, a);
+      asm volatile("" : "+v"(a) : : "memory");
+      if (u + 3 < U) loadh(B, base(u + 3), lda);
+      if constexpr (PS == 1) {
+        acc[0] += a;
+      } else {
+        if (h == 0) acc[0] += a;
+        else acc[PS - 1] += a;
+      }
+    }
+#pragma unroll
+    for (int h = 0; h < PS; ++h) *reinterpret_cast<vec*>(&red[w][h * PW + VW * l]) = acc[h];
+    __syncthreads();
+    if (threadIdx.x < NB) {
+#pragma unroll
+      for (int q = 0; q < NW; ++q) sum += red[q][threadIdx.x];
+    }
+  }
+  if (ok && threadIdx.x < NB) part[(int64_t)(ctl.rowbase[it.k] + it.s) * NB + threadIdx.x] = sum;
+  group_publish(ctl.pflag + ctl.rowbase[it.k] + it.s, ctl.epoch);
+}
+
 // Forward items: row-block products sum_j M_j x_j; the lanes keep per-row
 // partials over all blocks and passes and reduce once.  PF: the first pass
 // of the first block is loaded into registers before the item's waits.
@@ -256,25 +434,26 @@ void trsv_fwd_kernel(const T* __restrict__ L, int64_t lda, const T* __restrict__
                      TrsvCtl ctl) {
   __shared__ int s_id, s_ok, s_ok2;
   typedef typename Vec<T>::type vec;
-  __shared__ __attribute__((aligned(32))) T xs[NB];
+  __shared__ __attribute__((aligned(32))) T xs[NB * MAXCHUNK];
   const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
   constexpr int RPW = Shape<T>::RPW, NT = Shape<T>::NT, PW = 64 * Shape<T>::VW;   // PW: columns per pass
   const int row = RPW * w + (l & (RPW - 1));     // row of the 256-block reduced into lanes l < RPW
   Item it;
   if (!take_ticket<T>(ctl, &it, &s_id)) return;
+  if (it.s >= 0) {
+    far_fwd<T>(it, L, lda, b, part, ctl, xs, &s_ok);
+    return;
+  }
   const int64_t k0 = (int64_t)it.k * NB;
-  // far chunk: sum_{j in [jlo, jhi)} L_kj y_j;  c step: Dinv_k t (t staged in LDS);
-  // chain: sum_d G_{k,d} y_{k-d} (the blocks that exist), one wait per block
+  // c step: Dinv_k t (t staged in LDS);  chain: sum_d G_{k,d} y_{k-d} (the
+  // blocks that exist), one wait per block
   const int64_t plane = (int64_t)ctl.nb * NB * NB;
   const T* M;
   int64_t ld, bstride = 0;
   int nblk;
   const T* x = nullptr;
   const unsigned* wflag = nullptr;
-  if (it.s >= 0) {
-    M = L + k0 * lda + (int64_t)it.jlo * NB; ld = lda; bstride = NB; nblk = it.jhi - it.jlo;
-    x = b + (int64_t)it.jlo * NB;
-  } else if (it.s == CSTEP) {
+  if (it.s == CSTEP) {
     M = planes + k0 * NB; ld = NB; nblk = 1;
   } else {
     nblk = min(it.k, NEAR - 1);
@@ -287,9 +466,7 @@ void trsv_fwd_kernel(const T* __restrict__ L, int64_t lda, const T* __restrict__
   }
   bool ok = true;
   T y = T(0);
-  if (it.s >= 0) {
-    ok = group_wait(ctl.yflag + it.jhi - 1, ctl.epoch, ctl.err, &s_ok);
-  } else if (it.s == CSTEP) {                                 // t = b_k - far partials (the nearest last)
+  if (it.s == CSTEP) {                                        // t = b_k - far partials (the nearest last)
     const int f0 = ctl.rowbase[it.k], nfar = ctl.rowbase[it.k + 1] - f0;
     T t = b[k0 + row];
     if (nfar > 1) {
@@ -337,10 +514,7 @@ void trsv_fwd_kernel(const T* __restrict__ L, int64_t lda, const T* __restrict__
     }
     sum = lane_sums<T>(p, l);
   }
-  if (it.s >= 0) {
-    if (ok && l < RPW) part[(int64_t)(ctl.rowbase[it.k] + it.s) * NB + row] = sum;
-    group_publish(ctl.pflag + ctl.rowbase[it.k] + it.s, ctl.epoch);
-  } else if (it.s == CSTEP) {
+  if (it.s == CSTEP) {
     if (ok && l < RPW) cbuf[k0 + row] = sum;
     group_publish(ctl.cflag + it.k, ctl.epoch);
   } else {
@@ -358,24 +532,25 @@ void trsv_bwd_kernel(const T* __restrict__ L, int64_t lda, const T* __restrict__
   typedef typename Vec<T>::type vec;
   __shared__ int s_id, s_ok, s_ok2;
   constexpr int RPW = Shape<T>::RPW, NW = Shape<T>::NW, NT = Shape<T>::NT, VW = Shape<T>::VW, PW = 64 * VW;
-  __shared__ __attribute__((aligned(32))) T xs[NB];
+  __shared__ __attribute__((aligned(32))) T xs[NB * MAXCHUNK];
   __shared__ __attribute__((aligned(32))) T red[NW][NB];
   const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
   Item it;
   if (!take_ticket<T>(ctl, &it, &s_id)) return;
+  if (it.s >= 0) {
+    far_bwd<T>(it, L, lda, b, part, ctl, xs, red, &s_ok);
+    return;
+  }
   const int64_t k0 = (int64_t)it.k * NB;
-  // far chunk: sum_{j in [jlo, jhi)} L_jk^T x_j;  c step: Dinv_k^T t (t staged in LDS);
-  // chain: sum_d H_{k,d}^T x_{k+d} (the blocks that exist), one wait per block
+  // c step: Dinv_k^T t (t staged in LDS);  chain: sum_d H_{k,d}^T x_{k+d}
+  // (the blocks that exist), one wait per block
   const int64_t plane = (int64_t)ctl.nb * NB * NB;
   const T* M;
   int64_t ld, bstride = 0, xstride = NB;
   int nblk;
   const T* x = nullptr;
   const unsigned* wflag = nullptr;
-  if (it.s >= 0) {
-    M = L + (int64_t)it.jlo * NB * lda + k0; ld = lda; bstride = (int64_t)NB * lda; nblk = it.jhi - it.jlo;
-    x = b + (int64_t)it.jlo * NB;
-  } else if (it.s == CSTEP) {
+  if (it.s == CSTEP) {
     M = planes + k0 * NB; ld = NB; nblk = 1;
   } else {
     nblk = min(ctl.nb - 1 - it.k, NEAR - 1);
@@ -388,9 +563,7 @@ void trsv_bwd_kernel(const T* __restrict__ L, int64_t lda, const T* __restrict__
   }
   bool ok = true;
   T cv = T(0);
-  if (it.s >= 0) {
-    ok = group_wait(ctl.yflag + it.jlo, ctl.epoch, ctl.err, &s_ok);
-  } else if (it.s == CSTEP) {                                 // t = y_k - far partials (the nearest last)
+  if (it.s == CSTEP) {                                        // t = y_k - far partials (the nearest last)
     const int f0 = ctl.rowbase[it.k], nfar = ctl.rowbase[it.k + 1] - f0;
     T t = threadIdx.x < NB ? b[k0 + threadIdx.x] : T(0);
     if (nfar > 1) {
@@ -447,10 +620,7 @@ void trsv_bwd_kernel(const T* __restrict__ L, int64_t lda, const T* __restrict__
       for (int q = 0; q < NW; ++q) sum += red[q][threadIdx.x];
     }
   }
-  if (it.s >= 0) {
-    if (ok && threadIdx.x < NB) part[(int64_t)(ctl.rowbase[it.k] + it.s) * NB + threadIdx.x] = sum;
-    group_publish(ctl.pflag + ctl.rowbase[it.k] + it.s, ctl.epoch);
-  } else if (it.s == CSTEP) {
+  if (it.s == CSTEP) {
     if (ok && threadIdx.x < NB) cbuf[k0 + threadIdx.x] = sum;
     group_publish(ctl.cflag + it.k, ctl.epoch);
   } else {
@@ -537,7 +707,12 @@ static int trsv_prepare(gk_handle h, int nb, hipStream_t st, char** base, TrsvLa
   std::vector<Item> fwd, bwd;
   std::vector<int> fb, bb;
   const bool rebuild = h->trsv_nb != nb || h->ws[WS_TRSV] == nullptr;
-  if (rebuild) build_tables(nb, fwd, fb, bwd, bb);
+  if (rebuild) {
+    build_tables(nb, fwd, fb, bwd, bb);
+    for (const std::vector<Item>* tab : {&fwd, &bwd})
+      for (const Item& q : *tab)
+        GK_REQUIRE(q.s < 0 || q.jhi - q.jlo <= MAXCHUNK, GK_ERR_ARG, "triangular solve: far chunk too large");
+  }
   const int slots = rebuild ? std::max(fb[nb], bb[nb]) : h->trsv_slots;
   const int nfwd = rebuild ? (int)fwd.size() : h->trsv_nfwd;
   const int nbwd = rebuild ? (int)bwd.size() : h->trsv_nbwd;
