@@ -190,6 +190,45 @@ def test_triangular_solve_many_blocks(gpu, dtype):
         assert err <= tol
 
 
+@pytest.mark.parametrize("dtype", ["f32", "f64"])
+def test_triangular_solve_repeated_large(gpu, dtype):
+    """Ten solves in a row at nb = 96 (N = 24576): every call, under the full
+    load of far chunks streaming the factor, must match the FP64 solution (a
+    result published ahead of its data would show here)."""
+    nb, calls = 96, 10
+    say(f"triangular solve repeated {dtype} nb={nb} calls={calls}")
+    L = _lib()
+    lda = 256 * nb
+    tdt = torch.float32 if dtype == "f32" else torch.float64
+    g = torch.Generator(device=gpu).manual_seed(3)
+    M = torch.randn((lda, lda), device=gpu, dtype=torch.float64, generator=g) * (0.5 / lda ** 0.5)
+    M = torch.tril(M, -1)
+    M.diagonal().add_(1.5)
+    B = lambda k: slice(256 * k, 256 * (k + 1))
+    P = torch.zeros((L.GK_CHOL_PLANES, lda, 256), device=gpu, dtype=torch.float64)
+    for k in range(nb):
+        P[0, B(k)] = torch.tril(torch.linalg.inv(M[B(k), B(k)]))
+    depth = (L.GK_CHOL_PLANES - 1) // 2
+    for k in range(nb):
+        for d in range(1, depth + 1):
+            if k >= d:
+                P[depth + 1 - d, B(k)] = P[0, B(k)] @ M[B(k), B(k - d)]
+            if k + d < nb:
+                P[2 * depth + 1 - d, B(k)] = M[B(k + d), B(k)] @ P[0, B(k)]
+    tM, tP = M.to(tdt), P.to(tdt)
+    fn = L.load().gk_cholesky_solve_f32 if dtype == "f32" else L.load().gk_cholesky_solve_f64
+    tol = 2e-5 if dtype == "f32" else 1e-12
+    for call in range(calls):
+        b = torch.randn(lda, device=gpu, dtype=torch.float64, generator=g)
+        want = torch.linalg.solve_triangular(M.T, torch.linalg.solve_triangular(M, b[:, None], upper=False),
+                                             upper=True)[:, 0]
+        tb = b.to(tdt).clone()                     # solved in place
+        L.check("trsv", fn(_h(), _p(tM), _p(tP), lda, _p(tb), _st()))
+        err = ((tb.double() - want).abs().max() / want.abs().max()).item()
+        say(f"  call {call}: max rel err {err:.2e}")
+        assert err <= tol
+
+
 def test_cholesky_f64_and_solve(gpu):
     say("cholesky f64 n=700 lda=768")
     L = _lib()
