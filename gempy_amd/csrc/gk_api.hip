@@ -85,7 +85,7 @@ int gk_side_stream(gk_handle h, hipStream_t* s2, hipEvent_t** ev) {
 extern "C" int gk_eval_impl(gk_handle h, const gk_system* sys, const double* weights, gk_eval_desc* d,
                             hipStream_t st, bool force_mask, const double* src_box);
 extern "C" int gk_matvec_impl(gk_handle h, const gk_system* sys, int32_t precision, const double* x,
-                              double* y, hipStream_t st, void* scratch);
+                              double* y, hipStream_t st, void* scratch, const double* src_box);
 extern "C" size_t gk_matvec_scratch_doubles(const gk_system* sys);
 extern "C" int gk_assemble_impl(gk_handle h, const gk_system* sys, int32_t dtype, void* A, int64_t lda,
                                 double* rhs, double* dscale, hipStream_t st, int lower_only);
@@ -135,13 +135,33 @@ extern "C" int gk_eval(gk_handle h, const gk_system* sys, const double* weights,
   return gk_eval_impl(h, sys, weights, desc, st, false, pbox);
 }
 
-extern "C" int gk_matvec(gk_handle h, const gk_system* sys, int32_t precision, const double* x, double* y,
-                         gk_stream stream) {
+// Operator product; with `src_box` (the bounding box of the system's points,
+// from gk_source_box) the r < a mask is dropped when no pair can reach the range.
+extern "C" int gk_matvec_boxed(gk_handle h, const gk_system* sys, int32_t precision, const double* x, double* y,
+                               gk_stream stream, const double* src_box) {
   GK_REQUIRE(h && sys && x && y, GK_ERR_ARG, "gk_matvec: NULL argument");
   GK_REQUIRE(precision == GK_F32 || precision == GK_F64, GK_ERR_ARG, "gk_matvec: bad precision");
   void* scratch = nullptr;
   GK_TRY(gk_workspace(h, WS_MATVEC, gk_matvec_scratch_doubles(sys) * sizeof(double), &scratch));
-  return gk_matvec_impl(h, sys, precision, x, y, (hipStream_t)stream, scratch);
+  return gk_matvec_impl(h, sys, precision, x, y, (hipStream_t)stream, scratch, src_box);
+}
+
+extern "C" int gk_matvec(gk_handle h, const gk_system* sys, int32_t precision, const double* x, double* y,
+                         gk_stream stream) {
+  return gk_matvec_boxed(h, sys, precision, x, y, stream, nullptr);
+}
+
+// Bounding box of the system's points ([lo xyz, hi xyz]), synchronously.
+extern "C" int gk_source_box(gk_handle h, const gk_system* sys, double* box, gk_stream stream) {
+  GK_REQUIRE(h && sys && box, GK_ERR_ARG, "gk_source_box: NULL argument");
+  GK_REQUIRE(sys->n_ori + sys->n_rest + sys->n_surf > 0, GK_ERR_ARG, "gk_source_box: empty system");
+  hipStream_t st = (hipStream_t)stream;
+  void* ws = nullptr;
+  GK_TRY(gk_workspace(h, WS_SMALL, 256, &ws));
+  hipLaunchKernelGGL(source_box_kernel, dim3(1), dim3(256), 0, st, *sys, static_cast<double*>(ws));
+  GK_HIP(hipMemcpyAsync(box, ws, 6 * sizeof(double), hipMemcpyDeviceToHost, st));
+  GK_HIP(hipStreamSynchronize(st));
+  return GK_OK;
 }
 
 extern "C" int gk_assemble(gk_handle h, const gk_system* sys, int32_t dtype, void* A, int64_t lda,

@@ -103,8 +103,12 @@ template <typename T, bool TRANS_B, bool LOWER, typename TL, int WPE>
 __global__ __launch_bounds__(TL::NT) __attribute__((amdgpu_waves_per_eu(WPE, 0))) void gemm_kernel(int64_t m, int64_t n, int64_t k, const T* __restrict__ A,
                                                    int64_t lda, const T* __restrict__ B, int64_t ldb,
                                                    T alpha, T beta, T* C, int64_t ldc, int64_t tiles_n,
-                                                   int64_t n_tiles) {
+                                                   int64_t n_tiles, int64_t sA, int64_t sB, int64_t sC) {
   using M = Mfma<T>;
+  // batched launches: product blockIdx.y at A + y sA, B + y sB, C + y sC
+  A += blockIdx.y * sA;
+  B += blockIdx.y * sB;
+  C += blockIdx.y * sC;
   constexpr int TM = TL::TM, NT = TL::NT, FM = TL::FM, FN = TL::FN;
   constexpr int BK = M::BK;
   constexpr int KS = BK / 4;              // k values per lane per stage
@@ -321,26 +325,30 @@ __global__ __launch_bounds__(TL::NT) __attribute__((amdgpu_waves_per_eu(WPE, 0))
 template <typename T, typename TL, int WPE>
 void gemm_dispatch(dim3 grid, bool trans_b, bool lower, hipStream_t st, int64_t m, int64_t n, int64_t k, const T* A,
                    int64_t lda, const T* B, int64_t ldb, T alpha, T beta, T* C, int64_t ldc, int64_t tn,
-                   int64_t tiles) {
+                   int64_t tiles, int64_t sA, int64_t sB, int64_t sC) {
   dim3 block(TL::NT);
   if (trans_b) {
-    if (lower) hipLaunchKernelGGL((gemm_kernel<T, true, true, TL, WPE>), grid, block, 0, st, m, n, k, A, lda, B, ldb, alpha, beta, C, ldc, tn, tiles);
-    else hipLaunchKernelGGL((gemm_kernel<T, true, false, TL, WPE>), grid, block, 0, st, m, n, k, A, lda, B, ldb, alpha, beta, C, ldc, tn, tiles);
+    if (lower) hipLaunchKernelGGL((gemm_kernel<T, true, true, TL, WPE>), grid, block, 0, st, m, n, k, A, lda, B, ldb, alpha, beta, C, ldc, tn, tiles, sA, sB, sC);
+    else hipLaunchKernelGGL((gemm_kernel<T, true, false, TL, WPE>), grid, block, 0, st, m, n, k, A, lda, B, ldb, alpha, beta, C, ldc, tn, tiles, sA, sB, sC);
   } else {
-    if (lower) hipLaunchKernelGGL((gemm_kernel<T, false, true, TL, WPE>), grid, block, 0, st, m, n, k, A, lda, B, ldb, alpha, beta, C, ldc, tn, tiles);
-    else hipLaunchKernelGGL((gemm_kernel<T, false, false, TL, WPE>), grid, block, 0, st, m, n, k, A, lda, B, ldb, alpha, beta, C, ldc, tn, tiles);
+    if (lower) hipLaunchKernelGGL((gemm_kernel<T, false, true, TL, WPE>), grid, block, 0, st, m, n, k, A, lda, B, ldb, alpha, beta, C, ldc, tn, tiles, sA, sB, sC);
+    else hipLaunchKernelGGL((gemm_kernel<T, false, false, TL, WPE>), grid, block, 0, st, m, n, k, A, lda, B, ldb, alpha, beta, C, ldc, tn, tiles, sA, sB, sC);
   }
 }
 
+// C = alpha A op(B) + beta C; `batch` > 1 runs that many products, the i-th at
+// A + i sA, B + i sB, C + i sC (one launch, products on blockIdx.y).
 template <typename T>
 int gemm_launch(int64_t m, int64_t n, int64_t k, const T* A, int64_t lda, const T* B, int64_t ldb,
-                bool trans_b, T alpha, T beta, T* C, int64_t ldc, bool lower, hipStream_t st) {
+                bool trans_b, T alpha, T beta, T* C, int64_t ldc, bool lower, hipStream_t st, int64_t batch = 1,
+                int64_t sA = 0, int64_t sB = 0, int64_t sC = 0) {
   constexpr int BK = Mfma<T>::BK;
   GK_REQUIRE(m % 128 == 0 && n % 128 == 0 && k % BK == 0, GK_ERR_ARG,
              "gk_gemm: m, n must be multiples of 128 and k of %d (got %lld %lld %lld)", BK,
              (long long)m, (long long)n, (long long)k);
   GK_REQUIRE(lda % 4 == 0 && ldb % 4 == 0 && ldc >= n, GK_ERR_ARG, "gk_gemm: bad leading dimensions");
   GK_REQUIRE(!lower || m == n, GK_ERR_ARG, "gk_gemm: lower_only needs m == n");
+  GK_REQUIRE(batch >= 1 && batch <= 65535 && sA % 4 == 0 && sB % 4 == 0, GK_ERR_ARG, "gk_gemm: bad batch");
   if (m == 0 || n == 0) return GK_OK;
   if (k == 0) {
     GK_REQUIRE(beta == T(1), GK_ERR_ARG, "gk_gemm: k == 0 only with beta == 1");
@@ -350,11 +358,12 @@ int gemm_launch(int64_t m, int64_t n, int64_t k, const T* A, int64_t lda, const 
   const bool prof = gk_prof_begin(st, &pa);
   const int64_t tm = m / 128, tn = n / 128;
   const int64_t tiles = lower ? tm * (tm + 1) / 2 : tm * tn;
-  gemm_dispatch<T, typename GemmTile<T>::type, Mfma<T>::WPE>(dim3((unsigned)tiles), trans_b, lower, st, m, n, k, A,
-                                                             lda, B, ldb, alpha, beta, C, ldc, tn, tiles);
+  gemm_dispatch<T, typename GemmTile<T>::type, Mfma<T>::WPE>(dim3((unsigned)tiles, (unsigned)batch), trans_b, lower,
+                                                             st, m, n, k, A, lda, B, ldb, alpha, beta, C, ldc, tn,
+                                                             tiles, sA, sB, sC);
   if (prof)
     gk_prof_end(st, pa, 4 * (sizeof(T) == 8) + 2 * trans_b + lower,
-                lower ? (double)m * (double)(m + 1) * (double)k : 2.0 * (double)m * (double)n * (double)k);
+                (double)batch * (lower ? (double)m * (double)(m + 1) * (double)k : 2.0 * (double)m * (double)n * (double)k));
   GK_HIP(hipGetLastError());
   return GK_OK;
 }
@@ -536,13 +545,14 @@ __global__ __launch_bounds__(1024) void potrf128_f64_kernel(double* A, int64_t l
   }
 }
 
-// Blocked Cholesky + inverse of a DB x DB block (one workgroup of 1024):
-// 8-column panels are factored by wave 0 in registers (cross-lane
-// broadcasts, no workgroup barrier), the rank-8 trailing update by all
-// threads; the inverse X = L^-1 goes by 8-row blocks (the update from earlier
-// rows by all threads, the 8 rows themselves by wave 0 in registers).  Two
-// barriers per 8 columns instead of three per column.  Storage: FP32 square
-// LDS arrays; FP64 packed lower triangles (the square ones do not fit).
+// Blocked Cholesky + inverse of a DB x DB block (one workgroup of NT threads),
+// two barriers per PB columns.  Phase 1 (wave 0, registers, cross-lane
+// broadcasts): factor the PB-column panel, then the PB rows of X = L^-1,
+//   X[i][c] = (delta_ic - acc[i][c] - sum_{jb<=k<i} L[i][k] X[k][c]) / L[i][i],
+// where acc[i][c] = sum_{k<jb} L[i][k] X[k][c] was applied eagerly.  Phase 2
+// (all threads): the rank-PB trailing update of L and the eager update of the
+// later rows of X by the new panel and rows, in one sweep per row.  Storage:
+// packed lower triangles of L and X in LDS.
 #ifndef GK_POTRF_PB_F32
 #define GK_POTRF_PB_F32 8
 #endif
@@ -577,9 +587,9 @@ __device__ void potrf_blocked(BlkStore<T, SQ> B, T* A, int64_t lda, T* X, int64_
     if (c <= i) { B.L(i, c) = A[i * lda + c]; B.X(i, c) = T(0); }
   }
   __syncthreads();
-  // ---- factorization
   for (int jb = 0; jb < DB; jb += PB) {
     if (wave == 0) {
+      // ---- panel: lane l holds rows jb + l (a) and jb + 64 + l (b)
       const int ra = jb + lane, rb = jb + 64 + lane;
       T a[PB], b[PB];
 #pragma unroll
@@ -608,55 +618,53 @@ __device__ void potrf_blocked(BlkStore<T, SQ> B, T* A, int64_t lda, T* X, int64_
         if (ra < DB && jb + c <= ra) B.L(ra, jb + c) = a[c];
         if (rb < DB) B.L(rb, jb + c) = b[c];
       }
-    }
-    __syncthreads();
-    const int c0 = jb + PB;
-    for (int i = c0 + ty; i < DB; i += RS) {
-      T li[PB];
-#pragma unroll
-      for (int t = 0; t < PB; ++t) li[t] = B.L(i, jb + t);
-      for (int c = c0 + tx; c <= i; c += 32) {
-        T sacc = T(0);
-#pragma unroll
-        for (int t = 0; t < PB; ++t) sacc = fma(li[t], B.L(c, jb + t), sacc);
-        B.L(i, c) -= sacc;
-      }
-    }
-    __syncthreads();
-  }
-  // ---- inverse by 8-row blocks: X[i][c] = (delta_ic - sum_{c<=k<i} L[i][k] X[k][c]) / L[i][i]
-  for (int ib = 0; ib < DB; ib += PB) {
-    // contributions of rows k < ib: X[i][c] := -sum_{c<=k<ib} L[i][k] X[k][c], c < ib
-    for (int e = tid; e < PB * ib; e += NT) {
-      const int i = ib + e / ib, c = e % ib;
-      T sacc = T(0);
-      for (int k = c; k < ib; ++k) sacc = fma(B.L(i, k), B.X(k, c), sacc);
-      B.X(i, c) = -sacc;
-    }
-    __syncthreads();
-    if (wave == 0) {                          // the 8 rows, lane owns columns lane and lane + 64
+      // ---- rows jb .. jb+PB-1 of X: lane owns columns lane and lane + 64
       T xa[PB], xb[PB];
       const int ca = lane, cb = lane + 64;
 #pragma unroll
       for (int r = 0; r < PB; ++r) {
-        const int i = ib + r;
-        T va = ca < ib ? B.X(i, ca) : (ca == i ? T(1) : T(0));
-        T vb = cb < ib ? B.X(i, cb) : (cb == i ? T(1) : T(0));
+        const int i = jb + r;
+        T va = ca < jb ? B.X(i, ca) : (ca == i ? T(1) : T(0));   // B.X holds -acc[i][c] for c < jb
+        T vb = cb < jb ? B.X(i, cb) : (cb == i ? T(1) : T(0));
 #pragma unroll
         for (int k = 0; k < r; ++k) {
-          const T lik = B.L(i, ib + k);
+          const T lik = __shfl(a[k], r);          // L[jb+r][jb+k]
           va -= lik * xa[k];
           vb -= lik * xb[k];
         }
-        const T ip = T(1) / B.L(i, i);
+        const T ip = T(1) / __shfl(a[r], r);
         xa[r] = va * ip;
         xb[r] = vb * ip;
       }
 #pragma unroll
       for (int r = 0; r < PB; ++r) {
-        const int i = ib + r;
+        const int i = jb + r;
         if (ca <= i) B.X(i, ca) = xa[r];
         if (cb <= i) B.X(i, cb) = xb[r];
+      }
+    }
+    __syncthreads();
+    // ---- rows i >= c0: L[i][c] -= sum_t L[i][jb+t] L[c][jb+t]   (c0 <= c <= i)
+    //                    -acc[i][c] -= sum_t L[i][jb+t] X[jb+t][c] (c < c0, X lower)
+    const int c0 = jb + PB;
+    for (int i = c0 + ty; i < DB; i += RS) {
+      T li[PB];
+#pragma unroll
+      for (int t = 0; t < PB; ++t) li[t] = B.L(i, jb + t);
+      for (int c = tx; c <= i; c += 32) {
+        T sacc = T(0);
+        if (c >= c0) {
+#pragma unroll
+          for (int t = 0; t < PB; ++t) sacc = fma(li[t], B.L(c, jb + t), sacc);
+          B.L(i, c) -= sacc;
+        } else {
+#pragma unroll
+          for (int t = 0; t < PB; ++t) {
+            const T xv = B.X(jb + t, c);
+            sacc = fma(li[t], jb + t >= c ? xv : T(0), sacc);
+          }
+          B.X(i, c) -= sacc;
+        }
       }
     }
     __syncthreads();
@@ -905,35 +913,37 @@ static int chol32_panel(float* A, int64_t lda, int64_t kb, float* dinv, int32_t*
   return GK_OK;
 }
 
-// Chain planes of the triangular solves for block row kb (planes of
-// lda x 256, D = GK_CHAIN_DEPTH): plane D+1-d: G_d = Dinv_kb L_{kb,kb-d};
-// plane 2D+1-d: H_d = L_{kb+d,kb} Dinv_kb (d = 1..D).  Lb = L_{kb+1:,kb}
-// (ld ldb): the finished panel below the diagonal block.
+// Chain planes of the triangular solves (planes of lda x 256, D =
+// GK_CHAIN_DEPTH), for all block rows kb at once from the finished factor:
+// plane D+1-d: G_d = Dinv_kb L_{kb,kb-d} (kb >= d); plane 2D+1-d: H_d =
+// L_{kb+d,kb} Dinv_kb (kb + d < nb).  One batched launch per plane; every
+// product is 256 x 256 x 256 and the block addresses advance by a fixed stride
+// per block row (NB x NB in dinv, NB rows + NB columns in A).
 template <typename T>
-static int chain_planes(const T* A, int64_t lda, int64_t kb, T* dinv, const T* Lb, int64_t ldb, hipStream_t s) {
+static int chain_planes(const T* A, int64_t lda, T* dinv, hipStream_t s) {
   constexpr int D = GK_CHAIN_DEPTH;
-  const int64_t k0 = kb * NB, nb = lda / NB;
-  const T* Dk = dinv + k0 * NB;
+  const int64_t nb = lda / NB, sD = (int64_t)NB * NB, sL = (int64_t)NB * lda + NB;
   for (int d = 1; d <= D; ++d) {
-    T* Gd = dinv + (D + 1 - d) * lda * NB + k0 * NB;
-    T* Hd = dinv + (2 * D + 1 - d) * lda * NB + k0 * NB;
-    if (kb >= d)
-      GK_TRY(gemm_launch<T>(NB, NB, NB, Dk, NB, A + k0 * lda + k0 - d * NB, lda, false, T(1), T(0), Gd, NB, false,
-                            s));
-    if (kb + d < nb)
-      GK_TRY(gemm_launch<T>(NB, NB, NB, Lb + (d - 1) * NB * ldb, ldb, Dk, NB, false, T(1), T(0), Hd, NB, false, s));
+    const int64_t cnt = nb - d;
+    if (cnt <= 0) continue;
+    T* G = dinv + (D + 1 - d) * lda * NB;
+    T* H = dinv + (2 * D + 1 - d) * lda * NB;
+    GK_TRY(gemm_launch<T>(NB, NB, NB, dinv + d * sD, NB, A + d * NB * lda, lda, false, T(1), T(0), G + d * sD, NB,
+                          false, s, cnt, sD, sL, sD));
+    GK_TRY(gemm_launch<T>(NB, NB, NB, A + d * NB * lda, lda, dinv, NB, false, T(1), T(0), H, NB, false, s, cnt, sL,
+                          sD, sD));
   }
   return GK_OK;
 }
 
 // The rest of the panel, off the sequential path: W back into A21 (columns no
-// other kernel touches until the end) and the chain planes.
-static int chol32_panel_post(float* A, int64_t lda, int64_t kb, float* dinv, const float* W, hipStream_t s) {
+// other kernel touches until the end).
+static int chol32_panel_post(float* A, int64_t lda, int64_t kb, const float* W, hipStream_t s) {
   const int64_t k0 = kb * NB, rem = lda - k0 - NB;
   if (rem > 0)
     hipLaunchKernelGGL(copy_block_kernel<float>, dim3((unsigned)gk_ceil_div(rem * NB, 256)), dim3(256), 0, s, W,
                        (int64_t)NB, A + (k0 + NB) * lda + k0, lda, rem, (int64_t)NB);
-  return chain_planes<float>(A, lda, kb, dinv, W, NB, s);
+  return GK_OK;
 }
 
 template <typename T>
@@ -964,8 +974,8 @@ static int cholesky_impl(gk_handle h, T* A, int64_t n, int64_t lda, T* dinv, int
     // recursive order needs a ~60x larger diagonal shift to factor in FP32.
     // Look-ahead: the caller's stream first updates the next panel's columns
     // and signals; the side stream factors that panel (diagonal block, panel
-    // solve) and signals, then finishes it (copy back, chain planes) while the
-    // caller's stream applies the rest of the trailing update.  W alternates
+    // solve) and signals, then copies it back while the caller's stream
+    // applies the rest of the trailing update.  W alternates
     // between two buffers: the side stream writes panel kb+2 only after the
     // caller's stream has passed the update that last read it.
     if constexpr (sizeof(T) == 4) {
@@ -977,7 +987,7 @@ static int cholesky_impl(gk_handle h, T* A, int64_t n, int64_t lda, T* dinv, int
       GK_HIP(hipStreamWaitEvent(s2, ev[0], 0));
       if (nb > 0) GK_TRY(chol32_panel(A, lda, 0, dinv, dinfo, Wb[0], Tb, s2));
       GK_HIP(hipEventRecord(ev[1], s2));                    // panel 0 factored
-      if (nb > 0) GK_TRY(chol32_panel_post(A, lda, 0, dinv, Wb[0], s2));
+      if (nb > 0) GK_TRY(chol32_panel_post(A, lda, 0, Wb[0], s2));
       for (int64_t kb = 0; kb + 1 < nb; ++kb) {
         const int64_t k0 = kb * NB, rem = lda - k0 - NB;
         T* Wk = Wb[kb & 1];
@@ -989,7 +999,7 @@ static int cholesky_impl(gk_handle h, T* A, int64_t n, int64_t lda, T* dinv, int
         GK_HIP(hipStreamWaitEvent(s2, ev[0], 0));
         GK_TRY(chol32_panel(A, lda, kb + 1, dinv, dinfo, Wb[(kb + 1) & 1], Tb, s2));
         GK_HIP(hipEventRecord(ev[1], s2));
-        GK_TRY(chol32_panel_post(A, lda, kb + 1, dinv, Wb[(kb + 1) & 1], s2));
+        GK_TRY(chol32_panel_post(A, lda, kb + 1, Wb[(kb + 1) & 1], s2));
         if (rem > NB)                                       // the rest of the trailing update (lower)
           GK_TRY(gemm_launch<T>(rem - NB, rem - NB, NB, Wk + NB * NB, NB, Wk + NB * NB, NB, true, T(-1), T(1),
                                 A + (k0 + 2 * NB) * lda + k0 + 2 * NB, lda, true, st));
@@ -1004,9 +1014,7 @@ static int cholesky_impl(gk_handle h, T* A, int64_t n, int64_t lda, T* dinv, int
       GK_HIP(hipStreamWaitEvent(st, ev[2], 0));
     }
   }
-  if (sizeof(T) == 8)                     // chain planes of the triangular solves
-    for (int64_t kb = 0; kb < nb; ++kb)
-      GK_TRY(chain_planes<T>(A, lda, kb, dinv, A + (kb + 1) * NB * lda + kb * NB, lda, st));
+  GK_TRY(chain_planes<T>(A, lda, dinv, st));   // chain planes of the triangular solves
   GK_HIP(hipGetLastError());
   int32_t hinfo = 0;
   GK_HIP(hipMemcpyAsync(&hinfo, dinfo, sizeof(int32_t), hipMemcpyDeviceToHost, st));

@@ -420,19 +420,21 @@ __global__ void eval_finalize_kernel(EvalArgs a, int grad) {
 // with weight -i_res c_o W_s (W_s = sum of w_i over the surface): the
 // rest-minus-reference differencing of the interface covariance collapses
 // into per-surface reference sources.
-__global__ void surface_sums_kernel(const double* wI, const int32_t* surf, int64_t R, double* W) {
-  __shared__ double red[256];
+__global__ __launch_bounds__(1024) void surface_sums_kernel(const double* wI, const int32_t* surf, int64_t R,
+                                                             double* W) {
+  __shared__ double red[16];
   const int s = blockIdx.x;
   double acc = 0.0;
-  for (int64_t i = threadIdx.x; i < R; i += 256)
+  for (int64_t i = threadIdx.x; i < R; i += 1024)
     if (surf[i] == s) acc += wI[i];
-  red[threadIdx.x] = acc;
+  for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
   __syncthreads();
-  for (int w = 128; w > 0; w >>= 1) {
-    if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
-    __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int w = 0; w < 16; ++w) t += red[w];
+    W[s] = t;
   }
-  if (threadIdx.x == 0) W[s] = red[0];
 }
 
 template <typename T>
@@ -522,7 +524,7 @@ static int prepare_sources(gk_handle h, const gk_system* sys, const double* weig
   void* p8 = base + off;
   (void)h;
   if (S > 0)
-    hipLaunchKernelGGL(surface_sums_kernel, dim3((unsigned)S), dim3(256), 0, st,
+    hipLaunchKernelGGL(surface_sums_kernel, dim3((unsigned)S), dim3(1024), 0, st,
                        weights + 3 * no, sys->rest_surf, R, W);
   if (R + S > 0) {
     unsigned nb = (unsigned)gk_ceil_div(R + S, 256);
@@ -669,8 +671,12 @@ extern "C" int gk_eval_impl(gk_handle h, const gk_system* sys, const double* wei
   GK_TRY(gk_workspace(h, WS_EVAL_SRC, sources_bytes(sys), &ws));
   GK_TRY(prepare_sources(h, sys, weights, d->precision, st, &a.pts, &a.ori, ws));
   bool masked = true;
-  if (!force_mask && d->kind == GK_POINTS_LATTICE && src_box) {
+  if (!force_mask && src_box) {
+    // explicit points: only callers whose points lie inside the source box
+    // (the data points themselves) pass a box for them
     double sl[3] = {src_box[0], src_box[1], src_box[2]}, sh[3] = {src_box[3], src_box[4], src_box[5]};
+    if (d->kind == GK_POINTS_EXPLICIT)
+      for (int k = 0; k < 3; ++k) { lo[k] = sl[k]; hi[k] = sh[k]; }
     masked = needs_mask(sys, lo, hi, sl, sh);
   }
   GK_TRY(run_eval(h, a, d->kind == GK_POINTS_LATTICE, d->precision, d->gradient != nullptr, masked, st));
@@ -765,7 +771,7 @@ __global__ void gather_data_points_kernel(gk_system sys, double* pts) {
 }  // namespace
 
 extern "C" int gk_matvec_impl(gk_handle h, const gk_system* sys, int32_t precision, const double* x,
-                              double* y, hipStream_t st, void* scratch) {
+                              double* y, hipStream_t st, void* scratch, const double* src_box) {
   // scratch: >= (R+S)*3 + (R+S) + n_ori*4 doubles
   const int64_t R = sys->n_rest, S = sys->n_surf, no = sys->n_ori;
   double* pts = static_cast<double*>(scratch);
@@ -784,7 +790,7 @@ extern "C" int gk_matvec_impl(gk_handle h, const gk_system* sys, int32_t precisi
     d.n_points = R + S;
     d.xyz = pts;
     d.scalar = s_data;
-    GK_TRY(gk_eval_impl(h, &nofault, x, &d, st, true, nullptr));
+    GK_TRY(gk_eval_impl(h, &nofault, x, &d, st, src_box == nullptr, src_box));
   }
   if (no > 0) {
     gk_eval_desc d{};
@@ -794,7 +800,7 @@ extern "C" int gk_matvec_impl(gk_handle h, const gk_system* sys, int32_t precisi
     d.xyz = sys->ori_xyz;
     d.scalar = s_ori;
     d.gradient = g_ori;
-    GK_TRY(gk_eval_impl(h, &nofault, x, &d, st, true, nullptr));
+    GK_TRY(gk_eval_impl(h, &nofault, x, &d, st, src_box == nullptr, src_box));
   }
   const int64_t NG = 3 * no;
   if (NG + R > 0)

@@ -21,8 +21,9 @@ extern "C" int gk_rhs_impl(const gk_system* sys, double* b, hipStream_t st);
 extern "C" int gk_diag_scale_impl(const gk_system* sys, double* d, hipStream_t st);
 extern "C" int gk_spai_apply_impl(const gk_system* sys, const int32_t* idx, const float* Y, const double* u, double* y,
                                   hipStream_t st);
-extern "C" int gk_matvec(gk_handle h, const gk_system* sys, int32_t precision, const double* x, double* y,
-                         gk_stream stream);
+extern "C" int gk_matvec_boxed(gk_handle h, const gk_system* sys, int32_t precision, const double* x, double* y,
+                               gk_stream stream, const double* src_box);
+extern "C" int gk_source_box(gk_handle h, const gk_system* sys, double* box, gk_stream stream);
 extern "C" int gk_knn(gk_handle h, const gk_system* sys, int32_t* idx, float* hk, gk_stream stream);
 extern "C" int gk_spai_local_inverse(gk_handle h, const gk_system* sys, const int32_t* idx, const double* dscale,
                                      float* Y, gk_stream stream);
@@ -35,24 +36,21 @@ namespace {
 
 constexpr int RED_ROWS = 4096;   // rows per block of the column-dot kernels
 
-__global__ void dots_partial_kernel(const double* V, int64_t ldv, int ncols, const double* w, int64_t n,
-                                    double* partial) {
-  __shared__ double red[256];
+// partial[b][c] = sum over rows [b*RED_ROWS, (b+1)*RED_ROWS) of V[:, c] * w; one
+// workgroup per (row block, column), so a Gram-Schmidt step's dots fill the chip.
+__global__ __launch_bounds__(256) void dots_partial_kernel(const double* V, int64_t ldv, int ncols, const double* w,
+                                                           int64_t n, double* partial) {
+  __shared__ double red[4];
+  const int c = blockIdx.y;
   const int64_t r0 = (int64_t)blockIdx.x * RED_ROWS;
   const int64_t r1 = std::min<int64_t>(n, r0 + RED_ROWS);
-  for (int c = 0; c < ncols; ++c) {
-    double acc = 0.0;
-    const double* v = V + (int64_t)c * ldv;
-    for (int64_t i = r0 + threadIdx.x; i < r1; i += 256) acc = fma(v[i], w[i], acc);
-    red[threadIdx.x] = acc;
-    __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
-      if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
-      __syncthreads();
-    }
-    if (threadIdx.x == 0) partial[(int64_t)blockIdx.x * ncols + c] = red[0];
-    __syncthreads();
-  }
+  const double* v = V + (int64_t)c * ldv;
+  double acc = 0.0;
+  for (int64_t i = r0 + threadIdx.x; i < r1; i += 256) acc = fma(v[i], w[i], acc);
+  for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) partial[(int64_t)blockIdx.x * ncols + c] = (red[0] + red[1]) + (red[2] + red[3]);
 }
 
 __global__ void reduce_cols_kernel(const double* partial, int nblk, int ncols, double* out) {
@@ -222,6 +220,8 @@ struct Solver {
   const int32_t* spai_idx = nullptr;
   const float* spai_Y = nullptr;
   int32_t mv_prec = GK_F64;          // precision of the Krylov matvec
+  double box[6] = {0, 0, 0, 0, 0, 0}; // bounding box of the system's points (drops the operator's range mask)
+  const double* mv_box = nullptr;
   double shift = 0.0;                // diagonal shift of the factored block
   void* H = nullptr;
   void* dinv = nullptr;
@@ -247,7 +247,8 @@ struct Solver {
   int host_dots(const double* Vc, int64_t ldv, int ncols, const double* vec, int64_t n, double* out_dev,
                 double* out_host) {
     const unsigned nb = nblk(n, RED_ROWS);
-    hipLaunchKernelGGL(dots_partial_kernel, dim3(nb), dim3(256), 0, st, Vc, ldv, ncols, vec, n, partial);
+    hipLaunchKernelGGL(dots_partial_kernel, dim3(nb, (unsigned)ncols), dim3(256), 0, st, Vc, ldv, ncols, vec, n,
+                       partial);
     hipLaunchKernelGGL(reduce_cols_kernel, dim3(1), dim3(128), 0, st, partial, (int)nb, ncols, out_dev);
     if (out_host) {
       GK_HIP(hipMemcpyAsync(out_host, out_dev, ncols * sizeof(double), hipMemcpyDeviceToHost, st));
@@ -273,7 +274,7 @@ struct Solver {
   // v = u - H y on the equilibrated covariance block (H = D A_cc D), matrix-free
   int scaled_residual(const double* u, const double* y, double* v) {
     hipLaunchKernelGGL(unscale_pad_kernel, dim3(nblk(N)), dim3(256), 0, st, y, dscale, pc_x, Nc, N);
-    GK_TRY(gk_matvec(h, sys, GK_F64, pc_x, pc_w, st));
+    GK_TRY(gk_matvec_boxed(h, sys, GK_F64, pc_x, pc_w, st, mv_box));
     hipLaunchKernelGGL(scaled_residual_kernel, dim3(nblk(Nc)), dim3(256), 0, st, u, dscale, pc_w, v, Nc);
     return GK_OK;
   }
@@ -456,7 +457,7 @@ struct Solver {
       return GK_OK;
     };
     auto true_rel = [&](const double* xv, double* out) -> int {
-      GK_TRY(gk_matvec(h, sys, GK_F64, xv, w, st));
+      GK_TRY(gk_matvec_boxed(h, sys, GK_F64, xv, w, st, mv_box));
       hipLaunchKernelGGL(sub_kernel, dim3(nblk(N)), dim3(256), 0, st, b, w, r, N);
       double nr = 0.0;
       GK_TRY(norm(r, N, &nr));
@@ -479,7 +480,7 @@ struct Solver {
         double* vj = V + (int64_t)j * Npad;
         double* zj = Z + (int64_t)j * Npad;
         GK_TRY(apply_M(vj, zj));
-        GK_TRY(gk_matvec(h, sys, mv_prec, zj, w, st));
+        GK_TRY(gk_matvec_boxed(h, sys, mv_prec, zj, w, st, mv_box));
         std::vector<double> hc(j + 1, 0.0);
         for (int pass = 0; pass < 2; ++pass) {      // classical Gram-Schmidt, two passes
           GK_TRY(host_dots(V, Npad, j + 1, w, N, hdev, hv.data()));
@@ -557,7 +558,7 @@ struct Solver {
     double prev = 1e300, rel = 0.0;
     int k = 0;
     for (; k < max_steps; ++k) {
-      GK_TRY(gk_matvec(h, sys, GK_F64, x, w, st));
+      GK_TRY(gk_matvec_boxed(h, sys, GK_F64, x, w, st, mv_box));
       hipLaunchKernelGGL(sub_kernel, dim3(nblk(N)), dim3(256), 0, st, b, w, r, N);
       double nr = 0.0;
       GK_TRY(norm(r, N, &nr));
@@ -576,7 +577,7 @@ struct Solver {
   int true_relres(const double* x, double* out) {
     double nb = 0.0, nr = 0.0;
     GK_TRY(norm(b, N, &nb));
-    GK_TRY(gk_matvec(h, sys, GK_F64, x, w, st));
+    GK_TRY(gk_matvec_boxed(h, sys, GK_F64, x, w, st, mv_box));
     hipLaunchKernelGGL(sub_kernel, dim3(nblk(N)), dim3(256), 0, st, b, w, r, N);
     GK_TRY(norm(r, N, &nr));
     *out = nb > 0 ? nr / nb : nr;
@@ -627,6 +628,10 @@ static int init_solver(gk_handle h, const gk_system* sys, hipStream_t st, int fo
   const int64_t Lt = L + sys->n_fault;
   const int64_t Nc = N - Lt;
   S.h = h; S.sys_copy = *sys; S.sys = &S.sys_copy; S.st = st;
+  if (sys->n_ori + sys->n_rest + sys->n_surf > 0) {
+    GK_TRY(gk_source_box(h, sys, S.box, st));
+    S.mv_box = S.box;
+  }
   S.N = N; S.Nc = Nc; S.Lt = Lt; S.Npad = Npad; S.Ncpad = gk_round_up(std::max<int64_t>(Nc, 1), GK_BLOCK);
   const bool fast = form != GK_SOLVE_FP64_LU && form != GK_SOLVE_FP64_CHOL;   // needs the Krylov basis
   const int m = fast ? S.m : 1;

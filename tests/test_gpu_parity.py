@@ -249,6 +249,39 @@ def test_cholesky_f64_and_solve(gpu):
     np.testing.assert_allclose(tb[:n].cpu().numpy(), np.linalg.solve(A, b), rtol=1e-9, atol=1e-11)
 
 
+@pytest.mark.parametrize("dtype", ["f32", "f64"])
+def test_cholesky_planes_and_inverse_blocks(gpu, dtype):
+    """The factor's side outputs over 9 block rows: Dinv_k is the inverse of
+    the k-th diagonal block of L, and the chain planes equal the products of
+    the blocks of L (both from numpy on the returned factor)."""
+    n, lda = 2200, 2304
+    nb = lda // 256
+    say(f"cholesky {dtype} planes n={n} lda={lda}")
+    L = _lib()
+    tdt = torch.float32 if dtype == "f32" else torch.float64
+    A = _spd(n, 21)
+    tA = torch.zeros((lda, lda), dtype=tdt, device=gpu)
+    tA[:n, :n] = torch.tensor(A, dtype=tdt)
+    dinv = torch.empty(L.GK_CHOL_PLANES * lda * 256, dtype=tdt, device=gpu)
+    info = C.c_int32(-1)
+    fn = L.load().gk_cholesky_f32 if dtype == "f32" else L.load().gk_cholesky_f64
+    L.check("chol", fn(_h(), _p(tA), n, lda, _p(dinv), C.byref(info), _st()))
+    torch.cuda.synchronize()
+    assert info.value == 0
+    Lm = torch.tril(tA).double().cpu().numpy()
+    want = _chain_planes(Lm, nb)
+    got = dinv.double().cpu().numpy().reshape(L.GK_CHOL_PLANES, lda, 256)
+    tol = 1e-4 if dtype == "f32" else 1e-11
+    for p in range(L.GK_CHOL_PLANES):
+        for k in range(nb):
+            rows = slice(256 * k, 256 * (k + 1))
+            w = want[p, rows]
+            if not w.any():
+                continue                    # planes with no block (k < d or k + d >= nb)
+            err = np.abs(got[p, rows] - w).max() / max(np.abs(w).max(), 1e-30)
+            assert err <= tol, (p, k, err)
+
+
 def test_lu_f64_and_solve(gpu):
     say("lu f64 n=600 lda=768 (saddle-point like)")
     L = _lib()
@@ -347,6 +380,43 @@ def test_matvec_matches_oracle(gpu, deg, fault):
     assert np.all(np.abs(y - ref) <= 1e-11 * scale + 1e-300)
     y32 = ds.matvec(torch.tensor(x, device=gpu), precision=_lib().GK_F32).cpu().numpy()
     assert np.all(np.abs(y32 - ref) <= 1e-5 * scale)
+
+
+@pytest.mark.parametrize("short_range", [False, True])
+def test_matvec_without_range_mask_is_identical(gpu, short_range):
+    """The solver's operator drops the r < a test when the points' bounding box
+    is within the range; the product must equal the masked one, and with a
+    range shorter than the box the mask must stay."""
+    say(f"matvec boxed short_range={short_range}")
+    from gempy_amd.engine import DeviceStack
+    L = _lib()
+    lib = L.load()
+    lib.gk_source_box.restype = C.c_int
+    lib.gk_source_box.argtypes = [C.c_void_p, C.POINTER(L.GkSystem), C.POINTER(C.c_double), C.c_void_p]
+    lib.gk_matvec_boxed.restype = C.c_int
+    lib.gk_matvec_boxed.argtypes = [C.c_void_p, C.POINTER(L.GkSystem), C.c_int32, C.c_void_p, C.c_void_p,
+                                    C.c_void_p, C.POINTER(C.c_double)]
+    st, fsp = _small_stack(3, fault=False)
+    ds = DeviceStack(st, _kopts(1), [500, 500, 500], fsp)
+    box = (C.c_double * 6)()
+    L.check("gk_source_box", lib.gk_source_box(ds.h, C.byref(ds.sys), box, _st()))
+    diag = np.sqrt(sum((box[3 + k] - box[k]) ** 2 for k in range(3)))
+    assert diag > 0
+    if short_range:
+        ds.sys.range = 0.4 * diag          # pairs beyond the range exist: C = 0 there
+    else:
+        assert diag < ds.sys.range          # the configured range covers every pair
+    x = torch.tensor(np.random.default_rng(8).standard_normal(ds.N), device=gpu)
+    for prec in (L.GK_F64, L.GK_F32):
+        y_mask = ds.matvec(x, precision=prec)
+        y_box = torch.empty_like(y_mask)
+        L.check("gk_matvec_boxed", lib.gk_matvec_boxed(ds.h, C.byref(ds.sys), prec, _p(x), _p(y_box), _st(), box))
+        torch.cuda.synchronize()
+        if short_range:                     # both keep the mask: the same kernel
+            assert torch.equal(y_mask, y_box)
+        else:                               # the same operations without the select
+            err = (y_mask - y_box).abs().max().item()
+            assert err <= 1e-12 * y_mask.abs().max().item(), (prec, err)
 
 
 @pytest.mark.parametrize("precision", ["f64", "f32"])
