@@ -375,50 +375,6 @@ int gemm_launch(int64_t m, int64_t n, int64_t k, const T* A, int64_t lda, const 
 // quadrant of the inverse).
 constexpr int DB = 128;
 
-// Cholesky of a DB x DB FP32 block (lower) in LDS and the inverse of its factor.
-__global__ __launch_bounds__(1024) void potrf128_kernel(float* A, int64_t lda, float* X, int64_t ldx,
-                                                        int32_t* info, int64_t gbase) {
-  __shared__ float L[DB][DB + 1];
-  __shared__ float Xs[DB][DB + 1];
-  const int tid = threadIdx.x, tx = tid & 31, ty = tid >> 5;
-  for (int e = tid; e < DB * DB; e += 1024) {
-    const int i = e / DB, c = e % DB;
-    L[i][c] = (c <= i) ? A[i * lda + c] : 0.0f;
-    Xs[i][c] = (i == c) ? 1.0f : 0.0f;
-  }
-  __syncthreads();
-  for (int j = 0; j < DB; ++j) {
-    float d = L[j][j];
-    const bool bad = !(d > 0.0f);
-    if (bad) d = 1.0f;
-    const float piv = sqrtf(d), ip = 1.0f / piv;
-    __syncthreads();
-    if (tid == 0) {
-      L[j][j] = piv;
-      if (bad) atomicCAS(info, 0, (int32_t)(gbase + j + 1));
-    }
-    if (tid > j && tid < DB) L[tid][j] *= ip;
-    __syncthreads();
-    for (int i = j + 1 + ty; i < DB; i += 32)
-      for (int c = j + 1 + tx; c <= i; c += 32) L[i][c] -= L[i][j] * L[c][j];
-    __syncthreads();
-  }
-  // X = L^-1, right-looking forward substitution
-  for (int k = 0; k < DB; ++k) {
-    const float ip = 1.0f / L[k][k];
-    if (tid <= k) Xs[k][tid] *= ip;
-    __syncthreads();
-    for (int i = k + 1 + ty; i < DB; i += 32)
-      for (int c = tx; c <= k; c += 32) Xs[i][c] -= L[i][k] * Xs[k][c];
-    __syncthreads();
-  }
-  for (int e = tid; e < DB * DB; e += 1024) {
-    const int i = e / DB, c = e % DB;
-    if (c <= i) A[i * lda + c] = L[i][c];
-    X[i * ldx + c] = Xs[i][c];
-  }
-}
-
 // LU without pivoting of a DB x DB FP64 block in LDS, then the inverses of its
 // unit-lower factor (Li) and of its upper factor (Ui), each computed with the
 // factor and the inverse packed in the same LDS buffer.
@@ -496,119 +452,105 @@ __global__ __launch_bounds__(1024) void getrf128_kernel(double* A, int64_t lda, 
   }
 }
 
-// Cholesky of a DB x DB FP64 block and the inverse of its factor, both packed
-// (lower triangle) in one LDS buffer.
-__global__ __launch_bounds__(1024) void potrf128_f64_kernel(double* A, int64_t lda, double* X, int64_t ldx,
-                                                            int32_t* info, int64_t gbase) {
-  __shared__ double S[DB * (DB + 1)];
-  double* Lp = S;                         // packed lower incl. diagonal
-  double* Xp = S + DB * (DB + 1) / 2;
-  auto pl = [](int i, int c) { return i * (i + 1) / 2 + c; };
-  const int tid = threadIdx.x, tx = tid & 31, ty = tid >> 5;
-  for (int e = tid; e < DB * DB; e += 1024) {
-    const int i = e / DB, c = e % DB;
-    if (c <= i) { Lp[pl(i, c)] = A[i * lda + c]; Xp[pl(i, c)] = (i == c) ? 1.0 : 0.0; }
-  }
-  __syncthreads();
-  for (int j = 0; j < DB; ++j) {
-    double d = Lp[pl(j, j)];
-    const bool bad = !(d > 0.0);
-    if (bad) d = 1.0;
-    const double piv = sqrt(d), ip = 1.0 / piv;
-    __syncthreads();
-    if (tid == 0) {
-      Lp[pl(j, j)] = piv;
-      if (bad) atomicCAS(info, 0, (int32_t)(gbase + j + 1));
-    }
-    if (tid > j && tid < DB) Lp[pl(tid, j)] *= ip;
-    __syncthreads();
-    for (int i = j + 1 + ty; i < DB; i += 32) {
-      const double lij = Lp[pl(i, j)];
-      for (int c = j + 1 + tx; c <= i; c += 32) Lp[pl(i, c)] -= lij * Lp[pl(c, j)];
-    }
-    __syncthreads();
-  }
-  for (int k = 0; k < DB; ++k) {
-    const double ip = 1.0 / Lp[pl(k, k)];
-    if (tid <= k) Xp[pl(k, tid)] *= ip;
-    __syncthreads();
-    for (int i = k + 1 + ty; i < DB; i += 32) {
-      const double l = Lp[pl(i, k)];
-      for (int c = tx; c <= k; c += 32) Xp[pl(i, c)] -= l * Xp[pl(k, c)];
-    }
-    __syncthreads();
-  }
-  for (int e = tid; e < DB * DB; e += 1024) {
-    const int i = e / DB, c = e % DB;
-    if (c <= i) A[i * lda + c] = Lp[pl(i, c)];
-    X[i * ldx + c] = (c <= i) ? Xp[pl(i, c)] : 0.0;
-  }
-}
-
 // Blocked Cholesky + inverse of a DB x DB block (one workgroup of NT threads),
-// two barriers per PB columns.  Phase 1 (wave 0, registers, cross-lane
-// broadcasts): factor the PB-column panel, then the PB rows of X = L^-1,
+// two barriers per PB columns.  Phase 1 (wave 0, registers, lane values
+// broadcast through scalar registers): factor the PB-column panel, then the PB
+// rows of X = L^-1,
 //   X[i][c] = (delta_ic - acc[i][c] - sum_{jb<=k<i} L[i][k] X[k][c]) / L[i][i],
 // where acc[i][c] = sum_{k<jb} L[i][k] X[k][c] was applied eagerly.  Phase 2
 // (all threads): the rank-PB trailing update of L and the eager update of the
-// later rows of X by the new panel and rows, in one sweep per row.  Storage:
-// packed lower triangles of L and X in LDS.
+// later rows of X by the new panel and rows, in one sweep per row.
 #ifndef GK_POTRF_PB_F32
 #define GK_POTRF_PB_F32 8
 #endif
 #ifndef GK_POTRF_PB_F64
 #define GK_POTRF_PB_F64 4
 #endif
-// L and X = L^-1 in LDS: packed lower triangles in one array (FP32 66 KB,
-// FP64 132 KB), or (SQ) L square with a padded row and X packed (FP32 99 KB).
-template <typename T, bool SQ = false>
+// LDS: L and X = L^-1 as packed lower triangles, then two panel buffers P[b]
+// holding the PB columns of the current / next panel column-major (P[b][t][i]
+// = L[i][jb + t]), so the panel is read with consecutive addresses (the packed
+// rows would put lanes on the same banks).  FP32 72.5 KB, FP64 137 KB.
+template <typename T, int PB>
 struct BlkStore {
   T* S;
+  static constexpr int SIZE = DB * (DB + 1) + 2 * PB * DB;
   __device__ static int pl(int i, int c) { return i * (i + 1) / 2 + c; }
-  __device__ T& L(int i, int c) { return SQ ? S[i * (DB + 1) + c] : S[pl(i, c)]; }
-  __device__ T& X(int i, int c) { return S[(SQ ? DB * (DB + 1) : DB * (DB + 1) / 2) + pl(i, c)]; }
+  __device__ T& L(int i, int c) { return S[pl(i, c)]; }
+  __device__ T& X(int i, int c) { return S[DB * (DB + 1) / 2 + pl(i, c)]; }
+  __device__ T& P(int b, int t, int i) { return S[DB * (DB + 1) + (b * PB + t) * DB + i]; }
 };
-// FP32 diagonal-block kernel: threads and layout.  Measured in the look-ahead
-// factorization at N = 35072 (factor ms): 1024 packed 197.5, 512 packed
-// 199.7, 512 square-L 204, 1024 square-L 219.
+// FP32 diagonal-block kernel: threads.  It runs on the side stream beside the
+// trailing update, whose workgroups (512 threads, 80 VGPRs, 40 KB LDS) sit
+// three to a CU: with 512 threads, < 96 VGPRs and < 80 KB of LDS it fits on a
+// CU where one of them has finished, while a 1024-thread version needs an
+// empty CU and waits for the end of the update.
 #ifndef GK_POTRF_NT_F32
-#define GK_POTRF_NT_F32 1024
-#endif
-#ifndef GK_POTRF_SQ_F32
-#define GK_POTRF_SQ_F32 0
+#define GK_POTRF_NT_F32 512
 #endif
 
-template <typename T, int PB, int NT, bool SQ>
-__device__ void potrf_blocked(BlkStore<T, SQ> B, T* A, int64_t lda, T* X, int64_t ldx, int32_t* info, int64_t gbase) {
+// Value of v in lane l (l uniform) as a scalar-register broadcast.
+__device__ __forceinline__ float lane_value(float v, int l) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+__device__ __forceinline__ double lane_value(double v, int l) {
+  const int lo = __builtin_amdgcn_readlane(__double2loint(v), l);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), l);
+  return __hiloint2double(hi, lo);
+}
+
+// Pivot d > 0 -> (sqrt(d), 1/sqrt(d)).  FP32: the hardware reciprocal square
+// root (within 1 ulp), the factor being an FP32 preconditioner; FP64: IEEE.
+__device__ __forceinline__ void pivot_pair(float d, float& piv, float& ip) {
+  ip = __builtin_amdgcn_rsqf(d);
+  piv = d * ip;
+}
+__device__ __forceinline__ void pivot_pair(double d, double& piv, double& ip) {
+  piv = sqrt(d);
+  ip = 1.0 / piv;
+}
+
+template <typename T, int PB, int NT>
+__device__ void potrf_blocked(BlkStore<T, PB> B, T* A, int64_t lda, T* X, int64_t ldx, int32_t* info, int64_t gbase) {
   constexpr int RS = NT / 32;                 // rows of the trailing update in flight
   const int tid = threadIdx.x, tx = tid & 31, ty = tid >> 5, lane = tid & 63, wave = tid >> 6;
   for (int e = tid; e < DB * DB; e += NT) {
     const int i = e / DB, c = e % DB;
-    if (c <= i) { B.L(i, c) = A[i * lda + c]; B.X(i, c) = T(0); }
+    if (c <= i) {
+      const T v = A[i * lda + c];
+      B.L(i, c) = v;
+      B.X(i, c) = T(0);
+      if (c < PB) B.P(0, c, i) = v;
+    }
   }
   __syncthreads();
-  for (int jb = 0; jb < DB; jb += PB) {
+  int bad_col = 0;                            // first non-positive pivot + 1 (wave 0)
+  for (int jb = 0, pb = 0; jb < DB; jb += PB, pb ^= 1) {
     if (wave == 0) {
       // ---- panel: lane l holds rows jb + l (a) and jb + 64 + l (b)
       const int ra = jb + lane, rb = jb + 64 + lane;
-      T a[PB], b[PB];
+      const int rac = min(ra, DB - 1), rbc = min(rb, DB - 1);
+      T a[PB], b[PB], ipv[PB];
 #pragma unroll
       for (int c = 0; c < PB; ++c) {
-        a[c] = (ra < DB && jb + c <= ra) ? B.L(ra, jb + c) : T(0);
-        b[c] = (rb < DB) ? B.L(rb, jb + c) : T(0);
+        const T va = B.P(pb, c, rac), vb = B.P(pb, c, rbc);
+        a[c] = (ra < DB && jb + c <= ra) ? va : T(0);
+        b[c] = rb < DB ? vb : T(0);
       }
 #pragma unroll
       for (int j = 0; j < PB; ++j) {
-        T d = __shfl(a[j], j);                    // L[jb+j][jb+j], held by lane j
-        const bool bad = !(d > T(0));
-        if (bad) d = T(1);
-        const T piv = sqrt(d), ip = T(1) / piv;
-        if (bad && lane == 0) atomicCAS(info, 0, (int32_t)(gbase + jb + j + 1));
+        T d = lane_value(a[j], j);                // L[jb+j][jb+j], held by lane j
+        if (!(d > T(0))) {
+          if (bad_col == 0) bad_col = jb + j + 1;
+          d = T(1);
+        }
+        T piv, ip;
+        pivot_pair(d, piv, ip);
+        ipv[j] = ip;
         a[j] = lane == j ? piv : (lane > j ? a[j] * ip : a[j]);
         b[j] *= ip;
 #pragma unroll
         for (int c = j + 1; c < PB; ++c) {
-          const T lcj = __shfl(a[j], c);          // L[jb+c][jb+j]
+          const T lcj = lane_value(a[j], c);      // L[jb+c][jb+j]
           if (lane > j) a[c] -= a[j] * lcj;
           b[c] -= b[j] * lcj;
         }
@@ -617,6 +559,8 @@ __device__ void potrf_blocked(BlkStore<T, SQ> B, T* A, int64_t lda, T* X, int64_
       for (int c = 0; c < PB; ++c) {
         if (ra < DB && jb + c <= ra) B.L(ra, jb + c) = a[c];
         if (rb < DB) B.L(rb, jb + c) = b[c];
+        if (ra < DB) B.P(pb, c, ra) = a[c];
+        if (rb < DB) B.P(pb, c, rb) = b[c];
       }
       // ---- rows jb .. jb+PB-1 of X: lane owns columns lane and lane + 64
       T xa[PB], xb[PB];
@@ -628,13 +572,12 @@ __device__ void potrf_blocked(BlkStore<T, SQ> B, T* A, int64_t lda, T* X, int64_
         T vb = cb < jb ? B.X(i, cb) : (cb == i ? T(1) : T(0));
 #pragma unroll
         for (int k = 0; k < r; ++k) {
-          const T lik = __shfl(a[k], r);          // L[jb+r][jb+k]
+          const T lik = lane_value(a[k], r);      // L[jb+r][jb+k]
           va -= lik * xa[k];
           vb -= lik * xb[k];
         }
-        const T ip = T(1) / __shfl(a[r], r);
-        xa[r] = va * ip;
-        xb[r] = vb * ip;
+        xa[r] = va * ipv[r];
+        xb[r] = vb * ipv[r];
       }
 #pragma unroll
       for (int r = 0; r < PB; ++r) {
@@ -644,19 +587,22 @@ __device__ void potrf_blocked(BlkStore<T, SQ> B, T* A, int64_t lda, T* X, int64_
       }
     }
     __syncthreads();
-    // ---- rows i >= c0: L[i][c] -= sum_t L[i][jb+t] L[c][jb+t]   (c0 <= c <= i)
+    // ---- rows i >= c0: L[i][c] -= sum_t L[i][jb+t] L[c][jb+t]   (c0 <= c <= i;
+    //      the next panel's columns also into P[pb^1])
     //                    -acc[i][c] -= sum_t L[i][jb+t] X[jb+t][c] (c < c0, X lower)
     const int c0 = jb + PB;
     for (int i = c0 + ty; i < DB; i += RS) {
       T li[PB];
 #pragma unroll
-      for (int t = 0; t < PB; ++t) li[t] = B.L(i, jb + t);
+      for (int t = 0; t < PB; ++t) li[t] = B.P(pb, t, i);
       for (int c = tx; c <= i; c += 32) {
         T sacc = T(0);
         if (c >= c0) {
 #pragma unroll
-          for (int t = 0; t < PB; ++t) sacc = fma(li[t], B.L(c, jb + t), sacc);
-          B.L(i, c) -= sacc;
+          for (int t = 0; t < PB; ++t) sacc = fma(li[t], B.P(pb, t, c), sacc);
+          const T v = B.L(i, c) - sacc;
+          B.L(i, c) = v;
+          if (c < c0 + PB) B.P(pb ^ 1, c - c0, i) = v;
         } else {
 #pragma unroll
           for (int t = 0; t < PB; ++t) {
@@ -669,6 +615,7 @@ __device__ void potrf_blocked(BlkStore<T, SQ> B, T* A, int64_t lda, T* X, int64_
     }
     __syncthreads();
   }
+  if (wave == 0 && lane == 0 && bad_col != 0) atomicCAS(info, 0, (int32_t)(gbase + bad_col));
   for (int e = tid; e < DB * DB; e += NT) {
     const int i = e / DB, c = e % DB;
     if (c <= i) A[i * lda + c] = B.L(i, c);
@@ -676,20 +623,20 @@ __device__ void potrf_blocked(BlkStore<T, SQ> B, T* A, int64_t lda, T* X, int64_
   }
 }
 
-// FP32: packed 66 KB of LDS, so the kernel fits on a CU beside workgroups of
-// the concurrent trailing update (the look-ahead factorization runs it on a
+// FP32: 72.5 KB of LDS, so the kernel fits on a CU beside workgroups of the
+// concurrent trailing update (the look-ahead factorization runs it on a
 // second stream).
 __global__ __launch_bounds__(GK_POTRF_NT_F32) void potrf128b_f32_kernel(float* A, int64_t lda, float* X, int64_t ldx,
                                                                         int32_t* info, int64_t gbase) {
-  __shared__ float S[(GK_POTRF_SQ_F32 ? DB * (DB + 1) : 0) + DB * (DB + 1) / 2 * (GK_POTRF_SQ_F32 ? 1 : 2)];
-  potrf_blocked<float, GK_POTRF_PB_F32, GK_POTRF_NT_F32, GK_POTRF_SQ_F32>(BlkStore<float, GK_POTRF_SQ_F32>{S}, A,
-                                                                          lda, X, ldx, info, gbase);
+  __shared__ float S[BlkStore<float, GK_POTRF_PB_F32>::SIZE];
+  potrf_blocked<float, GK_POTRF_PB_F32, GK_POTRF_NT_F32>(BlkStore<float, GK_POTRF_PB_F32>{S}, A, lda, X, ldx, info,
+                                                         gbase);
 }
 
 __global__ __launch_bounds__(1024) void potrf128b_f64_kernel(double* A, int64_t lda, double* X, int64_t ldx,
                                                              int32_t* info, int64_t gbase) {
-  __shared__ double S[DB * (DB + 1)];
-  potrf_blocked<double, GK_POTRF_PB_F64, 1024, false>(BlkStore<double>{S}, A, lda, X, ldx, info, gbase);
+  __shared__ double S[BlkStore<double, GK_POTRF_PB_F64>::SIZE];
+  potrf_blocked<double, GK_POTRF_PB_F64, 1024>(BlkStore<double, GK_POTRF_PB_F64>{S}, A, lda, X, ldx, info, gbase);
 }
 
 template <typename T> struct Potrf128;

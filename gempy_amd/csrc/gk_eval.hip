@@ -703,50 +703,51 @@ __global__ void matvec_combine_kernel(gk_system sys, const double* x, const doub
 }
 
 // Drift / fault rows: deterministic block reductions, one block per row.
-__global__ void matvec_drift_rows_kernel(gk_system sys, const double* x, double* y) {
-  __shared__ double red[256];
+// Drift rows: y[NG+R+l] = sum over the orientation and interface unknowns of
+// the drift function's derivative / difference times x (one workgroup per row).
+__global__ __launch_bounds__(1024) void matvec_drift_rows_kernel(gk_system sys, const double* x, double* y) {
+  __shared__ double red[16];
   const int l = blockIdx.x;                  // 0..L+F-1
   const int64_t no = sys.n_ori, R = sys.n_rest, NG = 3 * no;
   const int L = sys.uni_degree == 0 ? 0 : (sys.uni_degree == 1 ? 3 : 9);
   double acc = 0.0;
-  for (int64_t i = threadIdx.x; i < NG + R; i += 256) {
-    double v = 0.0;
-    if (l < L) {
-      if (i < NG) {
-        const int64_t u = i / no, g = i % no;
-        const double* q = sys.ori_xyz + 3 * g;
+  if (l < L) {
+    for (int64_t g = threadIdx.x; g < no; g += 1024) {
+      const double* q = sys.ori_xyz + 3 * g;
+#pragma unroll
+      for (int u = 0; u < 3; ++u) {
         double df;
         if (l < 3) df = (l == u) ? 1.0 : 0.0;
         else if (l < 6) df = (l - 3 == u) ? 2.0 * q[u] : 0.0;
         else if (l == 6) df = (u == 0) ? q[1] : (u == 1 ? q[0] : 0.0);
         else if (l == 7) df = (u == 0) ? q[2] : (u == 2 ? q[0] : 0.0);
         else df = (u == 1) ? q[2] : (u == 2 ? q[1] : 0.0);
-        v = df * x[i];
-      } else {
-        const int64_t r = i - NG;
-        const double* p = sys.rest_xyz + 3 * r;
-        const double* p0 = sys.ref_xyz + 3 * sys.rest_surf[r];
-        double f1, f0;
-        if (l < 3) { f1 = p[l]; f0 = p0[l]; }
-        else if (l < 6) { f1 = p[l - 3] * p[l - 3]; f0 = p0[l - 3] * p0[l - 3]; }
-        else if (l == 6) { f1 = p[0] * p[1]; f0 = p0[0] * p0[1]; }
-        else if (l == 7) { f1 = p[0] * p[2]; f0 = p0[0] * p0[2]; }
-        else { f1 = p[1] * p[2]; f0 = p0[1] * p0[2]; }
-        v = sys.gi_res * (f1 - f0) * x[i];
+        acc += df * x[u * no + g];
       }
-    } else if (i >= NG) {
-      const int64_t r = i - NG;
-      v = sys.gi_res * sys.fault_rest[r * sys.n_fault + (l - L)] * x[i];
     }
-    acc += v;
+    for (int64_t r = threadIdx.x; r < R; r += 1024) {
+      const double* p = sys.rest_xyz + 3 * r;
+      const double* p0 = sys.ref_xyz + 3 * sys.rest_surf[r];
+      double f1, f0;
+      if (l < 3) { f1 = p[l]; f0 = p0[l]; }
+      else if (l < 6) { f1 = p[l - 3] * p[l - 3]; f0 = p0[l - 3] * p0[l - 3]; }
+      else if (l == 6) { f1 = p[0] * p[1]; f0 = p0[0] * p0[1]; }
+      else if (l == 7) { f1 = p[0] * p[2]; f0 = p0[0] * p0[2]; }
+      else { f1 = p[1] * p[2]; f0 = p0[1] * p0[2]; }
+      acc += sys.gi_res * (f1 - f0) * x[NG + r];
+    }
+  } else {
+    for (int64_t r = threadIdx.x; r < R; r += 1024)
+      acc += sys.gi_res * sys.fault_rest[r * sys.n_fault + (l - L)] * x[NG + r];
   }
-  red[threadIdx.x] = acc;
+  for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
   __syncthreads();
-  for (int w = 128; w > 0; w >>= 1) {
-    if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
-    __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int w = 0; w < 16; ++w) t += red[w];
+    y[NG + R + l] = t;
   }
-  if (threadIdx.x == 0) y[NG + R + l] = red[0];
 }
 
 // Fault-drift contribution to the interface rows (not part of the field eval).
@@ -811,7 +812,7 @@ extern "C" int gk_matvec_impl(gk_handle h, const gk_system* sys, int32_t precisi
                        st, *sys, x, y);
   const int L = n_drift_of(sys->uni_degree);
   if (L + sys->n_fault > 0)
-    hipLaunchKernelGGL(matvec_drift_rows_kernel, dim3(L + sys->n_fault), dim3(256), 0, st, *sys, x, y);
+    hipLaunchKernelGGL(matvec_drift_rows_kernel, dim3(L + sys->n_fault), dim3(1024), 0, st, *sys, x, y);
   GK_HIP(hipGetLastError());
   return GK_OK;
 }
