@@ -61,6 +61,25 @@ __global__ void reduce_cols_kernel(const double* partial, int nblk, int ncols, d
   out[c] = s;
 }
 
+// Second Gram-Schmidt pass: tmp = column sums (the pass's coefficients) and
+// acc += tmp (the Hessenberg column).
+__global__ void reduce_cols_acc_kernel(const double* partial, int nblk, int ncols, double* tmp, double* acc) {
+  const int c = threadIdx.x;
+  if (c >= ncols) return;
+  double s = 0.0;
+  for (int b = 0; b < nblk; ++b) s += partial[(int64_t)b * ncols + c];
+  tmp[c] = s;
+  acc[c] += s;
+}
+
+// v = w / |w| with |w|^2 read from device memory (0 for a zero vector).
+__global__ void normalize_kernel(const double* w, const double* nrm2, double* v, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double hn = sqrt(fmax(*nrm2, 0.0));
+  v[i] = hn > 0.0 ? w[i] * (1.0 / hn) : 0.0;
+}
+
 // w -= V h   (V columns of length n, h on device)
 __global__ void gemv_sub_kernel(const double* V, int64_t ldv, int ncols, const double* h, double* w,
                                 int64_t n) {
@@ -241,6 +260,7 @@ struct Solver {
   double* w = nullptr;
   double* r = nullptr;
   double* hdev = nullptr;
+  double* hcol = nullptr;            // Hessenberg columns on the device: column j at j (m + 2)
   double* b = nullptr;
   double* xt = nullptr;
 
@@ -255,6 +275,19 @@ struct Solver {
       GK_HIP(hipStreamSynchronize(st));
     }
     return GK_OK;
+  }
+
+  // out_dev[c] = <V[:, c], vec> (c < ncols) on the stream, no host copy.
+  void dev_dots(const double* Vc, int64_t ldv, int ncols, const double* vec, int64_t n, double* out_dev,
+                double* acc_dev = nullptr) {
+    const unsigned nb = nblk(n, RED_ROWS);
+    hipLaunchKernelGGL(dots_partial_kernel, dim3(nb, (unsigned)ncols), dim3(256), 0, st, Vc, ldv, ncols, vec, n,
+                       partial);
+    if (acc_dev)
+      hipLaunchKernelGGL(reduce_cols_acc_kernel, dim3(1), dim3(128), 0, st, partial, (int)nb, ncols, out_dev,
+                         acc_dev);
+    else
+      hipLaunchKernelGGL(reduce_cols_kernel, dim3(1), dim3(128), 0, st, partial, (int)nb, ncols, out_dev);
   }
 
   int norm(const double* v, int64_t n, double* out) {
@@ -438,7 +471,7 @@ struct Solver {
     int total = 0;
     double last_true = 1.0;
     std::vector<double> hist;                 // true residual at each check
-    std::vector<double> Hh((m + 1) * m), cs(m), sn(m), g(m + 1), hv(m + 2), yv(m);
+    std::vector<double> Hh((m + 1) * m), cs(m), sn(m), g(m + 1), yv(m), hv_cols((size_t)m * (m + 2));
     *ok = false;
     bool done = false;
     // out = x + Z y(j), y from the first j columns of the rotated Hessenberg
@@ -476,41 +509,56 @@ struct Solver {
       g[0] = beta;
       int j = 0;
       bool restart = false;
+      int jdone = 0;                        // Hessenberg columns already rotated on the host
       while (!done && !restart) {
         double* vj = V + (int64_t)j * Npad;
         double* zj = Z + (int64_t)j * Npad;
         GK_TRY(apply_M(vj, zj));
         GK_TRY(gk_matvec_boxed(h, sys, mv_prec, zj, w, st, mv_box));
-        std::vector<double> hc(j + 1, 0.0);
-        for (int pass = 0; pass < 2; ++pass) {      // classical Gram-Schmidt, two passes
-          GK_TRY(host_dots(V, Npad, j + 1, w, N, hdev, hv.data()));
-          hipLaunchKernelGGL(gemv_sub_kernel, dim3(nblk(N)), dim3(256), 0, st, V, Npad, j + 1, hdev, w, N);
-          for (int i = 0; i <= j; ++i) hc[i] += hv[i];
-        }
-        double hn = 0.0;
-        GK_TRY(norm(w, N, &hn));
-        for (int i = 0; i <= j; ++i) Hh[i * m + j] = hc[i];
-        Hh[(j + 1) * m + j] = hn;
-        if (hn > 0.0)
-          hipLaunchKernelGGL(scale_kernel, dim3(nblk(N)), dim3(256), 0, st, w, 1.0 / hn,
-                             V + (int64_t)(j + 1) * Npad, N);
-        for (int i = 0; i < j; ++i) {
-          const double a = Hh[i * m + j], c = Hh[(i + 1) * m + j];
-          Hh[i * m + j] = cs[i] * a + sn[i] * c;
-          Hh[(i + 1) * m + j] = -sn[i] * a + cs[i] * c;
-        }
-        const double a = Hh[j * m + j], c = Hh[(j + 1) * m + j];
-        const double den = std::hypot(a, c);
-        cs[j] = den > 0 ? a / den : 1.0;
-        sn[j] = den > 0 ? c / den : 0.0;
-        Hh[j * m + j] = den;
-        Hh[(j + 1) * m + j] = 0.0;
-        g[j + 1] = -sn[j] * g[j];
-        g[j] = cs[j] * g[j];
+        // classical Gram-Schmidt, two passes, and the normalisation, all on the
+        // device: the Hessenberg column goes to the host at the next check
+        double* hc = hcol + (int64_t)j * (m + 2);
+        dev_dots(V, Npad, j + 1, w, N, hc);
+        hipLaunchKernelGGL(gemv_sub_kernel, dim3(nblk(N)), dim3(256), 0, st, V, Npad, j + 1, hc, w, N);
+        dev_dots(V, Npad, j + 1, w, N, hdev, hc);
+        hipLaunchKernelGGL(gemv_sub_kernel, dim3(nblk(N)), dim3(256), 0, st, V, Npad, j + 1, hdev, w, N);
+        dev_dots(w, N, 1, w, N, hc + j + 1);
+        hipLaunchKernelGGL(normalize_kernel, dim3(nblk(N)), dim3(256), 0, st, w, hc + j + 1,
+                           V + (int64_t)(j + 1) * Npad, N);
         ++total;
         ++j;
+        if (!(j == m || total >= max_iter || j % check == 0)) continue;
+        // rotate the new columns (j - jdone of them) on the host
+        GK_HIP(hipMemcpyAsync(hv_cols.data(), hcol + (int64_t)jdone * (m + 2),
+                              (size_t)(j - jdone) * (m + 2) * sizeof(double), hipMemcpyDeviceToHost, st));
+        GK_HIP(hipStreamSynchronize(st));
+        double hn = 1.0;
+        for (int jj = jdone; jj < j; ++jj) {
+          const double* col = hv_cols.data() + (int64_t)(jj - jdone) * (m + 2);
+          hn = std::sqrt(std::max(col[jj + 1], 0.0));
+          for (int i = 0; i <= jj; ++i) Hh[i * m + jj] = col[i];
+          Hh[(jj + 1) * m + jj] = hn;
+          for (int i = 0; i < jj; ++i) {
+            const double a = Hh[i * m + jj], c = Hh[(i + 1) * m + jj];
+            Hh[i * m + jj] = cs[i] * a + sn[i] * c;
+            Hh[(i + 1) * m + jj] = -sn[i] * a + cs[i] * c;
+          }
+          const double a = Hh[jj * m + jj], c = Hh[(jj + 1) * m + jj];
+          const double den = std::hypot(a, c);
+          cs[jj] = den > 0 ? a / den : 1.0;
+          sn[jj] = den > 0 ? c / den : 0.0;
+          Hh[jj * m + jj] = den;
+          Hh[(jj + 1) * m + jj] = 0.0;
+          g[jj + 1] = -sn[jj] * g[jj];
+          g[jj] = cs[jj] * g[jj];
+          if (hn == 0.0) {                  // breakdown: the Krylov space ends at column jj
+            j = jj + 1;
+            break;
+          }
+        }
+        jdone = j;
         const bool full = (j == m) || hn == 0.0 || total >= max_iter;
-        if (full || j % check == 0) {
+        {
           GK_TRY(combine(j, xt));
           double rel = 0.0;
           GK_TRY(true_rel(xt, &rel));
@@ -652,6 +700,7 @@ static int init_solver(gk_handle h, const gk_system* sys, hipStream_t st, int fo
   const size_t o_w = take(Npad * sizeof(double));
   const size_t o_r = take(Npad * sizeof(double));
   const size_t o_h = take(256 * sizeof(double));
+  const size_t o_hc = take((size_t)m * (m + 2) * sizeof(double));
   const size_t o_b = take(Npad * sizeof(double));
   const size_t o_xt = take(Npad * sizeof(double));
   const bool pc = form == GK_SOLVE_FP32_REFINE || form == 3;
@@ -682,6 +731,7 @@ static int init_solver(gk_handle h, const gk_system* sys, hipStream_t st, int fo
   S.w = reinterpret_cast<double*>(base + o_w);
   S.r = reinterpret_cast<double*>(base + o_r);
   S.hdev = reinterpret_cast<double*>(base + o_h);
+  S.hcol = reinterpret_cast<double*>(base + o_hc);
   S.b = reinterpret_cast<double*>(base + o_b);
   S.xt = reinterpret_cast<double*>(base + o_xt);
   if (pc) {
