@@ -46,6 +46,7 @@ enum gk_ws_slot {
   WS_ADJOINT,        // vectors of the adjoint
   WS_SMALL,          // small scratch (bounding boxes, counts)
   WS_TRSV,           // work tables, readiness flags and partial sums of the triangular solves
+  WS_ASSEMBLE,       // reference-point tables of the assembly
   WS_COUNT
 };
 struct gk_context {
