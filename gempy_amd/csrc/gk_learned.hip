@@ -141,12 +141,14 @@ __global__ __launch_bounds__(256) void local_blocks_kernel(Geo G, int64_t nq, co
 // Exact local sparse approximate inverse: for point p, the rows of
 // H_loc^-1 that belong to p's own unknowns, as K 3x3 blocks Y[p][j] (the same
 // layout the network predicts).  H_loc (FP64, equilibrated, dummy slots
-// identity) is inverted in LDS by Gauss-Jordan without pivoting (SPD).
+// identity) is symmetric positive definite, so those rows are the columns
+// H_loc^-1 e_{3 self + r}: Gauss-Jordan without pivoting on [H_loc | three unit
+// columns] in LDS (the full inverse is not formed).
 __global__ __launch_bounds__(256) void local_inverse_kernel(Geo G, int64_t n, const int32_t* idx,
                                                             const double* dscale, float* Y) {
   const int64_t p = blockIdx.x;
-  __shared__ double A[LD][LD + 1];
-  __shared__ double X[LD][LD + 1];
+  constexpr int W = LD + 3;                   // columns of the augmented block
+  __shared__ double A[LD][W + 1];
   __shared__ int64_t un[LD];
   __shared__ double piv;
   __shared__ int self;
@@ -160,38 +162,37 @@ __global__ __launch_bounds__(256) void local_inverse_kernel(Geo G, int64_t n, co
       if (idx[p * K + j] == p) { self = j; break; }
   }
   __syncthreads();
-  for (int e = threadIdx.x; e < LD * LD; e += 256) {
-    const int r = e / LD, c = e % LD;
-    const int64_t ur = un[r], uc = un[c];
-    A[r][c] = (ur < 0 || uc < 0) ? (r == c ? 1.0 : 0.0) : entry(G, ur, uc) * dscale[ur] * dscale[uc];
-    X[r][c] = r == c ? 1.0 : 0.0;
+  for (int e = threadIdx.x; e < LD * W; e += 256) {
+    const int r = e / W, c = e % W;
+    const int64_t ur = un[r];
+    double v;
+    if (c < LD) {
+      const int64_t uc = un[c];
+      v = (ur < 0 || uc < 0) ? (r == c ? 1.0 : 0.0) : entry(G, ur, uc) * dscale[ur] * dscale[uc];
+    } else {
+      v = r == 3 * self + (c - LD) ? 1.0 : 0.0;
+    }
+    A[r][c] = v;
   }
   __syncthreads();
   for (int k = 0; k < LD; ++k) {
     if (threadIdx.x == 0) piv = 1.0 / A[k][k];
     __syncthreads();
-    // scale row k
-    for (int c = threadIdx.x; c < 2 * LD; c += 256) {
-      if (c < LD) A[k][c] *= piv;
-      else X[k][c - LD] *= piv;
-    }
+    // scale row k (columns right of k; the pivot column is not read again)
+    for (int c = k + 1 + threadIdx.x; c < W; c += 256) A[k][c] *= piv;
     __syncthreads();
     // eliminate column k from every other row
-    for (int e = threadIdx.x; e < LD * 2 * LD; e += 256) {
-      const int r = e / (2 * LD), c = e % (2 * LD);
+    for (int e = threadIdx.x; e < LD * (W - k - 1); e += 256) {
+      const int r = e / (W - k - 1), c = k + 1 + e % (W - k - 1);
       if (r == k) continue;
-      const double f = A[r][k];
-      if (c < LD) { if (c != k) A[r][c] -= f * A[k][c]; }
-      else X[r][c - LD] -= f * X[k][c - LD];
+      A[r][c] -= A[r][k] * A[k][c];
     }
     __syncthreads();
-    if (threadIdx.x < LD && threadIdx.x != k) A[threadIdx.x][k] = 0.0;
-    __syncthreads();
   }
-  // Y[p][j][3 r + c] = X[3 self + r][3 j + c]
+  // Y[p][j][3 r + c] = (H_loc^-1)[3 self + r][3 j + c] = X[3 j + c][r]
   for (int e = threadIdx.x; e < K * 9; e += 256) {
     const int j = e / 9, rc = e % 9, r = rc / 3, c = rc % 3;
-    Y[(p * K + j) * 9 + rc] = (float)X[3 * self + r][3 * j + c];
+    Y[(p * K + j) * 9 + rc] = (float)A[3 * j + c][LD + r];
   }
 }
 
