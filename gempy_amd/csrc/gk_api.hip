@@ -83,7 +83,7 @@ int gk_side_stream(gk_handle h, hipStream_t* s2, hipEvent_t** ev) {
 
 // implemented in the other translation units
 extern "C" int gk_eval_impl(gk_handle h, const gk_system* sys, const double* weights, gk_eval_desc* d,
-                            hipStream_t st, bool force_mask, const double* src_box);
+                            hipStream_t st, bool force_mask, const double* src_box, bool reuse_sources);
 extern "C" int gk_matvec_impl(gk_handle h, const gk_system* sys, int32_t precision, const double* x,
                               double* y, hipStream_t st, void* scratch, const double* src_box);
 extern "C" size_t gk_matvec_scratch_doubles(const gk_system* sys);
@@ -132,7 +132,7 @@ extern "C" int gk_eval(gk_handle h, const gk_system* sys, const double* weights,
     GK_HIP(hipStreamSynchronize(st));
     pbox = box;
   }
-  return gk_eval_impl(h, sys, weights, desc, st, false, pbox);
+  return gk_eval_impl(h, sys, weights, desc, st, false, pbox, false);
 }
 
 // Operator product; with `src_box` (the bounding box of the system's points,

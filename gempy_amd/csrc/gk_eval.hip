@@ -513,7 +513,7 @@ void dispatch_eval(const EvalArgs& a, bool lattice, bool grad, bool masked, hipS
 // Prepare the packed sources for `weights` in the handle workspace.
 // Layout: [W_s (n_surf doubles, 256-B aligned) | pts V4<T> (R+S) | ori V8<T> (n_ori)].
 static int prepare_sources(gk_handle h, const gk_system* sys, const double* weights, int precision,
-                           hipStream_t st, const void** pts, const void** ori, void* ws) {
+                           hipStream_t st, const void** pts, const void** ori, void* ws, bool launch = true) {
   const int64_t R = sys->n_rest, S = sys->n_surf, no = sys->n_ori;
   const size_t tsz = precision == GK_F64 ? 8 : 4;
   char* base = static_cast<char*>(ws);
@@ -523,6 +523,9 @@ static int prepare_sources(gk_handle h, const gk_system* sys, const double* weig
   off += gk_round_up((R + S) * 4 * tsz, 256);
   void* p8 = base + off;
   (void)h;
+  *pts = p4;
+  *ori = p8;
+  if (!launch) return GK_OK;             // sources already prepared for these weights
   if (S > 0)
     hipLaunchKernelGGL(surface_sums_kernel, dim3((unsigned)S), dim3(1024), 0, st,
                        weights + 3 * no, sys->rest_surf, R, W);
@@ -544,8 +547,6 @@ static int prepare_sources(gk_handle h, const gk_system* sys, const double* weig
       hipLaunchKernelGGL(prep_ori_kernel<float>, dim3(nb), dim3(256), 0, st, *sys, weights,
                          reinterpret_cast<V8<float>*>(p8));
   }
-  *pts = p4;
-  *ori = p8;
   GK_HIP(hipGetLastError());
   return GK_OK;
 }
@@ -622,7 +623,7 @@ static int run_eval(gk_handle h, EvalArgs& a, bool lattice, int precision, bool 
 
 extern "C" int gk_eval_impl(gk_handle h, const gk_system* sys, const double* weights,
                             gk_eval_desc* d, hipStream_t st, bool force_mask,
-                            const double* src_box /*6 or null*/) {
+                            const double* src_box /*6 or null*/, bool reuse_sources) {
   GK_TRY(check_system(sys));
   GK_REQUIRE(d && weights && d->scalar, GK_ERR_ARG, "gk_eval: desc, weights and scalar required");
   GK_REQUIRE(d->precision == GK_F32 || d->precision == GK_F64, GK_ERR_ARG, "gk_eval: bad precision");
@@ -669,7 +670,7 @@ extern "C" int gk_eval_impl(gk_handle h, const gk_system* sys, const double* wei
 
   void* ws = nullptr;
   GK_TRY(gk_workspace(h, WS_EVAL_SRC, sources_bytes(sys), &ws));
-  GK_TRY(prepare_sources(h, sys, weights, d->precision, st, &a.pts, &a.ori, ws));
+  GK_TRY(prepare_sources(h, sys, weights, d->precision, st, &a.pts, &a.ori, ws, !reuse_sources));
   bool masked = true;
   if (!force_mask && src_box) {
     // explicit points: only callers whose points lie inside the source box
@@ -791,7 +792,7 @@ extern "C" int gk_matvec_impl(gk_handle h, const gk_system* sys, int32_t precisi
     d.n_points = R + S;
     d.xyz = pts;
     d.scalar = s_data;
-    GK_TRY(gk_eval_impl(h, &nofault, x, &d, st, src_box == nullptr, src_box));
+    GK_TRY(gk_eval_impl(h, &nofault, x, &d, st, src_box == nullptr, src_box, false));
   }
   if (no > 0) {
     gk_eval_desc d{};
@@ -801,7 +802,7 @@ extern "C" int gk_matvec_impl(gk_handle h, const gk_system* sys, int32_t precisi
     d.xyz = sys->ori_xyz;
     d.scalar = s_ori;
     d.gradient = g_ori;
-    GK_TRY(gk_eval_impl(h, &nofault, x, &d, st, src_box == nullptr, src_box));
+    GK_TRY(gk_eval_impl(h, &nofault, x, &d, st, src_box == nullptr, src_box, R + S > 0));
   }
   const int64_t NG = 3 * no;
   if (NG + R > 0)

@@ -954,7 +954,7 @@ extern "C" int gk_eval_sources(gk_handle h, double range, double c_o, const doub
                                const double* src_xyz, const double* src_w, int64_t n_points, const double* xyz,
                                double* scalar, double* gradient, gk_stream stream);
 extern "C" int gk_eval_impl(gk_handle h, const gk_system* sys, const double* weights, gk_eval_desc* d,
-                            hipStream_t st, bool force_mask, const double* src_box);
+                            hipStream_t st, bool force_mask, const double* src_box, bool reuse_sources);
 
 static Solver* kept_solver(gk_handle h, const gk_system* sys) {
   Solver* S = static_cast<Solver*>(h->solver_state);
@@ -1042,7 +1042,7 @@ extern "C" int gk_adjoint(gk_handle h, const gk_system* sys, const double* w, in
     d.scalar = E;                 // scratch (E is no longer needed)
     d.gradient = pass == 0 ? gSw : gSl;
     d.fault_values = sys->n_fault > 0 ? fz : nullptr;   // no gradient from the fault block
-    GK_TRY(gk_eval_impl(h, sys, pass == 0 ? w : lam, &d, st, true, nullptr));
+    GK_TRY(gk_eval_impl(h, sys, pass == 0 ? w : lam, &d, st, true, nullptr, false));
   }
   // 4. combine
   if (Sf > 0)
