@@ -720,23 +720,27 @@ __global__ __launch_bounds__(256) void diag_apply_kernel(const T* Dk, T* b) {
 }
 
 // Factor + invert one 256-wide diagonal block from two 128-wide halves.
-// Cholesky: L = [L11 0; L21 L22], L^-1 = [X11 0; -X22 L21 X11  X22].
+// Cholesky: L = [L11 0; L21 L22], L^-1 = [X11 0; -X22 L21 X11  X22].  L21 is
+// formed in T1; with copy_l21 false the caller copies it into A later (no
+// later step of the factorization reads it from A).  The upper-right quadrant
+// of Dinv is zeroed once per factorization by the caller.
 template <typename T>
-int diag256_chol(T* A, int64_t lda, int64_t k0, T* dinv, int32_t* dinfo, T* T1, T* T2, hipStream_t st) {
+int diag256_chol(T* A, int64_t lda, int64_t k0, T* dinv, int32_t* dinfo, T* T1, T* T2, hipStream_t st,
+                 bool copy_l21 = true) {
   T* A11 = A + k0 * lda + k0;
   T* A21 = A + (k0 + DB) * lda + k0;
   T* A22 = A21 + DB;
   T* D = dinv + k0 * NB;                 // 256 x 256, ld NB
   Potrf128<T>::launch(A11, lda, D, NB, dinfo, k0, st);
   GK_TRY(gemm_launch<T>(DB, DB, DB, A21, lda, D, NB, true, T(1), T(0), T1, DB, false, st));        // L21
-  hipLaunchKernelGGL(copy_block_kernel<T>, dim3(DB * DB / 256), dim3(256), 0, st, T1, (int64_t)DB, A21,
-                     lda, (int64_t)DB, (int64_t)DB);
+  if (copy_l21)
+    hipLaunchKernelGGL(copy_block_kernel<T>, dim3(DB * DB / 256), dim3(256), 0, st, T1, (int64_t)DB, A21,
+                       lda, (int64_t)DB, (int64_t)DB);
   GK_TRY(gemm_launch<T>(DB, DB, DB, T1, DB, T1, DB, true, T(-1), T(1), A22, lda, false, st));      // syrk
   Potrf128<T>::launch(A22, lda, D + DB * NB + DB, NB, dinfo, k0 + DB, st);
   GK_TRY(gemm_launch<T>(DB, DB, DB, T1, DB, D, NB, false, T(1), T(0), T2, DB, false, st));         // L21 X11
   GK_TRY(gemm_launch<T>(DB, DB, DB, D + DB * NB + DB, NB, T2, DB, false, T(-1), T(0), D + DB * NB, NB,
                         false, st));
-  hipLaunchKernelGGL(zero_block_kernel<T>, dim3(DB * DB / 256), dim3(256), 0, st, D + DB, (int64_t)NB, DB, DB);
   return GK_OK;
 }
 
@@ -854,7 +858,7 @@ struct CholRec {
 static int chol32_panel(float* A, int64_t lda, int64_t kb, float* dinv, int32_t* dinfo, float* W, float* Tb,
                         hipStream_t s) {
   const int64_t k0 = kb * NB, rem = lda - k0 - NB;
-  GK_TRY(diag256_chol<float>(A, lda, k0, dinv, dinfo, Tb, Tb + DB * DB, s));
+  GK_TRY(diag256_chol<float>(A, lda, k0, dinv, dinfo, Tb, Tb + DB * DB, s, false));
   if (rem > 0)
     GK_TRY(gemm_launch<float>(rem, NB, NB, A + (k0 + NB) * lda + k0, lda, dinv + k0 * NB, NB, true, 1.0f, 0.0f, W,
                               NB, false, s));
@@ -885,9 +889,12 @@ static int chain_planes(const T* A, int64_t lda, T* dinv, hipStream_t s) {
 }
 
 // The rest of the panel, off the sequential path: W back into A21 (columns no
-// other kernel touches until the end).
-static int chol32_panel_post(float* A, int64_t lda, int64_t kb, const float* W, hipStream_t s) {
+// other kernel touches until the end) and the diagonal block's L21 (from T1,
+// before the next panel reuses it).
+static int chol32_panel_post(float* A, int64_t lda, int64_t kb, const float* W, const float* T1, hipStream_t s) {
   const int64_t k0 = kb * NB, rem = lda - k0 - NB;
+  hipLaunchKernelGGL(copy_block_kernel<float>, dim3(DB * DB / 256), dim3(256), 0, s, T1, (int64_t)DB,
+                     A + (k0 + DB) * lda + k0, lda, (int64_t)DB, (int64_t)DB);
   if (rem > 0)
     hipLaunchKernelGGL(copy_block_kernel<float>, dim3((unsigned)gk_ceil_div(rem * NB, 256)), dim3(256), 0, s, W,
                        (int64_t)NB, A + (k0 + NB) * lda + k0, lda, rem, (int64_t)NB);
@@ -906,6 +913,7 @@ static int cholesky_impl(gk_handle h, T* A, int64_t n, int64_t lda, T* dinv, int
   T* W = reinterpret_cast<T*>(static_cast<char*>(ws) + 256);       // two panel buffers (FP32 look-ahead)
   T* Tb = W + 2 * (size_t)lda * NB;
   GK_HIP(hipMemsetAsync(dinfo, 0, sizeof(int32_t), st));
+  GK_HIP(hipMemsetAsync(dinv, 0, (size_t)lda * NB * sizeof(T), st));   // Dinv plane: upper-right quadrants stay 0
   if (n < lda) {
     const int64_t tot = (lda - n) * lda + n * (lda - n);
     hipLaunchKernelGGL(pad_identity_kernel<T>, dim3((unsigned)gk_ceil_div(tot, 256)), dim3(256), 0, st, A, lda, n);
@@ -935,7 +943,7 @@ static int cholesky_impl(gk_handle h, T* A, int64_t n, int64_t lda, T* dinv, int
       GK_HIP(hipStreamWaitEvent(s2, ev[0], 0));
       if (nb > 0) GK_TRY(chol32_panel(A, lda, 0, dinv, dinfo, Wb[0], Tb, s2));
       GK_HIP(hipEventRecord(ev[1], s2));                    // panel 0 factored
-      if (nb > 0) GK_TRY(chol32_panel_post(A, lda, 0, Wb[0], s2));
+      if (nb > 0) GK_TRY(chol32_panel_post(A, lda, 0, Wb[0], Tb, s2));
       for (int64_t kb = 0; kb + 1 < nb; ++kb) {
         const int64_t k0 = kb * NB, rem = lda - k0 - NB;
         T* Wk = Wb[kb & 1];
@@ -947,7 +955,7 @@ static int cholesky_impl(gk_handle h, T* A, int64_t n, int64_t lda, T* dinv, int
         GK_HIP(hipStreamWaitEvent(s2, ev[0], 0));
         GK_TRY(chol32_panel(A, lda, kb + 1, dinv, dinfo, Wb[(kb + 1) & 1], Tb, s2));
         GK_HIP(hipEventRecord(ev[1], s2));
-        GK_TRY(chol32_panel_post(A, lda, kb + 1, Wb[(kb + 1) & 1], s2));
+        GK_TRY(chol32_panel_post(A, lda, kb + 1, Wb[(kb + 1) & 1], Tb, s2));
         if (rem > NB)                                       // the rest of the trailing update (lower)
           GK_TRY(gemm_launch<T>(rem - NB, rem - NB, NB, Wk + NB * NB, NB, Wk + NB * NB, NB, true, T(-1), T(1),
                                 A + (k0 + 2 * NB) * lda + k0 + 2 * NB, lda, true, st));
