@@ -356,11 +356,20 @@ int gemm_launch(int64_t m, int64_t n, int64_t k, const T* A, int64_t lda, const 
   }
   hipEvent_t pa;
   const bool prof = gk_prof_begin(st, &pa);
-  const int64_t tm = m / 128, tn = n / 128;
+  // Tall narrow FP32 products (the panel solve and the next panel's update:
+  // m in the thousands, n = 256) have too few 128 x 128 tiles to fill the
+  // chip; 64 x 64 tiles of 4 waves give 4x as many workgroups.
+  const bool narrow = sizeof(T) == 4 && !lower && n <= 256 && m >= 2048 && batch == 1;
+  const int64_t TMe = narrow ? 64 : 128;
+  const int64_t tm = m / TMe, tn = n / TMe;
   const int64_t tiles = lower ? tm * (tm + 1) / 2 : tm * tn;
-  gemm_dispatch<T, typename GemmTile<T>::type, Mfma<T>::WPE>(dim3((unsigned)tiles, (unsigned)batch), trans_b, lower,
-                                                             st, m, n, k, A, lda, B, ldb, alpha, beta, C, ldc, tn,
-                                                             tiles, sA, sB, sC);
+  if (narrow)
+    gemm_dispatch<T, Tile<64, 2, 2>, Mfma<T>::WPE>(dim3((unsigned)tiles, 1), trans_b, false, st, m, n, k, A, lda, B,
+                                                   ldb, alpha, beta, C, ldc, tn, tiles, 0, 0, 0);
+  else
+    gemm_dispatch<T, typename GemmTile<T>::type, Mfma<T>::WPE>(dim3((unsigned)tiles, (unsigned)batch), trans_b, lower,
+                                                               st, m, n, k, A, lda, B, ldb, alpha, beta, C, ldc, tn,
+                                                               tiles, sA, sB, sC);
   if (prof)
     gk_prof_end(st, pa, 4 * (sizeof(T) == 8) + 2 * trans_b + lower,
                 (double)batch * (lower ? (double)m * (double)(m + 1) * (double)k : 2.0 * (double)m * (double)n * (double)k));

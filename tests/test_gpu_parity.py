@@ -83,23 +83,25 @@ def test_gemm_lower_only(gpu):
                 np.testing.assert_array_equal(out[blk], Cm[blk].astype(np.float32))
 
 
-@pytest.mark.parametrize("case", ["nt", "nn", "lower"])
+@pytest.mark.parametrize("case", ["nt", "nn", "lower", "narrow_nt", "narrow_nn"])
 def test_gemm_f32_large(gpu, case):
     """Trailing-update sizes (thousands of tiles, XCD-remapped tile order),
     against an FP64 product; lower_only leaves the 256-blocks above the
-    diagonal untouched."""
-    m = 6144 if case == "lower" else 4096
+    diagonal untouched; the narrow cases (n = 256, the panel-solve shape) take
+    the 64 x 64 tiles."""
+    m = 6144 if case in ("lower", "narrow_nt", "narrow_nn") else 4096
+    n = 256 if case.startswith("narrow") else m
     k = 256
-    say(f"gemm f32 large {case} m=n={m} k={k}")
+    say(f"gemm f32 large {case} m={m} n={n} k={k}")
     L = _lib()
     g = torch.Generator(device=gpu).manual_seed(5)
     A = torch.randn((m, k), device=gpu, generator=g)
-    B = A if case == "lower" else (torch.randn((m, k), device=gpu, generator=g) if case == "nt"
-                                   else torch.randn((k, m), device=gpu, generator=g))
-    C0 = torch.randn((m, m), device=gpu, generator=g)
+    B = A if case == "lower" else (torch.randn((n, k), device=gpu, generator=g) if case.endswith("nt")
+                                   else torch.randn((k, n), device=gpu, generator=g))
+    C0 = torch.randn((m, n), device=gpu, generator=g)
     tC = C0.clone()
-    trans_b = 0 if case == "nn" else 1
-    L.check("gemm", L.load().gk_gemm_f32(_h(), m, m, k, _p(A), k, _p(B), B.shape[1], trans_b, 1.0, _p(tC), m,
+    trans_b = 0 if case.endswith("nn") else 1
+    L.check("gemm", L.load().gk_gemm_f32(_h(), m, n, k, _p(A), k, _p(B), B.shape[1], trans_b, 1.0, _p(tC), n,
                                          1 if case == "lower" else 0, _st()))
     torch.cuda.synchronize()
     Bd = B.double()
