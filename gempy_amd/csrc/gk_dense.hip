@@ -357,9 +357,10 @@ int gemm_launch(int64_t m, int64_t n, int64_t k, const T* A, int64_t lda, const 
   hipEvent_t pa;
   const bool prof = gk_prof_begin(st, &pa);
   // Tall narrow FP32 products (the panel solve and the next panel's update:
-  // m in the thousands, n = 256) have too few 128 x 128 tiles to fill the
-  // chip; 64 x 64 tiles of 4 waves give 4x as many workgroups.
-  const bool narrow = sizeof(T) == 4 && !lower && n <= 256 && m >= 2048 && batch == 1;
+  // m in the thousands, n = 256) and the 128^3 products of the diagonal
+  // blocks have too few 128 x 128 tiles to fill the chip (one, for the
+  // latter); 64 x 64 tiles of 4 waves give 4x as many workgroups.
+  const bool narrow = sizeof(T) == 4 && !lower && batch == 1 && ((n <= 256 && m >= 2048) || m * n <= 256 * 256);
   const int64_t TMe = narrow ? 64 : 128;
   const int64_t tm = m / TMe, tn = n / TMe;
   const int64_t tiles = lower ? tm * (tm + 1) / 2 : tm * tn;

@@ -44,11 +44,12 @@ def _p(t):
 # ------------------------------------------------------------ dense blocks
 @pytest.mark.parametrize("dtype", ["f32", "f64"])
 @pytest.mark.parametrize("trans_b", [0, 1])
-def test_gemm_matches_numpy(gpu, dtype, trans_b):
-    say(f"gemm {dtype} trans_b={trans_b} m=384 n=256 k=256")
+@pytest.mark.parametrize("shape", [(384, 256, 256), (128, 128, 128)])
+def test_gemm_matches_numpy(gpu, dtype, trans_b, shape):
+    m, n, k = shape
+    say(f"gemm {dtype} trans_b={trans_b} m={m} n={n} k={k}")
     L = _lib()
     rng = np.random.default_rng(0)
-    m, n, k = 384, 256, 256
     tdt = torch.float32 if dtype == "f32" else torch.float64
     A = rng.standard_normal((m, k))
     B = rng.standard_normal((n, k) if trans_b else (k, n))
