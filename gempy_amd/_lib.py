@@ -102,6 +102,7 @@ SIGNATURES = {
                                    P, P]),
     "gk_octree_refine": (C.c_int, [P, C.c_int32, P, C.c_int64, P, C.POINTER(C.c_int64), P, P,
                                    C.POINTER(C.c_int64), P]),
+    "gk_test_trsv_set_error": (C.c_int, [P, C.c_int32, P]),
 }
 
 _lib = None

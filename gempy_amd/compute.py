@@ -25,14 +25,18 @@ from .data import (GeoModelInput, OctreeLevel, Solutions, SolverForm, StackRelat
                    StackSolution)
 from .engine import DeviceStack, _ptr, _stream, handle
 
-_FORMS = {SolverForm.FP64_DIRECT: 0, SolverForm.FP32_REFINE: 1, SolverForm.FP64_CHOL: 2}
+# GK_SOLVE_* of each solver form; the learned form is driven from the host
+# (network-preconditioned matrix-free GMRES, then the FP32 form on stagnation)
+GMRES_LEARNED = -1
+_FORMS = {SolverForm.FP64_DIRECT: _lib.GK_SOLVE_FP64_LU, SolverForm.FP32_REFINE: _lib.GK_SOLVE_FP32_REFINE,
+          SolverForm.FP64_CHOL: _lib.GK_SOLVE_FP64_CHOL, SolverForm.GMRES_LEARNED: GMRES_LEARNED}
 
 
 def _form(model: GeoModelInput, cfg: GemPyEngineConfig) -> int:
     """engine dtype 'float32' -> FP32 factor + FP64 refinement; 'float64' -> the
     options' FP64 form (Cholesky + Schur unless the LU is asked for)."""
     if cfg.dtype == "float32":
-        return _lib.GK_SOLVE_FP32_REFINE
+        return GMRES_LEARNED if model.options.solver == SolverForm.GMRES_LEARNED else _lib.GK_SOLVE_FP32_REFINE
     if cfg.dtype == "float64":
         return (_lib.GK_SOLVE_FP64_LU if model.options.solver == SolverForm.FP64_DIRECT
                 else _lib.GK_SOLVE_FP64_CHOL)
@@ -74,15 +78,22 @@ class _Model:
                 pts = torch.tensor(st.sp_xyz, dtype=torch.float64, device="cuda")
                 fault_sp = np.stack([self._fault_ids(j, pts).cpu().numpy() for j in st.faults], 1)
             ds = DeviceStack(st, model.options.kernel_options, center, fault_sp)
-            w, info = ds.solve(form)
+            if form == GMRES_LEARNED:
+                from .learned import solve_learned
+                w, info = solve_learned(ds)
+            else:
+                w, info = ds.solve(form)
             thr, s_sp = ds.surface_thresholds(w)
             self.stacks.append(ds)
             self.sols.append(StackSolution(weights=w, scalar_field_at_surface_points=s_sp.cpu().numpy(),
                                            thresholds=thr, solver_info=info))
 
     def _fault_ids(self, j, pts):
-        ds = self.stacks[j]
-        _, _, fid = ds.evaluate_points(pts, precision=_lib.GK_F64, thresholds=ds.thresholds)
+        """Block values of fault stack j at pts; a fault offset by older faults
+        is evaluated with their block values at the same points."""
+        ds, st = self.stacks[j], self.model.stacks[j]
+        fv = (torch.stack([self._fault_ids(i, pts) for i in st.faults], 1).contiguous() if st.faults else None)
+        _, _, fid = ds.evaluate_points(pts, precision=_lib.GK_F64, thresholds=ds.thresholds, fault_values=fv)
         return fid.double()
 
     def at_points(self, pts: torch.Tensor, precision=_lib.GK_F64, keep_scalar=False):

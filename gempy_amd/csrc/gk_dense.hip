@@ -997,13 +997,18 @@ extern "C" int gk_cholesky_f64(gk_handle h, double* A, int64_t n, int64_t lda, d
                                gk_stream stream) {
   return cholesky_impl<double>(h, A, n, lda, dinv, info, (hipStream_t)stream);
 }
+// The public solves read the sync-free kernels' error word before returning
+// (one stream synchronisation): a wait that reached its spin limit leaves b
+// unusable and is reported here instead of by a later, unrelated solve.
 extern "C" int gk_cholesky_solve_f32(gk_handle h, const float* L, const float* dinv, int64_t lda, float* b,
                                      gk_stream stream) {
-  return gk_trsv_chol_impl<float>(h, L, dinv, lda, b, (hipStream_t)stream);
+  GK_TRY(gk_trsv_chol_impl<float>(h, L, dinv, lda, b, (hipStream_t)stream));
+  return gk_trsv_check(h, (hipStream_t)stream);
 }
 extern "C" int gk_cholesky_solve_f64(gk_handle h, const double* L, const double* dinv, int64_t lda, double* b,
                                      gk_stream stream) {
-  return gk_trsv_chol_impl<double>(h, L, dinv, lda, b, (hipStream_t)stream);
+  GK_TRY(gk_trsv_chol_impl<double>(h, L, dinv, lda, b, (hipStream_t)stream));
+  return gk_trsv_check(h, (hipStream_t)stream);
 }
 
 extern "C" int gk_lu_f64(gk_handle h, double* A, int64_t n, int64_t lda, double* dinv, int32_t* info,

@@ -347,12 +347,12 @@ struct Solver {
     if (f64) {
       double* u = static_cast<double*>(uvec);
       hipLaunchKernelGGL(to_scaled_kernel<double>, dim3(nblk(Ncpad)), dim3(256), 0, st, rin, dscale, u, Nc, Ncpad);
-      GK_TRY(gk_cholesky_solve_f64(h, static_cast<double*>(H), static_cast<double*>(dinv), Ncpad, u, st));
+      GK_TRY(gk_trsv_chol_impl<double>(h, static_cast<double*>(H), static_cast<double*>(dinv), Ncpad, u, st));
       hipLaunchKernelGGL(from_scaled_kernel<double>, dim3(nblk(Nc)), dim3(256), 0, st, u, dscale, yout, Nc);
     } else {
       float* u = static_cast<float*>(uvec);
       hipLaunchKernelGGL(to_scaled_kernel<float>, dim3(nblk(Ncpad)), dim3(256), 0, st, rin, dscale, u, Nc, Ncpad);
-      GK_TRY(gk_cholesky_solve_f32(h, static_cast<float*>(H), static_cast<float*>(dinv), Ncpad, u, st));
+      GK_TRY(gk_trsv_chol_impl<float>(h, static_cast<float*>(H), static_cast<float*>(dinv), Ncpad, u, st));
       hipLaunchKernelGGL(from_scaled_kernel<float>, dim3(nblk(Nc)), dim3(256), 0, st, u, dscale, yout, Nc);
     }
     return GK_OK;
@@ -618,6 +618,8 @@ struct Solver {
     }
     *steps = k;
     *relres = rel;
+    // a triangular solve whose wait reached its spin limit leaves x unusable
+    GK_TRY(gk_trsv_check(h, st));
     return GK_OK;
   }
 
@@ -808,7 +810,11 @@ extern "C" int gk_solve(gk_handle h, const gk_system* sys, int32_t form, double 
   // FP32 shifted factor + exact local-inverse correction, FP64 GMRES refinement
   GK_HIP(hipEventRecord(ev[0], st));
   GK_TRY(gk_diag_scale_impl(sys, S.dscale, st));
-  {                          // neighbourhoods and local inverses overlap the factorization
+  // The local-inverse correction needs GK_SPAI_K data points for its
+  // neighbourhoods; smaller systems use the shifted factor alone (their shift,
+  // 4 u32 sqrt(n), is tiny, and FGMRES removes it).
+  const bool local = sys->n_ori + sys->n_rest >= GK_SPAI_K;
+  if (local) {               // neighbourhoods and local inverses overlap the factorization
     hipStream_t s3;
     hipEvent_t* aev;
     GK_TRY(gk_aux_stream(h, &s3, &aev));
@@ -818,10 +824,10 @@ extern "C" int gk_solve(gk_handle h, const gk_system* sys, int32_t form, double 
     GK_TRY(gk_spai_local_inverse(h, sys, S.own_idx, S.dscale, S.own_Y, (gk_stream)s3));
     GK_HIP(hipEventRecord(aev[1], s3));
     S.pc_ready = aev[1];
+    S.spai_idx = S.own_idx;
+    S.spai_Y = S.own_Y;
   }
-  S.spai_idx = S.own_idx;
-  S.spai_Y = S.own_Y;
-  S.pcp = true;
+  S.pcp = local;
   int32_t finfo = 0;
   int rc = S.setup_preconditioner(false, &finfo);
   info->info = finfo;
@@ -956,11 +962,22 @@ extern "C" int gk_eval_sources(gk_handle h, double range, double c_o, const doub
 extern "C" int gk_eval_impl(gk_handle h, const gk_system* sys, const double* weights, gk_eval_desc* d,
                             hipStream_t st, bool force_mask, const double* src_box, bool reuse_sources);
 
+// The kept factorization belongs to the system of the last gk_solve: every
+// field of the descriptor (sizes, options, frame centre and the device
+// pointers of the data) must match.  Data changed in place at the same
+// addresses cannot be detected; callers solve again after such a change.
 static Solver* kept_solver(gk_handle h, const gk_system* sys) {
   Solver* S = static_cast<Solver*>(h->solver_state);
-  if (!S || gk_system_n(sys) != S->N || sys->n_ori != S->sys_copy.n_ori || sys->n_rest != S->sys_copy.n_rest)
-    return nullptr;
-  return S;
+  if (!S) return nullptr;
+  const gk_system& k = S->sys_copy;
+  const bool same = gk_system_n(sys) == S->N && sys->n_ori == k.n_ori && sys->n_rest == k.n_rest &&
+                    sys->n_surf == k.n_surf && sys->uni_degree == k.uni_degree && sys->n_fault == k.n_fault &&
+                    sys->range == k.range && sys->c_o == k.c_o && sys->i_res == k.i_res && sys->gi_res == k.gi_res &&
+                    sys->ori_xyz == k.ori_xyz && sys->ori_grad == k.ori_grad && sys->ori_nugget == k.ori_nugget &&
+                    sys->rest_xyz == k.rest_xyz && sys->rest_surf == k.rest_surf &&
+                    sys->rest_nugget == k.rest_nugget && sys->ref_xyz == k.ref_xyz && sys->fault_rest == k.fault_rest &&
+                    sys->center[0] == k.center[0] && sys->center[1] == k.center[1] && sys->center[2] == k.center[2];
+  return same ? S : nullptr;
 }
 
 extern "C" int gk_solve_rhs(gk_handle h, const gk_system* sys, const double* rhs, double* x, gk_solve_info* info,

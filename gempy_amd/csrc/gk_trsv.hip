@@ -808,3 +808,18 @@ int gk_trsv_check(gk_handle h, hipStream_t st) {
   }
   return GK_OK;
 }
+
+// Test hook: store `code` into the error word of the handle's triangular-solve
+// workspace, as a readiness wait reaching its spin limit would (the kernels
+// then skip their waits and drain).  Needs a solve to have run on the handle.
+extern "C" int gk_test_trsv_set_error(gk_handle h, int32_t code, gk_stream stream) {
+  GK_REQUIRE(h, GK_ERR_ARG, "gk_test_trsv_set_error: NULL handle");
+  GK_REQUIRE(h->ws[WS_TRSV] && h->trsv_nb != 0, GK_ERR_ARG,
+             "gk_test_trsv_set_error: no triangular-solve workspace yet");
+  hipStream_t st = (hipStream_t)stream;
+  TrsvLayout lay = trsv_layout(h->trsv_nb, h->trsv_nfwd, h->trsv_nbwd, h->trsv_slots);
+  int v = code;
+  GK_HIP(hipMemcpyAsync(static_cast<char*>(h->ws[WS_TRSV]) + lay.o_err, &v, sizeof(int), hipMemcpyHostToDevice, st));
+  GK_HIP(hipStreamSynchronize(st));
+  return GK_OK;
+}

@@ -217,3 +217,35 @@ def solve_spai(ds: DeviceStack, idx=None, y=None, tol: float = 1e-8, max_iter: i
     return x, {"form": int(info.form), "iterations": int(info.iterations), "fallback": int(info.fallback),
                "relres": float(info.relres), "shift": float(info.shift), "t_setup_ms": float(info.t_factor_ms),
                "t_krylov_ms": float(info.t_refine_ms)}
+
+
+_NET = {}
+
+
+def default_network() -> SpaiNetwork:
+    """The trained network shipped with the package (spai_weights.npz), loaded once."""
+    if "net" not in _NET:
+        _NET["net"] = SpaiNetwork.load(WEIGHTS)
+    return _NET["net"]
+
+
+def solve_learned(ds: DeviceStack, tol: float = 1e-10, max_iter: int = 48):
+    """The learned solve form: matrix-free GMRES (FP32 Krylov matvecs on the
+    recomputed kernel tiles) preconditioned by the network's blocks; when it
+    stagnates above 1e-6, the shifted FP32 factor with the exact local inverse
+    and FP64 FGMRES (GK_SOLVE_FP32_REFINE, itself falling back to the FP64
+    Cholesky form) solves the system.  ``info['learned']`` records the
+    network-preconditioned attempt."""
+    if ds.sys.n_ori + ds.sys.n_rest < K:             # too few points for a neighbourhood
+        return ds.solve(_lib.GK_SOLVE_FP32_REFINE)
+    idx, y = spai_blocks(default_network(), ds)
+    x, first = solve_spai(ds, idx, y, tol=tol, max_iter=max_iter, krylov_precision=_lib.GK_F32,
+                          with_factor=False, no_fallback=True, sandwich=False)
+    if first["relres"] <= 1e-6:
+        first["learned"] = dict(first)
+        first["fallback"] = 0
+        return x, first
+    x, info = ds.solve(_lib.GK_SOLVE_FP32_REFINE)
+    info["learned"] = first
+    info["fallback"] = 1
+    return x, info

@@ -216,9 +216,10 @@ def config2(seed: int = 2, scale: float = 1.0, resolution=(200, 200, 200),
         sp_nugget=np.full(n_f, SP_NUGGET), ori_xyz=np.stack([xo, yo, zo], 1), ori_grad=gf,
         ori_nugget=np.full(no_f, ORI_NUGGET), relation=StackRelationType.FAULT,
         surface_names=["fault_0"])
-    erosive = _layer_stack(rng, "erosive", [800.0, 870.0], n_s, no_s, faulted=False)
-    upper = _layer_stack(rng, "upper", [540.0, 610.0, 680.0], n_s, no_s, faulted=True)
-    lower = _layer_stack(rng, "lower", [220.0, 300.0, 380.0], n_s, no_s, faulted=True)
+    # surfaces of every stack top first (youngest first), as the ids count them
+    erosive = _layer_stack(rng, "erosive", [870.0, 800.0], n_s, no_s, faulted=False)
+    upper = _layer_stack(rng, "upper", [680.0, 610.0, 540.0], n_s, no_s, faulted=True)
+    lower = _layer_stack(rng, "lower", [380.0, 300.0, 220.0], n_s, no_s, faulted=True)
     upper.faults = [0]
     lower.faults = [0]
     opts = _options(number_octree_levels=octree_levels)

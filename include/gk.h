@@ -303,6 +303,15 @@ int gk_solve_spai(gk_handle h, const gk_system* sys, const int32_t* idx, const f
 int gk_profile(gk_handle h, int32_t enable);
 int gk_profile_read(gk_handle h, int32_t kind, int64_t* launches, double* ms, double* work);
 
+/* ---- testing ------------------------------------------------------------
+ * No counterpart in the reference.  Stores `code` (1 = spin limit reached,
+ * 2 = ticket out of range) into the error word of the sync-free triangular
+ * solves of `h`, as a failing readiness wait would: the next solves skip their
+ * waits, and the entry points that check the word (gk_cholesky_solve_*, the
+ * gk_solve forms, gk_solve_rhs) return GK_ERR_HIP and reset it.  Needs one
+ * triangular solve to have run on the handle. */
+int gk_test_trsv_set_error(gk_handle h, int32_t code, gk_stream stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -370,8 +370,11 @@ def compute_model(stacks, opts, x, gradient=False):
             kw = dict(a=opts["range"], c_o=opts["c_o"], uni_degree=opts.get("uni_degree", 1),
                       i_res=opts.get("i_res", 4.0), gi_res=opts.get("gi_res", 2.0))
 
-            def block_fn(pts, st=st, r=r, kw=kw):
-                s = evaluate(pts, st["sp_xyz"], st["counts"], st["ori_xyz"], r["weights"], **kw)
+            def block_fn(pts, st=st, r=r, kw=kw, fl=fl):
+                # a fault offset by older faults needs their blocks at pts too
+                fpts = (np.stack([fault_vals_sp_fn[j](pts) for j in fl], 1) if fl else None)
+                s = evaluate(pts, st["sp_xyz"], st["counts"], st["ori_xyz"], r["weights"],
+                             fault_x=fpts, **kw)
                 return segment(s, r["thresholds"]).astype(np.float64)
             fault_vals_sp_fn[k] = block_fn
             fault_vals_x[k] = r["ids"].astype(np.float64)

@@ -1,0 +1,184 @@
+"""Whole-model GPU tests through compute_model: small reference models with the
+default engine configuration, multi-fault models against the FP64 oracle,
+the learned solve form, and the error word of the sync-free triangular solves.
+"""
+import ctypes as C
+import os
+import sys
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import cokriging as ok
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def say(msg):
+    print(f"[gpu-case] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+def _two_layers_model(solver=None):
+    from gempy_amd import GeoModelInput, InterpolationOptions, KernelOptions, StackInput
+    d = np.load(os.path.join(GOLD, "two_layers.npz"))
+    st = StackInput("Strat_Series", d["sp_xyz"], list(d["counts"]), d["sp_nugget"], d["ori_xyz"],
+                    d["ori_grad"], d["ori_nugget"])
+    kw = {} if solver is None else {"solver": solver}
+    opts = InterpolationOptions(KernelOptions(range=float(d["range"]), c_o=float(d["c_o"])), **kw)
+    return GeoModelInput([st], opts), d["expected_thresholds"]
+
+
+@pytest.mark.parametrize("solver", [None, "gmres_learned", "fp64_direct"])
+def test_compute_model_two_layers_default_config(gpu, solver):
+    """The reference's 2-layer model (1 orientation, 2 rest points: fewer data
+    points than a neighbourhood) through the default engine configuration;
+    the recorded interface values 0.09 / -0.248333 (planar closed form)."""
+    say(f"compute_model two_layers solver={solver}")
+    from gempy_amd import GemPyEngineConfig, compute_model
+    m, expected = _two_layers_model(solver)
+    sol = compute_model(m, GemPyEngineConfig())
+    say(f"two_layers {solver}: thresholds {sol.stacks[0].thresholds} info {sol.stacks[0].solver_info}")
+    np.testing.assert_allclose(sol.stacks[0].thresholds, expected, rtol=0, atol=1e-7)
+
+
+def _two_fault_model(scale=0.05, res=20):
+    """config-2 geometry plus a second planar fault that the first offsets;
+    both faults offset the two lower stacks."""
+    from gempy_amd import synthetic
+    from gempy_amd.data import StackInput, StackRelationType
+    m = synthetic.config2(scale=scale, resolution=(res,) * 3, octree_levels=1)
+    rng = np.random.default_rng(77)
+    n_pts, n_ori = 12, 3
+    nrm = np.array([0.0, np.sin(np.deg2rad(70.0)), np.cos(np.deg2rad(70.0))])
+    x = rng.uniform(0, 1000, n_pts)
+    z = rng.uniform(80, 920, n_pts)
+    y = 420.0 - (nrm[2] / nrm[1]) * (z - 500.0)
+    z = synthetic._offset(x, y, z)
+    xo = rng.uniform(100, 900, n_ori)
+    zo = rng.uniform(150, 850, n_ori)
+    yo = 420.0 - (nrm[2] / nrm[1]) * (zo - 500.0)
+    f2 = StackInput("fault2", np.stack([x, y, z], 1), [n_pts], np.full(n_pts, synthetic.SP_NUGGET),
+                    np.stack([xo, yo, zo], 1), np.tile(nrm, (n_ori, 1)), np.full(n_ori, synthetic.ORI_NUGGET),
+                    relation=StackRelationType.FAULT, faults=[0])
+    fault, erosive, upper, lower = m.stacks
+    upper.faults = [0, 1]
+    lower.faults = [0, 1]
+    m.stacks = [fault, f2, erosive, upper, lower]
+    return m
+
+
+def test_compute_model_two_faults_fault_on_fault_matches_oracle(gpu):
+    """Two fault stacks, the second offset by the first (its own fault-drift
+    column), both entering the two lower stacks: lith block and both fault
+    blocks against the oracle, every stack's field to 1e-6 of its range."""
+    say("compute_model two faults (fault on fault) vs oracle, 20^3")
+    from gempy_amd import GemPyEngineConfig, compute_model
+    m = _two_fault_model()
+    X = m.grid.values()
+    res = ok.compute_model([s.as_oracle_dict() for s in m.stacks], m.options.as_dict(), X)
+    for dt in ("float64", "float32"):
+        sol = compute_model(m, GemPyEngineConfig(dtype=dt))
+        band = np.zeros(len(X), dtype=bool)
+        for s_gpu, r in zip(sol.stacks, res["stacks"]):
+            rng_s = r["scalar"].max() - r["scalar"].min()
+            band |= np.min(np.abs(r["scalar"][:, None] - r["thresholds"][None, :]), 1) <= 1e-6 * rng_s
+            err = np.abs(s_gpu.scalar_field.cpu().numpy() - r["scalar"]).max() / rng_s
+            assert err <= 1e-6, err
+        lith = sol.lith_block.cpu().numpy()
+        mism = int(((lith != res["lith_block"]) & ~band).sum())
+        say(f"two faults {dt}: lith mismatches outside band {mism}, band {int(band.sum())}, "
+            f"forms {[s.solver_info['form'] for s in sol.stacks]}")
+        assert mism == 0
+        assert len(sol.fault_blocks) == 2
+        for fb, rb in zip(sol.fault_blocks, res["fault_blocks"]):
+            fbn = fb.cpu().numpy()
+            assert int(((fbn != rb) & ~band).sum()) == 0
+        # the second fault block really is offset by the first
+        assert len(np.unique(res["fault_blocks"][1])) == 2
+
+
+def test_learned_form_through_compute_model(gpu):
+    """SolverForm.GMRES_LEARNED reached from compute_model: the network-
+    preconditioned matrix-free GMRES attempt is recorded, and the field meets
+    the 1e-6 bound against the oracle (directly or through the fallback)."""
+    say("compute_model learned form, config1-like stack")
+    from gempy_amd import GemPyEngineConfig, compute_model, synthetic
+    from gempy_amd.data import SolverForm
+    m = synthetic.config3(n_points=600, n_ori=150, resolution=(16, 16, 16))
+    m.options.solver = SolverForm.GMRES_LEARNED
+    sol = compute_model(m, GemPyEngineConfig())
+    info = sol.stacks[0].solver_info
+    say(f"learned form: {info}")
+    assert "learned" in info and info["learned"]["iterations"] > 0
+    st = m.stacks[0].as_oracle_dict()
+    r = ok.compute_stack(st, m.options.as_dict(), m.grid.values())
+    rng_s = r["scalar"].max() - r["scalar"].min()
+    err = np.abs(sol.stacks[0].scalar_field.cpu().numpy() - r["scalar"]).max() / rng_s
+    say(f"learned form: max|dS|/range {err:.3e}")
+    assert err <= 1e-6
+
+
+def test_triangular_solve_error_word_is_reported(gpu):
+    """A wait that reached its spin limit (error word forced from the host)
+    is returned as GK_ERR_HIP by the FP64 Cholesky form and by the public
+    Cholesky solve, and the next calls run normally again."""
+    say("trsv error word: FP64_CHOL solve and gk_cholesky_solve_f64")
+    from gempy_amd import _lib, synthetic
+    from gempy_amd.engine import DeviceStack, _stream, handle
+    lib = _lib.load()
+    m = synthetic.config3(n_points=400, n_ori=100, resolution=(8, 8, 8))
+    ds = DeviceStack(m.stacks[0], m.options.kernel_options, [500.0, 500.0, 500.0])
+    w0, _ = ds.solve(_lib.GK_SOLVE_FP64_CHOL)
+    _lib.check("set_error", lib.gk_test_trsv_set_error(handle(0), 1, _stream()))
+    with pytest.raises(_lib.GkError) as e:
+        ds.solve(_lib.GK_SOLVE_FP64_CHOL)
+    assert e.value.code == -2 and "spin limit" in str(e.value)
+    w1, info = ds.solve(_lib.GK_SOLVE_FP64_CHOL)
+    assert info["relres"] < 1e-8
+    assert torch.allclose(w0, w1, rtol=0, atol=1e-9 * float(w0.abs().max()))
+    # public solve on a small SPD matrix
+    n = 512
+    rng = np.random.default_rng(3)
+    Bm = rng.standard_normal((n, n))
+    A = Bm @ Bm.T / n + np.eye(n)
+    tA = torch.tensor(A, dtype=torch.float64, device=gpu).contiguous()
+    dinv = torch.zeros(_lib.GK_CHOL_PLANES * n * _lib.GK_BLOCK, dtype=torch.float64, device=gpu)
+    info32 = C.c_int32(0)
+    _lib.check("chol", lib.gk_cholesky_f64(handle(0), C.c_void_p(tA.data_ptr()), n, n, C.c_void_p(dinv.data_ptr()),
+                                           C.byref(info32), _stream()))
+    b = rng.standard_normal(n)
+    tb = torch.tensor(b, dtype=torch.float64, device=gpu)
+    _lib.check("solve", lib.gk_cholesky_solve_f64(handle(0), C.c_void_p(tA.data_ptr()), C.c_void_p(dinv.data_ptr()),
+                                                  n, C.c_void_p(tb.data_ptr()), _stream()))
+    _lib.check("set_error", lib.gk_test_trsv_set_error(handle(0), 1, _stream()))
+    tb2 = torch.tensor(b, dtype=torch.float64, device=gpu)
+    rc = lib.gk_cholesky_solve_f64(handle(0), C.c_void_p(tA.data_ptr()), C.c_void_p(dinv.data_ptr()), n,
+                                   C.c_void_p(tb2.data_ptr()), _stream())
+    assert rc == -2
+    tb3 = torch.tensor(b, dtype=torch.float64, device=gpu)
+    _lib.check("solve", lib.gk_cholesky_solve_f64(handle(0), C.c_void_p(tA.data_ptr()), C.c_void_p(dinv.data_ptr()),
+                                                  n, C.c_void_p(tb3.data_ptr()), _stream()))
+    x = np.linalg.solve(A, b)
+    assert np.abs(tb3.cpu().numpy() - x).max() <= 1e-10 * np.abs(x).max()
+    assert np.abs(tb.cpu().numpy() - x).max() <= 1e-10 * np.abs(x).max()
+
+
+def test_kept_factorization_is_tied_to_its_system(gpu):
+    """gk_solve_rhs refuses a system whose data live elsewhere (same sizes,
+    another stack), instead of reusing the kept factor of the last solve."""
+    say("kept factorization vs another system of the same sizes")
+    from gempy_amd import _lib, synthetic
+    from gempy_amd.engine import DeviceStack, _ptr, _stream
+    lib = _lib.load()
+    m = synthetic.config3(n_points=300, n_ori=60, resolution=(8, 8, 8))
+    a = DeviceStack(m.stacks[0], m.options.kernel_options, [500.0, 500.0, 500.0])
+    b = DeviceStack(m.stacks[0].copy(), m.options.kernel_options, [500.0, 500.0, 500.0])
+    a.solve(_lib.GK_SOLVE_FP64_CHOL)
+    rhs = torch.ones(a.N, dtype=torch.float64, device=gpu)
+    x = torch.empty_like(rhs)
+    info = _lib.GkSolveInfo()
+    assert lib.gk_solve_rhs(a.h, C.byref(a.sys), _ptr(rhs), _ptr(x), C.byref(info), _stream()) == 0
+    assert lib.gk_solve_rhs(b.h, C.byref(b.sys), _ptr(rhs), _ptr(x), C.byref(info), _stream()) == -1
