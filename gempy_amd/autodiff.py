@@ -46,6 +46,7 @@ class ScalarFieldAt(torch.autograd.Function):
                 raise ValueError("fault_obs is required when the stack has fault-drift columns")
             fo = torch.as_tensor(np.asarray(fault_obs, dtype=np.float64).reshape(obs_d.shape[0], -1),
                                  device="cuda").contiguous()
+            fo = ds.used_fault_values(fo)
         S, _, _ = ds.evaluate_points(obs_d, w, _lib.GK_F64, fault_values=fo)
         ctx.ds, ctx.w, ctx.obs, ctx.fo = ds, w, obs_d, fo
         ctx.epoch = DeviceStack.solve_epoch

@@ -78,12 +78,20 @@ class DeviceStack:
         self.rest_nugget = torch.tensor(nug, **f64)
         self.ref_xyz = torch.tensor(sp[ref_idx], **f64).contiguous()
         self.sp_xyz = torch.tensor(sp, **f64).contiguous()
-        self.n_fault = 0 if fault_sp is None else int(np.asarray(fault_sp).shape[1])
+        # fault columns: a fault whose block is the same at every rest point and
+        # its reference point leaves an all-zero drift column (a singular
+        # system) and says nothing about this stack; it is dropped (weight 0)
+        self.n_fault_in = 0 if fault_sp is None else int(np.asarray(fault_sp).reshape(len(sp), -1).shape[1])
+        self.fault_cols = np.zeros(0, dtype=np.int64)
         self.fault_rest = None
-        if self.n_fault:
+        if self.n_fault_in:
             fs = np.asarray(fault_sp, dtype=np.float64).reshape(len(sp), -1)
-            self.fault_rest = torch.tensor(fs[rest_idx] - fs[ref_idx][surf], **f64).contiguous()
+            inc = fs[rest_idx] - fs[ref_idx][surf]
+            self.fault_cols = np.nonzero(np.any(inc != 0.0, axis=0))[0]
+            fs, inc = fs[:, self.fault_cols], inc[:, self.fault_cols]
+            self.fault_rest = torch.tensor(inc, **f64).contiguous()
             self.fault_sp = torch.tensor(fs, **f64).contiguous()
+        self.n_fault = int(len(self.fault_cols))
         s = _lib.GkSystem()
         s.n_ori = stack.n_orientations
         s.n_rest = len(rest_idx)
@@ -148,6 +156,19 @@ class DeviceStack:
         return y
 
     # ------------------------------------------------------------- evaluate
+    def used_fault_values(self, fault_values):
+        """The columns of the fault-block values (one per fault of the stack)
+        that enter this stack's system."""
+        if fault_values is None or self.n_fault == 0:
+            return None
+        fv = fault_values.reshape(fault_values.shape[0], -1)
+        if fv.shape[1] == self.n_fault:            # already the used columns
+            return fault_values
+        if fv.shape[1] != self.n_fault_in:
+            raise ValueError(f"fault values with {fv.shape[1]} columns; the stack has {self.n_fault_in} faults")
+        cols = torch.as_tensor(self.fault_cols, device=fault_values.device)
+        return fv[:, cols].contiguous()
+
     def _eval(self, desc, weights):
         lib = _lib.load()
         _lib.check("gk_eval", lib.gk_eval(self.h, C.byref(self.sys), _ptr(weights), C.byref(desc), _stream()))
@@ -167,6 +188,7 @@ class DeviceStack:
         d.n_points = n
         d.xyz = _ptr(xyz)
         d.scalar, d.gradient, d.ids = _ptr(S), _ptr(G), _ptr(ids)
+        fault_values = self.used_fault_values(fault_values)
         d.fault_values = _ptr(fault_values)
         self._set_thresholds(d, thresholds)
         if n:
@@ -190,6 +212,7 @@ class DeviceStack:
             d.spacing[k] = grid.spacing[k]
             d.res[k] = grid.resolution[k]
         d.scalar, d.gradient, d.ids = _ptr(S), _ptr(G), _ptr(ids)
+        fault_values = self.used_fault_values(fault_values)
         d.fault_values = _ptr(fault_values)
         self._set_thresholds(d, thresholds)
         self._eval(d, w)

@@ -49,7 +49,7 @@ import numpy as np
 __all__ = [
     "cov", "dcov_r", "d2cov", "split_ref_rest", "drift_functions",
     "build_system", "solve_system", "evaluate", "surface_thresholds",
-    "segment", "grid_points", "compute_stack", "compute_model",
+    "segment", "grid_points", "compute_stack", "compute_model", "fault_columns_used",
 ]
 
 
@@ -140,12 +140,20 @@ def _pair(x, y):
     return h, r
 
 
+def _chunk_rows(n_cols, budget=2_000_000):
+    """Rows per chunk so that a (rows, n_cols, 3) temporary stays near `budget` entries."""
+    return max(1, budget // max(1, n_cols))
+
+
 def build_system(sp_xyz, counts, sp_nugget, ori_xyz, ori_grad, ori_nugget,
                  a, c_o, uni_degree=1, i_res=4.0, gi_res=2.0,
-                 fault_sp=None):
+                 fault_sp=None, out=None):
     """Co-kriging matrix A (N,N) and right-hand side b (N,) of one stack.
 
     fault_sp: optional (n_sp, F) fault-block values at the surface points.
+    The covariance blocks are filled in row chunks (temporaries of a few
+    million entries), so a config-3 system (N = 34,993) needs only the
+    matrix itself.  out: optional preallocated (N, N) float64 array.
     """
     ref, rest, rsurf, ref_idx, rest_idx = split_ref_rest(sp_xyz, counts)
     sp_nugget = np.asarray(sp_nugget, dtype=np.float64)
@@ -157,41 +165,57 @@ def build_system(sp_xyz, counts, sp_nugget, ori_xyz, ori_grad, ori_nugget,
     F = 0 if fault_sp is None else np.asarray(fault_sp).shape[1]
     NG = 3 * n_o
     N = NG + R + L + F
-    A = np.zeros((N, N))
+    if out is None:
+        A = np.zeros((N, N))
+    else:
+        A = out
+        A[...] = 0.0
     refp = ref[rsurf]                     # reference point of every rest point
 
     # --- GG block: -[d_uv C'/r + hh_uv (C'' - C'/r)], h = q_g - q_g'
     if n_o:
-        h, r = _pair(q, q)
-        d1 = dcov_r(r, a, c_o)
-        dd = _d2_minus_d1(r, a, c_o)
-        with np.errstate(invalid="ignore", divide="ignore"):
-            hh = np.where(r[..., None] > 0, h / np.where(r > 0, r, 1.0)[..., None], 0.0)
-        for u in range(3):
-            for v in range(3):
-                blk = -(dd * hh[..., u] * hh[..., v] + (d1 if u == v else 0.0))
-                A[u * n_o:(u + 1) * n_o, v * n_o:(v + 1) * n_o] = blk
+        cs = _chunk_rows(n_o)
+        for g0 in range(0, n_o, cs):
+            g1 = min(n_o, g0 + cs)
+            h, r = _pair(q[g0:g1], q)
+            d1 = dcov_r(r, a, c_o)
+            dd = _d2_minus_d1(r, a, c_o)
+            with np.errstate(invalid="ignore", divide="ignore"):
+                hh = np.where(r[..., None] > 0, h / np.where(r > 0, r, 1.0)[..., None], 0.0)
+            for u in range(3):
+                for v in range(3):
+                    blk = -(dd * hh[..., u] * hh[..., v] + (d1 if u == v else 0.0))
+                    A[u * n_o + g0:u * n_o + g1, v * n_o:(v + 1) * n_o] = blk
         A[np.arange(NG), np.arange(NG)] += np.tile(nug_o, 3)
 
     # --- GI block: gi_res [ (q-p_i)_u C'/r(|q-p_i|) - (q-p_ref)_u C'/r(|q-p_ref|) ]
     if n_o and R:
-        h1, r1 = _pair(q, rest)
-        h2, r2 = _pair(q, refp)
-        k1 = dcov_r(r1, a, c_o)
-        k2 = dcov_r(r2, a, c_o)
-        for u in range(3):
-            blk = gi_res * (h1[..., u] * k1 - h2[..., u] * k2)
-            A[u * n_o:(u + 1) * n_o, NG:NG + R] = blk
-            A[NG:NG + R, u * n_o:(u + 1) * n_o] = blk.T
+        cs = _chunk_rows(R)
+        for g0 in range(0, n_o, cs):
+            g1 = min(n_o, g0 + cs)
+            h1, r1 = _pair(q[g0:g1], rest)
+            h2, r2 = _pair(q[g0:g1], refp)
+            k1 = dcov_r(r1, a, c_o)
+            k2 = dcov_r(r2, a, c_o)
+            for u in range(3):
+                blk = gi_res * (h1[..., u] * k1 - h2[..., u] * k2)
+                A[u * n_o + g0:u * n_o + g1, NG:NG + R] = blk
+                A[NG:NG + R, u * n_o + g0:u * n_o + g1] = blk.T
 
     # --- II block: i_res [C(pi,pj) - C(pi,p0j) - C(p0i,pj) + C(p0i,p0j)]
     if R:
-        _, r_rr = _pair(rest, rest)
-        _, r_rf = _pair(rest, refp)
-        _, r_fr = _pair(refp, rest)
-        _, r_ff = _pair(refp, refp)
-        blk = cov(r_rr, a, c_o) - cov(r_rf, a, c_o) - cov(r_fr, a, c_o) + cov(r_ff, a, c_o)
-        A[NG:NG + R, NG:NG + R] = i_res * blk
+        cs = _chunk_rows(R)
+        for i0 in range(0, R, cs):
+            i1 = min(R, i0 + cs)
+            _, r_rr = _pair(rest[i0:i1], rest)
+            blk = cov(r_rr, a, c_o)
+            _, r_rf = _pair(rest[i0:i1], refp)
+            blk -= cov(r_rf, a, c_o)
+            _, r_fr = _pair(refp[i0:i1], rest)
+            blk -= cov(r_fr, a, c_o)
+            _, r_ff = _pair(refp[i0:i1], refp)
+            blk += cov(r_ff, a, c_o)
+            A[NG + i0:NG + i1, NG:NG + R] = i_res * blk
         nug = sp_nugget[rest_idx] + sp_nugget[ref_idx][rsurf]
         A[NG + np.arange(R), NG + np.arange(R)] += nug
 
@@ -216,9 +240,13 @@ def build_system(sp_xyz, counts, sp_nugget, ori_xyz, ori_grad, ori_nugget,
     return A, b
 
 
-def solve_system(A, b):
+def solve_system(A, b, overwrite=False):
     """Dense LU solve (numpy.linalg.solve): the reference's default direct solver
-    (kernel_solver 1, 2-layers.approved.txt:158)."""
+    (kernel_solver 1, 2-layers.approved.txt:158).  overwrite: factor A in place
+    (LAPACK gesv through scipy), for systems too large for a second copy."""
+    if overwrite:
+        import scipy.linalg
+        return scipy.linalg.solve(A, b, overwrite_a=True, check_finite=False)
     return np.linalg.solve(A, b)
 
 
@@ -327,8 +355,26 @@ def grid_points(extent, resolution):
 # --------------------------------------------------------------------------
 # stack + model drivers
 # --------------------------------------------------------------------------
+def fault_columns_used(counts, fault_sp):
+    """Indices of the fault columns whose block differs between some rest
+    point and its reference point.  A fault that does not separate any two
+    points of one surface leaves an all-zero drift column (a singular
+    system); it carries no information about the stack and is dropped
+    (its weight is 0)."""
+    fs = np.asarray(fault_sp, dtype=np.float64)
+    _, _, rsurf, ref_idx, rest_idx = split_ref_rest(np.zeros((fs.shape[0], 3)), counts)
+    inc = fs[rest_idx] - fs[ref_idx][rsurf]
+    return np.nonzero(np.any(inc != 0.0, axis=0))[0]
+
+
 def compute_stack(stack, opts, x, fault_sp=None, fault_x=None, gradient=False):
     """Assemble, solve and evaluate one stack.  Returns a dict."""
+    if fault_sp is not None:
+        keep = fault_columns_used(stack["counts"], fault_sp)
+        fault_sp = np.asarray(fault_sp)[:, keep] if len(keep) else None
+        if fault_x is not None:
+            fault_x = np.asarray(fault_x).reshape(np.asarray(x).reshape(-1, 3).shape[0], -1)[:, keep] \
+                if len(keep) else None
     A, b = build_system(stack["sp_xyz"], stack["counts"], stack["sp_nugget"],
                         stack["ori_xyz"], stack["ori_grad"], stack["ori_nugget"],
                         opts["range"], opts["c_o"], opts.get("uni_degree", 1),

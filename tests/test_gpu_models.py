@@ -182,3 +182,45 @@ def test_kept_factorization_is_tied_to_its_system(gpu):
     info = _lib.GkSolveInfo()
     assert lib.gk_solve_rhs(a.h, C.byref(a.sys), _ptr(rhs), _ptr(x), C.byref(info), _stream()) == 0
     assert lib.gk_solve_rhs(b.h, C.byref(b.sys), _ptr(rhs), _ptr(x), C.byref(info), _stream()) == -1
+
+
+@pytest.mark.parametrize("tag", ["m9a", "m9b"])
+def test_legacy_fault_models_match_v2_and_oracle(gpu, tag):
+    """The reference's legacy v2 fault models (Tutorial_ch1-9a / -9b) through
+    compute_model: every stack's field against the oracle (1e-6 of its range),
+    the youngest fault's field against v2's recorded one (up to v2's additive
+    constant), its block against v2's, and (9a) the lith block against v2's
+    outside the +-1e-6 band."""
+    say(f"legacy fault model {tag} through compute_model")
+    from test_oracle_legacy import band_of, check_fault_field, legacy_model
+    from gempy_amd import GemPyEngineConfig, compute_model
+    from gempy_amd.data import (GeoModelInput, InterpolationOptions, KernelOptions, RegularGrid, StackInput,
+                                StackRelationType)
+    stacks, opts, X, g = legacy_model(tag)
+    res = ok.compute_model(stacks, opts, X)
+    st_in = [StackInput(f"s{k}", s["sp_xyz"], s["counts"], s["sp_nugget"], s["ori_xyz"], s["ori_grad"],
+                        s["ori_nugget"], relation=StackRelationType.FAULT if s["relation"] == "fault"
+                        else StackRelationType.ERODE, faults=s["faults"]) for k, s in enumerate(stacks)]
+    m = GeoModelInput(st_in, InterpolationOptions(KernelOptions(range=opts["range"], c_o=opts["c_o"])),
+                      grid=RegularGrid(g["extent_r"], g["resolution"]))
+    for dt in ("float64", "float32"):
+        sol = compute_model(m, GemPyEngineConfig(dtype=dt))
+        band = np.zeros(len(X), dtype=bool)
+        for k, (s, r) in enumerate(zip(sol.stacks, res["stacks"])):
+            S = s.scalar_field.cpu().numpy()
+            err = np.abs(S - r["scalar"]).max() / np.ptp(r["scalar"])
+            say(f"{tag} {dt} stack {k}: max|dS|/range vs oracle {err:.3e} info {s.solver_info}")
+            assert err <= 1e-6
+            band |= band_of(r["scalar"], r["thresholds"])
+        S0 = sol.stacks[0].scalar_field.cpu().numpy()
+        slope, resid = check_fault_field(S0[g["sample_index"]], g["fault_scalar"][0])
+        say(f"{tag} {dt}: youngest fault vs v2: slope {slope:.8f} residual/range {resid:.3e}")
+        assert abs(slope - 1.0) < 2e-6 and resid < 3e-6
+        fb = sol.fault_blocks[0].cpu().numpy()
+        assert np.array_equal(fb[~band], g["fault_block"][0][~band])
+        if "lith_block" in g:
+            lith = sol.lith_block.cpu().numpy()
+            v2 = np.where(lith == lith.max(), 0, lith + 1)
+            mism = int(((v2 != g["lith_block"]) & ~band).sum())
+            say(f"{tag} {dt}: lith mismatches vs v2 outside band {mism} (band {int(band.sum())})")
+            assert mism == 0

@@ -1,0 +1,167 @@
+"""Full-size workloads on the GPU against the oracle's committed FP64 fixtures.
+
+* config 3 (precision form, N = 34,993, 200^3 grid) through compute_model in
+  the FP32-factor form and both FP64 forms: S at 65,536 sampled lattice cells
+  within 1e-6 of the field range, ids bit-exact outside the +-1e-6 band,
+  thresholds within 1e-6 of the range.  This runs the production-size factor
+  paths (136 panels with look-ahead, the chain planes of the triangular
+  solves, the FGMRES refinement).
+* config 2 at full scale (fault + 3 stacks, 5,000 points, 200^3 grid, 4 octree
+  levels): every stack's S, the lith block and the fault block at 65,536
+  sampled cells.
+* the FP32 Cholesky at n ~ 24k on a real equilibrated kernel block, checked in
+  FP64 with probe vectors; the FP32 form's fallback to the FP64 Cholesky form.
+
+Fixtures: tests/golden/make_scale_fixtures.py.
+"""
+import ctypes as C
+import os
+import sys
+
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def say(msg):
+    print(f"[gpu-case] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+def _band(S, thr, rng_s, tol=1e-6):
+    return np.min(np.abs(S[:, None] - np.asarray(thr)[None, :]), 1) <= tol * rng_s
+
+
+@pytest.mark.parametrize("dtype", ["float32", "float64", "fp64_lu"])
+def test_config3_full_size_matches_oracle_fixture(gpu, dtype):
+    say(f"config3 full size (N=34,993, 200^3) dtype={dtype} vs oracle fixture")
+    from gempy_amd import GemPyEngineConfig, compute_model, synthetic
+    from gempy_amd.data import SolverForm
+    d = np.load(os.path.join(GOLD, "config3_oracle.npz"))
+    m = synthetic.config3()
+    if dtype == "fp64_lu":
+        m.options.solver = SolverForm.FP64_DIRECT
+    sol = compute_model(m, GemPyEngineConfig(dtype="float64" if dtype == "fp64_lu" else dtype))
+    info = sol.stacks[0].solver_info
+    idx = torch.as_tensor(d["sample_index"], device=gpu)
+    S = sol.stacks[0].scalar_field[idx].cpu().numpy()
+    ids = sol.lith_block[idx].cpu().numpy()
+    So, thr_o = d["scalar"], d["thresholds"]
+    rng_s = So.max() - So.min()
+    err = np.abs(S - So).max() / rng_s
+    rms = np.sqrt(np.mean((S - So) ** 2)) / rng_s
+    terr = np.abs(np.asarray(sol.stacks[0].thresholds) - thr_o).max() / rng_s
+    band = _band(So, thr_o, rng_s)
+    mism = int(((ids != d["ids"]) & ~band).sum())
+    say(f"config3 {dtype}: max|dS|/range {err:.3e} rms {rms:.3e} thresholds {terr:.3e} "
+        f"id mismatches outside band {mism} (band {int(band.sum())}) relres {info['relres']:.3e} "
+        f"form {info['form']} fallback {info['fallback']} iterations {info['iterations']} shift {info['shift']:.3e} "
+        f"(oracle relres {float(d['relres']):.3e})")
+    assert err <= 1e-6 and terr <= 1e-6
+    assert mism == 0
+    assert info["fallback"] == 0
+    assert info["relres"] <= 1e-6
+
+
+def test_config2_full_scale_matches_oracle_fixture(gpu):
+    say("config2 full scale (fault + 3 stacks, 200^3, 4 octree levels) vs oracle fixture")
+    from gempy_amd import GemPyEngineConfig, compute_model, synthetic
+    d = np.load(os.path.join(GOLD, "config2_oracle.npz"))
+    m = synthetic.config2()
+    sol = compute_model(m, GemPyEngineConfig(dtype="float32"))
+    idx = torch.as_tensor(d["sample_index"], device=gpu)
+    band = np.zeros(len(d["sample_index"]), dtype=bool)
+    for k, s in enumerate(sol.stacks):
+        So, thr = d[f"scalar_{k}"], d[f"thresholds_{k}"]
+        rng_s = So.max() - So.min()
+        S = s.scalar_field[idx].cpu().numpy()
+        err = np.abs(S - So).max() / rng_s
+        terr = np.abs(np.asarray(s.thresholds) - thr).max() / rng_s
+        say(f"config2 stack {k}: N={len(d[f'weights_{k}'])} max|dS|/range {err:.3e} thresholds {terr:.3e} "
+            f"info {s.solver_info}")
+        assert err <= 1e-6 and terr <= 1e-6
+        assert s.solver_info["fallback"] == 0
+        band |= _band(So, thr, rng_s)
+    lith = sol.lith_block[idx].cpu().numpy()
+    fault = sol.fault_blocks[0][idx].cpu().numpy()
+    mism = int(((lith != d["lith_block"]) & ~band).sum())
+    fmism = int(((fault != d["fault_block"]) & ~band).sum())
+    say(f"config2: lith mismatches outside band {mism}, fault {fmism}, band {int(band.sum())}; "
+        f"octree levels {[int(o.voxel_index.numel()) for o in sol.octrees_output]}")
+    assert mism == 0 and fmism == 0
+    assert len(np.unique(d["lith_block"])) == 9          # every unit of the model is present
+    octs = sol.octrees_output
+    assert [o.resolution for o in octs] == [(25, 25, 25), (50, 50, 50), (100, 100, 100), (200, 200, 200)]
+    last = octs[-1]
+    dense = sol.lith_block
+    assert torch.equal(last.ids, dense[last.voxel_index])
+
+
+def test_cholesky_f32_large_kernel_block(gpu):
+    """FP32 Cholesky of the shifted equilibrated covariance block of a
+    config-3-style stack at n ~ 24k (94 panels): ||L L^T - (H + s I)|| / ||H||
+    estimated in FP64 with 8 probe vectors, and the solve's backward error."""
+    from gempy_amd import _lib, synthetic
+    from gempy_amd.engine import DeviceStack, _stream, handle
+    L_ = _lib
+    lib = L_.load()
+    m = synthetic.config3(n_points=14_000, n_ori=3_300, resolution=(8, 8, 8))
+    ds = DeviceStack(m.stacks[0], m.options.kernel_options, [500.0, 500.0, 500.0])
+    A, _, _ = ds.assemble(L_.GK_F32)
+    n = 3 * ds.sys.n_ori + ds.sys.n_rest
+    lda = A.shape[0]
+    say(f"cholesky f32 large: n={n} lda={lda}")
+    shift = 4.0 * 5.9604644775390625e-08 * np.sqrt(n)
+    A.diagonal()[:n] += shift
+    H = torch.tril(A[:n, :n].double())
+    H = H + torch.tril(H, -1).T
+    dinv = torch.zeros(L_.GK_CHOL_PLANES * lda * L_.GK_BLOCK, dtype=torch.float32, device=gpu)
+    info = C.c_int32(-1)
+    L_.check("chol", lib.gk_cholesky_f32(handle(0), C.c_void_p(A.data_ptr()), n, lda, C.c_void_p(dinv.data_ptr()),
+                                         C.byref(info), _stream()))
+    assert info.value == 0
+    Lf = torch.tril(A[:n, :n]).double()
+    V = torch.randn((n, 8), dtype=torch.float64, device=gpu, generator=torch.Generator(device=gpu).manual_seed(5))
+    HV = H @ V
+    rel = float(torch.linalg.norm(Lf @ (Lf.T @ V) - HV) / torch.linalg.norm(HV))
+    say(f"cholesky f32 large: ||(LL^T - H)V|| / ||HV|| = {rel:.3e} (shift {shift:.3e})")
+    assert rel <= 2e-5
+    b = torch.randn(n, dtype=torch.float64, device=gpu, generator=torch.Generator(device=gpu).manual_seed(6))
+    x = torch.zeros(lda, dtype=torch.float32, device=gpu)
+    x[:n] = b.float()
+    L_.check("solve", lib.gk_cholesky_solve_f32(handle(0), C.c_void_p(A.data_ptr()), C.c_void_p(dinv.data_ptr()),
+                                                lda, C.c_void_p(x.data_ptr()), _stream()))
+    xd = x[:n].double()
+    bwd = float(torch.linalg.norm(H @ xd - b) / (torch.linalg.norm(H @ xd) + torch.linalg.norm(b)))
+    say(f"cholesky f32 large: solve backward error {bwd:.3e}")
+    assert bwd <= 1e-4
+    assert torch.all(x[n:] == 0)
+
+
+def test_fp32_form_falls_back_to_fp64_cholesky(gpu):
+    """With an iteration budget too small for the refinement, the FP32 form
+    falls back to the FP64 Cholesky form and still meets the bound."""
+    say("fp32 form fallback (max_iter=2)")
+    from gempy_amd import _lib, synthetic
+    from gempy_amd.engine import DeviceStack
+    from oracle import cokriging as ok
+    m = synthetic.config3(n_points=3_000, n_ori=700, resolution=(8, 8, 8))
+    st = m.stacks[0]
+    ds = DeviceStack(st, m.options.kernel_options, [500.0, 500.0, 500.0])
+    w, info = ds.solve(_lib.GK_SOLVE_FP32_REFINE, max_iter=2)
+    say(f"fallback: {info}")
+    assert info["fallback"] == 1 and info["form"] == _lib.GK_SOLVE_FP64_CHOL
+    o = st.as_oracle_dict()
+    k = m.options.kernel_options
+    A, b = ok.build_system(o["sp_xyz"], o["counts"], o["sp_nugget"], o["ori_xyz"], o["ori_grad"], o["ori_nugget"],
+                           k.range, k.c_o)
+    wo = np.linalg.solve(A, b)
+    X = np.random.default_rng(9).uniform(0, 1000, (3000, 3))
+    So = ok.evaluate(X, o["sp_xyz"], o["counts"], o["ori_xyz"], wo, k.range, k.c_o)
+    S, _, _ = ds.evaluate_points(torch.tensor(X, device=gpu), w, _lib.GK_F64)
+    err = np.abs(S.cpu().numpy() - So).max() / (So.max() - So.min())
+    say(f"fallback: max|dS|/range {err:.3e}")
+    assert err <= 1e-6
