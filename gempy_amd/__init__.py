@@ -5,6 +5,7 @@ Public surface (mirrors the reference's compute seam, gempy/API/compute_API.py):
 ``AvailableBackends``, ``BackendTensor`` and the input/output records in
 :mod:`gempy_amd.data`.  Numerics run in libgkhip.so (HIP, gfx950).
 """
+from .adapters import Transform, model_input_from_geomodel
 from .config import AvailableBackends, BackendTensor, GemPyEngineConfig
 from .data import (EvaluationOptions, GeoModelInput, InterpolationOptions, KernelOptions,
                    RegularGrid, Solutions, SolverForm, StackInput, StackRelationType)
@@ -22,4 +23,5 @@ def compute_model_at(*args, **kwargs):
 
 __all__ = ["AvailableBackends", "BackendTensor", "GemPyEngineConfig", "EvaluationOptions",
            "GeoModelInput", "InterpolationOptions", "KernelOptions", "RegularGrid", "Solutions",
-           "SolverForm", "StackInput", "StackRelationType", "compute_model", "compute_model_at"]
+           "SolverForm", "StackInput", "StackRelationType", "Transform", "compute_model", "compute_model_at",
+           "model_input_from_geomodel"]

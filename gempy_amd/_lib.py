@@ -16,6 +16,7 @@ GK_ERR = {-1: "GK_ERR_ARG", -2: "GK_ERR_HIP", -3: "GK_ERR_NOT_SPD", -4: "GK_ERR_
 GK_F32, GK_F64 = 0, 1
 GK_POINTS_EXPLICIT, GK_POINTS_LATTICE = 0, 1
 GK_SOLVE_FP64_LU, GK_SOLVE_FP32_REFINE, GK_SOLVE_FP64_CHOL = 0, 1, 2
+GK_REL_ERODE, GK_REL_ONLAP = 1, 2
 GK_MAX_THRESHOLDS = 64
 GK_BLOCK = 256
 GK_CHOL_PLANES = 5
@@ -76,6 +77,8 @@ SIGNATURES = {
                               C.c_int32, C.c_double, P, C.c_int64, C.c_int32, P]),
     "gk_solve": (C.c_int, [P, C.POINTER(GkSystem), C.c_int32, C.c_double, C.c_int32, P,
                            C.POINTER(GkSolveInfo), P]),
+    "gk_solve_x0": (C.c_int, [P, C.POINTER(GkSystem), C.c_int32, C.c_double, C.c_int32, P, P,
+                              C.POINTER(GkSolveInfo), P]),
     "gk_eval": (C.c_int, [P, C.POINTER(GkSystem), P, C.POINTER(GkEvalDesc), P]),
     "gk_solve_rhs": (C.c_int, [P, C.POINTER(GkSystem), P, P, C.POINTER(GkSolveInfo), P]),
     "gk_adjoint": (C.c_int, [P, C.POINTER(GkSystem), P, C.c_int64, P, P, P, P, P, P, P]),
@@ -98,6 +101,9 @@ SIGNATURES = {
     "gk_profile_read": (C.c_int, [P, C.c_int32, C.POINTER(C.c_int64), C.POINTER(C.c_double),
                                   C.POINTER(C.c_double)]),
     "gk_combine_lith": (C.c_int, [P, C.c_int32, P, C.c_int64, C.POINTER(C.c_int32), P, P]),
+    "gk_block_values": (C.c_int, [P, P, C.c_int64, C.c_int32, C.POINTER(C.c_double), C.c_double, P, P]),
+    "gk_combine_stacks": (C.c_int, [P, C.c_int32, P, C.c_int64, C.POINTER(C.c_int32), C.POINTER(C.c_int32), P, P,
+                                    P]),
     "gk_octree_points": (C.c_int, [P, C.POINTER(C.c_double), C.POINTER(C.c_int64), P, C.c_int64, C.c_int32,
                                    P, P]),
     "gk_octree_refine": (C.c_int, [P, C.c_int32, P, C.c_int64, P, C.POINTER(C.c_int64), P, P,

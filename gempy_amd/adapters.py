@@ -1,0 +1,168 @@
+"""Input adapter: a gempy ``GeoModel`` (read through the attribute names the
+reference uses) to this library's :class:`GeoModelInput`.
+
+It mirrors ``interpolation_input_from_structural_frame``
+(gempy/modules/data_manipulation/_engine_factory.py:14-58): surface points and
+orientations are copied and moved into the model's input frame
+(``surface_points_copy_transformed`` / ``orientations_copy_transformed``,
+gempy/core/data/geo_model.py:135-168), the dense grid's extent is transformed
+the same way (``extent_transformed_transformed_by_input``, :193-199), the
+stack descriptor comes from the structural frame (groups in order, relation
+per group, the fault-relation matrix of structural_frame.py:243-273), and the
+previous solution's weights are passed on as the initial guess of the solve
+(_engine_factory.py:45-48).
+
+:class:`Transform` restates ``Transform.from_input_points`` of the engine (its
+source is not in the reference tree): position = -(min + max)/2 of all input
+points per axis, scale 1 / (2 max_d range_d) on every axis with a non-zero
+range; an axis along which every input point has the same coordinate gets
+1 / (2 r) with r the smallest non-zero range.  This reproduces every
+``input_transform`` record of the reference whose inputs are local:
+2-layers (test_gravity, position (-6, -0, -2.51), scale (1/12, 1/5.96, 1/12)),
+the anticline (fold.approved.txt, (-500, -500, -510), 1/2000) and the 2.5-D
+model (Model1.approved.txt, (-417.5, -0.5, 157), 1/778).  Only the 2-layers
+record has a zero-range axis: that rule rests on one record.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+from typing import List, Optional, Sequence
+
+import numpy as np
+
+from .data import (EvaluationOptions, GeoModelInput, InterpolationOptions, KernelOptions, RegularGrid, SolverForm,
+                   StackInput, StackRelationType)
+
+
+@dataclass
+class Transform:
+    """x' = (x + position) * scale (no rotation: the recorded rotations are 0)."""
+    position: np.ndarray = field(default_factory=lambda: np.zeros(3))
+    scale: np.ndarray = field(default_factory=lambda: np.ones(3))
+    rotation: np.ndarray = field(default_factory=lambda: np.zeros(3))
+
+    def __post_init__(self):
+        self.position = np.asarray(self.position, dtype=np.float64).reshape(3)
+        self.scale = np.asarray(self.scale, dtype=np.float64).reshape(3)
+        self.rotation = np.asarray(self.rotation, dtype=np.float64).reshape(3)
+        if np.any(self.rotation != 0.0):
+            raise NotImplementedError("rotated input transforms are not supported")
+
+    @classmethod
+    def from_input_points(cls, sp_xyz: np.ndarray, ori_xyz: Optional[np.ndarray] = None) -> "Transform":
+        pts = [np.asarray(sp_xyz, dtype=np.float64).reshape(-1, 3)]
+        if ori_xyz is not None:
+            pts.append(np.asarray(ori_xyz, dtype=np.float64).reshape(-1, 3))
+        p = np.concatenate(pts)
+        if p.shape[0] == 0:
+            return cls()
+        lo, hi = p.min(0), p.max(0)
+        rng = hi - lo
+        nz = rng[rng > 0]
+        if nz.size == 0:
+            return cls(position=-(lo + hi) / 2)
+        scale = np.where(rng > 0, 1.0 / (2.0 * nz.max()), 1.0 / (2.0 * nz.min()))
+        return cls(position=-(lo + hi) / 2, scale=scale)
+
+    def apply(self, xyz: np.ndarray) -> np.ndarray:
+        return (np.asarray(xyz, dtype=np.float64) + self.position) * self.scale
+
+    def apply_inverse(self, xyz: np.ndarray) -> np.ndarray:
+        return np.asarray(xyz, dtype=np.float64) / self.scale - self.position
+
+    def transform_gradient(self, grads: np.ndarray) -> np.ndarray:
+        """Gradient of a scalar field under x' = (x + p) s: d/dx' = (d/dx) / s,
+        kept at the input vectors' length."""
+        g = np.asarray(grads, dtype=np.float64).reshape(-1, 3)
+        t = g / self.scale
+        n0 = np.linalg.norm(g, axis=1, keepdims=True)
+        n1 = np.linalg.norm(t, axis=1, keepdims=True)
+        return np.where(n1 > 0, t * n0 / np.where(n1 > 0, n1, 1.0), t)
+
+
+_RELATIONS = {"ERODE": StackRelationType.ERODE, "ONLAP": StackRelationType.ONLAP,
+              "FAULT": StackRelationType.FAULT, "BASEMENT": StackRelationType.ERODE}
+
+
+def _relation(value) -> StackRelationType:
+    name = getattr(value, "name", value)
+    if isinstance(name, str):
+        return _RELATIONS[name.upper()]
+    return StackRelationType(int(value))
+
+
+def _xyz(table) -> np.ndarray:
+    return np.asarray(table.xyz, dtype=np.float64).reshape(-1, 3)
+
+
+def _fault_matrix(frame, n_groups: int) -> np.ndarray:
+    m = getattr(frame, "fault_relations", None)
+    if m is None:
+        return np.zeros((n_groups, n_groups), dtype=bool)
+    return np.asarray(m, dtype=bool).reshape(n_groups, n_groups)
+
+
+def model_input_from_geomodel(geo_model, solver: Optional[SolverForm] = None) -> GeoModelInput:
+    """GeoModelInput of a gempy GeoModel (duck-typed: structural_frame.
+    structural_groups[g].elements[e].{surface_points, orientations}.{xyz,
+    grads, nugget}, structural_relation, structural_frame.fault_relations,
+    input_transform, grid.regular_grid.{extent, resolution},
+    interpolation_options.{kernel_options, number_octree_levels}, solutions)."""
+    frame = geo_model.structural_frame
+    groups = list(frame.structural_groups)
+    tr = getattr(geo_model, "input_transform", None)
+    if tr is None or not hasattr(tr, "apply"):
+        sp_all = np.concatenate([_xyz(e.surface_points) for g in groups for e in g.elements] or [np.zeros((0, 3))])
+        ori_all = np.concatenate([_xyz(e.orientations) for g in groups for e in g.elements] or [np.zeros((0, 3))])
+        tr = Transform.from_input_points(sp_all, ori_all)
+    fm = _fault_matrix(frame, len(groups))
+    stacks: List[StackInput] = []
+    for gi, g in enumerate(groups):
+        pts, nug, counts, names, q, gr, qn = [], [], [], [], [], [], []
+        for e in g.elements:
+            sp = e.surface_points
+            x = _xyz(sp)
+            if len(x):
+                pts.append(tr.apply(x))
+                nug.append(np.broadcast_to(np.asarray(sp.nugget, dtype=np.float64), (len(x),)))
+                counts.append(len(x))
+                names.append(getattr(e, "name", f"element{len(names)}"))
+            o = e.orientations
+            xo = _xyz(o)
+            if len(xo):
+                q.append(tr.apply(xo))
+                gr.append(tr.transform_gradient(np.asarray(o.grads, dtype=np.float64)))
+                qn.append(np.broadcast_to(np.asarray(o.nugget, dtype=np.float64), (len(xo),)))
+        cat = lambda a, w: np.concatenate(a) if a else np.zeros((0, w)) if w else np.zeros(0)  # noqa: E731
+        stacks.append(StackInput(
+            name=getattr(g, "name", f"group{gi}"), sp_xyz=cat(pts, 3), counts=counts, sp_nugget=cat(nug, 0),
+            ori_xyz=cat(q, 3), ori_grad=cat(gr, 3), ori_nugget=cat(qn, 0),
+            relation=_relation(getattr(g, "structural_relation", "ERODE")), surface_names=names,
+            faults=[int(i) for i in np.nonzero(fm[:gi, gi])[0]]))
+    io = geo_model.interpolation_options
+    ko = io.kernel_options
+    kopts = KernelOptions(range=float(ko.range), c_o=float(ko.c_o), uni_degree=int(getattr(ko, "uni_degree", 1)),
+                          i_res=float(getattr(ko, "i_res", 4.0)), gi_res=float(getattr(ko, "gi_res", 2.0)))
+    levels = int(getattr(io, "number_octree_levels", 1) or 1)
+    eo = getattr(io, "evaluation_options", None)
+    grads = bool(getattr(eo, "compute_scalar_gradient", False)) if eo is not None else False
+    opts = InterpolationOptions(kopts, EvaluationOptions(number_octree_levels=levels, compute_scalar_gradient=grads),
+                                sigmoid_slope=float(getattr(io, "sigmoid_slope", 5e6)),
+                                solver=solver or SolverForm.FP32_REFINE)
+    grid = None
+    rg = getattr(getattr(geo_model, "grid", None), "regular_grid", None) or \
+        getattr(getattr(geo_model, "grid", None), "dense_grid", None)
+    if rg is not None:
+        ext = np.asarray(rg.extent, dtype=np.float64).reshape(6)
+        lo = tr.apply(ext[[0, 2, 4]][None])[0]
+        hi = tr.apply(ext[[1, 3, 5]][None])[0]
+        grid = RegularGrid([lo[0], hi[0], lo[1], hi[1], lo[2], hi[2]], [int(r) for r in rg.resolution])
+    weights = None
+    sol = getattr(geo_model, "solutions", None)
+    root = getattr(sol, "root_output", None) if sol is not None else None
+    if root is not None:
+        weights = [np.asarray(o.weights, dtype=np.float64) for o in root.outputs]
+    return GeoModelInput(stacks=stacks, options=opts, grid=grid, name=getattr(getattr(geo_model, "meta", None),
+                                                                             "name", "model"),
+                         weights=weights)
+

@@ -72,21 +72,27 @@ class _Model:
         self.stacks: List[DeviceStack] = []
         self.sols: List[StackSolution] = []
         center = _center(model)
-        for st in model.stacks:
+        slope = float(model.options.sigmoid_slope)
+        for k, st in enumerate(model.stacks):
             fault_sp = None
             if st.faults:
                 pts = torch.tensor(st.sp_xyz, dtype=torch.float64, device="cuda")
                 fault_sp = np.stack([self._fault_ids(j, pts).cpu().numpy() for j in st.faults], 1)
             ds = DeviceStack(st, model.options.kernel_options, center, fault_sp)
+            x0 = None
+            if model.weights is not None and k < len(model.weights) and model.weights[k] is not None \
+                    and np.asarray(model.weights[k]).size == ds.N:
+                x0 = model.weights[k]                 # warm start from the previous solution
             if form == GMRES_LEARNED:
                 from .learned import solve_learned
                 w, info = solve_learned(ds)
             else:
-                w, info = ds.solve(form)
+                w, info = ds.solve(form, x0=x0)
+            info["warm_start"] = x0 is not None
             thr, s_sp = ds.surface_thresholds(w)
             self.stacks.append(ds)
             self.sols.append(StackSolution(weights=w, scalar_field_at_surface_points=s_sp.cpu().numpy(),
-                                           thresholds=thr, solver_info=info))
+                                           thresholds=thr, solver_info=info, sigmoid_slope=slope))
 
     def _fault_ids(self, j, pts):
         """Block values of fault stack j at pts; a fault offset by older faults
@@ -96,21 +102,33 @@ class _Model:
         _, _, fid = ds.evaluate_points(pts, precision=_lib.GK_F64, thresholds=ds.thresholds, fault_values=fv)
         return fid.double()
 
-    def at_points(self, pts: torch.Tensor, precision=_lib.GK_F64, keep_scalar=False):
-        """Ids of every stack, the combined lith block and the fault blocks at pts."""
-        ids_rows, nsurf, faults, fault_of, scalars = [], [], [], {}, []
+    def at_points(self, pts: torch.Tensor, precision=_lib.GK_F64, keep_scalar=False, per_stack=None):
+        """Ids of every stack, the combined lith block and the fault blocks at
+        pts.  per_stack (a list): receives (S, ids, mask) of every stack."""
+        ids_rows, nsurf, rels, faults, fault_of, scalars, rows_of = [], [], [], [], {}, [], []
+        per = []
         for k, (st, ds) in enumerate(zip(self.model.stacks, self.stacks)):
             fv = (torch.stack([fault_of[j] for j in st.faults], 1).contiguous() if st.faults else None)
             S, _, ids = ds.evaluate_points(pts, precision=precision, thresholds=ds.thresholds, fault_values=fv)
             if keep_scalar:
                 scalars.append(S)
+            per.append([S, ids, None])
             if st.relation == StackRelationType.FAULT:
                 fault_of[k] = ids.double()
                 faults.append(ids)
             else:
                 ids_rows.append(ids)
                 nsurf.append(st.n_surfaces)
-        lith = _combine(ids_rows, nsurf, pts.shape[0]) if ids_rows else None
+                rels.append(st.relation)
+                rows_of.append(k)
+        masks = None
+        lith = None
+        if ids_rows:
+            lith, masks = _combine(ids_rows, nsurf, pts.shape[0], rels, want_masks=per_stack is not None)
+        if per_stack is not None:
+            for i, k in enumerate(rows_of):
+                per[k][2] = masks[i]
+            per_stack.extend(per)
         return lith, faults, scalars
 
 
@@ -130,7 +148,7 @@ def compute_model(model: GeoModelInput, engine_config: Optional[GemPyEngineConfi
     m = _Model(model, _form(model, cfg))
     out = Solutions(stacks=m.sols)
     if grid is not None:
-        ids_rows, nsurf, fault_grid = [], [], {}
+        ids_rows, nsurf, rels, rows_of, fault_grid = [], [], [], [], {}
         for k, (st, ds, sol) in enumerate(zip(model.stacks, m.stacks, m.sols)):
             fv = torch.stack([fault_grid[j] for j in st.faults], 1).contiguous() if st.faults else None
             S, G, ids = ds.evaluate_lattice(grid, sol.weights, eval_precision, gradient, sol.thresholds, fv)
@@ -139,11 +157,16 @@ def compute_model(model: GeoModelInput, engine_config: Optional[GemPyEngineConfi
             sol.ids_block = ids
             if st.relation == StackRelationType.FAULT:
                 fault_grid[k] = ids.double()
+                sol.mask = torch.ones(grid.n_points, dtype=torch.uint8, device="cuda")
             else:
                 ids_rows.append(ids)
                 nsurf.append(st.n_surfaces)
+                rels.append(st.relation)
+                rows_of.append(k)
         if ids_rows:
-            out.lith_block = _combine(ids_rows, nsurf, grid.n_points)
+            out.lith_block, masks = _combine(ids_rows, nsurf, grid.n_points, rels, want_masks=True)
+            for i, k in enumerate(rows_of):
+                m.sols[k].mask = masks[i]
         out.fault_blocks = [m.sols[k].ids_block for k in sorted(fault_grid)]
         if keep_scalar:
             out.scalar_field_matrix = torch.stack([s.scalar_field for s in m.sols])
@@ -177,9 +200,16 @@ def compute_octree(m: _Model, levels: int) -> List[OctreeLevel]:
         _lib.check("gk_octree_points", lib.gk_octree_points(handle(0), ext, cres, _ptr(vox), n, int(last),
                                                             _ptr(pts), _stream()))
         pts = pts[:npts]
-        lith, faults, scal = m.at_points(pts, keep_scalar=True)
+        per: list = []
+        lith, faults, scal = m.at_points(pts, keep_scalar=True, per_stack=per)
+        step = 1 if last else 9                 # voxel centres: every 9th probe point
+        outs = [StackSolution(weights=sol.weights, scalar_field_at_surface_points=sol.scalar_field_at_surface_points,
+                              thresholds=sol.thresholds, scalar_field=S[0::step], ids_block=ids[0::step],
+                              solver_info=sol.solver_info, sigmoid_slope=sol.sigmoid_slope,
+                              mask=None if msk is None else msk[0::step])
+                for sol, (S, ids, msk) in zip(m.sols, per)]
         if last:
-            out.append(OctreeLevel(tuple(res), vox, lith, scal[-1] if scal else None, None))
+            out.append(OctreeLevel(tuple(res), vox, lith, scal[-1] if scal else None, None, outs))
             break
         fields = torch.stack([lith] + faults).contiguous() if lith is not None else torch.stack(faults)
         refine = torch.empty(max(n, 1), dtype=torch.uint8, device="cuda")
@@ -190,20 +220,26 @@ def compute_octree(m: _Model, levels: int) -> List[OctreeLevel]:
                                                             _stream()))
         centre_scalar = scal[-1][0::9] if scal else None
         out.append(OctreeLevel(tuple(res), vox, lith[0::9] if lith is not None else None, centre_scalar,
-                               refine[:n].bool()))
+                               refine[:n].bool(), outs))
         vox = children[:nch.value].clone()
         res = [2 * r for r in res]
     return out
 
 
-def _combine(rows, nsurf, n) -> torch.Tensor:
+def _combine(rows, nsurf, n, relations=None, want_masks=False):
+    """Lith block of the non-fault stacks (youngest first) and, with
+    want_masks, each stack's mask of the points it holds."""
     lib = _lib.load()
     ids = torch.stack(rows).contiguous()
     lith = torch.empty(max(n, 1), dtype=torch.int32, device="cuda")
+    masks = torch.empty((len(rows), max(n, 1)), dtype=torch.uint8, device="cuda") if want_masks else None
     ns = (C.c_int32 * len(nsurf))(*nsurf)
-    _lib.check("gk_combine_lith", lib.gk_combine_lith(handle(0), len(rows), _ptr(ids), n, ns, _ptr(lith),
-                                                      _stream()))
-    return lith[:n]
+    rel = [_lib.GK_REL_ONLAP if r == StackRelationType.ONLAP else _lib.GK_REL_ERODE
+           for r in (relations or [StackRelationType.ERODE] * len(rows))]
+    rl = (C.c_int32 * len(rel))(*rel)
+    _lib.check("gk_combine_stacks", lib.gk_combine_stacks(handle(0), len(rows), _ptr(ids), n, ns, rl, _ptr(lith),
+                                                          _ptr(masks), _stream()))
+    return lith[:n], (None if masks is None else masks[:, :n])
 
 
 def _center(model: GeoModelInput):

@@ -824,45 +824,71 @@ extern "C" size_t gk_matvec_scratch_doubles(const gk_system* sys) {
 
 // ------------------------------------------------------------ stack combination
 namespace {
-// lith = id of the youngest non-fault stack whose domain (above its last
-// surface, id <= n_surf) holds the point, offset to global element ids;
-// basement otherwise.  ids: [n_stacks][n] int32 (stack order youngest first).
+// lith = global element id (offset per stack) of the stack that holds the
+// point, basement otherwise.  ids: [n_stacks][n] int32, youngest stack
+// first.  Stack s holds p when p is above its last surface
+// (id <= nsurf[s]) and no younger stack holds it; an onlapping stack (rel ==
+// GK_REL_ONLAP) additionally needs p above the first surface of the next
+// older stack, whose top is preserved.  masks (nullable): [n_stacks][n],
+// 1 where that stack holds the point.
 __global__ void combine_lith_kernel(const int32_t* ids, int32_t n_stacks, int64_t n, const int32_t* nsurf,
-                                    const int32_t* offset, int32_t basement, int32_t* out) {
+                                    const int32_t* offset, const int32_t* rel, int32_t basement, int32_t* out,
+                                    uint8_t* masks) {
   const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= n) return;
   int32_t v = basement;
+  int owner = -1;
   for (int s = 0; s < n_stacks; ++s) {
     const int32_t id = ids[(int64_t)s * n + p];
-    if (id <= nsurf[s]) { v = id + offset[s]; break; }
+    if (id > nsurf[s]) continue;
+    if (rel[s] == GK_REL_ONLAP && s + 1 < n_stacks && ids[(int64_t)(s + 1) * n + p] > 1) continue;
+    v = id + offset[s];
+    owner = s;
+    break;
   }
   out[p] = v;
+  if (masks)
+    for (int s = 0; s < n_stacks; ++s) masks[(int64_t)s * n + p] = s == owner ? 1 : 0;
 }
 }  // namespace
 
-extern "C" int gk_combine_lith(gk_handle h, int32_t n_stacks, const int32_t* ids, int64_t n,
-                               const int32_t* n_surf, int32_t* lith, gk_stream stream) {
+static int combine_impl(gk_handle h, int32_t n_stacks, const int32_t* ids, int64_t n, const int32_t* n_surf,
+                        const int32_t* relation, int32_t* lith, uint8_t* masks, hipStream_t st) {
   GK_REQUIRE(h && ids && lith && n_surf && n_stacks > 0 && n_stacks <= 64 && n >= 0, GK_ERR_ARG,
              "gk_combine_lith: bad arguments");
-  hipStream_t st = (hipStream_t)stream;
-  int32_t hbuf[128];
+  int32_t hbuf[192];
   int32_t off = 0;
   for (int s = 0; s < n_stacks; ++s) {
     GK_REQUIRE(n_surf[s] > 0, GK_ERR_ARG, "gk_combine_lith: stack %d has no surface", s);
+    const int32_t r = relation ? relation[s] : GK_REL_ERODE;
+    GK_REQUIRE(r == GK_REL_ERODE || r == GK_REL_ONLAP, GK_ERR_ARG,
+               "gk_combine_lith: stack %d: relation %d (erode or onlap expected)", s, r);
     hbuf[s] = n_surf[s];
     hbuf[64 + s] = off;
+    hbuf[128 + s] = r;
     off += n_surf[s];
   }
   void* ws = nullptr;
-  GK_TRY(gk_workspace(h, WS_SMALL, 128 * sizeof(int32_t), &ws));
+  GK_TRY(gk_workspace(h, WS_SMALL, 192 * sizeof(int32_t), &ws));
   int32_t* d = static_cast<int32_t*>(ws);
   GK_HIP(hipMemcpyAsync(d, hbuf, sizeof(hbuf), hipMemcpyHostToDevice, st));
   if (n > 0)
     hipLaunchKernelGGL(combine_lith_kernel, dim3((unsigned)gk_ceil_div(n, 256)), dim3(256), 0, st, ids,
-                       n_stacks, n, d, d + 64, off + 1, lith);
+                       n_stacks, n, d, d + 64, d + 128, off + 1, lith, masks);
   GK_HIP(hipGetLastError());
   GK_HIP(hipStreamSynchronize(st));   // hbuf is a stack buffer
   return GK_OK;
+}
+
+extern "C" int gk_combine_lith(gk_handle h, int32_t n_stacks, const int32_t* ids, int64_t n,
+                               const int32_t* n_surf, int32_t* lith, gk_stream stream) {
+  return combine_impl(h, n_stacks, ids, n, n_surf, nullptr, lith, nullptr, (hipStream_t)stream);
+}
+
+extern "C" int gk_combine_stacks(gk_handle h, int32_t n_stacks, const int32_t* ids, int64_t n,
+                                 const int32_t* n_surf, const int32_t* relation, int32_t* lith, uint8_t* masks,
+                                 gk_stream stream) {
+  return combine_impl(h, n_stacks, ids, n, n_surf, relation, lith, masks, (hipStream_t)stream);
 }
 
 // ------------------------------------------------------- raw-source fields
@@ -910,4 +936,41 @@ extern "C" int gk_eval_sources(gk_handle h, double range, double c_o, const doub
   a.scalar = scalar;
   a.grad = gradient;
   return run_eval(h, a, false, GK_F64, gradient != nullptr, true, st);
+}
+
+// Smooth ids (the engine's differentiable block, sigmoid_slope at
+// 2-layers.approved.txt:181): the step 1 + #{k: S < t_k} (or S > t_k for a
+// stack whose thresholds increase) with every unit step replaced by the
+// logistic function of slope * (t_k - S) (or slope * (S - t_k)).
+__global__ void block_values_kernel(const double* S, int64_t n, int32_t n_thr, const double* thr, int32_t rev,
+                                    double slope, double* out) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const double s = S[p];
+  double v = 1.0;
+  for (int k = 0; k < n_thr; ++k) {
+    const double z = rev ? slope * (s - thr[k]) : slope * (thr[k] - s);
+    v += 1.0 / (1.0 + exp(-z));
+  }
+  out[p] = v;
+}
+
+extern "C" int gk_block_values(gk_handle h, const double* scalar, int64_t n, int32_t n_thresholds,
+                               const double* thresholds, double sigmoid_slope, double* block, gk_stream stream) {
+  GK_REQUIRE(h && scalar && block && thresholds && n >= 0 && n_thresholds >= 0 &&
+             n_thresholds <= GK_MAX_THRESHOLDS, GK_ERR_ARG, "gk_block_values: bad arguments");
+  hipStream_t st = (hipStream_t)stream;
+  double hbuf[GK_MAX_THRESHOLDS];
+  for (int k = 0; k < n_thresholds; ++k) hbuf[k] = thresholds[k];
+  const int32_t rev = n_thresholds >= 2 && hbuf[0] < hbuf[n_thresholds - 1] ? 1 : 0;
+  void* ws = nullptr;
+  GK_TRY(gk_workspace(h, WS_SMALL, 192 * sizeof(int32_t) + GK_MAX_THRESHOLDS * sizeof(double), &ws));
+  double* d = reinterpret_cast<double*>(static_cast<char*>(ws) + 192 * sizeof(int32_t));
+  if (n_thresholds > 0) GK_HIP(hipMemcpyAsync(d, hbuf, n_thresholds * sizeof(double), hipMemcpyHostToDevice, st));
+  if (n > 0)
+    hipLaunchKernelGGL(block_values_kernel, dim3((unsigned)gk_ceil_div(n, 256)), dim3(256), 0, st, scalar, n,
+                       n_thresholds, d, rev, sigmoid_slope, block);
+  GK_HIP(hipGetLastError());
+  GK_HIP(hipStreamSynchronize(st));   // hbuf is a stack buffer
+  return GK_OK;
 }

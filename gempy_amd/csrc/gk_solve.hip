@@ -462,12 +462,17 @@ struct Solver {
   // The true FP64 residual is checked every `check` inner iterations: the
   // iteration stops when it is below tol, or when it no longer halves between
   // checks (the FP64 rounding floor of the operator has been reached).
-  int fgmres(double* x, double tol, double accept, int max_iter, int* iters, double* relres, bool* ok) {
+  // x0 (nullable): initial guess (the previous solution of a related system).
+  int fgmres(double* x, double tol, double accept, int max_iter, int* iters, double* relres, bool* ok,
+             const double* x0 = nullptr) {
     const int check = 8;
     double nb = 0.0;
     GK_TRY(norm(b, N, &nb));
     if (nb == 0.0) nb = 1.0;
-    GK_HIP(hipMemsetAsync(x, 0, N * sizeof(double), st));
+    if (x0)
+      GK_HIP(hipMemcpyAsync(x, x0, N * sizeof(double), hipMemcpyDeviceToDevice, st));
+    else
+      GK_HIP(hipMemsetAsync(x, 0, N * sizeof(double), st));
     int total = 0;
     double last_true = 1.0;
     std::vector<double> hist;                 // true residual at each check
@@ -751,9 +756,10 @@ static int init_solver(gk_handle h, const gk_system* sys, hipStream_t st, int fo
   return GK_OK;
 }
 
-extern "C" int gk_solve(gk_handle h, const gk_system* sys, int32_t form, double tol, int32_t max_iter,
-                        double* x, gk_solve_info* info, gk_stream stream) {
+static int solve_impl(gk_handle h, const gk_system* sys, int32_t form, double tol, int32_t max_iter,
+                      const double* x0, double* x, gk_solve_info* info, gk_stream stream) {
   GK_REQUIRE(h && sys && x && info, GK_ERR_ARG, "gk_solve: NULL argument");
+  GK_REQUIRE(x0 != x || x0 == nullptr, GK_ERR_ARG, "gk_solve_x0: x0 and x must not alias");
   GK_REQUIRE(form == GK_SOLVE_FP64_LU || form == GK_SOLVE_FP32_REFINE || form == GK_SOLVE_FP64_CHOL, GK_ERR_ARG,
              "gk_solve: bad form");
   hipStream_t st = (hipStream_t)stream;
@@ -834,7 +840,7 @@ extern "C" int gk_solve(gk_handle h, const gk_system* sys, int32_t form, double 
   GK_HIP(hipEventRecord(ev[1], st));
   bool ok = false;
   if (rc == GK_OK) {
-    GK_TRY(S.fgmres(x, tol, 1e-6, max_iter, &info->iterations, &info->relres, &ok));
+    GK_TRY(S.fgmres(x, tol, 1e-6, max_iter, &info->iterations, &info->relres, &ok, x0));
   } else if (rc != GK_ERR_NOT_SPD) {
     return rc;
   }
@@ -850,6 +856,16 @@ extern "C" int gk_solve(gk_handle h, const gk_system* sys, int32_t form, double 
   }
   keep_state();
   return GK_OK;
+}
+
+extern "C" int gk_solve(gk_handle h, const gk_system* sys, int32_t form, double tol, int32_t max_iter,
+                        double* x, gk_solve_info* info, gk_stream stream) {
+  return solve_impl(h, sys, form, tol, max_iter, nullptr, x, info, stream);
+}
+
+extern "C" int gk_solve_x0(gk_handle h, const gk_system* sys, int32_t form, double tol, int32_t max_iter,
+                           const double* x0, double* x, gk_solve_info* info, gk_stream stream) {
+  return solve_impl(h, sys, form, tol, max_iter, x0, x, info, stream);
 }
 
 // ---------------------------------------------------------------- adjoint

@@ -170,7 +170,8 @@ class StackInput:
         return {"sp_xyz": self.sp_xyz, "counts": self.counts, "sp_nugget": self.sp_nugget,
                 "ori_xyz": self.ori_xyz, "ori_grad": self.ori_grad,
                 "ori_nugget": self.ori_nugget,
-                "relation": "fault" if self.relation == StackRelationType.FAULT else "erode",
+                "relation": {StackRelationType.FAULT: "fault", StackRelationType.ONLAP: "onlap"}.get(
+                    self.relation, "erode"),
                 "faults": list(self.faults)}
 
 
@@ -182,6 +183,10 @@ class GeoModelInput:
     grid: Optional[RegularGrid] = None
     name: str = "model"
     custom_grid: Optional[np.ndarray] = None
+    # previous solution's weights per stack: the initial guess of the
+    # iterative refinement (the reference passes them on,
+    # gempy/modules/data_manipulation/_engine_factory.py:45-48)
+    weights: Optional[List[np.ndarray]] = None
 
     def system_size(self, k: int = 0) -> int:
         st = self.stacks[k]
@@ -190,9 +195,22 @@ class GeoModelInput:
 
 
 @dataclass
+class ExportedFields:
+    """Fields of one stack at one set of points (the engine's exported_fields:
+    scalar_field, gradient components, scalar_field_at_surface_points)."""
+    scalar_field: Optional[object] = None
+    gx_field: Optional[object] = None
+    gy_field: Optional[object] = None
+    gz_field: Optional[object] = None
+    scalar_field_at_surface_points: Optional[np.ndarray] = None
+
+
+@dataclass
 class StackSolution:
     """Per-stack output (the engine's ``InterpOutput``: weights, exported fields,
-    ids block; ``test_example_models_I.py:20-21``)."""
+    ids block, smooth block; read at test_example_models_I.py:20-21,
+    test_cg_solver.py:48-50, gp3_to_gp2_output.py:58-78, marching_cubes.py:36-46).
+    Arrays are device tensors unless stated."""
     weights: np.ndarray
     scalar_field_at_surface_points: np.ndarray
     thresholds: np.ndarray
@@ -200,17 +218,78 @@ class StackSolution:
     gradient: Optional[np.ndarray] = None
     ids_block: Optional[np.ndarray] = None
     solver_info: dict = field(default_factory=dict)
+    mask: Optional[object] = None              # points this stack holds in the lith block
+    sigmoid_slope: float = 5_000_000.0
+    _block: Optional[object] = None
+
+    @property
+    def exported_fields(self) -> ExportedFields:
+        g = self.gradient
+        return ExportedFields(scalar_field=self.scalar_field,
+                              gx_field=None if g is None else g[:, 0], gy_field=None if g is None else g[:, 1],
+                              gz_field=None if g is None else g[:, 2],
+                              scalar_field_at_surface_points=np.asarray(self.thresholds))
+
+    # the dense-grid and the generic accessors of the reference read the same fields here
+    exported_fields_dense_grid = exported_fields
+
+    @property
+    def scalar_fields(self) -> "StackSolution":
+        return self
+
+    @property
+    def block(self):
+        """Smooth ids (logistic steps of slope sigmoid_slope at the thresholds)."""
+        if self._block is None and self.scalar_field is not None:
+            from .engine import block_values
+            self._block = block_values(self.scalar_field, self.thresholds, self.sigmoid_slope)
+        return self._block
+
+    @property
+    def values_block(self):
+        b = self.block
+        return None if b is None else b.reshape(1, -1)
 
 
 @dataclass
 class OctreeLevel:
     """Points evaluated at one octree level: voxel indices at that level's
-    resolution, ids at the voxel centres and the mask of voxels refined next."""
+    resolution, ids at the voxel centres, the mask of voxels refined next and
+    every stack's output at the voxel centres (``outputs``)."""
     resolution: tuple
     voxel_index: np.ndarray
     ids: np.ndarray
     scalar: np.ndarray
     refine: np.ndarray
+    outputs: List[StackSolution] = field(default_factory=list)
+
+    @property
+    def last_output_center(self) -> Optional[StackSolution]:
+        return self.outputs[-1] if self.outputs else None
+
+
+@dataclass
+class RawArrays:
+    """Dense-grid arrays as host numpy arrays (``raw_arrays``: lith_block,
+    scalar_field_matrix, block_matrix, mask_matrix, custom;
+    compute_API.py:114, test_subduction.py:87)."""
+    lith_block: Optional[np.ndarray] = None
+    fault_block: Optional[np.ndarray] = None
+    scalar_field_matrix: Optional[np.ndarray] = None
+    block_matrix: Optional[np.ndarray] = None
+    mask_matrix: Optional[np.ndarray] = None
+    custom: Optional[np.ndarray] = None
+    vertices: Optional[list] = None
+    edges: Optional[list] = None
+
+    @property
+    def mask_matrix_squeezed(self):
+        return self.mask_matrix
+
+
+class _Root:
+    def __init__(self, outputs):
+        self.outputs = outputs
 
 
 @dataclass
@@ -223,3 +302,55 @@ class Solutions:
     octrees_output: List[OctreeLevel] = field(default_factory=list)
     custom: Optional[np.ndarray] = None
     timings: dict = field(default_factory=dict)
+    dc_meshes: Optional[list] = None
+    _raw: Optional[RawArrays] = None
+
+    @property
+    def root_output(self):
+        """Per-stack outputs whose weights seed the next solve (_engine_factory.py:45-48)."""
+        return _Root(self.stacks)
+
+    @property
+    def scalar_field_at_surface_points(self) -> np.ndarray:
+        """Threshold of every surface of every stack, in order (one per element
+        but the basement; geo_model.py:106-108)."""
+        return np.concatenate([np.asarray(s.thresholds, dtype=np.float64) for s in self.stacks]) \
+            if self.stacks else np.zeros(0)
+
+    @property
+    def _ordered_elements(self) -> List[np.ndarray]:
+        """Per stack, its surfaces ordered by their thresholds from the top
+        (geo_model.py:123-127)."""
+        out = []
+        for s in self.stacks:
+            t = np.asarray(s.thresholds, dtype=np.float64)
+            rev = len(t) >= 2 and t[0] < t[-1]
+            out.append(np.argsort(t if rev else -t, kind="stable"))
+        return out
+
+    @property
+    def block_solution_type(self) -> str:
+        return "octree" if self.octrees_output else "dense"
+
+    @property
+    def raw_arrays(self) -> RawArrays:
+        if self._raw is None:
+            host = lambda t: None if t is None else t.detach().cpu().numpy()  # noqa: E731
+            r = RawArrays()
+            if self.lith_block is not None:
+                r.lith_block = host(self.lith_block).astype(np.float64)
+            if self.fault_blocks:
+                r.fault_block = np.stack([host(f) for f in self.fault_blocks]).astype(np.float64)
+            if self.scalar_field_matrix is not None:
+                r.scalar_field_matrix = host(self.scalar_field_matrix)
+            if self.stacks and all(s.scalar_field is not None for s in self.stacks):
+                r.block_matrix = np.stack([host(s.block) for s in self.stacks])
+            if self.stacks and all(s.mask is not None for s in self.stacks):
+                r.mask_matrix = np.stack([host(s.mask).astype(bool) for s in self.stacks])
+            if self.custom is not None:
+                r.custom = host(self.custom).astype(np.float64)
+            if self.dc_meshes is not None:
+                r.vertices = [m.vertices for m in self.dc_meshes]
+                r.edges = [m.edges for m in self.dc_meshes]
+            self._raw = r
+        return self._raw

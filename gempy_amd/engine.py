@@ -115,12 +115,22 @@ class DeviceStack:
         self.thresholds: Optional[np.ndarray] = None
 
     # ------------------------------------------------------------------ solve
-    def solve(self, form: int = _lib.GK_SOLVE_FP32_REFINE, tol: float = 1e-10, max_iter: int = 300):
+    def solve(self, form: int = _lib.GK_SOLVE_FP32_REFINE, tol: float = 1e-10, max_iter: int = 300,
+              x0: Optional[torch.Tensor] = None):
+        """Solve the stack's system; x0 (length N): initial guess of the
+        FP32 form's refinement (e.g. the weights of a previous solve)."""
         lib = _lib.load()
         x = torch.empty(self.N, dtype=torch.float64, device=self.dev)
         info = _lib.GkSolveInfo()
-        _lib.check("gk_solve", lib.gk_solve(self.h, C.byref(self.sys), int(form), float(tol),
-                                             int(max_iter), _ptr(x), C.byref(info), _stream()))
+        if x0 is not None:
+            x0 = torch.as_tensor(x0, dtype=torch.float64, device=self.dev).reshape(-1).contiguous()
+            if x0.numel() != self.N:
+                raise ValueError(f"initial guess of length {x0.numel()}, the system has {self.N} unknowns")
+            _lib.check("gk_solve_x0", lib.gk_solve_x0(self.h, C.byref(self.sys), int(form), float(tol),
+                                                      int(max_iter), _ptr(x0), _ptr(x), C.byref(info), _stream()))
+        else:
+            _lib.check("gk_solve", lib.gk_solve(self.h, C.byref(self.sys), int(form), float(tol),
+                                                 int(max_iter), _ptr(x), C.byref(info), _stream()))
         DeviceStack.solve_epoch += 1
         self.weights = x
         return x, {"form": int(info.form), "iterations": int(info.iterations),
@@ -237,3 +247,17 @@ class DeviceStack:
         thr = S[torch.as_tensor(self.ref_idx, device=self.dev)].cpu().numpy()
         self.thresholds = thr
         return thr, S
+
+
+def block_values(scalar: torch.Tensor, thresholds, sigmoid_slope: float) -> torch.Tensor:
+    """Smooth ids of one stack (gk_block_values) at the points of `scalar`."""
+    lib = _lib.load()
+    t = [float(v) for v in thresholds]
+    if len(t) > _lib.GK_MAX_THRESHOLDS:
+        raise ValueError(f"at most {_lib.GK_MAX_THRESHOLDS} surfaces per stack")
+    S = scalar.reshape(-1).contiguous()
+    out = torch.empty_like(S)
+    arr = (C.c_double * max(len(t), 1))(*t)
+    _lib.check("gk_block_values", lib.gk_block_values(handle(S.device.index or 0), _ptr(S), S.numel(), len(t), arr,
+                                                      float(sigmoid_slope), _ptr(out), _stream()))
+    return out

@@ -41,6 +41,7 @@ enum {
 enum { GK_F32 = 0, GK_F64 = 1 };
 enum { GK_POINTS_EXPLICIT = 0, GK_POINTS_LATTICE = 1 };
 enum { GK_SOLVE_FP64_LU = 0, GK_SOLVE_FP32_REFINE = 1, GK_SOLVE_FP64_CHOL = 2 };
+enum { GK_REL_ERODE = 1, GK_REL_ONLAP = 2 };   /* stack relations (values of the reference's StackRelationType) */
 #define GK_MAX_THRESHOLDS 64
 #define GK_SPAI_K 16          /* neighbours per data point of the learned preconditioner */
 #define GK_SPAI_FEATURES 8    /* pair features fed to its network */
@@ -188,6 +189,14 @@ int gk_gemm_f64(gk_handle h, int64_t m, int64_t n, int64_t k, const double* A, i
 int gk_solve(gk_handle h, const gk_system* sys, int32_t form, double tol, int32_t max_iter,
              double* x, gk_solve_info* info, gk_stream stream);
 
+/* gk_solve with an initial guess x0 (device, length N; must not alias x): the
+ * weights of a previous solve of a related system, as the reference passes the
+ * previous solution's weights into the next solve
+ * (gempy/modules/data_manipulation/_engine_factory.py:45-48).  The FP32 form
+ * starts its FGMRES refinement from x0; the direct FP64 forms do not use it. */
+int gk_solve_x0(gk_handle h, const gk_system* sys, int32_t form, double tol, int32_t max_iter,
+                const double* x0, double* x, gk_solve_info* info, gk_stream stream);
+
 /* Solve A x = rhs with the factorization kept by the last gk_solve on this
  * handle (FP64_CHOL or FP32_REFINE forms; the same system). */
 int gk_solve_rhs(gk_handle h, const gk_system* sys, const double* rhs, double* x, gk_solve_info* info,
@@ -224,6 +233,22 @@ int gk_eval_sources(gk_handle h, double range, double c_o, const double* center,
  * last surface) holds p, basement = sum(n_surf) + 1 otherwise. */
 int gk_combine_lith(gk_handle h, int32_t n_stacks, const int32_t* ids, int64_t n, const int32_t* n_surf,
                     int32_t* lith, gk_stream stream);
+/* The same with each stack's relation to the older ones (host array,
+ * GK_REL_ERODE or GK_REL_ONLAP; StackRelationType of
+ * gempy/core/data/structural_frame.py:243-273): an onlapping stack holds a
+ * point only above the first surface of the next older stack (whose top is
+ * preserved).  masks (nullable, [n_stacks][n] uint8): 1 where that stack holds
+ * the point (the engine's mask_matrix; marching_cubes.py:44-46). */
+int gk_combine_stacks(gk_handle h, int32_t n_stacks, const int32_t* ids, int64_t n, const int32_t* n_surf,
+                      const int32_t* relation, int32_t* lith, uint8_t* masks, gk_stream stream);
+
+/* Smooth ids of one stack at n points (the engine's differentiable `block`,
+ * `values_block`; gempy/API/gp2_gp3_compatibility/gp3_to_gp2_output.py:97,124):
+ * block = 1 + sum_k logistic(sigmoid_slope (t_k - S)) for decreasing
+ * thresholds (host array), with (S - t_k) when they increase; it tends to the
+ * integer ids of gk_eval as the slope grows. */
+int gk_block_values(gk_handle h, const double* scalar, int64_t n, int32_t n_thresholds, const double* thresholds,
+                    double sigmoid_slope, double* block, gk_stream stream);
 
 /* ---- octree evaluation (the engine's octree levels: _number_octree_levels,
  * fault.approved.txt:210-214; resolution = base * 2^(levels-1),

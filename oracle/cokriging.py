@@ -396,7 +396,7 @@ def compute_model(stacks, opts, x, gradient=False):
     """Multi-stack model.
 
     stacks: list (youngest first) of dicts with keys sp_xyz, counts, sp_nugget,
-    ori_xyz, ori_grad, ori_nugget, relation ('erode' | 'fault'), faults (list
+    ori_xyz, ori_grad, ori_nugget, relation ('erode' | 'onlap' | 'fault'), faults (list
     of indices of fault stacks that offset it; they must come earlier).
     Returns dict with per-stack results, lith_block (global ids over the
     non-fault stacks, basement last) and fault_blocks.
@@ -424,19 +424,27 @@ def compute_model(stacks, opts, x, gradient=False):
                 return segment(s, r["thresholds"]).astype(np.float64)
             fault_vals_sp_fn[k] = block_fn
             fault_vals_x[k] = r["ids"].astype(np.float64)
-    # lithology: youngest non-fault stack whose domain contains the point
+    # lithology: youngest non-fault stack whose domain (above its last surface)
+    # contains the point; an onlapping stack ('onlap') only above the first
+    # surface of the next older non-fault stack, whose top it does not erode
     lith = np.zeros(x.shape[0], dtype=np.int32)
     assigned = np.zeros(x.shape[0], dtype=bool)
+    layer = [k for k, st in enumerate(stacks) if st.get("relation") != "fault"]
+    masks = {}
     next_id = 1
-    for st, r in zip(stacks, out):
-        if st.get("relation") == "fault":
-            continue
+    for pos, k in enumerate(layer):
+        st, r = stacks[k], out[k]
         ns = len(st["counts"])
-        inside = r["ids"] <= ns                 # above the stack's last surface
+        inside = r["ids"] <= ns
+        if st.get("relation") == "onlap" and pos + 1 < len(layer):
+            inside &= out[layer[pos + 1]]["ids"] == 1
         take = inside & ~assigned
         lith[take] = r["ids"][take] + next_id - 1
-        assigned |= inside
+        masks[k] = take
+        assigned |= take
         next_id += ns
     lith[~assigned] = next_id                   # basement
     faults = [r["ids"] for st, r in zip(stacks, out) if st.get("relation") == "fault"]
-    return dict(stacks=out, lith_block=lith, fault_blocks=faults, basement_id=next_id)
+    mask_matrix = [masks.get(k, np.ones(x.shape[0], dtype=bool)) for k in range(len(stacks))]
+    return dict(stacks=out, lith_block=lith, fault_blocks=faults, basement_id=next_id,
+                mask_matrix=mask_matrix)

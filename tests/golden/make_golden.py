@@ -15,6 +15,8 @@ where the reference tree is readable; the fixtures travel, the reference does no
   the 8 box corners [3,3,3,3,1,1,1,1] (test_custom_grid.py:50-65).
 * config1_oracle.npz -- the oracle's own FP64 outputs for config 1 (regression
   data for the GPU parity tests; pinned by the tests above, not independent).
+* transforms.npz -- raw input points and the recorded input_transform
+  (position, scale) of the 2-layers, anticline and 2.5-D (Model1) models.
 """
 from __future__ import annotations
 
@@ -85,8 +87,62 @@ def config1_oracle():
                         A=r["A"], b=r["b"])
 
 
+def _approved_transform(rel):
+    import json
+    text = open(os.path.join(REF, rel)).read()
+    d = json.loads(text[text.index("{"):])
+
+    def find(o):
+        if isinstance(o, dict):
+            if "input_transform" in o:
+                return o["input_transform"]
+            for v in o.values():
+                r = find(v)
+                if r is not None:
+                    return r
+        if isinstance(o, list):
+            for v in o:
+                r = find(v)
+                if r is not None:
+                    return r
+        return None
+    t = find(d)
+    return np.array(t["position"], dtype=float), np.array(t["scale"], dtype=float)
+
+
+def transforms():
+    """Raw input points and the recorded input_transform of three reference models:
+    2-layers (test_gravity.py:11-60; 2-layers.approved.txt), the anticline
+    (jan_models/model2_*.csv; test_custom_grid...fold.approved.txt) and the 2.5-D
+    model's first element (examples_generator.py:33-60; Model1.approved.txt)."""
+    out = {}
+    # 2-layers
+    sp = np.array([[3, 0, 3.05], [9, 0, 3.05], [3, 0, 1.02], [9, 0, 1.02]], dtype=float)
+    ori = np.array([[6, 0, 4]], dtype=float)
+    pos, sc = _approved_transform("test/test_modules/test_geophysics/test_gravity.test_gravity.verify/"
+                                  "2-layers.approved.txt")
+    out.update(two_layers_sp=sp, two_layers_ori=ori, two_layers_position=pos, two_layers_scale=sc)
+    # anticline
+    base = os.path.join(REF, "examples/data/input_data/jan_models")
+    sp = np.array([[float(r["X"]), float(r["Y"]), float(r["Z"])]
+                   for r in _read_csv(os.path.join(base, "model2_surface_points.csv"))])
+    ori = np.array([[float(r["X"]), float(r["Y"]), float(r["Z"])]
+                    for r in _read_csv(os.path.join(base, "model2_orientations.csv"))])
+    pos, sc = _approved_transform("test/test_modules/test_grids/test_custom_grid.test_custom_grid.verify/"
+                                  "fold.approved.txt")
+    out.update(fold_sp=sp, fold_ori=ori, fold_position=pos, fold_scale=sc)
+    # 2.5-D model, first element only (update_transform is called right after it)
+    sp = np.array([[223, 0.01, -94], [458, 0, -197], [612, 0, -14]], dtype=float)
+    ori = np.array([[350, 1, -300]], dtype=float)
+    pos, sc = _approved_transform("test/test_modules/test_grids/test_grids_sections.test_topography_II.verify/"
+                                  "Model1.approved.txt")
+    out.update(model1_sp=sp, model1_ori=ori, model1_position=pos, model1_scale=sc)
+    np.savez(os.path.join(HERE, "transforms.npz"), **out)
+
+
 if __name__ == "__main__":
     two_layers()
     anticline()
     config1_oracle()
+    transforms()
     print("fixtures written to", HERE)

@@ -224,3 +224,73 @@ def test_legacy_fault_models_match_v2_and_oracle(gpu, tag):
             mism = int(((v2 != g["lith_block"]) & ~band).sum())
             say(f"{tag} {dt}: lith mismatches vs v2 outside band {mism} (band {int(band.sum())})")
             assert mism == 0
+
+
+def test_warm_start_fewer_iterations_same_field(gpu):
+    """The previous solution's weights as the initial guess (the reference
+    passes them on, _engine_factory.py:45-48): a perturbed system solved in the
+    FP32 form from the old weights needs fewer refinement iterations and gives
+    the same field as the cold solve."""
+    say("warm start: perturbed config3-style system (N ~ 6k)")
+    from gempy_amd import GemPyEngineConfig, compute_model, synthetic
+    m = synthetic.config3(n_points=3_000, n_ori=1_000, resolution=(24, 24, 24))
+    sol0 = compute_model(m, GemPyEngineConfig(dtype="float32"))
+    pert = synthetic.config3(n_points=3_000, n_ori=1_000, resolution=(24, 24, 24))
+    rng = np.random.default_rng(12)
+    st = pert.stacks[0]
+    st.sp_xyz = st.sp_xyz + rng.normal(0, 0.5, st.sp_xyz.shape)       # a small update of the data
+    cold = compute_model(pert, GemPyEngineConfig(dtype="float32"))
+    pert.weights = [s.weights.cpu().numpy() for s in sol0.root_output.outputs]
+    warm = compute_model(pert, GemPyEngineConfig(dtype="float32"))
+    ic, iw = cold.stacks[0].solver_info, warm.stacks[0].solver_info
+    Sc = cold.stacks[0].scalar_field.cpu().numpy()
+    Sw = warm.stacks[0].scalar_field.cpu().numpy()
+    err = np.abs(Sc - Sw).max() / np.ptp(Sc)
+    say(f"warm start: cold {ic['iterations']} iterations (relres {ic['relres']:.2e}), warm {iw['iterations']} "
+        f"(relres {iw['relres']:.2e}, warm_start={iw['warm_start']}); max|dS|/range {err:.2e}")
+    assert iw["warm_start"] and not ic["warm_start"]
+    assert iw["iterations"] < ic["iterations"]
+    assert iw["relres"] <= 1e-6 and err <= 1e-6
+
+
+def test_solutions_view_matches_oracle(gpu):
+    """raw_arrays / masks / smooth block / octree outputs of a multi-stack model
+    with an onlapping stack against the oracle's lith block and mask matrix."""
+    say("solutions view + onlap: config2 scale 0.05, 24^3, 2 octree levels")
+    from gempy_amd import GemPyEngineConfig, compute_model, synthetic
+    from gempy_amd.data import StackRelationType
+    m = synthetic.config2(scale=0.05, resolution=(24, 24, 24), octree_levels=2)
+    m.stacks[2].relation = StackRelationType.ONLAP          # 'upper' onlaps 'lower'
+    X = m.grid.values()
+    res = ok.compute_model([s.as_oracle_dict() for s in m.stacks], m.options.as_dict(), X)
+    sol = compute_model(m, GemPyEngineConfig(dtype="float64"))
+    raw = sol.raw_arrays
+    band = np.zeros(len(X), dtype=bool)
+    for r in res["stacks"]:
+        band |= np.min(np.abs(r["scalar"][:, None] - r["thresholds"][None, :]), 1) <= 1e-6 * np.ptp(r["scalar"])
+    mism = int(((raw.lith_block != res["lith_block"]) & ~band).sum())
+    mmask = int(((raw.mask_matrix != np.stack(res["mask_matrix"])) & ~band[None]).sum())
+    say(f"onlap: lith mismatches outside band {mism}, mask mismatches {mmask}, band {int(band.sum())}")
+    assert mism == 0 and mmask == 0
+    # the onlap rule changes the lith block against pure erosion
+    ero = [dict(s.as_oracle_dict(), relation="erode" if s.relation == StackRelationType.ONLAP
+                else s.as_oracle_dict()["relation"]) for s in m.stacks]
+    res_e = ok.compute_model(ero, m.options.as_dict(), X)
+    assert int((res_e["lith_block"] != res["lith_block"]).sum()) > 0
+    assert raw.scalar_field_matrix.shape == (4, len(X)) and raw.fault_block.shape == (1, len(X))
+    # smooth block: equals the ids away from the thresholds (steep slope)
+    blk = raw.block_matrix
+    for k, r in enumerate(res["stacks"]):
+        far = np.min(np.abs(r["scalar"][:, None] - r["thresholds"][None, :]), 1) > 1e-3 * np.ptp(r["scalar"])
+        np.testing.assert_allclose(blk[k][far], r["ids"][far], rtol=0, atol=1e-9)
+    assert np.all(np.diff(np.sort(blk[1])) >= 0)
+    # octree outputs: every stack at the voxel centres of each level
+    assert len(sol.octrees_output) == 2
+    lvl = sol.octrees_output[-1]
+    assert len(lvl.outputs) == 4
+    assert lvl.last_output_center is lvl.outputs[-1]
+    dense_S = sol.stacks[3].scalar_field
+    np.testing.assert_allclose(lvl.outputs[3].exported_fields.scalar_field.cpu().numpy(),
+                               dense_S[lvl.voxel_index].cpu().numpy(), rtol=0, atol=1e-9 * float(dense_S.abs().max()))
+    np.testing.assert_array_equal(sol.octrees_output[0].outputs[3].weights.cpu().numpy(),
+                                  sol.stacks[3].weights.cpu().numpy())
