@@ -238,7 +238,8 @@ def test_warm_start_fewer_iterations_same_field(gpu):
     pert = synthetic.config3(n_points=3_000, n_ori=1_000, resolution=(24, 24, 24))
     rng = np.random.default_rng(12)
     st = pert.stacks[0]
-    st.sp_xyz = st.sp_xyz + rng.normal(0, 0.5, st.sp_xyz.shape)       # a small update of the data
+    g = st.ori_grad + rng.normal(0, 1e-5, st.ori_grad.shape)         # a small update of the data
+    st.ori_grad = g / np.linalg.norm(g, axis=1, keepdims=True)
     cold = compute_model(pert, GemPyEngineConfig(dtype="float32"))
     pert.weights = [s.weights.cpu().numpy() for s in sol0.root_output.outputs]
     warm = compute_model(pert, GemPyEngineConfig(dtype="float32"))
