@@ -22,6 +22,7 @@ GK_BLOCK = 256
 GK_CHOL_PLANES = 5
 GK_SPAI_WITH_FACTOR, GK_SPAI_NO_FALLBACK, GK_SPAI_SANDWICH = 1, 2, 4
 GK_SPAI_K, GK_SPAI_FEATURES, GK_SPAI_HIDDEN = 16, 8, 32
+GK_EVAL_SCRATCH_BYTES = 32 << 20
 
 P = C.c_void_p
 
@@ -80,6 +81,7 @@ SIGNATURES = {
     "gk_solve_x0": (C.c_int, [P, C.POINTER(GkSystem), C.c_int32, C.c_double, C.c_int32, P, P,
                               C.POINTER(GkSolveInfo), P]),
     "gk_eval": (C.c_int, [P, C.POINTER(GkSystem), P, C.POINTER(GkEvalDesc), P]),
+    "gk_eval_noalloc": (C.c_int, [P, C.POINTER(GkSystem), P, C.POINTER(GkEvalDesc), P]),
     "gk_solve_rhs": (C.c_int, [P, C.POINTER(GkSystem), P, P, C.POINTER(GkSolveInfo), P]),
     "gk_adjoint": (C.c_int, [P, C.POINTER(GkSystem), P, C.c_int64, P, P, P, P, P, P, P]),
     "gk_eval_sources": (C.c_int, [P, C.c_double, C.c_double, C.POINTER(C.c_double), C.c_int64, P, P,

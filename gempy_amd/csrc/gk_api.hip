@@ -41,6 +41,11 @@ extern "C" int gk_create(int device, gk_handle* out) {
   GK_HIP(hipSetDevice(device));
   gk_context* c = new gk_context();
   c->device = device;
+  const int rc = gk_eval_scratch_init(c);
+  if (rc != GK_OK) {
+    delete c;
+    return rc;
+  }
   *out = c;
   return GK_OK;
 }

@@ -588,7 +588,7 @@ static bool needs_mask(const gk_system* sys, const double* lo, const double* hi,
 // Split the sources over blockIdx.y when the points alone cannot fill the GPU,
 // then launch the evaluation kernel for the requested variant.
 static int run_eval(gk_handle h, EvalArgs& a, bool lattice, int precision, bool grad, bool masked,
-                    hipStream_t st) {
+                    hipStream_t st, bool external = false, char* part = nullptr, size_t part_bytes = 0) {
   a.nsplit = 1;
   a.pts_chunk = std::max<int64_t>(a.n_pts, 1);
   a.ori_chunk = std::max<int64_t>(a.n_ori, 1);
@@ -597,6 +597,11 @@ static int run_eval(gk_handle h, EvalArgs& a, bool lattice, int precision, bool 
     const int64_t blocks = gk_ceil_div(gk_ceil_div(a.n_points, ppt), TS);
     const int64_t tiles = std::max(gk_ceil_div(a.n_pts, TS), gk_ceil_div(a.n_ori, TS));
     int64_t want = std::min<int64_t>(std::min<int64_t>(gk_ceil_div(2048, blocks), 64), tiles);
+    if (external) {          // as many splits as the caller's scratch holds partials for
+      const size_t per_split = (size_t)a.n_points * (grad ? 4 : 1) * sizeof(double);
+      want = std::min<int64_t>(want, per_split > 0 ? (int64_t)((part_bytes > 512 ? part_bytes - 512 : 0) / per_split)
+                                                   : 1);
+    }
     if (want > 1) {
       // chunks are at least one tile, so an empty source set (n = 0) never divides by zero
       a.pts_chunk = std::max<int64_t>(gk_round_up(gk_ceil_div(a.n_pts, want), TS), TS);
@@ -604,9 +609,9 @@ static int run_eval(gk_handle h, EvalArgs& a, bool lattice, int precision, bool 
       a.nsplit = (int32_t)std::max<int64_t>(
           std::max(gk_ceil_div(a.n_pts, a.pts_chunk), gk_ceil_div(a.n_ori, a.ori_chunk)), 1);
       if (a.nsplit > 1) {
-        void* pw = nullptr;
+        void* pw = part;
         const size_t per = (size_t)a.n_points * (grad ? 4 : 1);
-        GK_TRY(gk_workspace(h, WS_EVAL_PART, per * a.nsplit * sizeof(double) + 512, &pw));
+        if (!external) GK_TRY(gk_workspace(h, WS_EVAL_PART, per * a.nsplit * sizeof(double) + 512, &pw));
         a.part_s = static_cast<double*>(pw);
         a.part_g = grad ? a.part_s + (size_t)a.nsplit * a.n_points : nullptr;
       } else {
@@ -621,9 +626,20 @@ static int run_eval(gk_handle h, EvalArgs& a, bool lattice, int precision, bool 
   return GK_OK;
 }
 
+// ext (nullable): caller-provided scratch for the packed sources and the
+// split partials instead of the handle's workspaces (the no-allocation entry
+// point); the split is then limited to what fits in it.
+static int eval_impl(gk_handle h, const gk_system* sys, const double* weights, gk_eval_desc* d, hipStream_t st,
+                     bool force_mask, const double* src_box, bool reuse_sources, char* ext, size_t ext_bytes);
+
 extern "C" int gk_eval_impl(gk_handle h, const gk_system* sys, const double* weights,
                             gk_eval_desc* d, hipStream_t st, bool force_mask,
                             const double* src_box /*6 or null*/, bool reuse_sources) {
+  return eval_impl(h, sys, weights, d, st, force_mask, src_box, reuse_sources, nullptr, 0);
+}
+
+static int eval_impl(gk_handle h, const gk_system* sys, const double* weights, gk_eval_desc* d, hipStream_t st,
+                     bool force_mask, const double* src_box, bool reuse_sources, char* ext, size_t ext_bytes) {
   GK_TRY(check_system(sys));
   GK_REQUIRE(d && weights && d->scalar, GK_ERR_ARG, "gk_eval: desc, weights and scalar required");
   GK_REQUIRE(d->precision == GK_F32 || d->precision == GK_F64, GK_ERR_ARG, "gk_eval: bad precision");
@@ -669,7 +685,19 @@ extern "C" int gk_eval_impl(gk_handle h, const gk_system* sys, const double* wei
   for (int k = 0; k < d->n_thresholds; ++k) a.thr[k] = d->thresholds[k];
 
   void* ws = nullptr;
-  GK_TRY(gk_workspace(h, WS_EVAL_SRC, sources_bytes(sys), &ws));
+  char* part = nullptr;
+  size_t part_bytes = 0;
+  if (ext) {
+    const size_t need = gk_round_up(sources_bytes(sys), 256);
+    GK_REQUIRE(need <= ext_bytes, GK_ERR_ARG,
+               "gk_eval_noalloc: the system's packed sources need %zu bytes, the scratch array holds %zu "
+               "(GK_EVAL_SCRATCH_BYTES)", need, ext_bytes);
+    ws = ext;
+    part = ext + need;
+    part_bytes = ext_bytes - need;
+  } else {
+    GK_TRY(gk_workspace(h, WS_EVAL_SRC, sources_bytes(sys), &ws));
+  }
   GK_TRY(prepare_sources(h, sys, weights, d->precision, st, &a.pts, &a.ori, ws, !reuse_sources));
   bool masked = true;
   if (!force_mask && src_box) {
@@ -680,7 +708,8 @@ extern "C" int gk_eval_impl(gk_handle h, const gk_system* sys, const double* wei
       for (int k = 0; k < 3; ++k) { lo[k] = sl[k]; hi[k] = sh[k]; }
     masked = needs_mask(sys, lo, hi, sl, sh);
   }
-  GK_TRY(run_eval(h, a, d->kind == GK_POINTS_LATTICE, d->precision, d->gradient != nullptr, masked, st));
+  GK_TRY(run_eval(h, a, d->kind == GK_POINTS_LATTICE, d->precision, d->gradient != nullptr, masked, st, ext != nullptr,
+                  part, part_bytes));
   GK_HIP(hipGetLastError());
   return GK_OK;
 }
@@ -973,4 +1002,27 @@ extern "C" int gk_block_values(gk_handle h, const double* scalar, int64_t n, int
   GK_HIP(hipGetLastError());
   GK_HIP(hipStreamSynchronize(st));   // hbuf is a stack buffer
   return GK_OK;
+}
+
+// ------------------------------------------------------ no-allocation entry
+// Scratch of gk_eval_noalloc: the packed sources and the split partials live
+// in this file-scope array (one copy per GPU, held until the process ends).
+// Host code never uses the array's name as an address: gk_eval_scratch_init
+// takes its device address once per handle (the device is current in
+// gk_create) and the entry point passes that address to the kernels.
+static __device__ __align__(256) unsigned char scratch[GK_EVAL_SCRATCH_BYTES];
+
+int gk_eval_scratch_init(gk_handle h) {
+  void* p = nullptr;
+  GK_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+  GK_REQUIRE(p != nullptr, GK_ERR_HIP, "gk_create: no device address for the evaluation scratch array");
+  h->eval_scratch = static_cast<char*>(p);
+  return GK_OK;
+}
+
+extern "C" int gk_eval_noalloc(gk_handle h, const gk_system* sys, const double* weights, gk_eval_desc* desc,
+                               gk_stream stream) {
+  GK_REQUIRE(h && h->eval_scratch, GK_ERR_ARG, "gk_eval_noalloc: handle without an evaluation scratch array");
+  return eval_impl(h, sys, weights, desc, (hipStream_t)stream, false, nullptr, false, h->eval_scratch,
+                   GK_EVAL_SCRATCH_BYTES);
 }

@@ -65,7 +65,13 @@ struct gk_context {
   // auxiliary stream (default priority) for set-up work that overlaps a factorization
   hipStream_t aux = nullptr;
   hipEvent_t aux_ev[2] = {};
+  // device address of gk_eval_noalloc's file-scope scratch array (this device's copy)
+  char* eval_scratch = nullptr;
 };
+
+// Takes the device address of the evaluation scratch array for h's device
+// (called by gk_create with that device current).
+int gk_eval_scratch_init(gk_handle h);
 
 // Depth of the chain step of the triangular solves: it multiplies the planes
 // Dinv_k L_{k,k-d} (forward) and L_{k+d,k} Dinv_k (backward) for d = 1..depth;

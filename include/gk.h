@@ -220,6 +220,21 @@ int gk_adjoint(gk_handle h, const gk_system* sys, const double* w, int64_t n_obs
 /* ---- (c) evaluation: scalar field, gradient, ids ------------------------ */
 int gk_eval(gk_handle h, const gk_system* sys, const double* weights, gk_eval_desc* desc,
             gk_stream stream);
+/* gk_eval without any device allocation, for callers that cannot allocate
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  No counterpart in the reference.  The packed sources and the
+ * split partial sums live in a fixed file-scope device array of
+ * GK_EVAL_SCRATCH_BYTES per GPU instead of the handle's growable workspace:
+ * systems whose packed sources (8 n_surf + 32 (n_rest + n_surf) + 64 n_ori
+ * bytes, FP64) exceed it are rejected with GK_ERR_ARG; explicit point sets
+ * use fewer source splits when their partials do not fit.  Results equal
+ * gk_eval's whenever the same split is used.  Every call on one GPU shares
+ * that array: calls that overlap on one GPU (other streams, other threads)
+ * race and must be serialised by the caller. */
+#define GK_EVAL_SCRATCH_BYTES (32u << 20)
+int gk_eval_noalloc(gk_handle h, const gk_system* sys, const double* weights, gk_eval_desc* desc,
+                    gk_stream stream);
+
 /* F(x) = sum_s w_s C(|x - s|) and grad F (nullable) at explicit points in FP64,
  * with explicitly given sources (range, c_o, frame centre as in gk_system). */
 int gk_eval_sources(gk_handle h, double range, double c_o, const double* center, int64_t n_src,
