@@ -84,6 +84,7 @@ SIGNATURES = {
     "gk_eval_noalloc": (C.c_int, [P, C.POINTER(GkSystem), P, C.POINTER(GkEvalDesc), P]),
     "gk_solve_rhs": (C.c_int, [P, C.POINTER(GkSystem), P, P, C.POINTER(GkSolveInfo), P]),
     "gk_adjoint": (C.c_int, [P, C.POINTER(GkSystem), P, C.c_int64, P, P, P, P, P, P, P]),
+    "gk_adjoint_full": (C.c_int, [P, C.POINTER(GkSystem), P, C.c_int64, P, P, P, P, P, P, P, P]),
     "gk_eval_sources": (C.c_int, [P, C.c_double, C.c_double, C.POINTER(C.c_double), C.c_int64, P, P,
                                   C.c_int64, P, P, P, P]),
     "gk_spai_nparams": (C.c_int, []),

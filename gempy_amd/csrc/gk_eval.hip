@@ -1026,3 +1026,192 @@ extern "C" int gk_eval_noalloc(gk_handle h, const gk_system* sys, const double* 
   return eval_impl(h, sys, weights, desc, (hipStream_t)stream, false, nullptr, false, h->eval_scratch,
                    GK_EVAL_SCRATCH_BYTES);
 }
+
+// ------------------------------------------------------------ Hessians
+// Second derivatives of a field at explicit points (FP64), for the adjoint
+// with respect to the orientation positions.  In the frame X = (x - c)/a with
+// unit polynomials (c_o folded into the source weights, as in the evaluation),
+// d1 = c'/rho, d2 = (c'' - c'/rho)/rho^2 = 105/4 (1 - rho^2)^2 / rho and
+// d3 = d2'/rho = -105/4 (1 - rho^2)(1 + 3 rho^2) / rho^3, H = X - source:
+//   point source W:        W (delta d1 + H H^T d2)
+//   orientation source V:  -(d2 (delta V.H + V H^T + H V^T) + H H^T (V.H) d3)
+// (the orientation term at rho = 0 is odd and contributes 0).  Output per
+// point: xx, yy, zz, xy, xz, yz of the Hessian in real coordinates.
+namespace {
+struct HessArgs {
+  const V4<double>* pts;
+  int64_t n_pts, pts_chunk;
+  const V8<double>* ori;
+  int64_t n_ori, ori_chunk;
+  const double* xyz;
+  int64_t n;
+  double cx, cy, cz, inv_a;
+  int32_t uni_degree;
+  double gi_res;
+  const double* lam;          // drift weights (degree 2 adds a constant Hessian)
+  int32_t nsplit;
+  double* part;               // [nsplit][n][6]
+};
+
+__global__ __launch_bounds__(256) void hessian_kernel(HessArgs a) {
+  __shared__ V4<double> tp[256];
+  __shared__ V8<double> to[256];
+  const int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const bool live = p < a.n;
+  double X = 0.0, Y = 0.0, Z = 0.0;
+  if (live) {
+    X = (a.xyz[3 * p] - a.cx) * a.inv_a;
+    Y = (a.xyz[3 * p + 1] - a.cy) * a.inv_a;
+    Z = (a.xyz[3 * p + 2] - a.cz) * a.inv_a;
+  }
+  double h[6] = {0, 0, 0, 0, 0, 0};
+  const int64_t p0 = (int64_t)blockIdx.y * a.pts_chunk;
+  const int64_t p1 = p0 + a.pts_chunk < a.n_pts ? p0 + a.pts_chunk : a.n_pts;
+  for (int64_t t0 = p0; t0 < p1; t0 += 256) {
+    __syncthreads();
+    if (t0 + threadIdx.x < p1) tp[threadIdx.x] = a.pts[t0 + threadIdx.x];
+    __syncthreads();
+    const int cnt = p1 - t0 < 256 ? (int)(p1 - t0) : 256;
+    if (!live) continue;
+    for (int k = 0; k < cnt; ++k) {
+      const V4<double> sv = tp[k];
+      const double hx = X - sv.x, hy = Y - sv.y, hz = Z - sv.z;
+      const double r2 = hx * hx + hy * hy + hz * hz;
+      if (r2 >= 1.0) continue;
+      const double rho = sqrt(r2);
+      const double d1 = -14.0 + rho * (26.25 + r2 * (-17.5 + 5.25 * r2));
+      const double om = 1.0 - r2;
+      const double d2 = rho > 0.0 ? 26.25 * om * om / rho : 0.0;
+      const double w = sv.w, wd2 = w * d2;
+      h[0] += w * d1 + wd2 * hx * hx;
+      h[1] += w * d1 + wd2 * hy * hy;
+      h[2] += w * d1 + wd2 * hz * hz;
+      h[3] += wd2 * hx * hy;
+      h[4] += wd2 * hx * hz;
+      h[5] += wd2 * hy * hz;
+    }
+  }
+  const int64_t q0 = (int64_t)blockIdx.y * a.ori_chunk;
+  const int64_t q1 = q0 + a.ori_chunk < a.n_ori ? q0 + a.ori_chunk : a.n_ori;
+  for (int64_t t0 = q0; t0 < q1; t0 += 256) {
+    __syncthreads();
+    if (t0 + threadIdx.x < q1) to[threadIdx.x] = a.ori[t0 + threadIdx.x];
+    __syncthreads();
+    const int cnt = q1 - t0 < 256 ? (int)(q1 - t0) : 256;
+    if (!live) continue;
+    for (int k = 0; k < cnt; ++k) {
+      const V8<double> sv = to[k];
+      const double hx = X - sv.x, hy = Y - sv.y, hz = Z - sv.z;
+      const double r2 = hx * hx + hy * hy + hz * hz;
+      if (r2 >= 1.0 || r2 == 0.0) continue;
+      const double rho = sqrt(r2);
+      const double om = 1.0 - r2;
+      const double d2 = 26.25 * om * om / rho;
+      const double d3 = -26.25 * om * (1.0 + 3.0 * r2) / (rho * r2);
+      const double vx = sv.wx, vy = sv.wy, vz = sv.wz;
+      const double vh = vx * hx + vy * hy + vz * hz;
+      const double t = vh * d3;
+      h[0] -= d2 * (vh + 2.0 * vx * hx) + hx * hx * t;
+      h[1] -= d2 * (vh + 2.0 * vy * hy) + hy * hy * t;
+      h[2] -= d2 * (vh + 2.0 * vz * hz) + hz * hz * t;
+      h[3] -= d2 * (vx * hy + vy * hx) + hx * hy * t;
+      h[4] -= d2 * (vx * hz + vz * hx) + hx * hz * t;
+      h[5] -= d2 * (vy * hz + vz * hy) + hy * hz * t;
+    }
+  }
+  if (live) {
+    double* o = a.part + ((int64_t)blockIdx.y * a.n + p) * 6;
+#pragma unroll
+    for (int c = 0; c < 6; ++c) o[c] = h[c];
+  }
+}
+
+__global__ void hessian_finalize_kernel(HessArgs a, double* out) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= a.n) return;
+  double h[6] = {0, 0, 0, 0, 0, 0};
+  for (int y = 0; y < a.nsplit; ++y)
+#pragma unroll
+    for (int c = 0; c < 6; ++c) h[c] += a.part[((int64_t)y * a.n + p) * 6 + c];
+  const double s = a.inv_a * a.inv_a;
+#pragma unroll
+  for (int c = 0; c < 6; ++c) h[c] *= s;
+  if (a.uni_degree >= 2 && a.lam) {       // x^2, y^2, z^2, xy, xz, yz drift terms
+    const double* l = a.lam;
+    h[0] += a.gi_res * 2.0 * l[3];
+    h[1] += a.gi_res * 2.0 * l[4];
+    h[2] += a.gi_res * 2.0 * l[5];
+    h[3] += a.gi_res * l[6];
+    h[4] += a.gi_res * l[7];
+    h[5] += a.gi_res * l[8];
+  }
+#pragma unroll
+  for (int c = 0; c < 6; ++c) out[p * 6 + c] = h[c];
+}
+
+int run_hessian(HessArgs& a, double* out, size_t part_cap, hipStream_t st) {
+  const int64_t nsrc = a.n_pts + a.n_ori;
+  int64_t ns = std::min<int64_t>(64, std::max<int64_t>(1, gk_ceil_div(nsrc, 2048)));
+  ns = std::max<int64_t>(1, std::min<int64_t>(ns, (int64_t)(part_cap / ((size_t)std::max<int64_t>(a.n, 1) * 48))));
+  a.nsplit = (int32_t)ns;
+  a.pts_chunk = std::max<int64_t>(gk_ceil_div(a.n_pts, ns), 1);
+  a.ori_chunk = std::max<int64_t>(gk_ceil_div(a.n_ori, ns), 1);
+  if (a.n == 0) return GK_OK;
+  hipLaunchKernelGGL(hessian_kernel, dim3((unsigned)gk_ceil_div(a.n, 256), (unsigned)ns), dim3(256), 0, st, a);
+  hipLaunchKernelGGL(hessian_finalize_kernel, dim3((unsigned)gk_ceil_div(a.n, 256)), dim3(256), 0, st, a, out);
+  GK_HIP(hipGetLastError());
+  return GK_OK;
+}
+}  // namespace
+
+// Hessian of the system's field for `weights` (covariance and drift terms; the
+// fault block is piecewise constant) at n explicit points.  part: scratch of
+// part_cap bytes (>= 48 n) for the source-split partials.
+int gk_hessian_system(gk_handle h, const gk_system* sys, const double* weights, int64_t n, const double* xyz,
+                      double* hess6, double* part, size_t part_cap, hipStream_t st) {
+  GK_TRY(check_system(sys));
+  GK_REQUIRE(part_cap >= (size_t)std::max<int64_t>(n, 1) * 48, GK_ERR_ARG, "hessian: partial buffer too small");
+  void* ws = nullptr;
+  GK_TRY(gk_workspace(h, WS_EVAL_SRC, sources_bytes(sys), &ws));
+  const void *pts = nullptr, *ori = nullptr;
+  GK_TRY(prepare_sources(h, sys, weights, GK_F64, st, &pts, &ori, ws, true));
+  HessArgs a{};
+  a.pts = static_cast<const V4<double>*>(pts);
+  a.n_pts = sys->n_rest + sys->n_surf;
+  a.ori = static_cast<const V8<double>*>(ori);
+  a.n_ori = sys->n_ori;
+  a.xyz = xyz;
+  a.n = n;
+  a.cx = sys->center[0]; a.cy = sys->center[1]; a.cz = sys->center[2];
+  a.inv_a = 1.0 / sys->range;
+  a.uni_degree = sys->uni_degree;
+  a.gi_res = sys->gi_res;
+  a.lam = weights + 3 * sys->n_ori + sys->n_rest;
+  a.part = part;
+  return run_hessian(a, hess6, part_cap, st);
+}
+
+// Hessian of F(x) = sum_s w_s C(|x - s|) (explicit sources) at n points.
+int gk_hessian_sources(gk_handle h, double range, double c_o, const double* center, int64_t n_src,
+                       const double* src_xyz, const double* src_w, int64_t n, const double* xyz, double* hess6,
+                       double* part, size_t part_cap, hipStream_t st) {
+  GK_REQUIRE(part_cap >= (size_t)std::max<int64_t>(n, 1) * 48, GK_ERR_ARG, "hessian: partial buffer too small");
+  void* ws = nullptr;
+  GK_TRY(gk_workspace(h, WS_EVAL_SRC, (size_t)std::max<int64_t>(n_src, 1) * 32 + 256, &ws));
+  V4<double>* src = static_cast<V4<double>*>(ws);
+  if (n_src > 0)
+    hipLaunchKernelGGL(prep_raw_kernel, dim3((unsigned)gk_ceil_div(n_src, 256)), dim3(256), 0, st, n_src, src_xyz,
+                       src_w, center[0], center[1], center[2], 1.0 / range, c_o, src);
+  HessArgs a{};
+  a.pts = src;
+  a.n_pts = n_src;
+  a.ori = nullptr;
+  a.n_ori = 0;
+  a.xyz = xyz;
+  a.n = n;
+  a.cx = center[0]; a.cy = center[1]; a.cz = center[2];
+  a.inv_a = 1.0 / range;
+  a.uni_degree = 0;
+  a.part = part;
+  return run_hessian(a, hess6, part_cap, st);
+}

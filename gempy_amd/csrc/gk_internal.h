@@ -139,3 +139,12 @@ __device__ __forceinline__ double gk_rsqrt(double t) {
   double r = fma(-t * y, y, 1.0);
   return fma(0.5 * y, r, y);
 }
+
+// Hessians (xx, yy, zz, xy, xz, yz per point, FP64) of the system's field for
+// `weights`, or of sum_s w_s C(|x - s|) over explicit sources, at n points;
+// part: scratch (>= 48 n bytes) for the source-split partial sums.
+int gk_hessian_system(gk_handle h, const gk_system* sys, const double* weights, int64_t n, const double* xyz,
+                      double* hess6, double* part, size_t part_cap, hipStream_t st);
+int gk_hessian_sources(gk_handle h, double range, double c_o, const double* center, int64_t n_src,
+                       const double* src_xyz, const double* src_w, int64_t n, const double* xyz, double* hess6,
+                       double* part, size_t part_cap, hipStream_t st);

@@ -963,6 +963,26 @@ __global__ void adjoint_points_kernel(gk_system sys, const double* w, const doub
     out[3 * a + u] = sys.i_res * Om * gE[3 * a + u] - La * gSw[3 * a + u] - Om * gSl[3 * a + u];
 }
 
+// dL/dq_g = gi_res Hess E(q_g) w_g - (Hess S_w(q_g) lam_g + Hess S_lam(q_g) w_g) / gi_res
+// (Hessians as xx, yy, zz, xy, xz, yz; w_g, lam_g the gradient weights of q_g)
+__global__ void adjoint_ori_pos_kernel(int64_t no, double gi, const double* w, const double* lam,
+                                       const double* HE, const double* HSw, const double* HSl, double* out) {
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= no) return;
+  const double wg[3] = {w[g], w[no + g], w[2 * no + g]};
+  const double lg[3] = {lam[g], lam[no + g], lam[2 * no + g]};
+  const int ix[3][3] = {{0, 3, 4}, {3, 1, 5}, {4, 5, 2}};
+  for (int v = 0; v < 3; ++v) {
+    double e = 0.0, m = 0.0;
+    for (int u = 0; u < 3; ++u) {
+      const int c = ix[v][u];
+      e += HE[6 * g + c] * wg[u];
+      m += HSw[6 * g + c] * lg[u] + HSl[6 * g + c] * wg[u];
+    }
+    out[3 * g + v] = gi * e - m / gi;
+  }
+}
+
 __global__ void adjoint_grad_kernel(int64_t no, const double* lam, double* d_grad) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= 3 * no) return;
@@ -1021,9 +1041,25 @@ extern "C" int gk_solve_rhs(gk_handle h, const gk_system* sys, const double* rhs
   return GK_OK;
 }
 
+static int adjoint_impl(gk_handle h, const gk_system* sys, const double* w, int64_t n_obs, const double* obs_xyz,
+                        const double* obs_fault, const double* e, double* d_points, double* d_grad, double* d_ori_xyz,
+                        double* lam_out, gk_stream stream);
+
 extern "C" int gk_adjoint(gk_handle h, const gk_system* sys, const double* w, int64_t n_obs, const double* obs_xyz,
                           const double* obs_fault, const double* e, double* d_points, double* d_grad, double* lam_out,
                           gk_stream stream) {
+  return adjoint_impl(h, sys, w, n_obs, obs_xyz, obs_fault, e, d_points, d_grad, nullptr, lam_out, stream);
+}
+
+extern "C" int gk_adjoint_full(gk_handle h, const gk_system* sys, const double* w, int64_t n_obs,
+                               const double* obs_xyz, const double* obs_fault, const double* e, double* d_points,
+                               double* d_grad, double* d_ori_xyz, double* lam_out, gk_stream stream) {
+  return adjoint_impl(h, sys, w, n_obs, obs_xyz, obs_fault, e, d_points, d_grad, d_ori_xyz, lam_out, stream);
+}
+
+static int adjoint_impl(gk_handle h, const gk_system* sys, const double* w, int64_t n_obs, const double* obs_xyz,
+                        const double* obs_fault, const double* e, double* d_points, double* d_grad, double* d_ori_xyz,
+                        double* lam_out, gk_stream stream) {
   GK_REQUIRE(h && sys && w && d_points && d_grad && n_obs >= 0 && (n_obs == 0 || (obs_xyz && e)), GK_ERR_ARG,
              "gk_adjoint: NULL argument");
   GK_REQUIRE(sys->n_fault == 0 || n_obs == 0 || obs_fault, GK_ERR_ARG,
@@ -1039,6 +1075,10 @@ extern "C" int gk_adjoint(gk_handle h, const gk_system* sys, const double* w, in
   const size_t o_pts = take(3 * NP), o_E = take(NP), o_gE = take(3 * NP), o_c = take(N), o_lam = take(N),
                o_gSw = take(3 * (R + Sf)), o_gSl = take(3 * (R + Sf)), o_Ww = take(Sf + 1), o_Wl = take(Sf + 1),
                o_fz = take((R + Sf) * std::max(sys->n_fault, 1));
+  // orientation positions: three Hessian sets and the source-split partials
+  const int64_t nh = d_ori_xyz ? no : 0;
+  const size_t part_doubles = (size_t)64 * 6 * std::max<int64_t>(nh, 1);
+  const size_t o_H = take(18 * std::max<int64_t>(nh, 1)), o_hp = take(d_ori_xyz ? part_doubles : 1);
   void* p = nullptr;
   GK_TRY(gk_workspace(h, WS_ADJOINT, need, &p));
   char* base = static_cast<char*>(p);
@@ -1086,6 +1126,18 @@ extern "C" int gk_adjoint(gk_handle h, const gk_system* sys, const double* w, in
                        gE + 3 * no, gSw, gSl, d_points);
   if (no > 0)
     hipLaunchKernelGGL(adjoint_grad_kernel, dim3(nblk(3 * no)), dim3(256), 0, st, no, lam, d_grad);
+  // 5. orientation positions: Hessians of E, S_w and S_lam at the orientations
+  if (d_ori_xyz && no > 0) {
+    double* H = (double*)(base + o_H);
+    double* hp = (double*)(base + o_hp);
+    const size_t cap = part_doubles * sizeof(double);
+    GK_TRY(gk_hessian_sources(h, sys->range, sys->c_o, sys->center, n_obs, obs_xyz, e, no, sys->ori_xyz, H, hp, cap,
+                              st));
+    GK_TRY(gk_hessian_system(h, sys, w, no, sys->ori_xyz, H + 6 * no, hp, cap, st));
+    GK_TRY(gk_hessian_system(h, sys, lam, no, sys->ori_xyz, H + 12 * no, hp, cap, st));
+    hipLaunchKernelGGL(adjoint_ori_pos_kernel, dim3(nblk(no)), dim3(256), 0, st, no, sys->gi_res, w, lam, H,
+                       H + 6 * no, H + 12 * no, d_ori_xyz);
+  }
   if (lam_out) GK_HIP(hipMemcpyAsync(lam_out, lam, N * sizeof(double), hipMemcpyDeviceToDevice, st));
   GK_HIP(hipGetLastError());
   return GK_OK;

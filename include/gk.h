@@ -217,6 +217,15 @@ int gk_adjoint(gk_handle h, const gk_system* sys, const double* w, int64_t n_obs
                const double* obs_fault, const double* e, double* d_points, double* d_grad, double* lam_out,
                gk_stream stream);
 
+/* gk_adjoint plus dL/d(orientation positions), d_ori_xyz [n_ori*3] (nullable;
+ * the reference's AD target dip_positions, test/test_modules/test_ad/
+ * test_ad_I.py:48-50): gi_res Hess E(q) w_q - (Hess S_w(q) lam_q +
+ * Hess S_lam(q) w_q) / gi_res with E the observation field, S_v the field of
+ * weights v and w_q, lam_q the orientation's gradient weights. */
+int gk_adjoint_full(gk_handle h, const gk_system* sys, const double* w, int64_t n_obs, const double* obs_xyz,
+                    const double* obs_fault, const double* e, double* d_points, double* d_grad, double* d_ori_xyz,
+                    double* lam_out, gk_stream stream);
+
 /* ---- (c) evaluation: scalar field, gradient, ids ------------------------ */
 int gk_eval(gk_handle h, const gk_system* sys, const double* weights, gk_eval_desc* desc,
             gk_stream stream);

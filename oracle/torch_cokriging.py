@@ -2,8 +2,9 @@
 
 It states the same system and evaluation as oracle/cokriging.py (whose
 conventions are pinned there) with torch operations, so that torch.autograd
-gives the gradient of a misfit w.r.t. the surface-point coordinates and the
-orientation gradients: the reference obtains such gradients by taping its
+gives the gradient of a misfit w.r.t. the surface-point coordinates, the
+orientation positions and the orientation gradients (fault-drift columns
+included, their block values held fixed): the reference obtains such gradients by taping its
 PyTorch backend (gempy/API/compute_API.py:49,66; test_ad_I.py:41-51), which is
 not available here.  Used only to check the hand-written adjoint.
 """
@@ -33,7 +34,9 @@ def _dd(rho, a, c_o):   # (C'' - C'/r) / r^2 * r^2 handled by the caller through
     return torch.where(rho < 1, v, torch.zeros_like(v))
 
 
-def build_and_solve(sp, counts, sp_nug, q, G, q_nug, a, c_o, i_res=4.0, gi_res=2.0):
+def build_and_solve(sp, counts, sp_nug, q, G, q_nug, a, c_o, i_res=4.0, gi_res=2.0, fault_sp=None):
+    """fault_sp (n_sp, F), optional: fault-block values at the surface points
+    (piecewise constant: constants of the differentiation)."""
     counts = [int(c) for c in counts]
     starts = [0]
     for c in counts[:-1]:
@@ -46,8 +49,7 @@ def build_and_solve(sp, counts, sp_nug, q, G, q_nug, a, c_o, i_res=4.0, gi_res=2
     refp = ref[rs]
     n_o, R = q.shape[0], rest.shape[0]
     NG = 3 * n_o
-    N = NG + R + 3
-    blocks = [[None] * 3 for _ in range(3)]
+    F = 0 if fault_sp is None else fault_sp.shape[1]
     # GG
     h = q[:, None, :] - q[None, :, :]
     rho = _rho(h, a)
@@ -76,25 +78,35 @@ def build_and_solve(sp, counts, sp_nug, q, G, q_nug, a, c_o, i_res=4.0, gi_res=2
     II = II + torch.diag(sp_nug[rest_idx] + sp_nug[ref_idx][rs])
     UG = torch.cat([torch.eye(3).repeat(n_o, 1).reshape(n_o, 3, 3)[:, :, u] for u in range(3)], 0)
     UI = gi_res * (rest - refp)
+    if F:
+        fs = torch.as_tensor(fault_sp, dtype=GG.dtype)
+        UG = torch.cat([UG.to(GG), torch.zeros(NG, F, dtype=GG.dtype)], 1)
+        UI = torch.cat([UI, gi_res * (fs[rest_idx] - fs[ref_idx][rs])], 1)
+    Lt = 3 + F
     top = torch.cat([GG, GI, UG.to(GG)], 1)
     mid = torch.cat([GI.T, II, UI], 1)
-    bot = torch.cat([UG.T.to(GG), UI.T, torch.zeros(3, 3, dtype=GG.dtype)], 1)
+    bot = torch.cat([UG.T.to(GG), UI.T, torch.zeros(Lt, Lt, dtype=GG.dtype)], 1)
     A = torch.cat([top, mid, bot], 0)
-    b = torch.cat([G.T.reshape(-1), torch.zeros(R + 3, dtype=G.dtype)])
+    b = torch.cat([G.T.reshape(-1), torch.zeros(R + Lt, dtype=G.dtype)])
     w = torch.linalg.solve(A, b)
     return w, rest, ref, rs
 
 
-def evaluate(x, w, rest, ref, rs, q, a, c_o, i_res=4.0, gi_res=2.0):
+def evaluate(x, w, rest, ref, rs, q, a, c_o, i_res=4.0, gi_res=2.0, fault_x=None):
+    """fault_x (P, F), optional: fault-block values at x."""
     n_o, R = q.shape[0], rest.shape[0]
     NG = 3 * n_o
     wG = w[:NG].reshape(3, n_o).T
     wI = w[NG:NG + R]
     lam = w[NG + R:NG + R + 3]
+    lamF = w[NG + R + 3:]
     W = torch.zeros(ref.shape[0], dtype=w.dtype).index_add(0, rs, wI)
     h = x[:, None, :] - q[None, :, :]
     d1 = _dcov_r(_rho(h, a), a, c_o)
     s = gi_res * ((-h * d1[..., None]) * wG[None]).sum((1, 2))
     s = s + i_res * (_cov(_rho(x[:, None, :] - rest[None], a), c_o) @ wI)
     s = s - i_res * (_cov(_rho(x[:, None, :] - ref[None], a), c_o) @ W)
-    return s + gi_res * (x @ lam)
+    s = s + gi_res * (x @ lam)
+    if fault_x is not None and lamF.numel():
+        s = s + gi_res * (torch.as_tensor(fault_x, dtype=w.dtype) @ lamF)
+    return s

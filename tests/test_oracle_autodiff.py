@@ -45,3 +45,50 @@ def test_torch_oracle_gradient_matches_fd():
         Gp[g, u] += h; Gm[g, u] -= h
         fd = (_loss_np(st, st["sp_xyz"], Gp, x, target) - _loss_np(st, st["sp_xyz"], Gm, x, target)) / (2 * h)
         assert abs(G.grad[g, u].item() - fd) <= 1e-5 * max(1.0, abs(fd))
+
+
+def _loss_np_f(st, sp, q, G, x, target, Fsp, Fx):
+    A, b = ok.build_system(sp, st["counts"], st["sp_nugget"], q, G, st["ori_nugget"], A_, C_, fault_sp=Fsp)
+    w = np.linalg.solve(A, b)
+    S = ok.evaluate(x, sp, st["counts"], q, w, A_, C_, fault_x=Fx)
+    return float(((S - target) ** 2).sum())
+
+
+def test_torch_oracle_fault_columns_and_orientation_positions_match_fd():
+    """With a fault-drift column (block values fixed): the torch restatement's
+    loss equals the NumPy oracle's, and its autograd gradients w.r.t. the
+    surface points, the orientation positions and the orientation gradients
+    equal central differences of the NumPy oracle."""
+    st = _rand_stack(12)
+    rng = np.random.default_rng(3)
+    x = rng.uniform(0, 1000, (60, 3))
+    target = rng.normal(size=60) * 50 + 800
+    hang = lambda p: ((np.asarray(p) - 500.0) @ np.array([0.8, 0.0, 0.6])) > 0   # noqa: E731
+    Fsp = np.where(hang(st["sp_xyz"]), 1.0, 2.0)[:, None]
+    Fx = np.where(hang(x), 1.0, 2.0)[:, None]
+    sp = _t(st["sp_xyz"]).requires_grad_(True)
+    q = _t(st["ori_xyz"]).requires_grad_(True)
+    G = _t(st["ori_grad"]).requires_grad_(True)
+    w, rest, ref, rs = tk.build_and_solve(sp, st["counts"], _t(st["sp_nugget"]), q, G, _t(st["ori_nugget"]),
+                                          A_, C_, fault_sp=_t(Fsp))
+    S = tk.evaluate(_t(x), w, rest, ref, rs, q, A_, C_, fault_x=_t(Fx))
+    loss = ((S - _t(target)) ** 2).sum()
+    loss.backward()
+    L0 = _loss_np_f(st, st["sp_xyz"], st["ori_xyz"], st["ori_grad"], x, target, Fsp, Fx)
+    assert abs(float(loss) - L0) <= 1e-9 * L0
+    assert abs(float(w[-1])) > 1e-6                     # the fault column is used
+    def fd(which, i, u, h):
+        arrs = {"sp": st["sp_xyz"], "q": st["ori_xyz"], "G": st["ori_grad"]}
+        p, m = {k: v.copy() for k, v in arrs.items()}, {k: v.copy() for k, v in arrs.items()}
+        p[which][i, u] += h
+        m[which][i, u] -= h
+        return (_loss_np_f(st, p["sp"], p["q"], p["G"], x, target, Fsp, Fx)
+                - _loss_np_f(st, m["sp"], m["q"], m["G"], x, target, Fsp, Fx)) / (2 * h)
+    for (i, u) in [(0, 2), (3, 0), (len(st["sp_xyz"]) - 1, 1)]:
+        f = fd("sp", i, u, 1e-3)
+        assert abs(sp.grad[i, u].item() - f) <= 1e-5 * max(1.0, abs(f))
+    for (g, u) in [(0, 0), (2, 2), (4, 1)]:
+        f = fd("q", g, u, 1e-3)
+        assert abs(q.grad[g, u].item() - f) <= 1e-5 * max(1.0, abs(f)), (q.grad[g, u].item(), f)
+        f = fd("G", g, u, 1e-5)
+        assert abs(G.grad[g, u].item() - f) <= 1e-5 * max(1.0, abs(f))
