@@ -17,6 +17,7 @@ where the reference tree is readable; the fixtures travel, the reference does no
   data for the GPU parity tests; pinned by the tests above, not independent).
 * transforms.npz -- raw input points and the recorded input_transform
   (position, scale) of the 2-layers, anticline and 2.5-D (Model1) models.
+* greenstone/ -- the Greenstone model's .gempy file and its legacy v2 tables.
 """
 from __future__ import annotations
 
@@ -140,9 +141,25 @@ def transforms():
     np.savez(os.path.join(HERE, "transforms.npz"), **out)
 
 
+def greenstone():
+    """The reference's Greenstone model as data: the .gempy file
+    (examples/data/gempy_models/Greenstone.gempy) and the same model's legacy
+    v2 surface-point / orientation tables (examples/data/gempy_models/
+    Greenstone/Greenstone_{surface_points,orientations}.csv)."""
+    import shutil
+    dst = os.path.join(HERE, "greenstone")
+    os.makedirs(dst, exist_ok=True)
+    src = os.path.join(REF, "examples/data/gempy_models")
+    shutil.copyfile(os.path.join(src, "Greenstone.gempy"), os.path.join(dst, "Greenstone.gempy"))
+    for name in ("surface_points", "orientations"):
+        shutil.copyfile(os.path.join(src, "Greenstone", f"Greenstone_{name}.csv"),
+                        os.path.join(dst, f"Greenstone_{name}.csv"))
+
+
 if __name__ == "__main__":
     two_layers()
     anticline()
     config1_oracle()
     transforms()
+    greenstone()
     print("fixtures written to", HERE)
