@@ -47,6 +47,7 @@ enum gk_ws_slot {
   WS_SMALL,          // small scratch (bounding boxes, counts)
   WS_TRSV,           // work tables, readiness flags and partial sums of the triangular solves
   WS_ASSEMBLE,       // reference-point tables of the assembly
+  WS_MESH,           // case tables, edge flags and scans of the isosurface extraction
   WS_COUNT
 };
 struct gk_context {
@@ -67,6 +68,8 @@ struct gk_context {
   hipEvent_t aux_ev[2] = {};
   // device address of gk_eval_noalloc's file-scope scratch array (this device's copy)
   char* eval_scratch = nullptr;
+  // where the marching-cubes case tables were last uploaded (inside WS_MESH)
+  void* mesh_tables = nullptr;
 };
 
 // Takes the device address of the evaluation scratch array for h's device

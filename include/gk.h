@@ -274,6 +274,21 @@ int gk_combine_stacks(gk_handle h, int32_t n_stacks, const int32_t* ids, int64_t
 int gk_block_values(gk_handle h, const double* scalar, int64_t n, int32_t n_thresholds, const double* thresholds,
                     double sigmoid_slope, double* block, gk_stream stream);
 
+/* ---- meshes: isosurface of a lattice field (marching cubes; the reference
+ * extracts one mesh per surface at its threshold with skimage's marching cubes,
+ * gempy/modules/mesh_extranction/marching_cubes.py:59-101) --------------------
+ * field [res0*res1*res2] (z fastest), mask (nullable, same layout: a cube is
+ * meshed when its first corner is set), vertices at origin + (index + t e_axis)
+ * * spacing on the lattice edges that cross `level` (one per edge, shared),
+ * faces as vertex-index triples wound so that normals point towards
+ * increasing values; the mesh is watertight away from the lattice boundary and
+ * the mask.  With verts or faces NULL only *n_verts / *n_faces are returned;
+ * otherwise the outputs must hold max_verts / max_faces entries.  Synchronises
+ * the stream. */
+int gk_marching_cubes(gk_handle h, const double* field, const int64_t* res, const uint8_t* mask, double level,
+                      const double* origin, const double* spacing, double* verts, int64_t max_verts, int32_t* faces,
+                      int64_t max_faces, int64_t* n_verts, int64_t* n_faces, gk_stream stream);
+
 /* ---- octree evaluation (the engine's octree levels: _number_octree_levels,
  * fault.approved.txt:210-214; resolution = base * 2^(levels-1),
  * gempy/core/data/grid_modules/regular_grid.py:36-43) ------------------------

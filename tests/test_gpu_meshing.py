@@ -1,0 +1,115 @@
+"""Marching cubes on the GPU (gk_marching_cubes) against the oracle's
+crossing-edge vertex set and the closed forms of level sets: a sphere
+(watertight, Euler characteristic 2, area 4 pi r^2, volume 4/3 pi r^3,
+normals towards increasing values), a torus (Euler characteristic 0), a
+plane (vertices on it), a masked sphere, and the surfaces of a multi-stack
+model (per surface at its threshold within the stack's mask)."""
+import sys
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import meshing as om
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(f"[gpu-case] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+def _grid(n, lo=-1.0, hi=1.0):
+    from gempy_amd.data import RegularGrid
+    return RegularGrid((lo, hi) * 3, (n, n + 1, n + 2))
+
+
+def _same_points(A, B, tol):
+    from scipy.spatial import cKDTree
+    assert A.shape == B.shape, (A.shape, B.shape)
+    d, _ = cKDTree(B).query(A)
+    return float(d.max()) <= tol
+
+
+def _field(grid, fn):
+    X = grid.values()
+    return X, torch.tensor(fn(X), dtype=torch.float64, device="cuda")
+
+
+def test_sphere_closed_forms(gpu):
+    from gempy_amd.meshing import marching_cubes
+    g = _grid(48)
+    r0 = 0.63
+    X, f = _field(g, lambda X: (X * X).sum(1))
+    V, F = marching_cubes(f, g, r0 * r0)
+    c = om.mesh_checks(V, F)
+    Vo = om.crossing_vertices(f.cpu().numpy().reshape(g.resolution), r0 * r0, g.origin, g.spacing)
+    say(f"sphere: {len(V)} vertices, {len(F)} faces, euler {c['euler']}, unpaired {c['unpaired']}, "
+        f"area {c['area']:.5f} ({4 * np.pi * r0 ** 2:.5f}), volume {c['volume']:.5f} ({4 / 3 * np.pi * r0 ** 3:.5f})")
+    assert _same_points(V, Vo, 1e-12)
+    assert c["unpaired"] == 0 and c["euler"] == 2
+    assert abs(c["area"] / (4 * np.pi * r0 ** 2) - 1) < 5e-3
+    assert abs(c["volume"] / (4 / 3 * np.pi * r0 ** 3) - 1) < 5e-3
+    assert np.all((c["normals"] * c["centroids"]).sum(1) > 0)       # outward = increasing r^2
+    np.testing.assert_allclose(np.linalg.norm(V, axis=1), r0, rtol=0, atol=2e-3)
+
+
+def test_torus_and_plane(gpu):
+    from gempy_amd.meshing import marching_cubes
+    g = _grid(56)
+    R, r = 0.55, 0.22
+    X, f = _field(g, lambda X: (np.sqrt(X[:, 0] ** 2 + X[:, 1] ** 2) - R) ** 2 + X[:, 2] ** 2)
+    V, F = marching_cubes(f, g, r * r)
+    c = om.mesh_checks(V, F)
+    say(f"torus: {len(V)} vertices, euler {c['euler']}, unpaired {c['unpaired']}, "
+        f"area {c['area']:.4f} ({4 * np.pi ** 2 * R * r:.4f})")
+    assert c["unpaired"] == 0 and c["euler"] == 0
+    assert abs(c["area"] / (4 * np.pi ** 2 * R * r) - 1) < 1e-2
+    n = np.array([0.3, -0.5, 0.81])
+    X, f = _field(g, lambda X: X @ n)
+    V, F = marching_cubes(f, g, 0.1)
+    np.testing.assert_allclose(V @ n, 0.1, rtol=0, atol=1e-12)
+    c = om.mesh_checks(V, F)
+    nrm = c["normals"] / np.linalg.norm(c["normals"], axis=1, keepdims=True)
+    np.testing.assert_allclose(nrm @ (n / np.linalg.norm(n)), 1.0, rtol=0, atol=1e-9)
+    Vo = om.crossing_vertices(f.cpu().numpy().reshape(g.resolution), 0.1, g.origin, g.spacing)
+    assert _same_points(V, Vo, 1e-12)
+
+
+def test_masked_sphere(gpu):
+    from gempy_amd.meshing import marching_cubes
+    g = _grid(40)
+    X, f = _field(g, lambda X: (X * X).sum(1))
+    mask = torch.tensor(X[:, 0] > 0.1, device="cuda")
+    V, F = marching_cubes(f, g, 0.4, mask)
+    Vo = om.crossing_vertices(f.cpu().numpy().reshape(g.resolution), 0.4, g.origin, g.spacing,
+                              mask.cpu().numpy().reshape(g.resolution))
+    say(f"masked sphere: {len(V)} vertices (oracle {len(Vo)}), {len(F)} faces")
+    assert _same_points(V, Vo, 1e-12)
+    assert F.max() < len(V) and F.min() >= 0
+
+
+def test_model_surface_meshes_match_oracle(gpu):
+    """Every surface of config 2 (scale 0.05, 40^3): vertex sets against the
+    oracle's crossing edges of the oracle's field within the stack's mask."""
+    say("model meshes: config2 scale 0.05, 40^3")
+    from gempy_amd import GemPyEngineConfig, compute_model, synthetic
+    from gempy_amd.meshing import compute_meshes
+    from oracle import cokriging as ok
+    m = synthetic.config2(scale=0.05, resolution=(40, 40, 40), octree_levels=1)
+    sol = compute_model(m, GemPyEngineConfig(dtype="float64"))
+    meshes = compute_meshes(sol, m.grid, reference_origin=False)
+    X = m.grid.values()
+    res = ok.compute_model([s.as_oracle_dict() for s in m.stacks], m.options.as_dict(), X)
+    shape = m.grid.resolution
+    total = 0
+    for k, (ms, r) in enumerate(zip(meshes, res["stacks"])):
+        mask = res["mask_matrix"][k].reshape(shape)
+        for t, mesh in zip(r["thresholds"], ms):
+            Vo = om.crossing_vertices(r["scalar"].reshape(shape), t, m.grid.origin, m.grid.spacing, mask)
+            assert _same_points(mesh.vertices, Vo, 1e-6 * max(m.grid.spacing))
+            total += len(Vo)
+    say(f"model meshes: {sum(len(x) for x in meshes)} surfaces, {total} vertices, "
+        f"dc_meshes {len(sol.dc_meshes)}, raw vertices {len(sol.raw_arrays.vertices)}")
+    assert total > 0 and len(sol.dc_meshes) == sum(len(s.counts) for s in m.stacks)
