@@ -114,6 +114,23 @@ class RegularGrid:
         e, dx = self.extent, self.spacing
         return tuple(e[2 * d] + dx[d] / 2 for d in range(3))
 
+    @classmethod
+    def octree(cls, extent: Sequence[float], octree_levels: int, base_resolution: Optional[Sequence[int]] = None,
+               legacy: bool = False) -> "RegularGrid":
+        """Finest lattice of an octree: base_resolution * 2^(levels-1), the
+        base resolution defaulting to round(length / min length) * 2 per axis
+        ([2, 2, 2] with legacy) (gempy/core/data/grid.py:128-145,
+        grid_modules/regular_grid.py:36-43)."""
+        e = np.asarray(extent, dtype=np.float64).reshape(6)
+        if legacy:
+            base = np.array([2, 2, 2])
+        elif base_resolution is None:
+            lengths = np.array([e[1] - e[0], e[3] - e[2], e[5] - e[4]])
+            base = np.round(lengths / lengths.min()).astype(int) * 2
+        else:
+            base = np.asarray(base_resolution, dtype=int)
+        return cls(e, base * 2 ** (int(octree_levels) - 1))
+
     def values(self) -> np.ndarray:
         o, dx, r = self.origin, self.spacing, self.resolution
         axes = [o[d] + dx[d] * np.arange(r[d]) for d in range(3)]

@@ -50,6 +50,7 @@ __all__ = [
     "cov", "dcov_r", "d2cov", "split_ref_rest", "drift_functions",
     "build_system", "solve_system", "evaluate", "surface_thresholds",
     "segment", "grid_points", "compute_stack", "compute_model", "fault_columns_used",
+    "octree_base_resolution", "octree_levels",
 ]
 
 
@@ -448,3 +449,52 @@ def compute_model(stacks, opts, x, gradient=False):
     mask_matrix = [masks.get(k, np.ones(x.shape[0], dtype=bool)) for k in range(len(stacks))]
     return dict(stacks=out, lith_block=lith, fault_blocks=faults, basement_id=next_id,
                 mask_matrix=mask_matrix)
+
+
+# --------------------------------------------------------------------------
+# octree levels (the refinement the GPU path implements; the engine's own
+# criterion -- octree_error_threshold, fault.approved.txt:210-214 -- is not in
+# the reference tree: this restates the library's rule, refine a voxel when any
+# id field differs among its centre and 8 corners)
+# --------------------------------------------------------------------------
+def octree_base_resolution(extent):
+    """round(length / min length) * 2 per axis (gempy/core/data/grid.py:128-145)."""
+    e = np.asarray(extent, dtype=np.float64)
+    lengths = np.array([e[1] - e[0], e[3] - e[2], e[5] - e[4]])
+    return (np.round(lengths / lengths.min()).astype(int) * 2)
+
+
+def octree_levels(fields_at, extent, base_resolution, levels):
+    """Level by level from the base lattice: probe every active voxel at its
+    centre and 8 corners (corner c at offsets ((c>>2)&1, (c>>1)&1, c&1)),
+    refine when any field differs among the 9 probes, the 8 children of each
+    refined voxel (same corner order, resolution doubled) in voxel order form
+    the next level; the last level is probed at the centres only.
+    fields_at(points (n, 3)) -> int array (n_fields, n).  Returns per level a
+    dict(resolution, voxels, ids (fields at the centres), refine)."""
+    e = np.asarray(extent, dtype=np.float64)
+    res = np.asarray(base_resolution, dtype=np.int64)
+    vox = np.arange(int(np.prod(res)), dtype=np.int64)
+    out = []
+    for lvl in range(levels):
+        last = lvl == levels - 1
+        h = np.array([(e[1] - e[0]) / res[0], (e[3] - e[2]) / res[1], (e[5] - e[4]) / res[2]])
+        ix, iy, iz = vox // (res[1] * res[2]), (vox // res[2]) % res[1], vox % res[2]
+        base = np.stack([ix, iy, iz], 1).astype(np.float64)
+        offs = [np.array([0.5, 0.5, 0.5])] if last else \
+            [np.array([0.5, 0.5, 0.5])] + [np.array([(c >> 2) & 1, (c >> 1) & 1, c & 1], float) for c in range(8)]
+        pts = np.stack([e[[0, 2, 4]] + (base + o) * h for o in offs], 1).reshape(-1, 3)
+        f = np.asarray(fields_at(pts)).reshape(-1, len(vox), len(offs))
+        if last:
+            out.append(dict(resolution=tuple(int(r) for r in res), voxels=vox, ids=f[:, :, 0], refine=None))
+            break
+        refine = np.any(f[:, :, 1:] != f[:, :, :1], axis=(0, 2))
+        out.append(dict(resolution=tuple(int(r) for r in res), voxels=vox, ids=f[:, :, 0], refine=refine))
+        r = vox[refine]
+        rx, ry, rz = r // (res[1] * res[2]), (r // res[2]) % res[1], r % res[2]
+        nres = 2 * res
+        kids = [((2 * rx + ((c >> 2) & 1)) * nres[1] + (2 * ry + ((c >> 1) & 1))) * nres[2] + (2 * rz + (c & 1))
+                for c in range(8)]
+        vox = np.stack(kids, 1).reshape(-1)
+        res = nres
+    return out

@@ -321,3 +321,30 @@ def test_greenstone_gempy_file_matches_oracle(gpu):
             band |= np.min(np.abs(r["scalar"][:, None] - r["thresholds"][None, :]), 1) <= 1e-6 * np.ptp(r["scalar"])
         lith = sol.lith_block.cpu().numpy()[idx]
         assert int(((lith != res["lith_block"]) & ~band).sum()) == 0
+
+
+def test_octree_levels_match_oracle(gpu):
+    """compute_model's octree levels (config 2 geometry, base 6^3, 3 levels)
+    against the oracle's restatement of the refinement: the same active
+    voxels, refinement masks and ids at the voxel centres at every level."""
+    say("octree vs oracle: config2 scale 0.05, base 6^3, 3 levels")
+    from gempy_amd import GemPyEngineConfig, compute_model, synthetic
+    from gempy_amd.data import RegularGrid
+    m = synthetic.config2(scale=0.05, resolution=(24, 24, 24), octree_levels=3)
+    stacks = [s.as_oracle_dict() for s in m.stacks]
+
+    def fields_at(pts):
+        r = ok.compute_model(stacks, m.options.as_dict(), pts)
+        return np.stack([r["lith_block"]] + r["fault_blocks"])
+    ref = ok.octree_levels(fields_at, m.grid.extent, (6, 6, 6), 3)
+    sol = compute_model(m, GemPyEngineConfig(dtype="float64"))
+    octs = sol.octrees_output
+    assert len(octs) == len(ref) == 3
+    for lvl, (o, r) in enumerate(zip(octs, ref)):
+        say(f"octree level {lvl}: res {o.resolution} voxels {o.voxel_index.numel()} (oracle {len(r['voxels'])})")
+        assert o.resolution == r["resolution"]
+        np.testing.assert_array_equal(o.voxel_index.cpu().numpy(), r["voxels"])
+        np.testing.assert_array_equal(o.ids.cpu().numpy(), r["ids"][0])
+        if r["refine"] is not None:
+            np.testing.assert_array_equal(o.refine.cpu().numpy(), r["refine"])
+    assert RegularGrid.octree((0, 1000, 0, 1000, 0, 1000), 3).resolution == (8, 8, 8)

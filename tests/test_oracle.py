@@ -259,3 +259,21 @@ def test_config1_fixture_reproduces():
                            st["ori_nugget"], m.options.kernel_options.range, m.options.kernel_options.c_o)
     np.testing.assert_allclose(A, d["A"], rtol=1e-14, atol=0)
     np.testing.assert_allclose(ok.solve_system(A, b), d["weights"], rtol=1e-9)
+
+
+def test_octree_oracle_planar_closed_form():
+    """One planar interface z = c: at every level exactly the voxels whose
+    z-extent [z0, z1] straddles c (z0 <= c < z1) are refined, and the base
+    resolution follows round(length / min length) * 2 (grid.py:128-145)."""
+    np.testing.assert_array_equal(ok.octree_base_resolution([0, 2500, 0, 1000, 0, 1000]), [4, 2, 2])
+    np.testing.assert_array_equal(ok.octree_base_resolution([0, 10, 0, 30, 0, 10]), [2, 6, 2])
+    c = 0.37
+    ext = [0, 1, 0, 1, 0, 1]
+    lv = ok.octree_levels(lambda p: (p[:, 2] > c).astype(int)[None], ext, (3, 3, 3), 4)
+    for l in lv[:-1]:
+        res = np.array(l["resolution"])
+        iz = l["voxels"] % res[2]
+        z0, z1 = iz / res[2], (iz + 1) / res[2]
+        np.testing.assert_array_equal(l["refine"], (z0 <= c) & (c < z1))
+    assert lv[-1]["resolution"] == (24, 24, 24)
+    assert len(lv[-1]["voxels"]) == 24 * 24 * 2          # the two child layers of the one refined layer
