@@ -206,8 +206,9 @@ def main():
         log(f"  {out['adjoint']}")
         from gempy_amd import learned
         if os.path.exists(learned.WEIGHTS):
-            log("learned preconditioner: config-4 system (N~130k, budgeted GMRES) + 35k judge")
-            out["learned"] = workloads.learned_gmres(eval_grid=False)
+            log("learned form: config-4 system (N~130k: budgeted network GMRES, then the direct solve, 256^3 grid) "
+                "+ 35k judge")
+            out["learned"] = workloads.learned_gmres(eval_grid=True)
             log(f"  {out['learned']}")
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(model, out_fast[0].cpu().numpy(), args.cpu_seconds, N, P)

@@ -229,15 +229,18 @@ def default_network() -> SpaiNetwork:
     return _NET["net"]
 
 
-def solve_learned(ds: DeviceStack, tol: float = 1e-10, max_iter: int = 48):
+def solve_learned(ds: DeviceStack, tol: float = 1e-10, max_iter: int = 48,
+                  fallback: int = _lib.GK_SOLVE_FP64_CHOL):
     """The learned solve form: matrix-free GMRES (FP32 Krylov matvecs on the
     recomputed kernel tiles) preconditioned by the network's blocks; when it
-    stagnates above 1e-6, the shifted FP32 factor with the exact local inverse
-    and FP64 FGMRES (GK_SOLVE_FP32_REFINE, itself falling back to the FP64
-    Cholesky form) solves the system.  ``info['learned']`` records the
+    stagnates above 1e-6, the plain direct solve takes over (``fallback``:
+    the FP64 Cholesky + Schur form by default; GK_SOLVE_FP32_REFINE, the
+    shifted FP32 factor with FP64 refinement, is the faster choice up to a
+    few 10^4 unknowns, but at config-4 size its factor needs a larger shift and
+    the refinement stagnates).  ``info['learned']`` records the
     network-preconditioned attempt."""
     if ds.sys.n_ori + ds.sys.n_rest < K:             # too few points for a neighbourhood
-        return ds.solve(_lib.GK_SOLVE_FP32_REFINE)
+        return ds.solve(fallback)
     idx, y = spai_blocks(default_network(), ds)
     x, first = solve_spai(ds, idx, y, tol=tol, max_iter=max_iter, krylov_precision=_lib.GK_F32,
                           with_factor=False, no_fallback=True, sandwich=False)
@@ -245,7 +248,7 @@ def solve_learned(ds: DeviceStack, tol: float = 1e-10, max_iter: int = 48):
         first["learned"] = dict(first)
         first["fallback"] = 0
         return x, first
-    x, info = ds.solve(_lib.GK_SOLVE_FP32_REFINE)
+    x, info = ds.solve(fallback)
     info["learned"] = first
     info["fallback"] = 1
     return x, info

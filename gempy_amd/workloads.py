@@ -86,11 +86,13 @@ def learned_gmres(n_points=100_000, n_ori=10_000, judge_points=None, tol=1e-9, b
     preconditioned by the network's sparse approximate inverse alone, against
     the same GMRES without it (identity blocks), for a fixed iteration budget
     (a purely local inverse cannot remove the smooth error of this dense
-    kernel matrix, so neither reaches 1e-6 at this size).  Judge: a ~35k
-    sub-sample of the same model, solved by the FP64 direct (Cholesky) form,
-    the shifted FP32 factor alone, the factor with the network's correction
-    and with the exact local inverse; field error on 200k points against the
-    direct solve."""
+    kernel matrix, so neither reaches 1e-6 at this size); then the learned
+    form end to end (that attempt, stagnation, the FP64 direct solve: the
+    system is solved) and the 256^3 grid evaluated with its weights.  Judge: a
+    ~35k sub-sample of the same model, solved by the FP64 direct (Cholesky)
+    form, the shifted FP32 factor alone, the factor with the network's
+    correction and with the exact local inverse; field error on 200k points
+    against the direct solve."""
     from . import learned
     net = learned.SpaiNetwork.load(weights or learned.WEIGHTS)
     m = synthetic.config4(n_points=n_points, n_ori=n_ori, resolution=(grid_res,) * 3)
@@ -110,13 +112,23 @@ def learned_gmres(n_points=100_000, n_ori=10_000, judge_points=None, tol=1e-9, b
                                      no_fallback=True)
         torch.cuda.synchronize()
         full[name] = {"solve_s": time.perf_counter() - t0, **info}
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    w, info = learned.solve_learned(ds)
+    torch.cuda.synchronize()
+    full["learned_form"] = {"solve_s": time.perf_counter() - t0, **{k: v for k, v in info.items() if k != "learned"},
+                            "network_attempt": info.get("learned")}
     out["full"] = full
     if eval_grid:
+        ds.evaluate_lattice(m.grid, w, _lib.GK_F64, want_scalar=True)      # warm-up of the evaluation path
+        torch.cuda.synchronize()
         t0 = time.perf_counter()
-        ds.evaluate_lattice(m.grid, w, _lib.GK_F64)
+        S, _, _ = ds.evaluate_lattice(m.grid, w, _lib.GK_F64)
         torch.cuda.synchronize()
         out["grid_eval_s"] = time.perf_counter() - t0
         out["grid_points"] = m.grid.n_points
+        out["grid_gpts_s"] = m.grid.n_points / out["grid_eval_s"] / 1e9
+        del S
     nj = judge_points or (20_000, 5_000)
     sub = synthetic.config4(n_points=nj[0], n_ori=nj[1], resolution=(8, 8, 8)).stacks[0]
     dj = DeviceStack(sub, ko, center)

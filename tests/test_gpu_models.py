@@ -348,3 +348,28 @@ def test_octree_levels_match_oracle(gpu):
         if r["refine"] is not None:
             np.testing.assert_array_equal(o.refine.cpu().numpy(), r["refine"])
     assert RegularGrid.octree((0, 1000, 0, 1000, 0, 1000), 3).resolution == (8, 8, 8)
+
+
+def test_config4_scaled_learned_form_matches_oracle(gpu):
+    """Config 4 (the learned-preconditioner workload) at 6,000 points / 600
+    orientations (N = 7,783) through compute_model in the learned form: the
+    network-preconditioned matrix-free GMRES attempt, then the direct solve,
+    against the oracle's FP64 LU on sampled cells of the grid."""
+    say("config4 scaled (N=7,783) learned form vs oracle")
+    from gempy_amd import GemPyEngineConfig, compute_model, synthetic
+    from gempy_amd.data import SolverForm
+    m = synthetic.config4(n_points=6_000, n_ori=600, resolution=(32, 32, 32))
+    m.options.solver = SolverForm.GMRES_LEARNED
+    sol = compute_model(m, GemPyEngineConfig())
+    info = sol.stacks[0].solver_info
+    X = m.grid.values()
+    idx = np.random.default_rng(4).choice(len(X), 5000, replace=False)
+    r = ok.compute_stack(m.stacks[0].as_oracle_dict(), m.options.as_dict(), X[idx])
+    S = sol.stacks[0].scalar_field.cpu().numpy()[idx]
+    err = np.abs(S - r["scalar"]).max() / np.ptp(r["scalar"])
+    band = np.min(np.abs(r["scalar"][:, None] - r["thresholds"][None, :]), 1) <= 1e-6 * np.ptp(r["scalar"])
+    mism = int(((sol.lith_block.cpu().numpy()[idx] != r["ids"]) & ~band).sum())
+    say(f"config4 scaled: N={len(r['b'])} max|dS|/range {err:.3e} id mismatches {mism} relres {info['relres']:.2e} "
+        f"form {info['form']} fallback {info['fallback']} network attempt {info['learned']['iterations']} its "
+        f"relres {info['learned']['relres']:.3e}")
+    assert err <= 1e-6 and mism == 0 and info["relres"] <= 1e-6
