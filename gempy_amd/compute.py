@@ -177,6 +177,9 @@ def compute_model(model: GeoModelInput, engine_config: Optional[GemPyEngineConfi
         pts = torch.tensor(np.asarray(model.custom_grid, dtype=np.float64).reshape(-1, 3), dtype=torch.float64,
                            device="cuda")
         out.custom, _, _ = m.at_points(pts)
+    if model.geophysics is not None:
+        from .gravity import gravity_of_model
+        out.gravity = gravity_of_model(m, model.geophysics)
     torch.cuda.synchronize()
     out.timings["model_s"] = time.perf_counter() - t0
     return out

@@ -204,6 +204,9 @@ class GeoModelInput:
     # iterative refinement (the reference passes them on,
     # gempy/modules/data_manipulation/_engine_factory.py:45-48)
     weights: Optional[List[np.ndarray]] = None
+    # gravity input (gempy_amd.gravity.GeophysicsInput: centred grid, tz,
+    # densities); Solutions.gravity holds the result
+    geophysics: Optional[object] = None
 
     def system_size(self, k: int = 0) -> int:
         st = self.stacks[k]
@@ -320,6 +323,7 @@ class Solutions:
     custom: Optional[np.ndarray] = None
     timings: dict = field(default_factory=dict)
     dc_meshes: Optional[list] = None
+    gravity: Optional[np.ndarray] = None
     _raw: Optional[RawArrays] = None
 
     @property

@@ -289,6 +289,18 @@ int gk_marching_cubes(gk_handle h, const double* field, const int64_t* res, cons
                       const double* origin, const double* spacing, double* verts, int64_t max_verts, int32_t* faces,
                       int64_t max_faces, int64_t* n_verts, int64_t* n_faces, gk_stream stream);
 
+/* ---- gravity on a centred grid (geophysics input tz + densities ->
+ * Solutions.gravity; gempy/API/grid_API.py:99-108, test_gravity.py:67-89) ----
+ * gk_gravity_tz: vertical attraction (unit density, G given: 6.674e-3 for mGal
+ * with m and g/cm^3) at the device of n prisms, cell centres relative to the
+ * device and half-sizes towards lower / higher coordinates [n*3] each.
+ * gk_gravity: out[d] = sum_k tz[k] densities[ids[d*n_kernel + k] - 1] (ids are
+ * lith ids 1..n_densities; others contribute 0; densities a host array). */
+int gk_gravity_tz(gk_handle h, int64_t n, const double* centres, const double* half_lo, const double* half_hi,
+                  double G, double* tz, gk_stream stream);
+int gk_gravity(gk_handle h, int64_t n_devices, int64_t n_kernel, const double* tz, const int32_t* ids,
+               int32_t n_densities, const double* densities, double* out, gk_stream stream);
+
 /* ---- octree evaluation (the engine's octree levels: _number_octree_levels,
  * fault.approved.txt:210-214; resolution = base * 2^(levels-1),
  * gempy/core/data/grid_modules/regular_grid.py:36-43) ------------------------
