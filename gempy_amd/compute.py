@@ -89,6 +89,13 @@ class _Model:
             else:
                 w, info = ds.solve(form, x0=x0)
             info["warm_start"] = x0 is not None
+            ko = model.options.kernel_options
+            if ko.compute_condition_number:
+                if info["form"] not in (_lib.GK_SOLVE_FP64_CHOL, _lib.GK_SOLVE_FP32_REFINE):
+                    ds.solve(_lib.GK_SOLVE_FP64_CHOL)         # a kept factor for the inverse iteration
+                info["condition_number"], info["condition"] = ds.condition_number()
+                # one value on the options, as the reference keeps it: the last stack's
+                ko.condition_number = info["condition_number"]
             thr, s_sp = ds.surface_thresholds(w)
             self.stacks.append(ds)
             self.sols.append(StackSolution(weights=w, scalar_field_at_surface_points=s_sp.cpu().numpy(),

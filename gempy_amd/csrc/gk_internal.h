@@ -48,6 +48,7 @@ enum gk_ws_slot {
   WS_TRSV,           // work tables, readiness flags and partial sums of the triangular solves
   WS_ASSEMBLE,       // reference-point tables of the assembly
   WS_MESH,           // case tables, edge flags and scans of the isosurface extraction
+  WS_LANCZOS,        // Lanczos basis of the condition-number estimate
   WS_COUNT
 };
 struct gk_context {

@@ -1041,6 +1041,166 @@ extern "C" int gk_solve_rhs(gk_handle h, const gk_system* sys, const double* rhs
   return GK_OK;
 }
 
+namespace {
+
+// Number of eigenvalues of the symmetric tridiagonal T (diagonal a, off-diagonal
+// b) below x (Sturm sequence of the LDL^T pivots).
+int sturm_count(const std::vector<double>& a, const std::vector<double>& b, int k, double x) {
+  int c = 0;
+  double d = 1.0;
+  for (int i = 0; i < k; ++i) {
+    const double bb = i > 0 ? b[i - 1] * b[i - 1] : 0.0;
+    d = a[i] - x - (i > 0 ? bb / d : 0.0);
+    if (d == 0.0) d = -1e-300;
+    if (d < 0.0) ++c;
+  }
+  return c;
+}
+
+// Smallest and largest eigenvalue of T by bisection inside Gershgorin bounds.
+void tridiag_extremes(const std::vector<double>& a, const std::vector<double>& b, int k, double* lo, double* hi) {
+  double glo = 1e300, ghi = -1e300;
+  for (int i = 0; i < k; ++i) {
+    const double r = (i > 0 ? std::fabs(b[i - 1]) : 0.0) + (i + 1 < k ? std::fabs(b[i]) : 0.0);
+    glo = std::min(glo, a[i] - r);
+    ghi = std::max(ghi, a[i] + r);
+  }
+  auto bisect = [&](int target) {          // smallest x with count(x) > target
+    double l = glo, u = ghi;
+    for (int it = 0; it < 200 && u - l > 1e-16 * std::max(std::fabs(l), std::fabs(u)); ++it) {
+      const double mid = 0.5 * (l + u);
+      if (sturm_count(a, b, k, mid) > target) u = mid; else l = mid;
+    }
+    return 0.5 * (l + u);
+  };
+  *lo = bisect(0);
+  *hi = bisect(k - 1);
+}
+
+__global__ void lanczos_start_kernel(double* v, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  // deterministic start vector with no special alignment to the system's structure
+  uint64_t z = (uint64_t)(i + 1) * 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  z ^= z >> 31;
+  v[i] = (double)(z >> 11) * (1.0 / 9007199254740992.0) - 0.5;
+}
+
+// Largest |eigenvalue| of the symmetric operator `op` by Lanczos with full
+// reorthogonalisation (classical Gram-Schmidt, twice); Vb holds kmax + 1
+// columns of leading dimension ld.  Stops when the estimate changes by less
+// than tol (relative) over 3 steps, at an invariant subspace, or at kmax.
+// part: (ceil(N / RED_ROWS) + 1) * 64 partial sums; coef: 2 (kmax + 1).
+template <class Op>
+int lanczos_sigma_max(Solver& S, Op op, int kmax, double tol, double* Vb, int64_t ld, double* wv, double* part,
+                      double* coef, double* sig, int* steps) {
+  const int64_t N = S.N;
+  hipStream_t st = S.st;
+  std::vector<double> a, bt, c(kmax + 1);
+  const unsigned nb = nblk(N, RED_ROWS);
+  // c[0..n) = V[:, 0..n)^T w, 64 columns per launch
+  auto dots = [&](int n) -> int {
+    for (int c0 = 0; c0 < n; c0 += 64) {
+      const int nc = std::min(64, n - c0);
+      hipLaunchKernelGGL(dots_partial_kernel, dim3(nb, (unsigned)nc), dim3(256), 0, st, Vb + (int64_t)c0 * ld, ld, nc,
+                         wv, N, part);
+      hipLaunchKernelGGL(reduce_cols_kernel, dim3(1), dim3(128), 0, st, part, (int)nb, nc, coef + c0);
+    }
+    GK_HIP(hipMemcpyAsync(c.data(), coef, n * sizeof(double), hipMemcpyDeviceToHost, st));
+    GK_HIP(hipStreamSynchronize(st));
+    return GK_OK;
+  };
+  hipLaunchKernelGGL(lanczos_start_kernel, dim3(nblk(N)), dim3(256), 0, st, Vb, N);
+  double nv = 0.0;
+  GK_TRY(S.norm(Vb, N, &nv));
+  hipLaunchKernelGGL(scale_kernel, dim3(nblk(N)), dim3(256), 0, st, Vb, 1.0 / nv, Vb, N);
+  double est = 0.0;
+  int stable = 0, j = 0;
+  for (; j < kmax; ++j) {
+    double* vj = Vb + (int64_t)j * ld;
+    GK_TRY(op(vj, wv));
+    // alpha_j = <v_j, w>; then w -= V c over all previous columns, twice
+    double alpha = 0.0;
+    for (int pass = 0; pass < 2; ++pass) {
+      GK_TRY(dots(j + 1));
+      if (pass == 0) alpha = c[j]; else alpha += c[j];
+      GK_HIP(hipMemcpyAsync(coef + kmax + 1, c.data(), (j + 1) * sizeof(double), hipMemcpyHostToDevice, st));
+      hipLaunchKernelGGL(gemv_sub_kernel, dim3(nblk(N)), dim3(256), 0, st, Vb, ld, j + 1, coef + kmax + 1, wv, N);
+    }
+    a.push_back(alpha);
+    double beta = 0.0;
+    GK_TRY(S.norm(wv, N, &beta));
+    double lo = 0.0, hi = 0.0;
+    tridiag_extremes(a, bt, j + 1, &lo, &hi);
+    const double cur = std::max(std::fabs(lo), std::fabs(hi));
+    stable = (j > 0 && std::fabs(cur - est) <= tol * cur) ? stable + 1 : 0;
+    est = cur;
+    if (stable >= 3 || beta <= 1e-14 * cur || j + 1 >= N) { ++j; break; }
+    bt.push_back(beta);
+    hipLaunchKernelGGL(scale_kernel, dim3(nblk(N)), dim3(256), 0, st, wv, 1.0 / beta, Vb + (int64_t)(j + 1) * ld, N);
+  }
+  GK_HIP(hipStreamSynchronize(st));
+  *sig = est;
+  *steps = j;
+  return GK_OK;
+}
+
+}  // namespace
+
+extern "C" int gk_condition_number(gk_handle h, const gk_system* sys, int32_t max_steps, double tol, double* cond,
+                                   double* sigma, int32_t* steps, gk_stream stream) {
+  GK_REQUIRE(h && sys && cond, GK_ERR_ARG, "gk_condition_number: NULL argument");
+  GK_REQUIRE(max_steps >= 2 && max_steps <= 300 && tol > 0.0, GK_ERR_ARG,
+             "gk_condition_number: max_steps must be in [2, 300] and tol > 0");
+  Solver* S = kept_solver(h, sys);
+  GK_REQUIRE(S != nullptr, GK_ERR_ARG,
+             "gk_condition_number: no factorization of this system is kept (call gk_solve with the FP64_CHOL or "
+             "FP32_REFINE form first)");
+  hipStream_t st = (hipStream_t)stream;
+  S->st = st;
+  const int64_t N = S->N, ld = S->Npad;
+  const int k = (int)std::min<int64_t>(max_steps, N);
+  void* p = nullptr;
+  const size_t n_part = (size_t)(gk_ceil_div(N, RED_ROWS) + 1) * 64;
+  GK_TRY(gk_workspace(h, WS_LANCZOS, ((size_t)(k + 1) * ld + 2 * ld + n_part + 2 * (k + 1)) * sizeof(double), &p));
+  double* Vb = static_cast<double*>(p);
+  double* wv = Vb + (size_t)(k + 1) * ld;
+  double* xo = wv + ld;
+  double* part = xo + ld;
+  double* coef = part + n_part;
+  auto op_A = [&](const double* v, double* out) -> int {
+    return gk_matvec_boxed(h, S->sys, GK_F64, v, out, st, S->mv_box);
+  };
+  auto op_inv = [&](const double* v, double* out) -> int {
+    GK_HIP(hipMemcpyAsync(S->b, v, N * sizeof(double), hipMemcpyDeviceToDevice, st));
+    if (S->f64) {
+      int it = 0;
+      double rel = 0.0;
+      GK_TRY(S->direct_refine(xo, 3, &it, &rel));
+    } else {
+      int it = 0;
+      double rel = 0.0;
+      bool ok = false;
+      // A^-1 v is dominated by the smallest singular direction: the FP64
+      // operator's residual floor is then near eps * cond(A), not eps
+      GK_TRY(S->fgmres(xo, 1e-12, 1e-5, 300, &it, &rel, &ok));
+      GK_REQUIRE(ok, GK_ERR_NOT_CONVERGED, "gk_condition_number: inner solve stagnated at relres %.3e", rel);
+    }
+    GK_HIP(hipMemcpyAsync(out, xo, N * sizeof(double), hipMemcpyDeviceToDevice, st));
+    return GK_OK;
+  };
+  double smax = 0.0, sinv = 0.0;
+  int s1 = 0, s2 = 0;
+  GK_TRY(lanczos_sigma_max(*S, op_A, k, tol, Vb, ld, wv, part, coef, &smax, &s1));
+  GK_TRY(lanczos_sigma_max(*S, op_inv, k, tol, Vb, ld, wv, part, coef, &sinv, &s2));
+  *cond = smax * sinv;
+  if (sigma) { sigma[0] = smax; sigma[1] = sinv > 0.0 ? 1.0 / sinv : 0.0; }
+  if (steps) { steps[0] = s1; steps[1] = s2; }
+  return GK_OK;
+}
+
 static int adjoint_impl(gk_handle h, const gk_system* sys, const double* w, int64_t n_obs, const double* obs_xyz,
                         const double* obs_fault, const double* e, double* d_points, double* d_grad, double* d_ori_xyz,
                         double* lam_out, gk_stream stream);

@@ -49,6 +49,7 @@ class KernelOptions:
     kernel_function: str = "cubic"
     kernel_solver: int = 1
     compute_condition_number: bool = False
+    condition_number: Optional[float] = None      # set by compute_model when compute_condition_number
 
 
 @dataclass

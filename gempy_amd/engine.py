@@ -139,6 +139,18 @@ class DeviceStack:
                    "t_factor_ms": float(info.t_factor_ms), "t_refine_ms": float(info.t_refine_ms),
                    "shift": float(info.shift)}
 
+    def condition_number(self, max_steps: int = 120, tol: float = 1e-10):
+        """2-norm condition number of the system matrix (needs the factor
+        kept by the last solve of this stack: FP64_CHOL or FP32_REFINE)."""
+        lib = _lib.load()
+        cond = C.c_double()
+        sig = (C.c_double * 2)()
+        steps = (C.c_int32 * 2)()
+        _lib.check("gk_condition_number", lib.gk_condition_number(self.h, C.byref(self.sys), int(max_steps),
+                                                                  float(tol), C.byref(cond), sig, steps, _stream()))
+        return float(cond.value), {"sigma_max": float(sig[0]), "sigma_min": float(sig[1]),
+                                   "lanczos_steps": [int(steps[0]), int(steps[1])]}
+
     # ------------------------------------------------------------ assembly
     def assemble(self, dtype: int = _lib.GK_F64):
         """Dense matrix (FP64 full system or FP32 equilibrated covariance block)."""

@@ -202,6 +202,18 @@ int gk_solve_x0(gk_handle h, const gk_system* sys, int32_t form, double tol, int
 int gk_solve_rhs(gk_handle h, const gk_system* sys, const double* rhs, double* x, gk_solve_info* info,
                  gk_stream stream);
 
+/* Condition number of the co-kriging matrix in the 2-norm, sigma_max /
+ * sigma_min (the reference's kernel_options.compute_condition_number ->
+ * kernel_options.condition_number, read at
+ * gempy/modules/optimize_nuggets/_ops.py:40 and examples/examples/real/Hecho.py:219).
+ * Lanczos with full reorthogonalisation on A (the matrix-free FP64 operator)
+ * and on A^-1 (the factorization kept by the last gk_solve of this system,
+ * FP64_CHOL or FP32_REFINE), each to a relative change below tol over three
+ * steps or max_steps (<= 300) steps.  sigma (nullable) [2]: sigma_max, sigma_min;
+ * steps (nullable) [2]: Lanczos steps on A and on A^-1. */
+int gk_condition_number(gk_handle h, const gk_system* sys, int32_t max_steps, double tol, double* cond, double* sigma,
+                        int32_t* steps, gk_stream stream);
+
 /* ---- differentiation (the reference tapes torch autograd through the
  * engine: compute_API.py:49,66; gradients w.r.t. sp_coords and the
  * orientation gradients, test_ad_I.py:41-51) ---------------------------------
