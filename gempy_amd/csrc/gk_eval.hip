@@ -866,7 +866,7 @@ __global__ void combine_lith_kernel(const int32_t* ids, int32_t n_stacks, int64_
   const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= n) return;
   int32_t v = basement;
-  int owner = -1;
+  int owner = n_stacks - 1;          // the oldest stack holds the basement
   for (int s = 0; s < n_stacks; ++s) {
     const int32_t id = ids[(int64_t)s * n + p];
     if (id > nsurf[s]) continue;

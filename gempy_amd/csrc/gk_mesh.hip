@@ -153,7 +153,8 @@ __global__ void mc_classify_kernel(McArgs a, uint8_t* cases, uint32_t* ntri, uin
   if (c >= ncx * ncy * ncz) return;
   const int64_t z = c % ncz, y = (c / ncz) % ncy, x = c / (ncz * ncy);
   int cs = 0;
-  bool ok = !a.mask || a.mask[pidx(a, x, y, z)];
+  // a cube is meshed when the mask is set at its far corner (+1, +1, +1)
+  bool ok = !a.mask || a.mask[pidx(a, x + 1, y + 1, z + 1)];
   if (ok) {
     for (int k = 0; k < 8; ++k) {
       const double v = a.f[pidx(a, x + (k & 1), y + ((k >> 1) & 1), z + ((k >> 2) & 1))];

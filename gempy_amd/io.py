@@ -187,3 +187,95 @@ def load_gempy(path_or_bytes, solver: SolverForm = SolverForm.FP32_REFINE) -> Lo
     return LoadedModel(model=model, name=model.name, transform=tr, element_names=enames,
                        scalar_field_at_interface=recorded, is_dirty=bool(sf.get("is_dirty", False)),
                        octree_extent=oct_ext, octree_resolution=oct_res, octree_levels=levels, header=header)
+
+
+# --------------------------------------------------------------------------
+# CSV tables (gempy/API/io_API.py: the ImporterHelper path of create_geomodel)
+# --------------------------------------------------------------------------
+_COLUMNS = {"X": ("X", "x"), "Y": ("Y", "y"), "Z": ("Z", "z"), "azimuth": ("azimuth", "Azimuth"),
+            "dip": ("dip", "Dip"), "polarity": ("polarity", "Polarity"),
+            "formation": ("formation", "Formation", "surface"), "G_x": ("G_x", "gradient_x"),
+            "G_y": ("G_y", "gradient_y"), "G_z": ("G_z", "gradient_z")}
+DEFAULT_SP_NUGGET, DEFAULT_ORI_NUGGET = 2e-5, 0.01      # surface_points.py:11, orientations.py:11
+
+
+def _read_table(path_or_text) -> Dict[str, list]:
+    import csv
+    if isinstance(path_or_text, str) and "\n" not in path_or_text:
+        with open(path_or_text, newline="") as f:
+            rows = list(csv.DictReader(f))
+    else:
+        rows = list(csv.DictReader(_io.StringIO(path_or_text)))
+    out: Dict[str, list] = {}
+    for std, names in _COLUMNS.items():
+        for n in names:
+            if rows and n in rows[0]:
+                out[std] = [r[n] for r in rows]
+                break
+    return out
+
+
+def pole_vectors(azimuth, dip, polarity) -> np.ndarray:
+    """Gradient (pole) vectors of orientations given as azimuth / dip /
+    polarity in degrees (io_API.py:96-101, manipulate_points.py:324-337)."""
+    az, dp, pol = (np.deg2rad(np.asarray(azimuth, dtype=np.float64)), np.deg2rad(np.asarray(dip, dtype=np.float64)),
+                   np.asarray(polarity, dtype=np.float64))
+    return np.stack([np.sin(dp) * np.sin(az) * pol, np.sin(dp) * np.cos(az) * pol, np.cos(dp) * pol], 1)
+
+
+def read_csv_model(surface_points, orientations, mapping: Dict[str, tuple], extent, resolution,
+                   relations: Optional[Dict[str, StackRelationType]] = None,
+                   fault_relations: Optional[np.ndarray] = None, transform: Optional[Transform] = None,
+                   options: Optional[InterpolationOptions] = None, name: str = "model") -> LoadedModel:
+    """A model from surface-point and orientation tables (CSV path or text;
+    columns X, Y, Z, formation and G_x/G_y/G_z or azimuth/dip/polarity) and a
+    group -> elements mapping (map_stack_to_surfaces; groups youngest first,
+    elements in the given order).  Points keep the table order per element
+    (the first is the element's reference point); nuggets take the reference's
+    defaults; the transform defaults to Transform.from_input_points of all
+    points and orientations; relations default to ERODE; fault_relations[i, j]
+    (bool) says that group i offsets group j."""
+    sp = _read_table(surface_points)
+    ori = _read_table(orientations)
+    xyz = np.array([sp["X"], sp["Y"], sp["Z"]], dtype=np.float64).T
+    fm = np.asarray(sp["formation"])
+    qx = np.array([ori["X"], ori["Y"], ori["Z"]], dtype=np.float64).T
+    qf = np.asarray(ori["formation"])
+    if "G_x" in ori:
+        grads = np.array([ori["G_x"], ori["G_y"], ori["G_z"]], dtype=np.float64).T
+    else:
+        grads = pole_vectors(ori["azimuth"], ori["dip"], ori["polarity"])
+    tr = transform or Transform.from_input_points(xyz, qx)
+    groups = list(mapping.items())
+    n = len(groups)
+    fr = np.zeros((n, n), dtype=bool) if fault_relations is None else np.asarray(fault_relations, dtype=bool)
+    stacks, enames = [], []
+    for gi, (gname, elements) in enumerate(groups):
+        elements = (elements,) if isinstance(elements, str) else tuple(elements)
+        P, counts, names, Q, G = [], [], [], [], []
+        for e in elements:
+            sel = fm == e
+            if sel.any():
+                P.append(xyz[sel])
+                counts.append(int(sel.sum()))
+                names.append(e)
+            qs = qf == e
+            if qs.any():
+                Q.append(qx[qs])
+                G.append(grads[qs])
+        rel = (relations or {}).get(gname, StackRelationType.ERODE)
+        Pc = np.concatenate(P) if P else np.zeros((0, 3))
+        Qc = np.concatenate(Q) if Q else np.zeros((0, 3))
+        stacks.append(StackInput(name=gname, sp_xyz=tr.apply(Pc), counts=counts,
+                                 sp_nugget=np.full(len(Pc), DEFAULT_SP_NUGGET), ori_xyz=tr.apply(Qc),
+                                 ori_grad=tr.transform_gradient(np.concatenate(G)) if G else np.zeros((0, 3)),
+                                 ori_nugget=np.full(len(Qc), DEFAULT_ORI_NUGGET), relation=rel, surface_names=names,
+                                 faults=[int(i) for i in np.nonzero(fr[:gi, gi])[0]]))
+        enames.append(names)
+    ext = np.asarray(extent, dtype=np.float64)
+    lo = tr.apply(ext[[0, 2, 4]][None])[0]
+    hi = tr.apply(ext[[1, 3, 5]][None])[0]
+    grid = RegularGrid([lo[0], hi[0], lo[1], hi[1], lo[2], hi[2]], [int(r) for r in resolution])
+    model = GeoModelInput(stacks=stacks, options=options or InterpolationOptions(), grid=grid, name=name)
+    return LoadedModel(model=model, name=name, transform=tr, element_names=enames,
+                       scalar_field_at_interface=[[None] * len(e) for e in enames], is_dirty=True)

@@ -30,7 +30,7 @@ def marching_cubes(field: torch.Tensor, grid: RegularGrid, level: float, mask: O
     """Isosurface ``field == level`` of a lattice field (device tensor of
     grid.n_points values, z fastest).  Vertices at origin + index * spacing
     (origin defaults to the first cell centre); mask (uint8/bool, same
-    layout): cubes whose first corner is set."""
+    layout): cubes whose far corner (+1, +1, +1) is set."""
     lib = _lib.load()
     f = field.reshape(-1).to(dtype=torch.float64).contiguous()
     if f.numel() != grid.n_points:

@@ -274,7 +274,8 @@ int gk_combine_lith(gk_handle h, int32_t n_stacks, const int32_t* ids, int64_t n
  * gempy/core/data/structural_frame.py:243-273): an onlapping stack holds a
  * point only above the first surface of the next older stack (whose top is
  * preserved).  masks (nullable, [n_stacks][n] uint8): 1 where that stack holds
- * the point (the engine's mask_matrix; marching_cubes.py:44-46). */
+ * the point, the oldest stack holding the basement too (the engine's squeezed
+ * mask matrix; marching_cubes.py:44-46). */
 int gk_combine_stacks(gk_handle h, int32_t n_stacks, const int32_t* ids, int64_t n, const int32_t* n_surf,
                       const int32_t* relation, int32_t* lith, uint8_t* masks, gk_stream stream);
 
@@ -290,7 +291,7 @@ int gk_block_values(gk_handle h, const double* scalar, int64_t n, int32_t n_thre
  * extracts one mesh per surface at its threshold with skimage's marching cubes,
  * gempy/modules/mesh_extranction/marching_cubes.py:59-101) --------------------
  * field [res0*res1*res2] (z fastest), mask (nullable, same layout: a cube is
- * meshed when its first corner is set), vertices at origin + (index + t e_axis)
+ * meshed when its far corner (+1, +1, +1) is set), vertices at origin + (index + t e_axis)
  * * spacing on the lattice edges that cross `level` (one per edge, shared),
  * faces as vertex-index triples wound so that normals point towards
  * increasing values; the mesh is watertight away from the lattice boundary and

@@ -445,6 +445,8 @@ def compute_model(stacks, opts, x, gradient=False):
         assigned |= take
         next_id += ns
     lith[~assigned] = next_id                   # basement
+    if layer:                                   # held by the oldest stack's mask
+        masks[layer[-1]] = masks[layer[-1]] | ~assigned
     faults = [r["ids"] for st, r in zip(stacks, out) if st.get("relation") == "fault"]
     mask_matrix = [masks.get(k, np.ones(x.shape[0], dtype=bool)) for k in range(len(stacks))]
     return dict(stacks=out, lith_block=lith, fault_blocks=faults, basement_id=next_id,

@@ -20,8 +20,8 @@ import numpy as np
 
 def crossing_vertices(F, level, origin, spacing, mask=None):
     """Vertices (n, 3) of all lattice edges with F > level at exactly one end;
-    with a mask (same shape as F), only the edges of cubes whose first corner
-    is set."""
+    with a mask (same shape as F), only the edges of cubes whose far corner
+    (+1, +1, +1) is set."""
     F = np.asarray(F, dtype=np.float64)
     nx, ny, nz = F.shape
     above = F > level
@@ -29,7 +29,7 @@ def crossing_vertices(F, level, origin, spacing, mask=None):
     if mask is None:
         cube = np.ones((nx - 1, ny - 1, nz - 1), dtype=bool)
     else:
-        cube = np.asarray(mask, dtype=bool)[:-1, :-1, :-1]
+        cube = np.asarray(mask, dtype=bool)[1:, 1:, 1:]
     for axis in range(3):
         # the 4 edges of each cube along this axis
         for o1 in (0, 1):

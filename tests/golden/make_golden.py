@@ -18,6 +18,11 @@ where the reference tree is readable; the fixtures travel, the reference does no
 * transforms.npz -- raw input points and the recorded input_transform
   (position, scale) of the 2-layers, anticline and 2.5-D (Model1) models.
 * greenstone/ -- the Greenstone model's .gempy file and its legacy v2 tables.
+* combination/ -- the COMBINATION example's input tables
+  (examples/data/input_data/jan_models/model7_{surface_points,orientations}.csv,
+  gempy/API/examples_generator.py:243-288); the reference's marching-cubes test
+  records its mesh vertex counts 600 / 860 / 1256 / 1680 on a 40x20x20 dense grid
+  (test/test_modules/test_marching_cubes.py:14-47).
 """
 from __future__ import annotations
 
@@ -156,10 +161,21 @@ def greenstone():
                         os.path.join(dst, f"Greenstone_{name}.csv"))
 
 
+def combination():
+    """The COMBINATION example's surface-point and orientation tables."""
+    import shutil
+    dst = os.path.join(HERE, "combination")
+    os.makedirs(dst, exist_ok=True)
+    src = os.path.join(REF, "examples/data/input_data/jan_models")
+    for name in ("surface_points", "orientations"):
+        shutil.copyfile(os.path.join(src, f"model7_{name}.csv"), os.path.join(dst, f"model7_{name}.csv"))
+
+
 if __name__ == "__main__":
     two_layers()
     anticline()
     config1_oracle()
     transforms()
     greenstone()
+    combination()
     print("fixtures written to", HERE)

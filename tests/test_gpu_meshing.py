@@ -113,3 +113,32 @@ def test_model_surface_meshes_match_oracle(gpu):
     say(f"model meshes: {sum(len(x) for x in meshes)} surfaces, {total} vertices, "
         f"dc_meshes {len(sol.dc_meshes)}, raw vertices {len(sol.raw_arrays.vertices)}")
     assert total > 0 and len(sol.dc_meshes) == sum(len(s.counts) for s in m.stacks)
+
+
+def test_combination_model_meshes(gpu):
+    """The reference's marching-cubes case (test_marching_cubes.py:14-47): the
+    COMBINATION model on a 40x20x20 dense grid, one mesh per element.  Every
+    vertex set equals the oracle's; the fault and rock3 vertex counts equal the
+    reference's 600 and 860, rock2 / rock1 come within 0.5% of 1256 / 1680."""
+    say("combination model meshes: 40x20x20")
+    from gempy_amd import GemPyEngineConfig, compute_model
+    from gempy_amd.data import StackRelationType
+    from gempy_amd.meshing import compute_meshes
+    from oracle import cokriging as ok
+    from test_io import COMBINATION_VERTICES, combination_model
+    m = combination_model().model
+    g = m.grid
+    sol = compute_model(m, GemPyEngineConfig(dtype="float64"))
+    meshes = compute_meshes(sol, g, reference_origin=False)
+    res = ok.compute_model([s.as_oracle_dict() for s in m.stacks], m.options.as_dict(), g.values())
+    counts = []
+    for k, (st, ms, r) in enumerate(zip(m.stacks, meshes, res["stacks"])):
+        mask = None if st.relation == StackRelationType.FAULT else res["mask_matrix"][k].reshape(g.resolution)
+        for t, mesh in zip(r["thresholds"], ms):
+            Vo = om.crossing_vertices(r["scalar"].reshape(g.resolution), t, g.origin, g.spacing, mask)
+            assert _same_points(mesh.vertices, Vo, 1e-6 * max(g.spacing))
+            counts.append(len(mesh.vertices))
+    say(f"combination meshes: vertex counts {counts}, reference {COMBINATION_VERTICES}")
+    assert counts[:2] == COMBINATION_VERTICES[:2]
+    for c, ref in zip(counts[2:], COMBINATION_VERTICES[2:]):
+        assert abs(c - ref) <= 0.005 * ref

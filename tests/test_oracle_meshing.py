@@ -23,11 +23,14 @@ def test_crossing_vertices_of_a_plane():
                     if all(q[i] < n[i] for i in range(3)):
                         count += (F[x, y, z] > level) != (F[q] > level)
     assert len(V) == count
-    # a mask removes exactly the edges of the masked cubes
+    # a mask removes exactly the edges of the masked cubes: a cube counts when
+    # its far corner (+1, +1, +1) is set, so mask[:4] = False keeps cubes x >= 3
     mask = np.ones(n, dtype=bool)
     mask[:4] = False
     Vm = om.crossing_vertices(F, level, (0.0, 0.0, 0.0), (1.0, 1.0, 1.0), mask)
-    assert 0 < len(Vm) < len(V) and np.all(Vm[:, 0] >= 4 - 1e-12)
+    assert 0 < len(Vm) < len(V) and np.all(Vm[:, 0] >= 3 - 1e-12) and np.any(Vm[:, 0] < 4)
+    keep = V[V[:, 0] >= 3 - 1e-12]
+    assert len(Vm) == len(keep)
 
 
 def test_mesh_checks_of_a_unit_cube():
