@@ -25,7 +25,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "model s & grid Gpts/s at N=35k: FP32+refine/learned vs FP64 direct, err≤1e-6"
-PEAK = {"fp64_valu": 78.6, "fp32_valu": 157.3, "fp32_mfma": 157.3, "fp64_mfma": 78.6, "hbm": 8000.0}
+PEAK = {"fp64_valu": 78.6, "fp32_valu": 157.3, "fp32_mfma": 157.3, "fp64_mfma": 78.6, "fp16_mfma": 2500.0,
+        "hbm": 8000.0}
 
 
 def log(msg):
@@ -118,7 +119,7 @@ def main():
         t = (time.perf_counter() - t0) / args.steps
         if profile:
             prof = {}
-            for kind in range(10):
+            for kind in range(12):
                 n, ms, work = C.c_int64(0), C.c_double(0), C.c_double(0)
                 _lib.check("gk_profile_read", lib.gk_profile_read(ds.h, kind, C.byref(n), C.byref(ms), C.byref(work)))
                 if n.value:
@@ -167,11 +168,18 @@ def main():
                                                 "true" if kind & 1 else "false")
             peak = PEAK["fp64_mfma"] if kind >= 4 else PEAK["fp32_mfma"]
             ach, unit, bound = work / (ms * 1e-3) / 1e12, "TFLOP/s", "mfma"
+        elif kind >= 10:
+            # split-fp16 rank-256 update: three fp16 MFMA products per product;
+            # achieved counts the fp16 flops executed (3x the algorithmic ones)
+            name = "syrk_h3_kernel<%s>" % ("true" if kind == 11 else "false")
+            peak, ach, unit, bound = PEAK["fp16_mfma"], 3 * work / (ms * 1e-3) / 1e12, "TFLOP/s", "mfma"
         else:
             name = "trsv_fwd_kernel+trsv_bwd_kernel<%s>" % ("double" if kind == 9 else "float")
             peak, ach, unit, bound = PEAK["hbm"], work / (ms * 1e-3) / 1e9, "GB/s", "hbm"
         kernels[name] = {"launches": n, "total_ms": ms, "avg_ms": ms / n, "work_per_launch": work / n,
                          "achieved": ach, "peak": peak, "unit": unit, "bound": bound, "frac": ach / peak}
+        if kind >= 10:
+            kernels[name]["algorithmic_tflops"] = work / (ms * 1e-3) / 1e12
     kernels["eval_kernel<double, true, false, false, 8>" if eprec == _lib.GK_F64 else "eval_kernel (FP32 lattice)"] = {
         "launches": args.steps, "total_ms": eval_ms * args.steps, "avg_ms": eval_ms, "work_per_launch": flop_eval,
         "achieved": ach_eval, "peak": peak_eval, "unit": "TFLOP/s", "bound": "valu", "frac": ach_eval / peak_eval}

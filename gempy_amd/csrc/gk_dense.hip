@@ -378,6 +378,187 @@ int gemm_launch(int64_t m, int64_t n, int64_t k, const T* A, int64_t lda, const 
   return GK_OK;
 }
 
+// ----------------------------------------------- split-fp16 rank-256 update
+// C -= A B^T with K = 256 on v_mfma_f32_16x16x32_f16, for factor panels whose
+// entries lie within [-16, 16] (rows of an equilibrated Cholesky factor: |L_ij|
+// <= sqrt(H_ii + shift) ~ 1).  Each FP32 operand x is split once per panel
+// into hi = fp16(x) and lo = fp16((x - hi) 2^11); then
+//   a b ~ 2^-11 ((2^11 hi_a) hi_b + hi_a lo_b + lo_a hi_b)
+// (2^11 hi_a is exact in fp16 below |x| = 16 x 2^11 / 2^11 ... i.e. |x| < 32),
+// three MFMAs into one FP32 accumulator.  The dropped lo_a lo_b term and the
+// two roundings leave ~2^-21 relative per product, against 2^-24 for FP32:
+// the factor's backward error is within 2x of the FP32 update's on the
+// co-kriging blocks, and fp16 MFMA runs at 16x the FP32 MFMA rate on gfx950.
+typedef _Float16 h8v __attribute__((ext_vector_type(8)));
+typedef _Float16 h2v __attribute__((ext_vector_type(2)));
+
+__global__ void split_h3_kernel(const float* __restrict__ W, int64_t n8, h8v* __restrict__ Wh,
+                                h8v* __restrict__ Wl) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n8) return;
+  const f4v a = reinterpret_cast<const f4v*>(W)[2 * i];
+  const f4v b = reinterpret_cast<const f4v*>(W)[2 * i + 1];
+  h8v hi, lo;
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    hi[e] = (_Float16)a[e];
+    hi[e + 4] = (_Float16)b[e];
+    lo[e] = (_Float16)((a[e] - (float)hi[e]) * 2048.0f);
+    lo[e + 4] = (_Float16)((b[e] - (float)hi[e + 4]) * 2048.0f);
+  }
+  Wh[i] = hi;
+  Wl[i] = lo;
+}
+
+constexpr int H3_K = 256;       // depth of the update (panel width)
+constexpr int H3_BK = 32;       // k per LDS stage (one MFMA)
+constexpr int H3_LD = H3_BK;      // LDS row stride in halves: a fragment read (16 rows x 4 vectors)
+                                  // and a stage store both cover 1 KB contiguously
+
+// 128 x 128 tile, 4 waves of 64 x 64 (4 x 4 fragments of 16 x 16).
+template <bool LOWER>
+__global__ __launch_bounds__(256, 2) void syrk_h3_kernel(
+    const _Float16* __restrict__ Ah, const _Float16* __restrict__ Al, const _Float16* __restrict__ Bh,
+    const _Float16* __restrict__ Bl, float* C, int64_t ldc, int64_t tiles_n) {
+  __shared__ __attribute__((aligned(16))) _Float16 S[2][4][128 * H3_LD];   // [buf][Ah, Al, Bh, Bl]
+  int64_t bid = blockIdx.x;
+  const int64_t nblk = gridDim.x;
+  if (nblk % 8 == 0) bid = (bid % 8) * (nblk / 8) + bid / 8;      // contiguous tiles per XCD
+  int64_t tm, tn;
+  if (LOWER) {
+    tm = (int64_t)((sqrt(8.0 * (double)bid + 1.0) - 1.0) * 0.5);
+    while ((tm + 1) * (tm + 2) / 2 <= bid) ++tm;
+    while (tm * (tm + 1) / 2 > bid) --tm;
+    tn = bid - tm * (tm + 1) / 2;
+  } else {
+    tm = bid / tiles_n;
+    tn = bid % tiles_n;
+  }
+  const int64_t m0 = tm * 128, n0 = tn * 128;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave >> 1, wn = wave & 1;
+  const int lrow = lane & 15, lk = lane >> 4;
+  const _Float16* src[4] = {Ah + m0 * H3_K, Al + m0 * H3_K, Bh + n0 * H3_K, Bl + n0 * H3_K};
+
+  // The accumulators start at -2^11 C (exact scaling), so the epilogue only
+  // stores: C - 2^-11 sum = -2^-11 (-2^11 C + sum).  The C loads are issued
+  // first and complete under the operand pipeline's start.  Lane layout of a
+  // 16 x 16 fragment (v_mfma_f32_16x16x32_f16, checked with an in-bounds
+  // probe): column lane % 16, rows 4 (lane / 16) + r.
+  f4v acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int64_t row = m0 + wm * 64 + i * 16 + lk * 4 + r;
+        const int64_t col = n0 + wn * 64 + j * 16 + lrow;
+        acc[i][j][r] = -2048.0f * C[row * ldc + col];
+      }
+
+  // one stage: 4 arrays x 128 rows x 32 halves = 2048 vectors of 8 halves, 8
+  // per thread; two register sets keep two stages in flight
+  h8v st_reg[2][8];
+  auto load_global = [&](int set, int k0) {
+#pragma unroll
+    for (int it = 0; it < 8; ++it) {
+      const int idx = tid + 256 * it;          // array it/2, row (idx % 512) / 4, vector idx % 4
+      const int arr = it >> 1, row = (idx & 511) >> 2, cv = idx & 3;
+      st_reg[set][it] = *reinterpret_cast<const h8v*>(src[arr] + (int64_t)row * H3_K + k0 + cv * 8);
+    }
+  };
+  auto store_lds = [&](int buf, int set) {
+#pragma unroll
+    for (int it = 0; it < 8; ++it) {
+      const int idx = tid + 256 * it;
+      const int arr = it >> 1, row = (idx & 511) >> 2, cv = idx & 3;
+      *reinterpret_cast<h8v*>(&S[buf][arr][row * H3_LD + cv * 8]) = st_reg[set][it];
+    }
+  };
+  auto multiply = [&](int buf) {
+    h8v ah[4], al[4], as[4], bh[4], bl[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int r = (wm * 64 + i * 16 + lrow) * H3_LD + lk * 8;
+      ah[i] = *reinterpret_cast<const h8v*>(&S[buf][0][r]);
+      al[i] = *reinterpret_cast<const h8v*>(&S[buf][1][r]);
+      as[i] = ah[i] * (_Float16)2048.0f;
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int r = (wn * 64 + j * 16 + lrow) * H3_LD + lk * 8;
+      bh[j] = *reinterpret_cast<const h8v*>(&S[buf][2][r]);
+      bl[j] = *reinterpret_cast<const h8v*>(&S[buf][3][r]);
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(as[i], bh[j], acc[i][j], 0, 0, 0);
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[i], bl[j], acc[i][j], 0, 0, 0);
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al[i], bh[j], acc[i][j], 0, 0, 0);
+      }
+  };
+
+  // stage kt: registers set kt & 1, then LDS buffer kt & 1 (loop fully
+  // unrolled so the register sets are static)
+  constexpr int NK = H3_K / H3_BK;
+  load_global(0, 0);
+  load_global(1, H3_BK);
+  store_lds(0, 0);
+  __syncthreads();
+#pragma unroll
+  for (int kt = 0; kt < NK; ++kt) {
+    if (kt + 2 < NK) load_global(kt & 1, (kt + 2) * H3_BK);
+    multiply(kt & 1);
+    if (kt + 1 < NK) store_lds((kt + 1) & 1, (kt + 1) & 1);
+    __syncthreads();
+  }
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int64_t row = m0 + wm * 64 + i * 16 + lk * 4 + r;
+        const int64_t col = n0 + wn * 64 + j * 16 + lrow;
+        C[row * ldc + col] = -0.00048828125f * acc[i][j][r];
+      }
+}
+
+// Split an FP32 panel (rows x 256, dense) into its fp16 hi / lo parts.
+int split_h3(const float* W, int64_t rows, _Float16* Wh, _Float16* Wl, hipStream_t st) {
+  const int64_t n8 = rows * H3_K / 8;
+  if (n8 > 0)
+    hipLaunchKernelGGL(split_h3_kernel, dim3((unsigned)gk_ceil_div(n8, 256)), dim3(256), 0, st, W, n8,
+                       reinterpret_cast<h8v*>(Wh), reinterpret_cast<h8v*>(Wl));
+  GK_HIP(hipGetLastError());
+  return GK_OK;
+}
+
+// C[m x n] -= A B^T (K = 256) from split panels: A rows (m), B rows (n), each
+// hi / lo with row stride 256; lower: only the tiles on or below the diagonal.
+int syrk_h3_launch(int64_t m, int64_t n, const _Float16* Ah, const _Float16* Al, const _Float16* Bh,
+                   const _Float16* Bl, float* C, int64_t ldc, bool lower, hipStream_t st) {
+  GK_REQUIRE(m % 128 == 0 && n % 128 == 0 && ldc >= n && (!lower || m == n), GK_ERR_ARG,
+             "syrk_h3: m, n must be multiples of 128 (m == n for the lower update)");
+  if (m == 0 || n == 0) return GK_OK;
+  hipEvent_t pa;
+  const bool prof = gk_prof_begin(st, &pa);
+  const int64_t tm = m / 128, tn = n / 128;
+  const int64_t tiles = lower ? tm * (tm + 1) / 2 : tm * tn;
+  if (lower)
+    hipLaunchKernelGGL(syrk_h3_kernel<true>, dim3((unsigned)tiles), dim3(256), 0, st, Ah, Al, Bh, Bl, C, ldc, tn);
+  else
+    hipLaunchKernelGGL(syrk_h3_kernel<false>, dim3((unsigned)tiles), dim3(256), 0, st, Ah, Al, Bh, Bl, C, ldc, tn);
+  if (prof)
+    gk_prof_end(st, pa, 10 + (lower ? 1 : 0),
+                lower ? (double)m * (double)(m + 1) * (double)H3_K : 2.0 * (double)m * (double)n * (double)H3_K);
+  GK_HIP(hipGetLastError());
+  return GK_OK;
+}
+
 // ------------------------------------------------------- diagonal blocks
 // A 256-wide diagonal block is factored as two 128-wide halves: each half is
 // factored and inverted by one workgroup with the block resident in LDS, and
@@ -865,13 +1046,17 @@ struct CholRec {
 // One FP32 panel of the look-ahead factorization, on the side stream.  The
 // part the next trailing update waits for: factor and invert diagonal block kb
 // and solve the panel, W = A21 Dinv_kb^T.
+// With Wh / Wl given, the panel is also split into fp16 hi / lo parts for the
+// split-fp16 updates.
 static int chol32_panel(float* A, int64_t lda, int64_t kb, float* dinv, int32_t* dinfo, float* W, float* Tb,
-                        hipStream_t s) {
+                        hipStream_t s, _Float16* Wh = nullptr, _Float16* Wl = nullptr) {
   const int64_t k0 = kb * NB, rem = lda - k0 - NB;
   GK_TRY(diag256_chol<float>(A, lda, k0, dinv, dinfo, Tb, Tb + DB * DB, s, false));
-  if (rem > 0)
+  if (rem > 0) {
     GK_TRY(gemm_launch<float>(rem, NB, NB, A + (k0 + NB) * lda + k0, lda, dinv + k0 * NB, NB, true, 1.0f, 0.0f, W,
                               NB, false, s));
+    if (Wh) GK_TRY(split_h3(W, rem, Wh, Wl, s));
+  }
   return GK_OK;
 }
 
@@ -911,17 +1096,24 @@ static int chol32_panel_post(float* A, int64_t lda, int64_t kb, const float* W, 
   return GK_OK;
 }
 
+// split16 (FP32 only): rank-256 updates with the split-fp16 MFMA product; for
+// matrices whose factor entries stay within [-16, 16] (unit-diagonal ones).
 template <typename T>
-static int cholesky_impl(gk_handle h, T* A, int64_t n, int64_t lda, T* dinv, int32_t* info, hipStream_t st) {
+static int cholesky_impl(gk_handle h, T* A, int64_t n, int64_t lda, T* dinv, int32_t* info, hipStream_t st,
+                         bool split16 = false) {
   GK_REQUIRE(A && dinv, GK_ERR_ARG, "gk_cholesky: A and dinv required");
   GK_REQUIRE(lda % NB == 0 && n <= lda && n >= 0, GK_ERR_ARG,
              "gk_cholesky: lda must be a multiple of %d and >= n", NB);
   const int64_t nb = lda / NB;
+  split16 = split16 && sizeof(T) == 4;
   void* ws = nullptr;
-  GK_TRY(gk_workspace(h, WS_FACTOR, 256 + (2 * (size_t)lda * NB + 2 * DB * DB) * sizeof(T), &ws));
+  const size_t split_bytes = split16 ? 2 * 2 * (size_t)lda * NB * sizeof(_Float16) : 0;
+  GK_TRY(gk_workspace(h, WS_FACTOR, 256 + (2 * (size_t)lda * NB + 2 * DB * DB) * sizeof(T) + split_bytes, &ws));
   int32_t* dinfo = static_cast<int32_t*>(ws);
   T* W = reinterpret_cast<T*>(static_cast<char*>(ws) + 256);       // two panel buffers (FP32 look-ahead)
   T* Tb = W + 2 * (size_t)lda * NB;
+  // split panels: [buffer 0 hi | buffer 0 lo | buffer 1 hi | buffer 1 lo], lda x NB halves each
+  _Float16* Hs = split16 ? reinterpret_cast<_Float16*>(Tb + 2 * DB * DB) : nullptr;
   GK_HIP(hipMemsetAsync(dinfo, 0, sizeof(int32_t), st));
   GK_HIP(hipMemsetAsync(dinv, 0, (size_t)lda * NB * sizeof(T), st));   // Dinv plane: upper-right quadrants stay 0
   if (n < lda) {
@@ -949,9 +1141,12 @@ static int cholesky_impl(gk_handle h, T* A, int64_t n, int64_t lda, T* dinv, int
       hipEvent_t* ev;
       GK_TRY(gk_side_stream(h, &s2, &ev));
       T* Wb[2] = {W, W + (size_t)lda * NB};
+      _Float16* Hb[2] = {Hs, Hs ? Hs + 2 * (size_t)lda * NB : nullptr};     // hi; lo follows at + lda NB
+      auto hi = [&](int64_t kb) { return Hb[kb & 1]; };
+      auto lo = [&](int64_t kb) { return Hb[kb & 1] ? Hb[kb & 1] + (size_t)lda * NB : nullptr; };
       GK_HIP(hipEventRecord(ev[0], st));
       GK_HIP(hipStreamWaitEvent(s2, ev[0], 0));
-      if (nb > 0) GK_TRY(chol32_panel(A, lda, 0, dinv, dinfo, Wb[0], Tb, s2));
+      if (nb > 0) GK_TRY(chol32_panel(A, lda, 0, dinv, dinfo, Wb[0], Tb, s2, hi(0), lo(0)));
       GK_HIP(hipEventRecord(ev[1], s2));                    // panel 0 factored
       if (nb > 0) GK_TRY(chol32_panel_post(A, lda, 0, Wb[0], Tb, s2));
       for (int64_t kb = 0; kb + 1 < nb; ++kb) {
@@ -959,16 +1154,25 @@ static int cholesky_impl(gk_handle h, T* A, int64_t n, int64_t lda, T* dinv, int
         T* Wk = Wb[kb & 1];
         GK_HIP(hipStreamWaitEvent(st, ev[1], 0));           // panel kb factored, Wk = its rows below
         // next panel's columns: A[k0+NB:, k0+NB : k0+2NB] -= Wk Wk[0:NB]^T
-        GK_TRY(gemm_launch<T>(rem, NB, NB, Wk, NB, Wk, NB, true, T(-1), T(1), A + (k0 + NB) * lda + k0 + NB, lda,
-                              false, st));
+        if (split16)
+          GK_TRY(syrk_h3_launch(rem, NB, hi(kb), lo(kb), hi(kb), lo(kb), A + (k0 + NB) * lda + k0 + NB, lda, false,
+                                st));
+        else
+          GK_TRY(gemm_launch<T>(rem, NB, NB, Wk, NB, Wk, NB, true, T(-1), T(1), A + (k0 + NB) * lda + k0 + NB, lda,
+                                false, st));
         GK_HIP(hipEventRecord(ev[0], st));
         GK_HIP(hipStreamWaitEvent(s2, ev[0], 0));
-        GK_TRY(chol32_panel(A, lda, kb + 1, dinv, dinfo, Wb[(kb + 1) & 1], Tb, s2));
+        GK_TRY(chol32_panel(A, lda, kb + 1, dinv, dinfo, Wb[(kb + 1) & 1], Tb, s2, hi(kb + 1), lo(kb + 1)));
         GK_HIP(hipEventRecord(ev[1], s2));
         GK_TRY(chol32_panel_post(A, lda, kb + 1, Wb[(kb + 1) & 1], Tb, s2));
-        if (rem > NB)                                       // the rest of the trailing update (lower)
-          GK_TRY(gemm_launch<T>(rem - NB, rem - NB, NB, Wk + NB * NB, NB, Wk + NB * NB, NB, true, T(-1), T(1),
-                                A + (k0 + 2 * NB) * lda + k0 + 2 * NB, lda, true, st));
+        if (rem > NB) {                                     // the rest of the trailing update (lower)
+          if (split16)
+            GK_TRY(syrk_h3_launch(rem - NB, rem - NB, hi(kb) + NB * NB, lo(kb) + NB * NB, hi(kb) + NB * NB,
+                                  lo(kb) + NB * NB, A + (k0 + 2 * NB) * lda + k0 + 2 * NB, lda, true, st));
+          else
+            GK_TRY(gemm_launch<T>(rem - NB, rem - NB, NB, Wk + NB * NB, NB, Wk + NB * NB, NB, true, T(-1), T(1),
+                                  A + (k0 + 2 * NB) * lda + k0 + 2 * NB, lda, true, st));
+        }
         if (kb % 16 == 15) {                                // stop early once a pivot has failed
           int32_t hinfo = 0;
           GK_HIP(hipMemcpyAsync(&hinfo, dinfo, sizeof(int32_t), hipMemcpyDeviceToHost, s2));
@@ -987,6 +1191,28 @@ static int cholesky_impl(gk_handle h, T* A, int64_t n, int64_t lda, T* dinv, int
   GK_HIP(hipStreamSynchronize(st));
   if (info) *info = hinfo;
   return hinfo == 0 ? GK_OK : GK_ERR_NOT_SPD;
+}
+
+extern "C" int gk_cholesky_f32_split16(gk_handle h, float* A, int64_t n, int64_t lda, float* dinv, int32_t* info,
+                                       gk_stream stream) {
+  return cholesky_impl<float>(h, A, n, lda, dinv, info, (hipStream_t)stream, true);
+}
+
+extern "C" int gk_test_syrk_split16(gk_handle h, int64_t m, int64_t n, const float* A, const float* B, float* C,
+                                    int64_t ldc, int32_t lower, gk_stream stream) {
+  GK_REQUIRE(h && A && B && C, GK_ERR_ARG, "gk_test_syrk_split16: NULL argument");
+  GK_REQUIRE(m > 0 && n > 0 && m % 128 == 0 && n % 128 == 0 && ldc >= n && (!lower || m == n), GK_ERR_ARG,
+             "gk_test_syrk_split16: m, n must be positive multiples of 128 (m == n when lower)");
+  hipStream_t st = (hipStream_t)stream;
+  void* ws = nullptr;
+  GK_TRY(gk_workspace(h, WS_FACTOR, 2 * (size_t)(m + n) * NB * sizeof(_Float16), &ws));
+  _Float16* Ah = static_cast<_Float16*>(ws);
+  _Float16* Al = Ah + (size_t)m * NB;
+  _Float16* Bh = Al + (size_t)m * NB;
+  _Float16* Bl = Bh + (size_t)n * NB;
+  GK_TRY(split_h3(A, m, Ah, Al, st));
+  GK_TRY(split_h3(B, n, Bh, Bl, st));
+  return syrk_h3_launch(m, n, Ah, Al, Bh, Bl, C, ldc, lower != 0, st);
 }
 
 extern "C" int gk_cholesky_f32(gk_handle h, float* A, int64_t n, int64_t lda, float* dinv, int32_t* info,

@@ -76,6 +76,10 @@ struct gk_context {
 // Takes the device address of the evaluation scratch array for h's device
 // (called by gk_create with that device current).
 int gk_eval_scratch_init(gk_handle h);
+// FP32 Cholesky whose rank-256 updates run on the split-fp16 MFMA product
+// (gk_dense.hip); for unit-diagonal matrices (factor entries within [-16, 16]).
+extern "C" int gk_cholesky_f32_split16(gk_handle h, float* A, int64_t n, int64_t lda, float* dinv, int32_t* info,
+                                       gk_stream stream);
 
 // Depth of the chain step of the triangular solves: it multiplies the planes
 // Dinv_k L_{k,k-d} (forward) and L_{k+d,k} Dinv_k (backward) for d = 1..depth;

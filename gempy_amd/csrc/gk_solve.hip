@@ -10,6 +10,7 @@
 //   solve falls back to the FP64 LU.
 #include <cmath>
 #include <cstdlib>
+#include <string>
 #include <vector>
 #include <algorithm>
 #include "gk_internal.h"
@@ -183,6 +184,16 @@ __global__ void get_col_kernel(const double* U, int Lt, int l, double* dst, int6
 }
 
 // GK_VERBOSE=1 in the environment prints the refinement's residual history
+// GK_FACTOR_UPDATE=fp32 runs the FP32 form's rank-256 updates on the FP32
+// MFMA GEMM instead of the split-fp16 product.
+inline bool factor_update_fp32() {
+  static const bool v = [] {
+    const char* e = std::getenv("GK_FACTOR_UPDATE");
+    return e && std::string(e) == "fp32";
+  }();
+  return v;
+}
+
 inline bool gk_verbose() {
   static const bool v = [] { const char* e = getenv("GK_VERBOSE"); return e && *e && *e != '0'; }();
   return v;
@@ -400,7 +411,12 @@ struct Solver {
       } else {
         hipLaunchKernelGGL(add_diag_kernel<float>, dim3(nblk(Nc)), dim3(256), 0, st, static_cast<float*>(H),
                            Ncpad, Nc, (float)shift);
-        rc = gk_cholesky_f32(h, static_cast<float*>(H), Nc, Ncpad, static_cast<float*>(dinv), &info, st);
+        // the equilibrated block has a unit diagonal: its factor's entries
+        // lie within [-1.01, 1.01], inside the split-fp16 update's range
+        rc = factor_update_fp32()
+                 ? gk_cholesky_f32(h, static_cast<float*>(H), Nc, Ncpad, static_cast<float*>(dinv), &info, st)
+                 : gk_cholesky_f32_split16(h, static_cast<float*>(H), Nc, Ncpad, static_cast<float*>(dinv), &info,
+                                           st);
       }
       if (gk_verbose())
         fprintf(stderr, "[gk] %s Cholesky of the covariance block, shift %.3e: %s (info %d)\n", f64 ? "FP64" : "FP32",
@@ -532,7 +548,11 @@ struct Solver {
                            V + (int64_t)(j + 1) * Npad, N);
         ++total;
         ++j;
-        if (!(j == m || total >= max_iter || j % check == 0)) continue;
+        // Between checks, past the first two, the Hessenberg column is also
+        // rotated on the host after every iteration ("peek"): the true residual
+        // is then checked as soon as the Arnoldi estimate reaches tol.
+        const bool peek = total >= 2 * check;
+        if (!(j == m || total >= max_iter || j % check == 0 || peek)) continue;
         // rotate the new columns (j - jdone of them) on the host
         GK_HIP(hipMemcpyAsync(hv_cols.data(), hcol + (int64_t)jdone * (m + 2),
                               (size_t)(j - jdone) * (m + 2) * sizeof(double), hipMemcpyDeviceToHost, st));
@@ -563,6 +583,7 @@ struct Solver {
         }
         jdone = j;
         const bool full = (j == m) || hn == 0.0 || total >= max_iter;
+        if (!full && j % check != 0 && std::fabs(g[j]) / nb > tol) continue;   // a peek above tol
         {
           GK_TRY(combine(j, xt));
           double rel = 0.0;

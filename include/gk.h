@@ -387,8 +387,10 @@ int gk_solve_spai(gk_handle h, const gk_system* sys, const int32_t* idx, const f
  * dense kernels (process-wide, cleared on enable; 0 stops).  gk_profile_read
  * sums the recorded launches of one kind: 0..7 = MFMA GEMM with
  * kind = 4*(FP64) + 2*(B transposed) + (lower tiles only), 8 / 9 = the FP32 /
- * FP64 triangular solves (one record per L^-T L^-1 b); `work` = algorithmic
- * flops (GEMM: 2mnk, lower: m(m+1)k) or factor bytes read (solves). */
+ * FP64 triangular solves (one record per L^-T L^-1 b), 10 / 11 = the
+ * split-fp16 rank-256 update of the FP32 factorization (full / lower tiles,
+ * three fp16 MFMA products per product); `work` = algorithmic flops (GEMM:
+ * 2mnk, lower: m(m+1)k) or factor bytes read (solves). */
 int gk_profile(gk_handle h, int32_t enable);
 int gk_profile_read(gk_handle h, int32_t kind, int64_t* launches, double* ms, double* work);
 
@@ -400,6 +402,12 @@ int gk_profile_read(gk_handle h, int32_t kind, int64_t* launches, double* ms, do
  * gk_solve forms, gk_solve_rhs) return GK_ERR_HIP and reset it.  Needs one
  * triangular solve to have run on the handle. */
 int gk_test_trsv_set_error(gk_handle h, int32_t code, gk_stream stream);
+/* Testing: the split-fp16 rank-256 update the FP32 factorization uses,
+ * C[m x n] -= A B^T with A (m x 256) and B (n x 256) dense FP32 rows (entries
+ * within [-16, 16]), m, n multiples of 128; lower: tiles on or below the
+ * diagonal only (m == n).  No counterpart in the reference. */
+int gk_test_syrk_split16(gk_handle h, int64_t m, int64_t n, const float* A, const float* B, float* C, int64_t ldc,
+                         int32_t lower, gk_stream stream);
 
 #ifdef __cplusplus
 }
