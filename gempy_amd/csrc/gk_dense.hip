@@ -392,8 +392,19 @@ int gemm_launch(int64_t m, int64_t n, int64_t k, const T* A, int64_t lda, const 
 typedef _Float16 h8v __attribute__((ext_vector_type(8)));
 typedef _Float16 h2v __attribute__((ext_vector_type(2)));
 
-__global__ void split_h3_kernel(const float* __restrict__ W, int64_t n8, h8v* __restrict__ Wh,
-                                h8v* __restrict__ Wl) {
+constexpr int H3_K = 256;       // depth of the update (panel width)
+constexpr int H3_BK = 32;       // k per LDS stage (one MFMA)
+constexpr int H3_NK = H3_K / H3_BK;
+constexpr int H3_TILE = 128 * H3_BK;          // halves of one array of one stage of a 128-row block
+constexpr int64_t H3_ROWBLK = 2 * 128 * H3_K; // halves of a 128-row block (both arrays, all stages)
+
+// Split panel layout ("tiled"): for 128-row block b and k-chunk c (32 wide),
+// the hi part (128 x 32, row-major) then the lo part, so that one stage of a
+// tile's operand is one contiguous 16 KB block, copied to LDS as it lies.
+// Element (r, k): ((r / 128) * H3_NK + k / 32) * 2 H3_TILE + [lo] H3_TILE
+// + (r % 128) * 32 + k % 32.  A row offset R (a multiple of 128) is R * 2 H3_K
+// halves.
+__global__ void split_h3_kernel(const float* __restrict__ W, int64_t n8, _Float16* __restrict__ P) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n8) return;
   const f4v a = reinterpret_cast<const f4v*>(W)[2 * i];
@@ -406,21 +417,24 @@ __global__ void split_h3_kernel(const float* __restrict__ W, int64_t n8, h8v* __
     lo[e] = (_Float16)((a[e] - (float)hi[e]) * 2048.0f);
     lo[e + 4] = (_Float16)((b[e] - (float)hi[e + 4]) * 2048.0f);
   }
-  Wh[i] = hi;
-  Wl[i] = lo;
+  const int64_t r = i / (H3_K / 8);
+  const int kv = (int)(i % (H3_K / 8)), c = kv >> 2, kk = (kv & 3) * 8;
+  _Float16* base = P + ((r >> 7) * H3_NK + c) * 2 * H3_TILE + (r & 127) * H3_BK + kk;
+  *reinterpret_cast<h8v*>(base) = hi;
+  *reinterpret_cast<h8v*>(base + H3_TILE) = lo;
 }
 
-constexpr int H3_K = 256;       // depth of the update (panel width)
-constexpr int H3_BK = 32;       // k per LDS stage (one MFMA)
-constexpr int H3_LD = H3_BK;      // LDS row stride in halves: a fragment read (16 rows x 4 vectors)
-                                  // and a stage store both cover 1 KB contiguously
+#ifndef H3_RS
+#define H3_RS 2                 // stages in flight in registers
+#endif
 
-// 128 x 128 tile, 4 waves of 64 x 64 (4 x 4 fragments of 16 x 16).
+// 128 x 128 tile, 4 waves of 64 x 64 (4 x 4 fragments of 16 x 16).  A, B:
+// tiled split panels at their first row.
 template <bool LOWER>
-__global__ __launch_bounds__(256, 2) void syrk_h3_kernel(
-    const _Float16* __restrict__ Ah, const _Float16* __restrict__ Al, const _Float16* __restrict__ Bh,
-    const _Float16* __restrict__ Bl, float* C, int64_t ldc, int64_t tiles_n) {
-  __shared__ __attribute__((aligned(16))) _Float16 S[2][4][128 * H3_LD];   // [buf][Ah, Al, Bh, Bl]
+__global__ __launch_bounds__(256, 2) void syrk_h3_kernel(const _Float16* __restrict__ A,
+                                                         const _Float16* __restrict__ B, float* C, int64_t ldc,
+                                                         int64_t tiles_n) {
+  __shared__ __attribute__((aligned(16))) _Float16 S[2][4 * H3_TILE];   // [buf][A hi | A lo | B hi | B lo]
   int64_t bid = blockIdx.x;
   const int64_t nblk = gridDim.x;
   if (nblk % 8 == 0) bid = (bid % 8) * (nblk / 8) + bid / 8;      // contiguous tiles per XCD
@@ -438,7 +452,8 @@ __global__ __launch_bounds__(256, 2) void syrk_h3_kernel(
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave >> 1, wn = wave & 1;
   const int lrow = lane & 15, lk = lane >> 4;
-  const _Float16* src[4] = {Ah + m0 * H3_K, Al + m0 * H3_K, Bh + n0 * H3_K, Bl + n0 * H3_K};
+  const h8v* srcA = reinterpret_cast<const h8v*>(A + tm * H3_ROWBLK);
+  const h8v* srcB = reinterpret_cast<const h8v*>(B + tn * H3_ROWBLK);
 
   // The accumulators start at -2^11 C (exact scaling), so the epilogue only
   // stores: C - 2^-11 sum = -2^-11 (-2^11 C + sum).  The C loads are issued
@@ -457,62 +472,65 @@ __global__ __launch_bounds__(256, 2) void syrk_h3_kernel(
         acc[i][j][r] = -2048.0f * C[row * ldc + col];
       }
 
-  // one stage: 4 arrays x 128 rows x 32 halves = 2048 vectors of 8 halves, 8
-  // per thread; two register sets keep two stages in flight
-  h8v st_reg[2][8];
-  auto load_global = [&](int set, int k0) {
+  // one stage = A's 16 KB block then B's: 2048 vectors of 8 halves, 8 per
+  // thread, contiguous in memory and in LDS; H3_RS register sets keep H3_RS
+  // stages in flight
+  h8v st_reg[H3_RS][8];
+  auto load_global = [&](int set, int c) {
 #pragma unroll
     for (int it = 0; it < 8; ++it) {
-      const int idx = tid + 256 * it;          // array it/2, row (idx % 512) / 4, vector idx % 4
-      const int arr = it >> 1, row = (idx & 511) >> 2, cv = idx & 3;
-      st_reg[set][it] = *reinterpret_cast<const h8v*>(src[arr] + (int64_t)row * H3_K + k0 + cv * 8);
+      const int v = tid + 256 * it;           // 0..1023: A, 1024..2047: B
+      st_reg[set][it] = it < 4 ? srcA[c * (2 * H3_TILE / 8) + v] : srcB[c * (2 * H3_TILE / 8) + v - 1024];
     }
   };
   auto store_lds = [&](int buf, int set) {
 #pragma unroll
-    for (int it = 0; it < 8; ++it) {
-      const int idx = tid + 256 * it;
-      const int arr = it >> 1, row = (idx & 511) >> 2, cv = idx & 3;
-      *reinterpret_cast<h8v*>(&S[buf][arr][row * H3_LD + cv * 8]) = st_reg[set][it];
-    }
+    for (int it = 0; it < 8; ++it) reinterpret_cast<h8v*>(S[buf])[tid + 256 * it] = st_reg[set][it];
   };
   auto multiply = [&](int buf) {
+    const _Float16* SA = S[buf];
+    const _Float16* SB = S[buf] + 2 * H3_TILE;
     h8v ah[4], al[4], as[4], bh[4], bl[4];
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-      const int r = (wm * 64 + i * 16 + lrow) * H3_LD + lk * 8;
-      ah[i] = *reinterpret_cast<const h8v*>(&S[buf][0][r]);
-      al[i] = *reinterpret_cast<const h8v*>(&S[buf][1][r]);
+      const int r = (wm * 64 + i * 16 + lrow) * H3_BK + lk * 8;
+      ah[i] = *reinterpret_cast<const h8v*>(SA + r);
+      al[i] = *reinterpret_cast<const h8v*>(SA + H3_TILE + r);
       as[i] = ah[i] * (_Float16)2048.0f;
     }
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
-      const int r = (wn * 64 + j * 16 + lrow) * H3_LD + lk * 8;
-      bh[j] = *reinterpret_cast<const h8v*>(&S[buf][2][r]);
-      bl[j] = *reinterpret_cast<const h8v*>(&S[buf][3][r]);
+      const int r = (wn * 64 + j * 16 + lrow) * H3_BK + lk * 8;
+      bh[j] = *reinterpret_cast<const h8v*>(SB + r);
+      bl[j] = *reinterpret_cast<const h8v*>(SB + H3_TILE + r);
     }
+    // term by term over all 16 fragments: an accumulator's next product is
+    // issued 16 MFMAs after its previous one
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(as[i], bh[j], acc[i][j], 0, 0, 0);
-        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[i], bl[j], acc[i][j], 0, 0, 0);
-        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al[i], bh[j], acc[i][j], 0, 0, 0);
-      }
+      for (int j = 0; j < 4; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(as[i], bh[j], acc[i][j], 0, 0, 0);
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[i], bl[j], acc[i][j], 0, 0, 0);
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al[i], bh[j], acc[i][j], 0, 0, 0);
   };
 
-  // stage kt: registers set kt & 1, then LDS buffer kt & 1 (loop fully
+  // stage kt: registers set kt % H3_RS, then LDS buffer kt & 1 (loop fully
   // unrolled so the register sets are static)
-  constexpr int NK = H3_K / H3_BK;
-  load_global(0, 0);
-  load_global(1, H3_BK);
+#pragma unroll
+  for (int q = 0; q < H3_RS; ++q) load_global(q, q);
   store_lds(0, 0);
   __syncthreads();
 #pragma unroll
-  for (int kt = 0; kt < NK; ++kt) {
-    if (kt + 2 < NK) load_global(kt & 1, (kt + 2) * H3_BK);
+  for (int kt = 0; kt < H3_NK; ++kt) {
+    if (kt + H3_RS < H3_NK) load_global(kt % H3_RS, kt + H3_RS);
     multiply(kt & 1);
-    if (kt + 1 < NK) store_lds((kt + 1) & 1, (kt + 1) & 1);
+    if (kt + 1 < H3_NK) store_lds((kt + 1) & 1, (kt + 1) % H3_RS);
     __syncthreads();
   }
 #pragma unroll
@@ -527,20 +545,21 @@ __global__ __launch_bounds__(256, 2) void syrk_h3_kernel(
       }
 }
 
-// Split an FP32 panel (rows x 256, dense) into its fp16 hi / lo parts.
-int split_h3(const float* W, int64_t rows, _Float16* Wh, _Float16* Wl, hipStream_t st) {
+// Split an FP32 panel (rows x 256, dense, rows a multiple of 128) into the
+// tiled fp16 layout.
+int split_h3(const float* W, int64_t rows, _Float16* P, hipStream_t st) {
+  GK_REQUIRE(rows % 128 == 0, GK_ERR_ARG, "split_h3: rows must be a multiple of 128");
   const int64_t n8 = rows * H3_K / 8;
   if (n8 > 0)
-    hipLaunchKernelGGL(split_h3_kernel, dim3((unsigned)gk_ceil_div(n8, 256)), dim3(256), 0, st, W, n8,
-                       reinterpret_cast<h8v*>(Wh), reinterpret_cast<h8v*>(Wl));
+    hipLaunchKernelGGL(split_h3_kernel, dim3((unsigned)gk_ceil_div(n8, 256)), dim3(256), 0, st, W, n8, P);
   GK_HIP(hipGetLastError());
   return GK_OK;
 }
 
-// C[m x n] -= A B^T (K = 256) from split panels: A rows (m), B rows (n), each
-// hi / lo with row stride 256; lower: only the tiles on or below the diagonal.
-int syrk_h3_launch(int64_t m, int64_t n, const _Float16* Ah, const _Float16* Al, const _Float16* Bh,
-                   const _Float16* Bl, float* C, int64_t ldc, bool lower, hipStream_t st) {
+// C[m x n] -= A B^T (K = 256) from tiled split panels (A: m rows, B: n rows);
+// lower: only the tiles on or below the diagonal.
+int syrk_h3_launch(int64_t m, int64_t n, const _Float16* A, const _Float16* B, float* C, int64_t ldc, bool lower,
+                   hipStream_t st) {
   GK_REQUIRE(m % 128 == 0 && n % 128 == 0 && ldc >= n && (!lower || m == n), GK_ERR_ARG,
              "syrk_h3: m, n must be multiples of 128 (m == n for the lower update)");
   if (m == 0 || n == 0) return GK_OK;
@@ -549,9 +568,9 @@ int syrk_h3_launch(int64_t m, int64_t n, const _Float16* Ah, const _Float16* Al,
   const int64_t tm = m / 128, tn = n / 128;
   const int64_t tiles = lower ? tm * (tm + 1) / 2 : tm * tn;
   if (lower)
-    hipLaunchKernelGGL(syrk_h3_kernel<true>, dim3((unsigned)tiles), dim3(256), 0, st, Ah, Al, Bh, Bl, C, ldc, tn);
+    hipLaunchKernelGGL(syrk_h3_kernel<true>, dim3((unsigned)tiles), dim3(256), 0, st, A, B, C, ldc, tn);
   else
-    hipLaunchKernelGGL(syrk_h3_kernel<false>, dim3((unsigned)tiles), dim3(256), 0, st, Ah, Al, Bh, Bl, C, ldc, tn);
+    hipLaunchKernelGGL(syrk_h3_kernel<false>, dim3((unsigned)tiles), dim3(256), 0, st, A, B, C, ldc, tn);
   if (prof)
     gk_prof_end(st, pa, 10 + (lower ? 1 : 0),
                 lower ? (double)m * (double)(m + 1) * (double)H3_K : 2.0 * (double)m * (double)n * (double)H3_K);
@@ -1046,16 +1065,16 @@ struct CholRec {
 // One FP32 panel of the look-ahead factorization, on the side stream.  The
 // part the next trailing update waits for: factor and invert diagonal block kb
 // and solve the panel, W = A21 Dinv_kb^T.
-// With Wh / Wl given, the panel is also split into fp16 hi / lo parts for the
-// split-fp16 updates.
+// With P given, the panel is also split into its tiled fp16 hi / lo form for
+// the split-fp16 updates.
 static int chol32_panel(float* A, int64_t lda, int64_t kb, float* dinv, int32_t* dinfo, float* W, float* Tb,
-                        hipStream_t s, _Float16* Wh = nullptr, _Float16* Wl = nullptr) {
+                        hipStream_t s, _Float16* P = nullptr) {
   const int64_t k0 = kb * NB, rem = lda - k0 - NB;
   GK_TRY(diag256_chol<float>(A, lda, k0, dinv, dinfo, Tb, Tb + DB * DB, s, false));
   if (rem > 0) {
     GK_TRY(gemm_launch<float>(rem, NB, NB, A + (k0 + NB) * lda + k0, lda, dinv + k0 * NB, NB, true, 1.0f, 0.0f, W,
                               NB, false, s));
-    if (Wh) GK_TRY(split_h3(W, rem, Wh, Wl, s));
+    if (P) GK_TRY(split_h3(W, rem, P, s));
   }
   return GK_OK;
 }
@@ -1112,7 +1131,7 @@ static int cholesky_impl(gk_handle h, T* A, int64_t n, int64_t lda, T* dinv, int
   int32_t* dinfo = static_cast<int32_t*>(ws);
   T* W = reinterpret_cast<T*>(static_cast<char*>(ws) + 256);       // two panel buffers (FP32 look-ahead)
   T* Tb = W + 2 * (size_t)lda * NB;
-  // split panels: [buffer 0 hi | buffer 0 lo | buffer 1 hi | buffer 1 lo], lda x NB halves each
+  // two tiled split panels of lda x NB x 2 halves
   _Float16* Hs = split16 ? reinterpret_cast<_Float16*>(Tb + 2 * DB * DB) : nullptr;
   GK_HIP(hipMemsetAsync(dinfo, 0, sizeof(int32_t), st));
   GK_HIP(hipMemsetAsync(dinv, 0, (size_t)lda * NB * sizeof(T), st));   // Dinv plane: upper-right quadrants stay 0
@@ -1141,12 +1160,11 @@ static int cholesky_impl(gk_handle h, T* A, int64_t n, int64_t lda, T* dinv, int
       hipEvent_t* ev;
       GK_TRY(gk_side_stream(h, &s2, &ev));
       T* Wb[2] = {W, W + (size_t)lda * NB};
-      _Float16* Hb[2] = {Hs, Hs ? Hs + 2 * (size_t)lda * NB : nullptr};     // hi; lo follows at + lda NB
-      auto hi = [&](int64_t kb) { return Hb[kb & 1]; };
-      auto lo = [&](int64_t kb) { return Hb[kb & 1] ? Hb[kb & 1] + (size_t)lda * NB : nullptr; };
+      _Float16* Hb[2] = {Hs, Hs ? Hs + 2 * (size_t)lda * NB : nullptr};
+      auto sp = [&](int64_t kb) { return Hb[kb & 1]; };
       GK_HIP(hipEventRecord(ev[0], st));
       GK_HIP(hipStreamWaitEvent(s2, ev[0], 0));
-      if (nb > 0) GK_TRY(chol32_panel(A, lda, 0, dinv, dinfo, Wb[0], Tb, s2, hi(0), lo(0)));
+      if (nb > 0) GK_TRY(chol32_panel(A, lda, 0, dinv, dinfo, Wb[0], Tb, s2, sp(0)));
       GK_HIP(hipEventRecord(ev[1], s2));                    // panel 0 factored
       if (nb > 0) GK_TRY(chol32_panel_post(A, lda, 0, Wb[0], Tb, s2));
       for (int64_t kb = 0; kb + 1 < nb; ++kb) {
@@ -1155,20 +1173,19 @@ static int cholesky_impl(gk_handle h, T* A, int64_t n, int64_t lda, T* dinv, int
         GK_HIP(hipStreamWaitEvent(st, ev[1], 0));           // panel kb factored, Wk = its rows below
         // next panel's columns: A[k0+NB:, k0+NB : k0+2NB] -= Wk Wk[0:NB]^T
         if (split16)
-          GK_TRY(syrk_h3_launch(rem, NB, hi(kb), lo(kb), hi(kb), lo(kb), A + (k0 + NB) * lda + k0 + NB, lda, false,
-                                st));
+          GK_TRY(syrk_h3_launch(rem, NB, sp(kb), sp(kb), A + (k0 + NB) * lda + k0 + NB, lda, false, st));
         else
           GK_TRY(gemm_launch<T>(rem, NB, NB, Wk, NB, Wk, NB, true, T(-1), T(1), A + (k0 + NB) * lda + k0 + NB, lda,
                                 false, st));
         GK_HIP(hipEventRecord(ev[0], st));
         GK_HIP(hipStreamWaitEvent(s2, ev[0], 0));
-        GK_TRY(chol32_panel(A, lda, kb + 1, dinv, dinfo, Wb[(kb + 1) & 1], Tb, s2, hi(kb + 1), lo(kb + 1)));
+        GK_TRY(chol32_panel(A, lda, kb + 1, dinv, dinfo, Wb[(kb + 1) & 1], Tb, s2, sp(kb + 1)));
         GK_HIP(hipEventRecord(ev[1], s2));
         GK_TRY(chol32_panel_post(A, lda, kb + 1, Wb[(kb + 1) & 1], Tb, s2));
         if (rem > NB) {                                     // the rest of the trailing update (lower)
           if (split16)
-            GK_TRY(syrk_h3_launch(rem - NB, rem - NB, hi(kb) + NB * NB, lo(kb) + NB * NB, hi(kb) + NB * NB,
-                                  lo(kb) + NB * NB, A + (k0 + 2 * NB) * lda + k0 + 2 * NB, lda, true, st));
+            GK_TRY(syrk_h3_launch(rem - NB, rem - NB, sp(kb) + NB * 2 * H3_K, sp(kb) + NB * 2 * H3_K,
+                                  A + (k0 + 2 * NB) * lda + k0 + 2 * NB, lda, true, st));
           else
             GK_TRY(gemm_launch<T>(rem - NB, rem - NB, NB, Wk + NB * NB, NB, Wk + NB * NB, NB, true, T(-1), T(1),
                                   A + (k0 + 2 * NB) * lda + k0 + 2 * NB, lda, true, st));
@@ -1206,13 +1223,11 @@ extern "C" int gk_test_syrk_split16(gk_handle h, int64_t m, int64_t n, const flo
   hipStream_t st = (hipStream_t)stream;
   void* ws = nullptr;
   GK_TRY(gk_workspace(h, WS_FACTOR, 2 * (size_t)(m + n) * NB * sizeof(_Float16), &ws));
-  _Float16* Ah = static_cast<_Float16*>(ws);
-  _Float16* Al = Ah + (size_t)m * NB;
-  _Float16* Bh = Al + (size_t)m * NB;
-  _Float16* Bl = Bh + (size_t)n * NB;
-  GK_TRY(split_h3(A, m, Ah, Al, st));
-  GK_TRY(split_h3(B, n, Bh, Bl, st));
-  return syrk_h3_launch(m, n, Ah, Al, Bh, Bl, C, ldc, lower != 0, st);
+  _Float16* Ap = static_cast<_Float16*>(ws);
+  _Float16* Bp = Ap + 2 * (size_t)m * NB;
+  GK_TRY(split_h3(A, m, Ap, st));
+  GK_TRY(split_h3(B, n, Bp, st));
+  return syrk_h3_launch(m, n, Ap, Bp, C, ldc, lower != 0, st);
 }
 
 extern "C" int gk_cholesky_f32(gk_handle h, float* A, int64_t n, int64_t lda, float* dinv, int32_t* info,
