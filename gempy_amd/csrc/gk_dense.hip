@@ -671,7 +671,7 @@ __global__ __launch_bounds__(1024) void getrf128_kernel(double* A, int64_t lda, 
 // (all threads): the rank-PB trailing update of L and the eager update of the
 // later rows of X by the new panel and rows, in one sweep per row.
 #ifndef GK_POTRF_PB_F32
-#define GK_POTRF_PB_F32 8
+#define GK_POTRF_PB_F32 16
 #endif
 #ifndef GK_POTRF_PB_F64
 #define GK_POTRF_PB_F64 4
@@ -724,13 +724,22 @@ template <typename T, int PB, int NT>
 __device__ void potrf_blocked(BlkStore<T, PB> B, T* A, int64_t lda, T* X, int64_t ldx, int32_t* info, int64_t gbase) {
   constexpr int RS = NT / 32;                 // rows of the trailing update in flight
   const int tid = threadIdx.x, tx = tid & 31, ty = tid >> 5, lane = tid & 63, wave = tid >> 6;
-  for (int e = tid; e < DB * DB; e += NT) {
-    const int i = e / DB, c = e % DB;
+  // all loads of the block first (in flight together), then the LDS stores
+  constexpr int PER = DB * DB / NT;
+  static_assert(PER * NT == DB * DB, "the block must be a whole number of elements per thread");
+  T vin[PER];
+#pragma unroll
+  for (int q = 0; q < PER; ++q) {
+    const int e = tid + q * NT, i = e / DB, c = e % DB;
+    vin[q] = c <= i ? A[i * lda + c] : T(0);
+  }
+#pragma unroll
+  for (int q = 0; q < PER; ++q) {
+    const int e = tid + q * NT, i = e / DB, c = e % DB;
     if (c <= i) {
-      const T v = A[i * lda + c];
-      B.L(i, c) = v;
+      B.L(i, c) = vin[q];
       B.X(i, c) = T(0);
-      if (c < PB) B.P(0, c, i) = v;
+      if (c < PB) B.P(0, c, i) = vin[q];
     }
   }
   __syncthreads();
@@ -798,10 +807,63 @@ __device__ void potrf_blocked(BlkStore<T, PB> B, T* A, int64_t lda, T* X, int64_
       }
     }
     __syncthreads();
+    const int c0 = jb + PB;
+    if constexpr (sizeof(T) == 4 && PB == 16) {
+      // ---- phase 2 on MFMA (16 x 16 x 4 f32), one 16 x 16 tile per wave at a time:
+      //      L tiles (ti >= tj >= c0/16):  L[i][c] -= sum_t L[i][jb+t] L[c][jb+t]
+      //      X tiles (ti >= c0/16 > tj):   X[i][c] -= sum_t L[i][jb+t] X[jb+t][c]
+      // Operands from the column-major panel buffer P[pb] (and the packed X rows
+      // of the panel); a lane holds C[4 (lane/16) + r][lane % 16] of a tile.
+      const int a = c0 / 16, nt = DB / 16 - a;           // first trailing tile row, tile rows left
+      const int n_l = nt * (nt + 1) / 2, n_x = nt * a;
+      const int lr = lane & 15, lq = lane >> 4;
+      for (int tile = wave; tile < n_l + n_x; tile += NT / 64) {
+        int ti, tj;
+        const bool is_l = tile < n_l;
+        if (is_l) {                                       // lower triangle of the trailing tiles
+          int t = tile, r = 0;
+          while (t > r) { t -= r + 1; ++r; }
+          ti = a + r;
+          tj = a + t;
+        } else {
+          ti = a + (tile - n_l) / a;
+          tj = (tile - n_l) % a;
+        }
+        const int i0 = ti * 16, cb = tj * 16;
+        f4v acc;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int i = i0 + lq * 4 + r, c = cb + lr;
+          acc[r] = is_l ? (c <= i ? B.L(i, c) : 0.0f) : B.X(i, c);
+        }
+#pragma unroll
+        for (int ks = 0; ks < PB / 4; ++ks) {
+          const int t = ks * 4 + lq;
+          const float av = -B.P(pb, t, i0 + lr);          // A[row][k] = L[i0+lr][jb+t]
+          float bv;
+          if (is_l) bv = B.P(pb, t, cb + lr);             // B[k][col] = L[cb+lr][jb+t]
+          else bv = (cb + lr <= jb + t) ? B.X(jb + t, cb + lr) : 0.0f;
+          acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv, acc, 0, 0, 0);
+        }
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int i = i0 + lq * 4 + r, c = cb + lr;
+          if (is_l) {
+            if (c <= i) {
+              B.L(i, c) = acc[r];
+              if (c < c0 + PB) B.P(pb ^ 1, c - c0, i) = acc[r];
+            }
+          } else {
+            B.X(i, c) = acc[r];
+          }
+        }
+      }
+      __syncthreads();
+      continue;
+    }
     // ---- rows i >= c0: L[i][c] -= sum_t L[i][jb+t] L[c][jb+t]   (c0 <= c <= i;
     //      the next panel's columns also into P[pb^1])
     //                    -acc[i][c] -= sum_t L[i][jb+t] X[jb+t][c] (c < c0, X lower)
-    const int c0 = jb + PB;
     for (int i = c0 + ty; i < DB; i += RS) {
       T li[PB];
 #pragma unroll

@@ -76,6 +76,17 @@ struct gk_context {
 // Takes the device address of the evaluation scratch array for h's device
 // (called by gk_create with that device current).
 int gk_eval_scratch_init(gk_handle h);
+// Neighbours per data point of the FP32 form's exact local inverse (the
+// learned network keeps GK_SPAI_K); gk_learned.hip.
+#ifndef GK_LOCAL_K
+#define GK_LOCAL_K 32
+#endif
+extern "C" int gk_knn_local(gk_handle h, const gk_system* sys, int32_t* idx, float* hk, gk_stream stream);
+extern "C" int gk_local_inverse_local(gk_handle h, const gk_system* sys, const int32_t* idx, const double* dscale,
+                                      float* Y, gk_stream stream);
+extern "C" int gk_spai_apply_k(const gk_system* sys, int32_t k, const int32_t* idx, const float* Y, const double* u,
+                               double* y, hipStream_t st);
+
 // FP32 Cholesky whose rank-256 updates run on the split-fp16 MFMA product
 // (gk_dense.hip); for unit-diagonal matrices (factor entries within [-16, 16]).
 extern "C" int gk_cholesky_f32_split16(gk_handle h, float* A, int64_t n, int64_t lda, float* dinv, int32_t* info,

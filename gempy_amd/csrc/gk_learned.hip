@@ -37,6 +37,7 @@ __device__ __forceinline__ int64_t unknown_of(const gk_system& s, int64_t p, int
 }
 
 // ---------------------------------------------------------------- neighbours
+template <int K>
 __global__ __launch_bounds__(256) void knn_kernel(gk_system sys, int64_t n, int32_t* idx, float* hk) {
   __shared__ float sx[256], sy[256], sz[256];
   const int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x;
@@ -141,58 +142,116 @@ __global__ __launch_bounds__(256) void local_blocks_kernel(Geo G, int64_t nq, co
 // Exact local sparse approximate inverse: for point p, the rows of
 // H_loc^-1 that belong to p's own unknowns, as K 3x3 blocks Y[p][j] (the same
 // layout the network predicts).  H_loc (FP64, equilibrated, dummy slots
-// identity) is symmetric positive definite, so those rows are the columns
-// H_loc^-1 e_{3 self + r}: Gauss-Jordan without pivoting on [H_loc | three unit
-// columns] in LDS (the full inverse is not formed).
+// identity) is symmetric positive definite; its unknowns are ordered with p's
+// own three last, so with H_loc = L L^T the wanted columns H_loc^-1 e_s are
+// L^-T z with z = L^-1 e_s nonzero only in its last three entries (the inverse
+// of L's last 3x3 block).  Cholesky in LDS (LD^3/6 multiply-adds, a third of a
+// Gauss-Jordan elimination's), then one wave solves L^T x = z for the three
+// columns with broadcasts instead of barriers.
+template <int K>
 __global__ __launch_bounds__(256) void local_inverse_kernel(Geo G, int64_t n, const int32_t* idx,
                                                             const double* dscale, float* Y) {
+  constexpr int LD = 3 * K;
   const int64_t p = blockIdx.x;
-  constexpr int W = LD + 3;                   // columns of the augmented block
-  __shared__ double A[LD][W + 1];
+  __shared__ double A[LD][LD + 1];
   __shared__ int64_t un[LD];
-  __shared__ double piv;
   __shared__ int self;
-  if (threadIdx.x < LD) {
-    const int j = threadIdx.x / 3, d = threadIdx.x % 3;
-    un[threadIdx.x] = unknown_of(G.s, idx[p * K + j], d);
-  }
-  if (threadIdx.x == 0) {
+  const int tid = threadIdx.x;
+  if (tid == 0) {
     self = 0;
     for (int j = 0; j < K; ++j)
       if (idx[p * K + j] == p) { self = j; break; }
   }
   __syncthreads();
-  for (int e = threadIdx.x; e < LD * W; e += 256) {
-    const int r = e / W, c = e % W;
-    const int64_t ur = un[r];
-    double v;
-    if (c < LD) {
-      const int64_t uc = un[c];
-      v = (ur < 0 || uc < 0) ? (r == c ? 1.0 : 0.0) : entry(G, ur, uc) * dscale[ur] * dscale[uc];
-    } else {
-      v = r == 3 * self + (c - LD) ? 1.0 : 0.0;
-    }
-    A[r][c] = v;
+  // slot s < K - 1: neighbour s (s + 1 past self); slot K - 1: p itself
+  if (tid < LD) {
+    const int sl = tid / 3, d = tid % 3;
+    const int j = sl == K - 1 ? self : (sl < self ? sl : sl + 1);
+    un[tid] = unknown_of(G.s, idx[p * K + j], d);
+  }
+  __syncthreads();
+  for (int e = tid; e < LD * LD; e += 256) {
+    const int r = e / LD, c = e % LD;
+    if (c > r) continue;
+    const int64_t ur = un[r], uc = un[c];
+    A[r][c] = (ur < 0 || uc < 0) ? (r == c ? 1.0 : 0.0) : entry(G, ur, uc) * dscale[ur] * dscale[uc];
   }
   __syncthreads();
   for (int k = 0; k < LD; ++k) {
-    if (threadIdx.x == 0) piv = 1.0 / A[k][k];
+    if (tid == 0) A[k][k] = sqrt(A[k][k]);
     __syncthreads();
-    // scale row k (columns right of k; the pivot column is not read again)
-    for (int c = k + 1 + threadIdx.x; c < W; c += 256) A[k][c] *= piv;
+    const double ip = 1.0 / A[k][k];
+    for (int r = k + 1 + tid; r < LD; r += 256) A[r][k] *= ip;
     __syncthreads();
-    // eliminate column k from every other row
-    for (int e = threadIdx.x; e < LD * (W - k - 1); e += 256) {
-      const int r = e / (W - k - 1), c = k + 1 + e % (W - k - 1);
-      if (r == k) continue;
-      A[r][c] -= A[r][k] * A[k][c];
+    // trailing lower triangle: rows k+1+i, columns k+1+t, t <= i
+    const int m = LD - k - 1;
+    for (int e = tid; e < m * (m + 1) / 2; e += 256) {
+      int i = (int)((sqrtf(8.0f * (float)e + 1.0f) - 1.0f) * 0.5f);
+      while ((i + 1) * (i + 2) / 2 <= e) ++i;
+      while (i * (i + 1) / 2 > e) --i;
+      const int t = e - i * (i + 1) / 2;
+      const int r = k + 1 + i, c = k + 1 + t;
+      A[r][c] -= A[r][k] * A[c][k];
     }
     __syncthreads();
   }
-  // Y[p][j][3 r + c] = (H_loc^-1)[3 self + r][3 j + c] = X[3 j + c][r]
-  for (int e = threadIdx.x; e < K * 9; e += 256) {
-    const int j = e / 9, rc = e % 9, r = rc / 3, c = rc % 3;
-    Y[(p * K + j) * 9 + rc] = (float)A[3 * j + c][LD + r];
+  if (tid < 64) {
+    // z = L^-1 e_{LD-3+q}: only z[LD-3 .. LD-1] are nonzero (forward substitution
+    // on L's last 3x3 block), then x = L^-T z; lane l owns rows l, l + 64
+    constexpr int RPL = (LD + 63) / 64;
+    double x[RPL][3];
+    const int b0 = LD - 3;
+    const double l00 = A[b0][b0], l10 = A[b0 + 1][b0], l11 = A[b0 + 1][b0 + 1];
+    const double l20 = A[b0 + 2][b0], l21 = A[b0 + 2][b0 + 1], l22 = A[b0 + 2][b0 + 2];
+    double z[3][3];                      // z[q][t]: column q, entry b0 + t
+    for (int q = 0; q < 3; ++q) {
+      const double e0 = q == 0, e1 = q == 1, e2 = q == 2;
+      z[q][0] = e0 / l00;
+      z[q][1] = (e1 - l10 * z[q][0]) / l11;
+      z[q][2] = (e2 - l20 * z[q][0] - l21 * z[q][1]) / l22;
+    }
+#pragma unroll
+    for (int u = 0; u < RPL; ++u) {
+      const int r = tid + 64 * u;
+#pragma unroll
+      for (int q = 0; q < 3; ++q) x[u][q] = r >= b0 && r < LD ? z[q][r - b0] : 0.0;
+    }
+    for (int k = LD - 1; k >= 0; --k) {
+      const double ikk = 1.0 / A[k][k];
+      const int owner = k & 63, uo = k >> 6;
+      double xk[3];
+#pragma unroll
+      for (int q = 0; q < 3; ++q) {
+        double v = 0.0;
+#pragma unroll
+        for (int u = 0; u < RPL; ++u)
+          if (u == uo) v = x[u][q];
+        v = __shfl(v, owner) * ikk;          // x_k = (z_k - sum_{i>k} L_ik x_i) / L_kk
+        xk[q] = v;
+      }
+#pragma unroll
+      for (int u = 0; u < RPL; ++u) {
+        const int r = tid + 64 * u;
+        if (r == k) {
+#pragma unroll
+          for (int q = 0; q < 3; ++q) x[u][q] = xk[q];
+        } else if (r < k) {
+          const double lkr = A[k][r];
+#pragma unroll
+          for (int q = 0; q < 3; ++q) x[u][q] -= lkr * xk[q];
+        }
+      }
+    }
+    // Y[p][j][3 r + c] = (H_loc^-1)[3 self + r][3 j + c] = x_r[slot(j), c]
+#pragma unroll
+    for (int u = 0; u < RPL; ++u) {
+      const int row = tid + 64 * u;
+      if (row >= LD) continue;
+      const int sl = row / 3, c = row % 3;
+      const int j = sl == K - 1 ? self : (sl < self ? sl : sl + 1);
+#pragma unroll
+      for (int q = 0; q < 3; ++q) Y[((int64_t)p * K + j) * 9 + 3 * q + c] = (float)x[u][q];
+    }
   }
 }
 
@@ -357,6 +416,7 @@ __global__ void adam_kernel(float* params, const float* grad, float* m, float* v
 
 // -------------------------------------------------------------- SPAI apply
 // y = M u on the covariance unknowns (u, y FP64 of length Nc; blocks FP32).
+template <int K>
 __global__ void spai_apply_kernel(gk_system sys, int64_t n, const int32_t* idx, const float* Y, const double* u,
                                   double* y) {
   const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -388,8 +448,47 @@ extern "C" int gk_knn(gk_handle h, const gk_system* sys, int32_t* idx, float* hk
   GK_REQUIRE(h && sys && idx && hk, GK_ERR_ARG, "gk_knn: NULL argument");
   const int64_t n = sys->n_ori + sys->n_rest;
   GK_REQUIRE(n >= K, GK_ERR_ARG, "gk_knn: needs at least %d data points (got %lld)", K, (long long)n);
-  hipLaunchKernelGGL(knn_kernel, dim3((unsigned)gk_ceil_div(n, 256)), dim3(256), 0, (hipStream_t)stream, *sys, n,
+  hipLaunchKernelGGL(knn_kernel<K>, dim3((unsigned)gk_ceil_div(n, 256)), dim3(256), 0, (hipStream_t)stream, *sys, n,
                      idx, hk);
+  GK_HIP(hipGetLastError());
+  return GK_OK;
+}
+
+// The neighbourhoods, local inverses and products of the FP32 form's exact
+// local inverse, with GK_LOCAL_K neighbours (the network keeps GK_SPAI_K).
+extern "C" int gk_knn_local(gk_handle h, const gk_system* sys, int32_t* idx, float* hk, gk_stream stream) {
+  GK_REQUIRE(h && sys && idx && hk, GK_ERR_ARG, "gk_knn_local: NULL argument");
+  const int64_t n = sys->n_ori + sys->n_rest;
+  GK_REQUIRE(n >= GK_LOCAL_K, GK_ERR_ARG, "gk_knn_local: needs at least %d data points (got %lld)", GK_LOCAL_K,
+             (long long)n);
+  hipLaunchKernelGGL(knn_kernel<GK_LOCAL_K>, dim3((unsigned)gk_ceil_div(n, 256)), dim3(256), 0, (hipStream_t)stream,
+                     *sys, n, idx, hk);
+  GK_HIP(hipGetLastError());
+  return GK_OK;
+}
+
+extern "C" int gk_local_inverse_local(gk_handle h, const gk_system* sys, const int32_t* idx, const double* dscale,
+                                      float* Y, gk_stream stream) {
+  GK_REQUIRE(h && sys && idx && dscale && Y, GK_ERR_ARG, "gk_local_inverse_local: bad arguments");
+  const int64_t n = sys->n_ori + sys->n_rest;
+  if (n == 0) return GK_OK;
+  hipLaunchKernelGGL(local_inverse_kernel<GK_LOCAL_K>, dim3((unsigned)n), dim3(256), 0, (hipStream_t)stream,
+                     make_geo(sys), n, idx, dscale, Y);
+  GK_HIP(hipGetLastError());
+  return GK_OK;
+}
+
+extern "C" int gk_spai_apply_k(const gk_system* sys, int32_t k, const int32_t* idx, const float* Y, const double* u,
+                               double* y, hipStream_t st) {
+  const int64_t n = sys->n_ori + sys->n_rest;
+  if (n == 0) return GK_OK;
+  GK_REQUIRE(k == K || k == GK_LOCAL_K, GK_ERR_ARG, "gk_spai_apply: %d neighbours not supported", k);
+  if (k == GK_LOCAL_K)
+    hipLaunchKernelGGL(spai_apply_kernel<GK_LOCAL_K>, dim3((unsigned)gk_ceil_div(n, 256)), dim3(256), 0, st, *sys, n,
+                       idx, Y, u, y);
+  else
+    hipLaunchKernelGGL(spai_apply_kernel<K>, dim3((unsigned)gk_ceil_div(n, 256)), dim3(256), 0, st, *sys, n, idx, Y,
+                       u, y);
   GK_HIP(hipGetLastError());
   return GK_OK;
 }
@@ -419,7 +518,7 @@ extern "C" int gk_spai_local_inverse(gk_handle h, const gk_system* sys, const in
   GK_REQUIRE(h && sys && idx && dscale && Y, GK_ERR_ARG, "gk_spai_local_inverse: bad arguments");
   const int64_t n = sys->n_ori + sys->n_rest;
   if (n == 0) return GK_OK;
-  hipLaunchKernelGGL(local_inverse_kernel, dim3((unsigned)n), dim3(256), 0, (hipStream_t)stream, make_geo(sys), n,
+  hipLaunchKernelGGL(local_inverse_kernel<K>, dim3((unsigned)n), dim3(256), 0, (hipStream_t)stream, make_geo(sys), n,
                      idx, dscale, Y);
   GK_HIP(hipGetLastError());
   return GK_OK;
@@ -474,7 +573,8 @@ extern "C" int gk_spai_apply_impl(const gk_system* sys, const int32_t* idx, cons
                                   hipStream_t st) {
   const int64_t n = sys->n_ori + sys->n_rest;
   if (n == 0) return GK_OK;
-  hipLaunchKernelGGL(spai_apply_kernel, dim3((unsigned)gk_ceil_div(n, 256)), dim3(256), 0, st, *sys, n, idx, Y, u, y);
+  hipLaunchKernelGGL(spai_apply_kernel<K>, dim3((unsigned)gk_ceil_div(n, 256)), dim3(256), 0, st, *sys, n, idx, Y, u,
+                     y);
   GK_HIP(hipGetLastError());
   return GK_OK;
 }

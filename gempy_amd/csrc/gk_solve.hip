@@ -248,6 +248,7 @@ struct Solver {
   float* own_Y = nullptr;
   float* own_hk = nullptr;
   const int32_t* spai_idx = nullptr;
+  int32_t spai_k = GK_SPAI_K;        // neighbours per point of spai_idx / spai_Y
   const float* spai_Y = nullptr;
   int32_t mv_prec = GK_F64;          // precision of the Krylov matvec
   double box[6] = {0, 0, 0, 0, 0, 0}; // bounding box of the system's points (drops the operator's range mask)
@@ -338,10 +339,10 @@ struct Solver {
       if (pcp_sandwich) {         // y = P u;  y += M (u - H y);  y += P (u - H y)
         GK_TRY(factor_apply_f32(pc_u, pc_y, false));
         GK_TRY(scaled_residual(pc_u, pc_y, pc_v));
-        GK_TRY(gk_spai_apply_impl(sys, spai_idx, spai_Y, pc_v, ytmp2, st));
+        GK_TRY(gk_spai_apply_k(sys, spai_k, spai_idx, spai_Y, pc_v, ytmp2, st));
         hipLaunchKernelGGL(axpy_kernel, dim3(nblk(Nc)), dim3(256), 0, st, ytmp2, pc_y, Nc);
       } else {
-        GK_TRY(gk_spai_apply_impl(sys, spai_idx, spai_Y, pc_u, pc_y, st));
+        GK_TRY(gk_spai_apply_k(sys, spai_k, spai_idx, spai_Y, pc_u, pc_y, st));
       }
       GK_TRY(scaled_residual(pc_u, pc_y, pc_v));
       GK_TRY(factor_apply_f32(pc_v, pc_y, true));
@@ -351,7 +352,7 @@ struct Solver {
     if (spai) {
       double* u = static_cast<double*>(uvec);
       hipLaunchKernelGGL(to_scaled_kernel<double>, dim3(nblk(Nc)), dim3(256), 0, st, rin, dscale, u, Nc, Nc);
-      GK_TRY(gk_spai_apply_impl(sys, spai_idx, spai_Y, u, ytmp2, st));
+      GK_TRY(gk_spai_apply_k(sys, spai_k, spai_idx, spai_Y, u, ytmp2, st));
       hipLaunchKernelGGL(from_scaled_kernel<double>, dim3(nblk(Nc)), dim3(256), 0, st, ytmp2, dscale, yout, Nc);
       return GK_OK;
     }
@@ -738,8 +739,8 @@ static int init_solver(gk_handle h, const gk_system* sys, hipStream_t st, int fo
   const size_t o_px = take(pc ? Npad * sizeof(double) : 0);
   const size_t o_pw = take(pc ? Npad * sizeof(double) : 0);
   const size_t o_pf = take(pc ? S.Ncpad * sizeof(float) : 0);
-  const size_t o_si = take(pc ? (size_t)(sys->n_ori + sys->n_rest) * GK_SPAI_K * sizeof(int32_t) : 0);
-  const size_t o_sy = take(pc ? (size_t)(sys->n_ori + sys->n_rest) * GK_SPAI_K * 9 * sizeof(float) : 0);
+  const size_t o_si = take(pc ? (size_t)(sys->n_ori + sys->n_rest) * GK_LOCAL_K * sizeof(int32_t) : 0);
+  const size_t o_sy = take(pc ? (size_t)(sys->n_ori + sys->n_rest) * GK_LOCAL_K * 9 * sizeof(float) : 0);
   const size_t o_hk = take(pc ? (size_t)(sys->n_ori + sys->n_rest) * sizeof(float) : 0);
   void* p = nullptr;
   GK_TRY(gk_workspace(h, WS_SOLVER, need, &p));
@@ -837,21 +838,22 @@ static int solve_impl(gk_handle h, const gk_system* sys, int32_t form, double to
   // FP32 shifted factor + exact local-inverse correction, FP64 GMRES refinement
   GK_HIP(hipEventRecord(ev[0], st));
   GK_TRY(gk_diag_scale_impl(sys, S.dscale, st));
-  // The local-inverse correction needs GK_SPAI_K data points for its
+  // The local-inverse correction needs GK_LOCAL_K data points for its
   // neighbourhoods; smaller systems use the shifted factor alone (their shift,
   // 4 u32 sqrt(n), is tiny, and FGMRES removes it).
-  const bool local = sys->n_ori + sys->n_rest >= GK_SPAI_K;
+  const bool local = sys->n_ori + sys->n_rest >= GK_LOCAL_K;
   if (local) {               // neighbourhoods and local inverses overlap the factorization
     hipStream_t s3;
     hipEvent_t* aev;
     GK_TRY(gk_aux_stream(h, &s3, &aev));
     GK_HIP(hipEventRecord(aev[0], st));
     GK_HIP(hipStreamWaitEvent(s3, aev[0], 0));
-    GK_TRY(gk_knn(h, sys, S.own_idx, S.own_hk, (gk_stream)s3));
-    GK_TRY(gk_spai_local_inverse(h, sys, S.own_idx, S.dscale, S.own_Y, (gk_stream)s3));
+    GK_TRY(gk_knn_local(h, sys, S.own_idx, S.own_hk, (gk_stream)s3));
+    GK_TRY(gk_local_inverse_local(h, sys, S.own_idx, S.dscale, S.own_Y, (gk_stream)s3));
     GK_HIP(hipEventRecord(aev[1], s3));
     S.pc_ready = aev[1];
     S.spai_idx = S.own_idx;
+    S.spai_k = GK_LOCAL_K;
     S.spai_Y = S.own_Y;
   }
   S.pcp = local;
