@@ -22,6 +22,7 @@ GK_BLOCK = 256
 GK_CHOL_PLANES = 5
 GK_SPAI_WITH_FACTOR, GK_SPAI_NO_FALLBACK, GK_SPAI_SANDWICH = 1, 2, 4
 GK_SPAI_K, GK_SPAI_FEATURES, GK_SPAI_HIDDEN = 16, 8, 32
+GK_LOCAL_NEIGHBOURS = 32
 GK_EVAL_SCRATCH_BYTES = 32 << 20
 
 P = C.c_void_p
@@ -109,6 +110,7 @@ SIGNATURES = {
                                     C.POINTER(C.c_int64), P]),
     "gk_condition_number": (C.c_int, [P, C.POINTER(GkSystem), C.c_int32, C.c_double, C.POINTER(C.c_double),
                                       C.POINTER(C.c_double), C.POINTER(C.c_int32), P]),
+    "gk_test_knn_local": (C.c_int, [P, C.POINTER(GkSystem), P, P, P]),
     "gk_test_syrk_split16": (C.c_int, [P, C.c_int64, C.c_int64, P, P, P, C.c_int64, C.c_int32, P]),
     "gk_gravity_tz": (C.c_int, [P, C.c_int64, P, P, P, C.c_double, P, P]),
     "gk_gravity": (C.c_int, [P, C.c_int64, C.c_int64, P, P, C.c_int32, C.POINTER(C.c_double), P, P]),

@@ -49,6 +49,7 @@ enum gk_ws_slot {
   WS_ASSEMBLE,       // reference-point tables of the assembly
   WS_MESH,           // case tables, edge flags and scans of the isosurface extraction
   WS_LANCZOS,        // Lanczos basis of the condition-number estimate
+  WS_KNN,            // cells and sorted points of the grid neighbour search
   WS_COUNT
 };
 struct gk_context {
@@ -78,10 +79,9 @@ struct gk_context {
 int gk_eval_scratch_init(gk_handle h);
 // Neighbours per data point of the FP32 form's exact local inverse (the
 // learned network keeps GK_SPAI_K); gk_learned.hip.
-#ifndef GK_LOCAL_K
-#define GK_LOCAL_K 32
-#endif
-extern "C" int gk_knn_local(gk_handle h, const gk_system* sys, int32_t* idx, float* hk, gk_stream stream);
+#define GK_LOCAL_K GK_LOCAL_NEIGHBOURS
+extern "C" int gk_knn_local(gk_handle h, const gk_system* sys, const double* box, int32_t* idx, float* hk,
+                            gk_stream stream);
 extern "C" int gk_local_inverse_local(gk_handle h, const gk_system* sys, const int32_t* idx, const double* dscale,
                                       float* Y, gk_stream stream);
 extern "C" int gk_spai_apply_k(const gk_system* sys, int32_t k, const int32_t* idx, const float* Y, const double* u,

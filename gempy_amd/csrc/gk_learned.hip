@@ -13,6 +13,8 @@
 #include "gk_internal.h"
 #include "gk_entry.h"
 
+extern "C" int gk_source_box(gk_handle h, const gk_system* sys, double* box, gk_stream stream);
+
 using namespace gk_detail;
 
 namespace {
@@ -90,6 +92,140 @@ __global__ __launch_bounds__(256) void knn_kernel(gk_system sys, int64_t n, int3
   }
 }
 
+#ifndef GK_LI_GRID
+#define GK_LI_GRID 128          // workgroups of the FP32 form's local-inverse kernel
+#endif
+
+// ------------------------------------------------------- grid neighbours
+// The exact K nearest data points of every data point (itself first) through a
+// uniform grid of cells (~2 points per cell on average): a query scans cubes of
+// cells of growing radius r (only the new shell each time) until its K-th
+// distance is within r cell sizes, which every point closer than that lies
+// inside.  Ties are broken by the lower index, as in the brute-force search.
+struct KnnGrid {
+  float lo[3];
+  float inv_c, c;
+  int n[3];
+};
+
+__device__ __forceinline__ void norm_point(const gk_system& s, int64_t p, double ia, float* o) {
+  double x[3];
+  point_xyz(s, p, x);
+  o[0] = (float)((x[0] - s.center[0]) * ia);
+  o[1] = (float)((x[1] - s.center[1]) * ia);
+  o[2] = (float)((x[2] - s.center[2]) * ia);
+}
+
+__device__ __forceinline__ int cell_coord(float v, float lo, float inv_c, int n) {
+  const int c = (int)floorf((v - lo) * inv_c);
+  return c < 0 ? 0 : (c >= n ? n - 1 : c);
+}
+
+__global__ void knn_cells_kernel(gk_system sys, int64_t n, KnnGrid g, int32_t* cell, int32_t* count) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  float x[3];
+  norm_point(sys, p, 1.0 / sys.range, x);
+  const int cx = cell_coord(x[0], g.lo[0], g.inv_c, g.n[0]), cy = cell_coord(x[1], g.lo[1], g.inv_c, g.n[1]),
+            cz = cell_coord(x[2], g.lo[2], g.inv_c, g.n[2]);
+  const int c = (cz * g.n[1] + cy) * g.n[0] + cx;
+  cell[p] = c;
+  atomicAdd(count + c, 1);
+}
+
+// exclusive scan of count[0..nc) into start[0..nc] (one workgroup of 1024)
+__global__ __launch_bounds__(1024) void knn_scan_kernel(const int32_t* count, int32_t nc, int32_t* start) {
+  __shared__ int32_t part[1024];
+  const int t = threadIdx.x;
+  const int per = (nc + 1023) / 1024;
+  const int b = t * per, e = min(nc, b + per);
+  int32_t sum = 0;
+  for (int i = b; i < e; ++i) sum += count[i];
+  part[t] = sum;
+  __syncthreads();
+  for (int off = 1; off < 1024; off <<= 1) {
+    const int32_t v = t >= off ? part[t - off] : 0;
+    __syncthreads();
+    part[t] += v;
+    __syncthreads();
+  }
+  int32_t run = part[t] - sum;
+  for (int i = b; i < e; ++i) {
+    start[i] = run;
+    run += count[i];
+  }
+  if (t == 1023) start[nc] = part[1023];
+}
+
+__global__ void knn_scatter_kernel(gk_system sys, int64_t n, const int32_t* cell, const int32_t* start,
+                                   int32_t* fill, float4* spos) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const int c = cell[p];
+  const int slot = start[c] + atomicAdd(fill + c, 1);
+  float x[3];
+  norm_point(sys, p, 1.0 / sys.range, x);
+  spos[slot] = make_float4(x[0], x[1], x[2], __int_as_float((int)p));
+}
+
+template <int K>
+__global__ __launch_bounds__(256) void knn_grid_kernel(gk_system sys, int64_t n, KnnGrid g, const int32_t* start,
+                                                       const float4* spos, int32_t* idx, float* hk) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  float x[3];
+  norm_point(sys, p, 1.0 / sys.range, x);
+  const int cx = cell_coord(x[0], g.lo[0], g.inv_c, g.n[0]), cy = cell_coord(x[1], g.lo[1], g.inv_c, g.n[1]),
+            cz = cell_coord(x[2], g.lo[2], g.inv_c, g.n[2]);
+  float bd[K];
+  int32_t bi[K];
+#pragma unroll
+  for (int j = 0; j < K; ++j) { bd[j] = 3.4e38f; bi[j] = 0x7fffffff; }
+  const int rmax = max(g.n[0], max(g.n[1], g.n[2]));
+  for (int r = 0; r <= rmax; ++r) {
+    for (int dz = -r; dz <= r; ++dz) {
+      const int z = cz + dz;
+      if (z < 0 || z >= g.n[2]) continue;
+      for (int dy = -r; dy <= r; ++dy) {
+        const int y = cy + dy;
+        if (y < 0 || y >= g.n[1]) continue;
+        const bool inner = abs(dz) < r && abs(dy) < r;
+        for (int dx = -r; dx <= r; dx += (inner ? 2 * r : 1)) {   // only the shell of the cube
+          const int xx = cx + dx;
+          if (r > 0 && inner && dx != -r && dx != r) continue;
+          if (xx < 0 || xx >= g.n[0]) continue;
+          const int c = (z * g.n[1] + y) * g.n[0] + xx;
+          for (int s = start[c], e = start[c + 1]; s < e; ++s) {
+            const float4 q = spos[s];
+            const float ddx = q.x - x[0], ddy = q.y - x[1], ddz = q.z - x[2];
+            float d = fmaf(ddz, ddz, fmaf(ddy, ddy, ddx * ddx));
+            int32_t id = __float_as_int(q.w);
+            if (d < bd[K - 1] || (d == bd[K - 1] && id < bi[K - 1])) {
+#pragma unroll
+              for (int t = 0; t < K; ++t) {
+                if (d < bd[t] || (d == bd[t] && id < bi[t])) {
+                  const float td = bd[t];
+                  const int32_t ti = bi[t];
+                  bd[t] = d; bi[t] = id;
+                  d = td; id = ti;
+                }
+              }
+            }
+          }
+          if (r == 0) break;
+        }
+      }
+    }
+    // every point within r cell sizes of the query has been seen (a margin
+    // for the rounding of the cell coordinates)
+    const float reach = ((float)r - 1e-3f) * g.c;
+    if (bi[K - 1] != 0x7fffffff && bd[K - 1] <= reach * reach) break;
+  }
+#pragma unroll
+  for (int j = 0; j < K; ++j) idx[p * K + j] = bi[j];
+  hk[p] = sqrtf(bd[K - 1]);
+}
+
 // ------------------------------------------------------------------ features
 // f = [ (q-p)/h_P (3), |q-p|/h_P, P is orientation, Q is orientation, P == Q, 10 h_P ]
 __global__ void features_kernel(gk_system sys, int64_t nq, const int32_t* pts, const int32_t* idx,
@@ -152,11 +288,14 @@ template <int K>
 __global__ __launch_bounds__(256) void local_inverse_kernel(Geo G, int64_t n, const int32_t* idx,
                                                             const double* dscale, float* Y) {
   constexpr int LD = 3 * K;
-  const int64_t p = blockIdx.x;
   __shared__ double A[LD][LD + 1];
   __shared__ int64_t un[LD];
   __shared__ int self;
   const int tid = threadIdx.x;
+  // a fixed grid walks the points: the kernel holds at most gridDim.x CUs'
+  // worth of LDS, so a factorization running beside it keeps the rest
+  for (int64_t p = blockIdx.x; p < n; p += gridDim.x) {
+  __syncthreads();
   if (tid == 0) {
     self = 0;
     for (int j = 0; j < K; ++j)
@@ -252,6 +391,7 @@ __global__ __launch_bounds__(256) void local_inverse_kernel(Geo G, int64_t n, co
 #pragma unroll
       for (int q = 0; q < 3; ++q) Y[((int64_t)p * K + j) * 9 + 3 * q + c] = (float)x[u][q];
     }
+  }
   }
 }
 
@@ -456,15 +596,67 @@ extern "C" int gk_knn(gk_handle h, const gk_system* sys, int32_t* idx, float* hk
 
 // The neighbourhoods, local inverses and products of the FP32 form's exact
 // local inverse, with GK_LOCAL_K neighbours (the network keeps GK_SPAI_K).
-extern "C" int gk_knn_local(gk_handle h, const gk_system* sys, int32_t* idx, float* hk, gk_stream stream) {
-  GK_REQUIRE(h && sys && idx && hk, GK_ERR_ARG, "gk_knn_local: NULL argument");
+// box: bounding box of the data points in the system's coordinates
+// (xmin, ymin, zmin, xmax, ymax, zmax), as gk_source_box returns it.
+extern "C" int gk_knn_local(gk_handle h, const gk_system* sys, const double* box, int32_t* idx, float* hk,
+                            gk_stream stream) {
+  GK_REQUIRE(h && sys && box && idx && hk, GK_ERR_ARG, "gk_knn_local: NULL argument");
   const int64_t n = sys->n_ori + sys->n_rest;
   GK_REQUIRE(n >= GK_LOCAL_K, GK_ERR_ARG, "gk_knn_local: needs at least %d data points (got %lld)", GK_LOCAL_K,
              (long long)n);
-  hipLaunchKernelGGL(knn_kernel<GK_LOCAL_K>, dim3((unsigned)gk_ceil_div(n, 256)), dim3(256), 0, (hipStream_t)stream,
-                     *sys, n, idx, hk);
+  hipStream_t st = (hipStream_t)stream;
+  // grid in the normalised frame (x - centre) / range, cells of ~2 points
+  KnnGrid g{};
+  double ext[3], vol = 1.0;
+  for (int d = 0; d < 3; ++d) {
+    const double a = (box[d] - sys->center[d]) / sys->range, b = (box[3 + d] - sys->center[d]) / sys->range;
+    g.lo[d] = (float)a - 1e-6f;
+    ext[d] = std::max(b - a, 1e-9) + 2e-6;
+  }
+  const double emax = std::max(ext[0], std::max(ext[1], ext[2]));
+  for (int d = 0; d < 3; ++d) vol *= std::max(ext[d], emax * 1e-3);
+  double c = std::cbrt(vol * 2.0 / (double)n);
+  int64_t ncell = 0;
+  for (int it = 0; it < 64; ++it) {
+    ncell = 1;
+    for (int d = 0; d < 3; ++d) ncell *= std::max<int64_t>(1, (int64_t)std::ceil(ext[d] / c));
+    if (ncell <= 4 * n + 64) break;
+    c *= 1.25;
+  }
+  for (int d = 0; d < 3; ++d) g.n[d] = (int)std::max<int64_t>(1, (int64_t)std::ceil(ext[d] / c));
+  g.c = (float)c;
+  g.inv_c = (float)(1.0 / c);
+  const int32_t nc = g.n[0] * g.n[1] * g.n[2];
+  void* ws = nullptr;
+  const size_t need = gk_round_up((size_t)n * 4, 256) + 3 * gk_round_up(((size_t)nc + 1) * 4, 256) +
+                      (size_t)n * sizeof(float4);
+  GK_TRY(gk_workspace(h, WS_KNN, need, &ws));
+  char* b = static_cast<char*>(ws);
+  int32_t* cell = reinterpret_cast<int32_t*>(b);
+  b += gk_round_up((size_t)n * 4, 256);
+  int32_t* count = reinterpret_cast<int32_t*>(b);
+  b += gk_round_up(((size_t)nc + 1) * 4, 256);
+  int32_t* start = reinterpret_cast<int32_t*>(b);
+  b += gk_round_up(((size_t)nc + 1) * 4, 256);
+  int32_t* fill = reinterpret_cast<int32_t*>(b);
+  b += gk_round_up(((size_t)nc + 1) * 4, 256);
+  float4* spos = reinterpret_cast<float4*>(b);
+  GK_HIP(hipMemsetAsync(count, 0, (size_t)nc * 4, st));
+  GK_HIP(hipMemsetAsync(fill, 0, (size_t)nc * 4, st));
+  const unsigned nb = (unsigned)gk_ceil_div(n, 256);
+  hipLaunchKernelGGL(knn_cells_kernel, dim3(nb), dim3(256), 0, st, *sys, n, g, cell, count);
+  hipLaunchKernelGGL(knn_scan_kernel, dim3(1), dim3(1024), 0, st, count, nc, start);
+  hipLaunchKernelGGL(knn_scatter_kernel, dim3(nb), dim3(256), 0, st, *sys, n, cell, start, fill, spos);
+  hipLaunchKernelGGL(knn_grid_kernel<GK_LOCAL_K>, dim3(nb), dim3(256), 0, st, *sys, n, g, start, spos, idx, hk);
   GK_HIP(hipGetLastError());
   return GK_OK;
+}
+
+extern "C" int gk_test_knn_local(gk_handle h, const gk_system* sys, int32_t* idx, float* hk, gk_stream stream) {
+  GK_REQUIRE(h && sys && idx && hk, GK_ERR_ARG, "gk_test_knn_local: NULL argument");
+  double box[6];
+  GK_TRY(gk_source_box(h, sys, box, stream));
+  return gk_knn_local(h, sys, box, idx, hk, stream);
 }
 
 extern "C" int gk_local_inverse_local(gk_handle h, const gk_system* sys, const int32_t* idx, const double* dscale,
@@ -472,8 +664,9 @@ extern "C" int gk_local_inverse_local(gk_handle h, const gk_system* sys, const i
   GK_REQUIRE(h && sys && idx && dscale && Y, GK_ERR_ARG, "gk_local_inverse_local: bad arguments");
   const int64_t n = sys->n_ori + sys->n_rest;
   if (n == 0) return GK_OK;
-  hipLaunchKernelGGL(local_inverse_kernel<GK_LOCAL_K>, dim3((unsigned)n), dim3(256), 0, (hipStream_t)stream,
-                     make_geo(sys), n, idx, dscale, Y);
+  // a fixed grid (the work spreads over the factorization it runs beside)
+  hipLaunchKernelGGL(local_inverse_kernel<GK_LOCAL_K>, dim3((unsigned)std::min<int64_t>(n, GK_LI_GRID)), dim3(256), 0,
+                     (hipStream_t)stream, make_geo(sys), n, idx, dscale, Y);
   GK_HIP(hipGetLastError());
   return GK_OK;
 }

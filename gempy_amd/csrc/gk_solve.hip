@@ -848,7 +848,7 @@ static int solve_impl(gk_handle h, const gk_system* sys, int32_t form, double to
     GK_TRY(gk_aux_stream(h, &s3, &aev));
     GK_HIP(hipEventRecord(aev[0], st));
     GK_HIP(hipStreamWaitEvent(s3, aev[0], 0));
-    GK_TRY(gk_knn_local(h, sys, S.own_idx, S.own_hk, (gk_stream)s3));
+    GK_TRY(gk_knn_local(h, sys, S.box, S.own_idx, S.own_hk, (gk_stream)s3));
     GK_TRY(gk_local_inverse_local(h, sys, S.own_idx, S.dscale, S.own_Y, (gk_stream)s3));
     GK_HIP(hipEventRecord(aev[1], s3));
     S.pc_ready = aev[1];

@@ -406,6 +406,13 @@ int gk_test_trsv_set_error(gk_handle h, int32_t code, gk_stream stream);
  * C[m x n] -= A B^T with A (m x 256) and B (n x 256) dense FP32 rows (entries
  * within [-16, 16]), m, n multiples of 128; lower: tiles on or below the
  * diagonal only (m == n).  No counterpart in the reference. */
+/* Testing: the grid search for the FP32 form's local-inverse neighbourhoods,
+ * idx[p * GK_LOCAL_NEIGHBOURS + j] = the j-th nearest data point of data point
+ * p (itself first; data points = orientations then rest points), hk[p] = the
+ * distance of the last (in units of the range).  No counterpart in the
+ * reference. */
+#define GK_LOCAL_NEIGHBOURS 32
+int gk_test_knn_local(gk_handle h, const gk_system* sys, int32_t* idx, float* hk, gk_stream stream);
 int gk_test_syrk_split16(gk_handle h, int64_t m, int64_t n, const float* A, const float* B, float* C, int64_t ldc,
                          int32_t lower, gk_stream stream);
 
