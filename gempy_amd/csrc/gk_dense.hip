@@ -428,6 +428,37 @@ __global__ void split_h3_kernel(const float* __restrict__ W, int64_t n8, _Float1
 #define H3_RS 2                 // stages in flight in registers
 #endif
 
+// Tile (tm >= tn) of the lower triangle of nt x nt tiles for launch index b,
+// in supertiles of H3_SUPER x H3_SUPER tiles (lower triangle of supertiles,
+// row by row; inside one, column by column; the diagonal supertiles are
+// triangles): an XCD's concurrently resident workgroups (its contiguous range
+// of indices) then share a few A and B row blocks, which stay in its L2,
+// instead of streaming a different B block for every tile of a tile row.
+constexpr int H3_SUPER = 8;
+__device__ __forceinline__ void h3_lower_tile(int64_t b, int64_t nt, int64_t* tm, int64_t* tn) {
+  const int64_t G = H3_SUPER, S = (nt + G - 1) / G;
+  // tiles before supertile row I (all rows above it full height G):
+  // G^2 I (I - 1) / 2 + I G (G + 1) / 2
+  auto before = [&](int64_t I) { return G * G * I * (I - 1) / 2 + I * G * (G + 1) / 2; };
+  int64_t I = (int64_t)(sqrt(2.0 * (double)b) / (double)G);
+  if (I > S - 1) I = S - 1;
+  while (I > 0 && before(I) > b) --I;
+  while (I + 1 < S && before(I + 1) <= b) ++I;
+  const int64_t h = min(G, nt - I * G);              // tile rows in supertile row I
+  int64_t l = b - before(I);
+  if (l < I * G * h) {                                // rectangle supertile (I, J), J < I
+    const int64_t J = l / (G * h), r = l % (G * h);
+    *tn = J * G + r / h;
+    *tm = I * G + r % h;
+  } else {                                            // diagonal supertile: triangle of h rows
+    l -= I * G * h;
+    int64_t c = 0;                                    // column c holds h - c tiles
+    while (l >= h - c) { l -= h - c; ++c; }
+    *tn = I * G + c;
+    *tm = I * G + c + l;
+  }
+}
+
 // 128 x 128 tile, 4 waves of 64 x 64 (4 x 4 fragments of 16 x 16).  A, B:
 // tiled split panels at their first row.
 template <bool LOWER>
@@ -440,10 +471,7 @@ __global__ __launch_bounds__(256, 2) void syrk_h3_kernel(const _Float16* __restr
   if (nblk % 8 == 0) bid = (bid % 8) * (nblk / 8) + bid / 8;      // contiguous tiles per XCD
   int64_t tm, tn;
   if (LOWER) {
-    tm = (int64_t)((sqrt(8.0 * (double)bid + 1.0) - 1.0) * 0.5);
-    while ((tm + 1) * (tm + 2) / 2 <= bid) ++tm;
-    while (tm * (tm + 1) / 2 > bid) --tm;
-    tn = bid - tm * (tm + 1) / 2;
+    h3_lower_tile(bid, tiles_n, &tm, &tn);
   } else {
     tm = bid / tiles_n;
     tn = bid % tiles_n;
