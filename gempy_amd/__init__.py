@@ -9,6 +9,7 @@ from .adapters import Transform, model_input_from_geomodel
 from .config import AvailableBackends, BackendTensor, GemPyEngineConfig
 from .data import (EvaluationOptions, GeoModelInput, InterpolationOptions, KernelOptions,
                    RegularGrid, Solutions, SolverForm, StackInput, StackRelationType)
+from .faults import EllipsoidFunction, FiniteFaultData, ellipsoid_3d_factory
 
 
 def compute_model(*args, **kwargs):
@@ -24,4 +25,4 @@ def compute_model_at(*args, **kwargs):
 __all__ = ["AvailableBackends", "BackendTensor", "GemPyEngineConfig", "EvaluationOptions",
            "GeoModelInput", "InterpolationOptions", "KernelOptions", "RegularGrid", "Solutions",
            "SolverForm", "StackInput", "StackRelationType", "Transform", "compute_model", "compute_model_at",
-           "model_input_from_geomodel"]
+           "model_input_from_geomodel", "EllipsoidFunction", "FiniteFaultData", "ellipsoid_3d_factory"]

@@ -45,8 +45,31 @@ class Transform:
         self.position = np.asarray(self.position, dtype=np.float64).reshape(3)
         self.scale = np.asarray(self.scale, dtype=np.float64).reshape(3)
         self.rotation = np.asarray(self.rotation, dtype=np.float64).reshape(3)
-        if np.any(self.rotation != 0.0):
-            raise NotImplementedError("rotated input transforms are not supported")
+
+    def rotation_matrix(self) -> np.ndarray:
+        """R = Rz Ry Rx of the rotation angles (degrees) about x, y, z."""
+        rx, ry, rz = np.deg2rad(self.rotation)
+        cx, sx, cy, sy, cz, sz = np.cos(rx), np.sin(rx), np.cos(ry), np.sin(ry), np.cos(rz), np.sin(rz)
+        Rx = np.array([[1, 0, 0], [0, cx, -sx], [0, sx, cx]])
+        Ry = np.array([[cy, 0, sy], [0, 1, 0], [-sy, 0, cy]])
+        Rz = np.array([[cz, -sz, 0], [sz, cz, 0], [0, 0, 1]])
+        return Rz @ Ry @ Rx
+
+    def _pivot_only(self):
+        if np.any(self.position != 0.0) or np.any(self.scale != 1.0):
+            raise NotImplementedError("pivot transforms support rotation only (position 0, scale 1)")
+
+    def apply_with_pivot(self, points: np.ndarray, pivot) -> np.ndarray:
+        """Rotate points about the pivot: R (x - pivot) + pivot."""
+        self._pivot_only()
+        p = np.asarray(pivot, dtype=np.float64).reshape(3)
+        return (np.asarray(points, dtype=np.float64).reshape(-1, 3) - p) @ self.rotation_matrix().T + p
+
+    def apply_inverse_with_pivot(self, points: np.ndarray, pivot) -> np.ndarray:
+        """The inverse rotation about the pivot: R^T (x - pivot) + pivot."""
+        self._pivot_only()
+        p = np.asarray(pivot, dtype=np.float64).reshape(3)
+        return (np.asarray(points, dtype=np.float64).reshape(-1, 3) - p) @ self.rotation_matrix() + p
 
     @classmethod
     def from_input_points(cls, sp_xyz: np.ndarray, ori_xyz: Optional[np.ndarray] = None) -> "Transform":
@@ -65,6 +88,8 @@ class Transform:
         return cls(position=-(lo + hi) / 2, scale=scale)
 
     def apply(self, xyz: np.ndarray) -> np.ndarray:
+        if np.any(self.rotation != 0.0):
+            raise NotImplementedError("rotated input transforms are not supported")
         return (np.asarray(xyz, dtype=np.float64) + self.position) * self.scale
 
     def apply_inverse(self, xyz: np.ndarray) -> np.ndarray:
@@ -100,6 +125,29 @@ def _fault_matrix(frame, n_groups: int) -> np.ndarray:
     if m is None:
         return np.zeros((n_groups, n_groups), dtype=bool)
     return np.asarray(m, dtype=bool).reshape(n_groups, n_groups)
+
+
+def _finite_fault(group):
+    """FiniteFaultData of a group's faults_input_data.finite_fault_data (the
+    reference's test_finite_faults.py:39-52): the implicit function is an
+    EllipsoidFunction or ellipsoid_3d_factory's partial (center, radii,
+    k_factors keywords); its transform a rotation about the pivot."""
+    fd = getattr(getattr(group, "faults_input_data", None), "finite_fault_data", None)
+    if fd is None:
+        return None
+    from .faults import EllipsoidFunction, FiniteFaultData
+    fn = fd.implicit_function
+    if not isinstance(fn, EllipsoidFunction):
+        kw = getattr(fn, "keywords", None) or {}
+        if not {"center", "radii", "k_factors"} <= set(kw):
+            raise NotImplementedError("finite faults support the ellipsoid implicit function")
+        fn = EllipsoidFunction(kw["center"], kw["radii"], kw["k_factors"])
+    t = fd.implicit_function_transform
+    tr = Transform(position=np.asarray(getattr(t, "position", np.zeros(3)), dtype=np.float64),
+                   scale=np.asarray(getattr(t, "scale", np.ones(3)), dtype=np.float64),
+                   rotation=np.asarray(getattr(t, "rotation", np.zeros(3)), dtype=np.float64)) if t is not None \
+        else Transform()
+    return FiniteFaultData(fn, tr, np.asarray(fd.pivot, dtype=np.float64))
 
 
 def model_input_from_geomodel(geo_model, solver: Optional[SolverForm] = None) -> GeoModelInput:
@@ -138,7 +186,7 @@ def model_input_from_geomodel(geo_model, solver: Optional[SolverForm] = None) ->
             name=getattr(g, "name", f"group{gi}"), sp_xyz=cat(pts, 3), counts=counts, sp_nugget=cat(nug, 0),
             ori_xyz=cat(q, 3), ori_grad=cat(gr, 3), ori_nugget=cat(qn, 0),
             relation=_relation(getattr(g, "structural_relation", "ERODE")), surface_names=names,
-            faults=[int(i) for i in np.nonzero(fm[:gi, gi])[0]]))
+            faults=[int(i) for i in np.nonzero(fm[:gi, gi])[0]], finite_fault=_finite_fault(g)))
     io = geo_model.interpolation_options
     ko = io.kernel_options
     kopts = KernelOptions(range=float(ko.range), c_o=float(ko.c_o), uni_degree=int(getattr(ko, "uni_degree", 1)),

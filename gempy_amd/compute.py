@@ -58,6 +58,8 @@ def _validate(model: GeoModelInput) -> None:
         for f in st.faults:
             if not (0 <= f < k) or model.stacks[f].relation != StackRelationType.FAULT:
                 raise ValueError(f"stack {st.name!r}: fault index {f} must name an earlier fault stack")
+        if st.finite_fault is not None and st.relation != StackRelationType.FAULT:
+            raise ValueError(f"stack {st.name!r}: finite fault data on a stack that is not a fault")
     if model.grid is not None:
         L = model.options.evaluation_options.number_octree_levels
         if L > 1 and any(r % (1 << (L - 1)) for r in model.grid.resolution):
@@ -107,7 +109,17 @@ class _Model:
         ds, st = self.stacks[j], self.model.stacks[j]
         fv = (torch.stack([self._fault_ids(i, pts) for i in st.faults], 1).contiguous() if st.faults else None)
         _, _, fid = ds.evaluate_points(pts, precision=_lib.GK_F64, thresholds=ds.thresholds, fault_values=fv)
-        return fid.double()
+        return self._fault_values(j, fid, pts=pts)
+
+    def _fault_values(self, j, ids, pts=None, lattice=None):
+        """Drift values of fault stack j from its block at the points: the ids,
+        or, for a finite fault, (id - 1) tapered by its implicit function."""
+        ff = self.model.stacks[j].finite_fault
+        if ff is None:
+            return ids.double()
+        # a copy: the block itself keeps its ids
+        v = ids.to(torch.float64, copy=True).contiguous()
+        return ff.taper(self.stacks[j].h, v, pts=pts.contiguous() if pts is not None else None, lattice=lattice)
 
     def at_points(self, pts: torch.Tensor, precision=_lib.GK_F64, keep_scalar=False, per_stack=None):
         """Ids of every stack, the combined lith block and the fault blocks at
@@ -121,7 +133,7 @@ class _Model:
                 scalars.append(S)
             per.append([S, ids, None])
             if st.relation == StackRelationType.FAULT:
-                fault_of[k] = ids.double()
+                fault_of[k] = self._fault_values(k, ids, pts=pts)
                 faults.append(ids)
             else:
                 ids_rows.append(ids)
@@ -163,7 +175,7 @@ def compute_model(model: GeoModelInput, engine_config: Optional[GemPyEngineConfi
             sol.gradient = G
             sol.ids_block = ids
             if st.relation == StackRelationType.FAULT:
-                fault_grid[k] = ids.double()
+                fault_grid[k] = m._fault_values(k, ids, lattice=(grid.origin, grid.spacing, grid.resolution))
                 sol.mask = torch.ones(grid.n_points, dtype=torch.uint8, device="cuda")
             else:
                 ids_rows.append(ids)

@@ -158,6 +158,8 @@ class StackInput:
     relation: StackRelationType = StackRelationType.ERODE
     surface_names: List[str] = field(default_factory=list)
     faults: List[int] = field(default_factory=list)
+    # fault stacks only: gempy_amd.faults.FiniteFaultData (tapered offset)
+    finite_fault: Optional[object] = None
 
     def __post_init__(self):
         self.sp_xyz = np.ascontiguousarray(np.asarray(self.sp_xyz, dtype=np.float64).reshape(-1, 3))
@@ -190,7 +192,8 @@ class StackInput:
                 "ori_nugget": self.ori_nugget,
                 "relation": {StackRelationType.FAULT: "fault", StackRelationType.ONLAP: "onlap"}.get(
                     self.relation, "erode"),
-                "faults": list(self.faults)}
+                "faults": list(self.faults),
+                "finite_fault": self.finite_fault.as_oracle_dict() if self.finite_fault is not None else None}
 
 
 @dataclass

@@ -302,6 +302,18 @@ int gk_marching_cubes(gk_handle h, const double* field, const int64_t* res, cons
                       const double* origin, const double* spacing, double* verts, int64_t max_verts, int32_t* faces,
                       int64_t max_faces, int64_t* n_verts, int64_t* n_faces, gk_stream stream);
 
+/* ---- finite faults (the reference's FiniteFaultData: implicit function,
+ * its transform and pivot; gempy/core/data/__init__.py:20, the ellipsoid of
+ * gempy/modules/custom_implicit_functions/ellipsoid_implicit_function.py:21-35).
+ * values[i] = (values[i] - 1) f(R^T (x_i - pivot) + pivot) in place, with
+ * f(x) = 1 - sigmoid(k1 k2 k3 (sum_d ((x_d - c_d) / (r_d k_d))^2 - 1)): the fault
+ * block's drift values fade out outside the ellipsoid.  Points: xyz [n*3], or
+ * (xyz NULL) the lattice given as lattice[9] = origin[3], spacing[3], res[3]
+ * (z fastest); rotation [9] row-major; all small arrays on the host. */
+int gk_finite_fault_values(gk_handle h, int64_t n, const double* xyz, const double* lattice, const double* rotation,
+                           const double* pivot, const double* center, const double* radii, const double* k,
+                           double* values, gk_stream stream);
+
 /* ---- gravity on a centred grid (geophysics input tz + densities ->
  * Solutions.gravity; gempy/API/grid_API.py:99-108, test_gravity.py:67-89) ----
  * gk_gravity_tz: vertical attraction (unit density, G given: 6.674e-3 for mGal

@@ -393,6 +393,22 @@ def compute_stack(stack, opts, x, fault_sp=None, fault_x=None, gradient=False):
                 gradient=ds, ids=segment(s, thr))
 
 
+def finite_fault_values(ids, pts, ff):
+    """Drift values of a fault block at pts: the ids, or for a finite fault
+    (ff: rotation R, pivot, center, radii, k) (id - 1) f(R^T (x - pivot) + pivot)
+    with the ellipsoid function of
+    gempy/modules/custom_implicit_functions/ellipsoid_implicit_function.py:21-35."""
+    if ff is None:
+        return ids
+    p = np.asarray(ff["pivot"], dtype=np.float64)
+    xr = (np.asarray(pts, dtype=np.float64).reshape(-1, 3) - p) @ np.asarray(ff["rotation"]) + p
+    k = np.asarray(ff["k"], dtype=np.float64)
+    v = (((xr - np.asarray(ff["center"])) / (np.asarray(ff["radii"]) * k)) ** 2).sum(1) - 1.0
+    with np.errstate(over="ignore"):            # exp -> inf gives f = 1
+        f = 1.0 - 1.0 / (1.0 + np.exp(-v * np.prod(k)))
+    return (ids - 1.0) * f
+
+
 def compute_model(stacks, opts, x, gradient=False):
     """Multi-stack model.
 
@@ -417,14 +433,16 @@ def compute_model(stacks, opts, x, gradient=False):
             kw = dict(a=opts["range"], c_o=opts["c_o"], uni_degree=opts.get("uni_degree", 1),
                       i_res=opts.get("i_res", 4.0), gi_res=opts.get("gi_res", 2.0))
 
-            def block_fn(pts, st=st, r=r, kw=kw, fl=fl):
+            ff = st.get("finite_fault")
+
+            def block_fn(pts, st=st, r=r, kw=kw, fl=fl, ff=ff):
                 # a fault offset by older faults needs their blocks at pts too
                 fpts = (np.stack([fault_vals_sp_fn[j](pts) for j in fl], 1) if fl else None)
                 s = evaluate(pts, st["sp_xyz"], st["counts"], st["ori_xyz"], r["weights"],
                              fault_x=fpts, **kw)
-                return segment(s, r["thresholds"]).astype(np.float64)
+                return finite_fault_values(segment(s, r["thresholds"]).astype(np.float64), pts, ff)
             fault_vals_sp_fn[k] = block_fn
-            fault_vals_x[k] = r["ids"].astype(np.float64)
+            fault_vals_x[k] = finite_fault_values(r["ids"].astype(np.float64), x, ff)
     # lithology: youngest non-fault stack whose domain (above its last surface)
     # contains the point; an onlapping stack ('onlap') only above the first
     # surface of the next older non-fault stack, whose top it does not erode

@@ -161,14 +161,23 @@ def greenstone():
                         os.path.join(dst, f"Greenstone_{name}.csv"))
 
 
-def combination():
-    """The COMBINATION example's surface-point and orientation tables."""
+def _jan_model(model, folder):
     import shutil
-    dst = os.path.join(HERE, "combination")
+    dst = os.path.join(HERE, folder)
     os.makedirs(dst, exist_ok=True)
     src = os.path.join(REF, "examples/data/input_data/jan_models")
     for name in ("surface_points", "orientations"):
-        shutil.copyfile(os.path.join(src, f"model7_{name}.csv"), os.path.join(dst, f"model7_{name}.csv"))
+        shutil.copyfile(os.path.join(src, f"{model}_{name}.csv"), os.path.join(dst, f"{model}_{name}.csv"))
+
+
+def combination():
+    """The COMBINATION example's surface-point and orientation tables."""
+    _jan_model("model7", "combination")
+
+
+def one_fault():
+    """The ONE_FAULT example's tables (examples_generator.py:198-240)."""
+    _jan_model("model5", "one_fault")
 
 
 if __name__ == "__main__":
@@ -178,4 +187,5 @@ if __name__ == "__main__":
     transforms()
     greenstone()
     combination()
+    one_fault()
     print("fixtures written to", HERE)
