@@ -11,7 +11,9 @@
 //    a panel solve is a GEMM with the inverted diagonal block.
 //  * Blocked triangular solves of the LU with the inverted diagonal blocks
 //    (the Cholesky solves are in gk_trsv.hip).
+#include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include "gk_internal.h"
 
 namespace {
@@ -121,9 +123,7 @@ __global__ __launch_bounds__(TL::NT) __attribute__((amdgpu_waves_per_eu(WPE, 0))
 
   // XCD-aware tile order: consecutive hardware block ids go to different
   // XCDs, so give each XCD a contiguous range of logical tiles.
-  int64_t bid = blockIdx.x;
-  const int64_t nblk = gridDim.x;
-  if (nblk % 8 == 0) bid = (bid % 8) * (nblk / 8) + bid / 8;
+  const int64_t bid = gk_xcd_tile(blockIdx.x, gridDim.x);
   // With GK_GEMM_GROUP = G > 0 the tiles go column by column within groups of
   // G tile rows, so a B tile is reused by the G rows while their A rows stay
   // in L2.
@@ -460,28 +460,22 @@ __device__ __forceinline__ void h3_lower_tile(int64_t b, int64_t nt, int64_t* tm
 }
 
 // 128 x 128 tile, 4 waves of 64 x 64 (4 x 4 fragments of 16 x 16).  A, B:
-// tiled split panels at their first row.
-template <bool LOWER>
-__global__ __launch_bounds__(256, 2) void syrk_h3_kernel(const _Float16* __restrict__ A,
-                                                         const _Float16* __restrict__ B, float* C, int64_t ldc,
-                                                         int64_t tiles_n) {
-  __shared__ __attribute__((aligned(16))) _Float16 S[2][4 * H3_TILE];   // [buf][A hi | A lo | B hi | B lo]
-  int64_t bid = blockIdx.x;
-  const int64_t nblk = gridDim.x;
-  if (nblk % 8 == 0) bid = (bid % 8) * (nblk / 8) + bid / 8;      // contiguous tiles per XCD
-  int64_t tm, tn;
-  if (LOWER) {
-    h3_lower_tile(bid, tiles_n, &tm, &tn);
-  } else {
-    tm = bid / tiles_n;
-    tn = bid % tiles_n;
-  }
+// tiled split panels at their first row; NP = 2 adds the product of a second
+// pair (A2, B2): a rank-512 update in one pass over C.
+template <bool LOWER, int NP>
+__device__ __forceinline__ void syrk_h3_tile(const _Float16* __restrict__ A, const _Float16* __restrict__ B,
+                                             const _Float16* __restrict__ A2, const _Float16* __restrict__ B2,
+                                             float* C, int64_t ldc, int64_t tm, int64_t tn,
+                                             _Float16 (*S)[4 * H3_TILE]) {
+  constexpr int NK = NP * H3_NK;              // stages
   const int64_t m0 = tm * 128, n0 = tn * 128;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave >> 1, wn = wave & 1;
   const int lrow = lane & 15, lk = lane >> 4;
   const h8v* srcA = reinterpret_cast<const h8v*>(A + tm * H3_ROWBLK);
   const h8v* srcB = reinterpret_cast<const h8v*>(B + tn * H3_ROWBLK);
+  const h8v* srcA2 = reinterpret_cast<const h8v*>((NP == 2 ? A2 : A) + tm * H3_ROWBLK);
+  const h8v* srcB2 = reinterpret_cast<const h8v*>((NP == 2 ? B2 : B) + tn * H3_ROWBLK);
 
   // The accumulators start at -2^11 C (exact scaling), so the epilogue only
   // stores: C - 2^-11 sum = -2^-11 (-2^11 C + sum).  The C loads are issued
@@ -505,10 +499,13 @@ __global__ __launch_bounds__(256, 2) void syrk_h3_kernel(const _Float16* __restr
   // stages in flight
   h8v st_reg[H3_RS][8];
   auto load_global = [&](int set, int c) {
+    const h8v* sa = c < H3_NK ? srcA : srcA2;
+    const h8v* sb = c < H3_NK ? srcB : srcB2;
+    const int cc = c % H3_NK;
 #pragma unroll
     for (int it = 0; it < 8; ++it) {
       const int v = tid + 256 * it;           // 0..1023: A, 1024..2047: B
-      st_reg[set][it] = it < 4 ? srcA[c * (2 * H3_TILE / 8) + v] : srcB[c * (2 * H3_TILE / 8) + v - 1024];
+      st_reg[set][it] = it < 4 ? sa[cc * (2 * H3_TILE / 8) + v] : sb[cc * (2 * H3_TILE / 8) + v - 1024];
     }
   };
   auto store_lds = [&](int buf, int set) {
@@ -555,10 +552,10 @@ __global__ __launch_bounds__(256, 2) void syrk_h3_kernel(const _Float16* __restr
   store_lds(0, 0);
   __syncthreads();
 #pragma unroll
-  for (int kt = 0; kt < H3_NK; ++kt) {
-    if (kt + H3_RS < H3_NK) load_global(kt % H3_RS, kt + H3_RS);
+  for (int kt = 0; kt < NK; ++kt) {
+    if (kt + H3_RS < NK) load_global(kt % H3_RS, kt + H3_RS);
     multiply(kt & 1);
-    if (kt + 1 < H3_NK) store_lds((kt + 1) & 1, (kt + 1) % H3_RS);
+    if (kt + 1 < NK) store_lds((kt + 1) & 1, (kt + 1) % H3_RS);
     __syncthreads();
   }
 #pragma unroll
@@ -573,6 +570,60 @@ __global__ __launch_bounds__(256, 2) void syrk_h3_kernel(const _Float16* __restr
       }
 }
 
+// Grid of g workgroups over t tiles (g <= t): workgroup b runs on XCD b % 8
+// (round-robin dispatch) and takes tiles from its XCD's contiguous range
+// (x t/8 + min(x, t%8), t/8 + [x < t%8] long), so the tiles resident at once
+// on an XCD are neighbours and share operand blocks in its L2.  g < t keeps
+// the workgroups resident (a persistent grid) and leaves the rest of the chip
+// to other streams; the range is then handed out in order: each workgroup's
+// first tile is its index l in the XCD, the next ones come from the XCD's
+// counter ctr[x] (tile nx + ticket), as a dispatcher would hand them out.
+// The workgroup that draws the XCD's last ticket (every workgroup draws one
+// past the range) sets the counter back to 0 for the next launch.  Without
+// counters the workgroups step through the range by their count.  Every tile
+// is visited once: XCD x holds nx = g/8 + [x < g%8] >= 1 workgroups whenever
+// g >= 8 or g == t.
+template <bool LOWER, int NP>
+__global__ __launch_bounds__(256, 2) void syrk_h3_kernel(const _Float16* __restrict__ A,
+                                                         const _Float16* __restrict__ B,
+                                                         const _Float16* __restrict__ A2,
+                                                         const _Float16* __restrict__ B2, float* C, int64_t ldc,
+                                                         int64_t tiles_n, int64_t tiles, int32_t* ctr) {
+  __shared__ __attribute__((aligned(16))) _Float16 S[2][4 * H3_TILE];   // [buf][A hi | A lo | B hi | B lo]
+  __shared__ int64_t next;
+  const int64_t g = gridDim.x, x = blockIdx.x % 8, l = blockIdx.x / 8;
+  const int64_t nx = g / 8 + (x < g % 8 ? 1 : 0);
+  const int64_t start = x * (tiles / 8) + min(x, tiles % 8), cnt = tiles / 8 + (x < tiles % 8 ? 1 : 0);
+  const bool dyn = ctr != nullptr && g < tiles;
+  const int64_t last_ticket = (cnt > nx ? cnt - nx : 0) + nx - 1;
+  int64_t j = l;
+  while (true) {
+    if (j < cnt) {
+      int64_t tm, tn;
+      if (LOWER) {
+        h3_lower_tile(start + j, tiles_n, &tm, &tn);
+      } else {
+        tm = (start + j) / tiles_n;
+        tn = (start + j) % tiles_n;
+      }
+      syrk_h3_tile<LOWER, NP>(A, B, A2, B2, C, ldc, tm, tn, S);
+    }
+    if (!dyn) {
+      j += nx;
+      if (j >= cnt) break;
+      continue;
+    }
+    if (threadIdx.x == 0) {
+      const int64_t v = atomicAdd(&ctr[x], 1);
+      if (v == last_ticket) atomicExch(&ctr[x], 0);
+      next = nx + v;
+    }
+    __syncthreads();
+    j = next;
+    if (j >= cnt) break;
+  }
+}
+
 // Split an FP32 panel (rows x 256, dense, rows a multiple of 128) into the
 // tiled fp16 layout.
 int split_h3(const float* W, int64_t rows, _Float16* P, hipStream_t st) {
@@ -584,10 +635,27 @@ int split_h3(const float* W, int64_t rows, _Float16* P, hipStream_t st) {
   return GK_OK;
 }
 
-// C[m x n] -= A B^T (K = 256) from tiled split panels (A: m rows, B: n rows);
+// Workgroups of the persistent trailing update: two per CU (64 KB of LDS
+// each) less GK_SYRK_RESERVE (default 16) slots, a multiple of 8 and >= 8.
+static int64_t syrk_h3_cap() {
+  static const int64_t cap = [] {
+    int dev = 0, cus = 256;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+      cus = 256;
+    const char* e = std::getenv("GK_SYRK_RESERVE");
+    const int64_t reserve = e && *e ? std::atoll(e) : 16;
+    return std::max<int64_t>(8, (2 * (int64_t)cus - reserve) / 8 * 8);
+  }();
+  return cap;
+}
+
+// C[m x n] -= A B^T (K = 256) from tiled split panels (A: m rows, B: n rows),
+// and -= A2 B2^T as well when A2 is given (K = 512 in one pass over C);
 // lower: only the tiles on or below the diagonal.
 int syrk_h3_launch(int64_t m, int64_t n, const _Float16* A, const _Float16* B, float* C, int64_t ldc, bool lower,
-                   hipStream_t st) {
+                   hipStream_t st, int32_t* ctr = nullptr, const _Float16* A2 = nullptr,
+                   const _Float16* B2 = nullptr) {
   GK_REQUIRE(m % 128 == 0 && n % 128 == 0 && ldc >= n && (!lower || m == n), GK_ERR_ARG,
              "syrk_h3: m, n must be multiples of 128 (m == n for the lower update)");
   if (m == 0 || n == 0) return GK_OK;
@@ -595,13 +663,27 @@ int syrk_h3_launch(int64_t m, int64_t n, const _Float16* A, const _Float16* B, f
   const bool prof = gk_prof_begin(st, &pa);
   const int64_t tm = m / 128, tn = n / 128;
   const int64_t tiles = lower ? tm * (tm + 1) / 2 : tm * tn;
-  if (lower)
-    hipLaunchKernelGGL(syrk_h3_kernel<true>, dim3((unsigned)tiles), dim3(256), 0, st, A, B, C, ldc, tn);
+  // the trailing (lower) update runs on a persistent grid of at most
+  // syrk_h3_cap() workgroups: the panel chain beside it keeps CUs to run on
+  const int64_t grid = lower ? std::min(tiles, syrk_h3_cap()) : tiles;
+  const int np = A2 ? 2 : 1;
+  GK_REQUIRE(!A2 == !B2, GK_ERR_ARG, "syrk_h3: A2 and B2 go together");
+  int32_t* no_ctr = nullptr;
+  if (lower && np == 2)
+    hipLaunchKernelGGL((syrk_h3_kernel<true, 2>), dim3((unsigned)grid), dim3(256), 0, st, A, B, A2, B2, C, ldc, tn,
+                       tiles, ctr);
+  else if (lower)
+    hipLaunchKernelGGL((syrk_h3_kernel<true, 1>), dim3((unsigned)grid), dim3(256), 0, st, A, B, A, B, C, ldc, tn,
+                       tiles, ctr);
+  else if (np == 2)
+    hipLaunchKernelGGL((syrk_h3_kernel<false, 2>), dim3((unsigned)grid), dim3(256), 0, st, A, B, A2, B2, C, ldc, tn,
+                       tiles, no_ctr);
   else
-    hipLaunchKernelGGL(syrk_h3_kernel<false>, dim3((unsigned)tiles), dim3(256), 0, st, A, B, C, ldc, tn);
+    hipLaunchKernelGGL((syrk_h3_kernel<false, 1>), dim3((unsigned)grid), dim3(256), 0, st, A, B, A, B, C, ldc, tn,
+                       tiles, no_ctr);
+  const double K = (double)H3_K * np;
   if (prof)
-    gk_prof_end(st, pa, 10 + (lower ? 1 : 0),
-                lower ? (double)m * (double)(m + 1) * (double)H3_K : 2.0 * (double)m * (double)n * (double)H3_K);
+    gk_prof_end(st, pa, 10 + (lower ? 1 : 0), lower ? (double)m * (double)(m + 1) * K : 2.0 * (double)m * (double)n * K);
   GK_HIP(hipGetLastError());
   return GK_OK;
 }
@@ -1205,6 +1287,13 @@ static int chol32_panel_post(float* A, int64_t lda, int64_t kb, const float* W, 
   return GK_OK;
 }
 
+// Paired rank-512 trailing updates of the split16 factorization (on unless
+// GK_FACTOR_PAIRS=0).
+static bool factor_pairs() {
+  static const bool v = [] { const char* e = std::getenv("GK_FACTOR_PAIRS"); return !(e && *e == '0'); }();
+  return v;
+}
+
 // split16 (FP32 only): rank-256 updates with the split-fp16 MFMA product; for
 // matrices whose factor entries stay within [-16, 16] (unit-diagonal ones).
 template <typename T>
@@ -1216,14 +1305,15 @@ static int cholesky_impl(gk_handle h, T* A, int64_t n, int64_t lda, T* dinv, int
   const int64_t nb = lda / NB;
   split16 = split16 && sizeof(T) == 4;
   void* ws = nullptr;
-  const size_t split_bytes = split16 ? 2 * 2 * (size_t)lda * NB * sizeof(_Float16) : 0;
+  const size_t split_bytes = split16 ? 4 * 2 * (size_t)lda * NB * sizeof(_Float16) : 0;
   GK_TRY(gk_workspace(h, WS_FACTOR, 256 + (2 * (size_t)lda * NB + 2 * DB * DB) * sizeof(T) + split_bytes, &ws));
   int32_t* dinfo = static_cast<int32_t*>(ws);
+  int32_t* tile_ctr = reinterpret_cast<int32_t*>(static_cast<char*>(ws) + 64);   // 8 XCD tile counters
   T* W = reinterpret_cast<T*>(static_cast<char*>(ws) + 256);       // two panel buffers (FP32 look-ahead)
   T* Tb = W + 2 * (size_t)lda * NB;
-  // two tiled split panels of lda x NB x 2 halves
+  // four tiled split panels of lda x NB x 2 halves
   _Float16* Hs = split16 ? reinterpret_cast<_Float16*>(Tb + 2 * DB * DB) : nullptr;
-  GK_HIP(hipMemsetAsync(dinfo, 0, sizeof(int32_t), st));
+  GK_HIP(hipMemsetAsync(ws, 0, 256, st));                              // info word and tile counters
   GK_HIP(hipMemsetAsync(dinv, 0, (size_t)lda * NB * sizeof(T), st));   // Dinv plane: upper-right quadrants stay 0
   if (n < lda) {
     const int64_t tot = (lda - n) * lda + n * (lda - n);
@@ -1245,13 +1335,22 @@ static int cholesky_impl(gk_handle h, T* A, int64_t n, int64_t lda, T* dinv, int
     // applies the rest of the trailing update.  W alternates
     // between two buffers: the side stream writes panel kb+2 only after the
     // caller's stream has passed the update that last read it.
+    // split16: the panels go in pairs (kb - 1, kb), kb odd: the trailing block
+    // receives both in one rank-512 update after kb, which halves the passes
+    // over it.  The next two column blocks take the pair's product first, so
+    // that the side stream factors both panels of the next pair (the second
+    // after applying the first to it) beside that update.  Four split
+    // buffers: the pair being applied and the two panels factored meanwhile.
     if constexpr (sizeof(T) == 4) {
       hipStream_t s2;
       hipEvent_t* ev;
       GK_TRY(gk_side_stream(h, &s2, &ev));
       T* Wb[2] = {W, W + (size_t)lda * NB};
-      _Float16* Hb[2] = {Hs, Hs ? Hs + 2 * (size_t)lda * NB : nullptr};
-      auto sp = [&](int64_t kb) { return Hb[kb & 1]; };
+      _Float16* Hb[4];
+      for (int b = 0; b < 4; ++b) Hb[b] = Hs ? Hs + 2 * b * (size_t)lda * NB : nullptr;
+      auto sp = [&](int64_t kb) { return Hb[kb % 4]; };
+      const int64_t RB = NB * 2 * H3_K;         // halves of NB rows of a split panel
+      const bool pairs = split16 && factor_pairs();
       GK_HIP(hipEventRecord(ev[0], st));
       GK_HIP(hipStreamWaitEvent(s2, ev[0], 0));
       if (nb > 0) GK_TRY(chol32_panel(A, lda, 0, dinv, dinfo, Wb[0], Tb, s2, sp(0)));
@@ -1259,26 +1358,56 @@ static int cholesky_impl(gk_handle h, T* A, int64_t n, int64_t lda, T* dinv, int
       if (nb > 0) GK_TRY(chol32_panel_post(A, lda, 0, Wb[0], Tb, s2));
       for (int64_t kb = 0; kb + 1 < nb; ++kb) {
         const int64_t k0 = kb * NB, rem = lda - k0 - NB;
-        T* Wk = Wb[kb & 1];
-        GK_HIP(hipStreamWaitEvent(st, ev[1], 0));           // panel kb factored, Wk = its rows below
-        // next panel's columns: A[k0+NB:, k0+NB : k0+2NB] -= Wk Wk[0:NB]^T
-        if (split16)
-          GK_TRY(syrk_h3_launch(rem, NB, sp(kb), sp(kb), A + (k0 + NB) * lda + k0 + NB, lda, false, st));
-        else
-          GK_TRY(gemm_launch<T>(rem, NB, NB, Wk, NB, Wk, NB, true, T(-1), T(1), A + (k0 + NB) * lda + k0 + NB, lda,
-                                false, st));
-        GK_HIP(hipEventRecord(ev[0], st));
-        GK_HIP(hipStreamWaitEvent(s2, ev[0], 0));
-        GK_TRY(chol32_panel(A, lda, kb + 1, dinv, dinfo, Wb[(kb + 1) & 1], Tb, s2, sp(kb + 1)));
-        GK_HIP(hipEventRecord(ev[1], s2));
-        GK_TRY(chol32_panel_post(A, lda, kb + 1, Wb[(kb + 1) & 1], Tb, s2));
-        if (rem > NB) {                                     // the rest of the trailing update (lower)
+        float* Cn = A + (k0 + NB) * lda + k0 + NB;          // column block kb + 1, rows below k0 + NB
+        if (pairs && !(kb & 1)) {
+          // kb opens a pair.  Column block kb + 1 holds every earlier panel's
+          // product (the last pair's update of two column blocks ran before
+          // panel kb was factored); the side stream applies panel kb to it
+          // and factors it, beside the trailing update of the last pair.
+          GK_TRY(syrk_h3_launch(rem, NB, sp(kb), sp(kb), Cn, lda, false, s2));
+          GK_TRY(chol32_panel(A, lda, kb + 1, dinv, dinfo, Wb[(kb + 1) & 1], Tb, s2, sp(kb + 1)));
+          GK_HIP(hipEventRecord(ev[1], s2));
+          GK_TRY(chol32_panel_post(A, lda, kb + 1, Wb[(kb + 1) & 1], Tb, s2));
+        } else if (pairs) {
+          // kb closes the pair (kb - 1, kb): its product with the next two
+          // column blocks (rank 512; the block above the diagonal of the
+          // second is computed and never read), then the next panel on the
+          // side stream, and the rest of the trailing block (column blocks
+          // from kb + 3) on the caller's stream.
+          GK_HIP(hipStreamWaitEvent(st, ev[1], 0));         // panel kb factored
+          GK_TRY(syrk_h3_launch(rem, std::min<int64_t>(2 * NB, rem), sp(kb), sp(kb), Cn, lda, false, st, nullptr,
+                                sp(kb - 1) + RB, sp(kb - 1) + RB));
+          GK_HIP(hipEventRecord(ev[0], st));
+          GK_HIP(hipStreamWaitEvent(s2, ev[0], 0));
+          GK_TRY(chol32_panel(A, lda, kb + 1, dinv, dinfo, Wb[(kb + 1) & 1], Tb, s2, sp(kb + 1)));
+          GK_HIP(hipEventRecord(ev[1], s2));
+          GK_TRY(chol32_panel_post(A, lda, kb + 1, Wb[(kb + 1) & 1], Tb, s2));
+          if (rem > 2 * NB)
+            GK_TRY(syrk_h3_launch(rem - 2 * NB, rem - 2 * NB, sp(kb) + 2 * RB, sp(kb) + 2 * RB,
+                                  A + (k0 + 3 * NB) * lda + k0 + 3 * NB, lda, true, st, tile_ctr,
+                                  sp(kb - 1) + 3 * RB, sp(kb - 1) + 3 * RB));
+        } else {
+          T* Wk = Wb[kb & 1];
+          GK_HIP(hipStreamWaitEvent(st, ev[1], 0));         // panel kb factored, Wk = its rows below
+          // next panel's columns: A[k0+NB:, k0+NB : k0+2NB] -= Wk Wk[0:NB]^T
           if (split16)
-            GK_TRY(syrk_h3_launch(rem - NB, rem - NB, sp(kb) + NB * 2 * H3_K, sp(kb) + NB * 2 * H3_K,
-                                  A + (k0 + 2 * NB) * lda + k0 + 2 * NB, lda, true, st));
+            GK_TRY(syrk_h3_launch(rem, NB, sp(kb), sp(kb), Cn, lda, false, st));
           else
-            GK_TRY(gemm_launch<T>(rem - NB, rem - NB, NB, Wk + NB * NB, NB, Wk + NB * NB, NB, true, T(-1), T(1),
-                                  A + (k0 + 2 * NB) * lda + k0 + 2 * NB, lda, true, st));
+            GK_TRY(gemm_launch<T>(rem, NB, NB, Wk, NB, Wk, NB, true, T(-1), T(1), A + (k0 + NB) * lda + k0 + NB, lda,
+                                  false, st));
+          GK_HIP(hipEventRecord(ev[0], st));
+          GK_HIP(hipStreamWaitEvent(s2, ev[0], 0));
+          GK_TRY(chol32_panel(A, lda, kb + 1, dinv, dinfo, Wb[(kb + 1) & 1], Tb, s2, sp(kb + 1)));
+          GK_HIP(hipEventRecord(ev[1], s2));
+          GK_TRY(chol32_panel_post(A, lda, kb + 1, Wb[(kb + 1) & 1], Tb, s2));
+          if (rem > NB) {                                   // the rest of the trailing update (lower)
+            if (split16)
+              GK_TRY(syrk_h3_launch(rem - NB, rem - NB, sp(kb) + RB, sp(kb) + RB,
+                                    A + (k0 + 2 * NB) * lda + k0 + 2 * NB, lda, true, st, tile_ctr));
+            else
+              GK_TRY(gemm_launch<T>(rem - NB, rem - NB, NB, Wk + NB * NB, NB, Wk + NB * NB, NB, true, T(-1), T(1),
+                                    A + (k0 + 2 * NB) * lda + k0 + 2 * NB, lda, true, st));
+          }
         }
         if (kb % 16 == 15) {                                // stop early once a pivot has failed
           int32_t hinfo = 0;
@@ -1289,6 +1418,7 @@ static int cholesky_impl(gk_handle h, T* A, int64_t n, int64_t lda, T* dinv, int
       }
       GK_HIP(hipEventRecord(ev[2], s2));
       GK_HIP(hipStreamWaitEvent(st, ev[2], 0));
+
     }
   }
   GK_TRY(chain_planes<T>(A, lda, dinv, st));   // chain planes of the triangular solves
@@ -1305,19 +1435,29 @@ extern "C" int gk_cholesky_f32_split16(gk_handle h, float* A, int64_t n, int64_t
   return cholesky_impl<float>(h, A, n, lda, dinv, info, (hipStream_t)stream, true);
 }
 
-extern "C" int gk_test_syrk_split16(gk_handle h, int64_t m, int64_t n, const float* A, const float* B, float* C,
-                                    int64_t ldc, int32_t lower, gk_stream stream) {
-  GK_REQUIRE(h && A && B && C, GK_ERR_ARG, "gk_test_syrk_split16: NULL argument");
+extern "C" int gk_test_syrk_split16(gk_handle h, int64_t m, int64_t n, const float* A, const float* B,
+                                    const float* A2, const float* B2, float* C, int64_t ldc, int32_t lower,
+                                    gk_stream stream) {
+  GK_REQUIRE(h && A && B && C && !A2 == !B2, GK_ERR_ARG, "gk_test_syrk_split16: NULL argument");
   GK_REQUIRE(m > 0 && n > 0 && m % 128 == 0 && n % 128 == 0 && ldc >= n && (!lower || m == n), GK_ERR_ARG,
              "gk_test_syrk_split16: m, n must be positive multiples of 128 (m == n when lower)");
   hipStream_t st = (hipStream_t)stream;
   void* ws = nullptr;
-  GK_TRY(gk_workspace(h, WS_FACTOR, 2 * (size_t)(m + n) * NB * sizeof(_Float16), &ws));
-  _Float16* Ap = static_cast<_Float16*>(ws);
+  const int np = A2 ? 2 : 1;
+  GK_TRY(gk_workspace(h, WS_FACTOR, 256 + 2 * np * (size_t)(m + n) * NB * sizeof(_Float16), &ws));
+  int32_t* ctr = static_cast<int32_t*>(ws);
+  _Float16* Ap = reinterpret_cast<_Float16*>(static_cast<char*>(ws) + 256);
   _Float16* Bp = Ap + 2 * (size_t)m * NB;
+  _Float16* Ap2 = Bp + 2 * (size_t)n * NB;
+  _Float16* Bp2 = Ap2 + 2 * (size_t)m * NB;
+  GK_HIP(hipMemsetAsync(ctr, 0, 256, st));
   GK_TRY(split_h3(A, m, Ap, st));
   GK_TRY(split_h3(B, n, Bp, st));
-  return syrk_h3_launch(m, n, Ap, Bp, C, ldc, lower != 0, st);
+  if (np == 2) {
+    GK_TRY(split_h3(A2, m, Ap2, st));
+    GK_TRY(split_h3(B2, n, Bp2, st));
+  }
+  return syrk_h3_launch(m, n, Ap, Bp, C, ldc, lower != 0, st, ctr, np == 2 ? Ap2 : nullptr, np == 2 ? Bp2 : nullptr);
 }
 
 extern "C" int gk_cholesky_f32(gk_handle h, float* A, int64_t n, int64_t lda, float* dinv, int32_t* info,

@@ -126,6 +126,16 @@ void gk_prof_end(hipStream_t st, hipEvent_t a, int kind, double work);
 static inline int64_t gk_ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
 static inline int64_t gk_round_up(int64_t a, int64_t b) { return gk_ceil_div(a, b) * b; }
 
+// Logical tile of hardware block bid of a grid of nblk: blocks are dispatched
+// to the 8 XCDs round-robin (XCD bid % 8), so XCD x runs ceil((nblk - x) / 8)
+// of them; it is given the contiguous range of logical tiles that starts at
+// x q + min(x, r) (q, r = nblk / 8, nblk % 8), so tiles that share operands
+// share that XCD's L2.  A bijection for every nblk.
+__device__ __forceinline__ int64_t gk_xcd_tile(int64_t bid, int64_t nblk) {
+  const int64_t q = nblk / 8, r = nblk % 8, x = bid % 8;
+  return x * q + (x < r ? x : r) + bid / 8;
+}
+
 // ------------------------------------------------------------------ covariance
 // All device code works in range-normalised coordinates (x / a), so the cubic
 // covariance is a polynomial in t = rho^2 and rho.

@@ -417,7 +417,9 @@ int gk_test_trsv_set_error(gk_handle h, int32_t code, gk_stream stream);
 /* Testing: the split-fp16 rank-256 update the FP32 factorization uses,
  * C[m x n] -= A B^T with A (m x 256) and B (n x 256) dense FP32 rows (entries
  * within [-16, 16]), m, n multiples of 128; lower: tiles on or below the
- * diagonal only (m == n).  No counterpart in the reference. */
+ * diagonal only (m == n).  With A2, B2 (same shapes; both or neither) also
+ * -= A2 B2^T in the same pass: the rank-512 update of paired panels.  No
+ * counterpart in the reference. */
 /* Testing: the grid search for the FP32 form's local-inverse neighbourhoods,
  * idx[p * GK_LOCAL_NEIGHBOURS + j] = the j-th nearest data point of data point
  * p (itself first; data points = orientations then rest points), hk[p] = the
@@ -425,8 +427,8 @@ int gk_test_trsv_set_error(gk_handle h, int32_t code, gk_stream stream);
  * reference. */
 #define GK_LOCAL_NEIGHBOURS 32
 int gk_test_knn_local(gk_handle h, const gk_system* sys, int32_t* idx, float* hk, gk_stream stream);
-int gk_test_syrk_split16(gk_handle h, int64_t m, int64_t n, const float* A, const float* B, float* C, int64_t ldc,
-                         int32_t lower, gk_stream stream);
+int gk_test_syrk_split16(gk_handle h, int64_t m, int64_t n, const float* A, const float* B, const float* A2,
+                         const float* B2, float* C, int64_t ldc, int32_t lower, gk_stream stream);
 
 #ifdef __cplusplus
 }

@@ -1,6 +1,6 @@
-"""The split-fp16 rank-256 update of the FP32 factorization
-(gk_test_syrk_split16: three fp16 MFMA products per product over hi / lo
-parts) against the FP64 product: its error stays within a small multiple of
+"""The split-fp16 rank-256 and paired rank-512 updates of the FP32
+factorization (gk_test_syrk_split16: three fp16 MFMA products per product
+over hi / lo parts) against the FP64 product: its error stays within a small multiple of
 an FP32 GEMM's on the same data for unit-scale entries (factor rows of an
 equilibrated block), within an absolute 1e-12 for entries below fp16's
 normal range, on full and lower-triangular launches."""
@@ -20,26 +20,32 @@ def say(msg):
 
 
 def _run(A, B, C0, lower):
+    """C0 - A B^T; A, B of 256 columns (one panel pair) or 512 (two pairs,
+    one rank-512 pass)."""
     from gempy_amd import _lib
     from gempy_amd.engine import _ptr, _stream, handle
     lib = _lib.load()
-    dA = torch.tensor(A, dtype=torch.float32, device="cuda")
-    dB = torch.tensor(B, dtype=torch.float32, device="cuda")
+    two = A.shape[1] == 512
+    dev = lambda X: torch.tensor(np.ascontiguousarray(X), dtype=torch.float32, device="cuda")  # noqa: E731
+    dA, dB = dev(A[:, :256]), dev(B[:, :256])
+    dA2, dB2 = (dev(A[:, 256:]), dev(B[:, 256:])) if two else (None, None)
     dC = torch.tensor(C0, dtype=torch.float32, device="cuda")
     m, n = A.shape[0], B.shape[0]
-    _lib.check("gk_test_syrk_split16", lib.gk_test_syrk_split16(handle(0), m, n, _ptr(dA), _ptr(dB), _ptr(dC), n,
-                                                                int(lower), _stream()))
+    _lib.check("gk_test_syrk_split16", lib.gk_test_syrk_split16(
+        handle(0), m, n, _ptr(dA), _ptr(dB), _ptr(dA2) if two else None, _ptr(dB2) if two else None, _ptr(dC), n,
+        int(lower), _stream()))
     return dC.cpu().numpy().astype(np.float64)
 
 
-@pytest.mark.parametrize("m,n,scale,lower", [(512, 384, 1.0, False), (640, 640, 1.0, True),
-                                             (384, 256, 1e-4, False), (256, 256, 3e-7, True)])
-def test_split16_update_matches_fp64(gpu, m, n, scale, lower):
-    say(f"split16 update: m {m} n {n} scale {scale} lower {lower}")
-    rng = np.random.default_rng(m + n)
-    A = (rng.uniform(-1, 1, (m, 256)) * scale).astype(np.float32)
-    B = A[:n].copy() if lower else (rng.uniform(-1, 1, (n, 256)) * scale).astype(np.float32)
-    C0 = rng.uniform(-1, 1, (m, n)).astype(np.float32) * np.float32(256 * scale * scale)
+@pytest.mark.parametrize("m,n,scale,lower,k", [(512, 384, 1.0, False, 256), (640, 640, 1.0, True, 256),
+                                               (384, 256, 1e-4, False, 256), (256, 256, 3e-7, True, 256),
+                                               (512, 256, 1.0, False, 512), (768, 768, 1.0, True, 512)])
+def test_split16_update_matches_fp64(gpu, m, n, scale, lower, k):
+    say(f"split16 update: m {m} n {n} k {k} scale {scale} lower {lower}")
+    rng = np.random.default_rng(m + n + k)
+    A = (rng.uniform(-1, 1, (m, k)) * scale).astype(np.float32)
+    B = A[:n].copy() if lower else (rng.uniform(-1, 1, (n, k)) * scale).astype(np.float32)
+    C0 = rng.uniform(-1, 1, (m, n)).astype(np.float32) * np.float32(k * scale * scale)
     got = _run(A, B, C0, lower)
     ref = C0.astype(np.float64) - A.astype(np.float64) @ B.astype(np.float64).T
     f32 = (C0 - A @ B.T).astype(np.float64)
@@ -55,3 +61,27 @@ def test_split16_update_matches_fp64(gpu, m, n, scale, lower):
     # far below the FP32 resolution (6e-8) of the unit-diagonal blocks the
     # factorization updates.
     assert err <= 4 * e32 + 1e-6 * np.abs(ref).max() + 1e-12
+
+
+@pytest.mark.parametrize("k", [256, 512])
+def test_split16_persistent_lower_update_twice(gpu, k):
+    """A lower update with more tiles (666 at m = 4608) than the persistent
+    grid holds: workgroups draw tiles from per-XCD counters, which the last
+    draw resets.  Two launches in a row give the same bits, and every tile on
+    or below the diagonal is updated (the counters were back at 0)."""
+    m = 4608
+    say(f"split16 persistent lower update: m {m}, k {k}, twice")
+    rng = np.random.default_rng(11)
+    A = rng.uniform(-1, 1, (m, k)).astype(np.float32)
+    C0 = rng.uniform(-1, 1, (m, m)).astype(np.float32) * np.float32(k)
+    g1 = _run(A, A, C0, True)
+    g2 = _run(A, A, C0, True)
+    assert np.array_equal(g1, g2)
+    ref = C0.astype(np.float64) - A.astype(np.float64) @ A.astype(np.float64).T
+    f32 = (C0 - A @ A.T).astype(np.float64)
+    blk = np.add.outer(np.arange(m) // 128, -(np.arange(m) // 128)) < 0
+    assert np.array_equal(g1[blk], C0.astype(np.float64)[blk])
+    err = np.abs(g1 - ref)[~blk].max()
+    e32 = np.abs(f32 - ref)[~blk].max()
+    say(f"split16 persistent: max err {err:.3e}, FP32 GEMM {e32:.3e}")
+    assert err <= 4 * e32 + 1e-6 * np.abs(ref).max()
