@@ -794,11 +794,12 @@ __global__ __launch_bounds__(1024) void getrf128_kernel(double* A, int64_t lda, 
 template <typename T, int PB>
 struct BlkStore {
   T* S;
-  static constexpr int SIZE = DB * (DB + 1) + 2 * PB * DB;
+  static constexpr int SIZE = DB * (DB + 1) + 2 * PB * DB + PB;
   __device__ static int pl(int i, int c) { return i * (i + 1) / 2 + c; }
   __device__ T& L(int i, int c) { return S[pl(i, c)]; }
   __device__ T& X(int i, int c) { return S[DB * (DB + 1) / 2 + pl(i, c)]; }
   __device__ T& P(int b, int t, int i) { return S[DB * (DB + 1) + (b * PB + t) * DB + i]; }
+  __device__ T& ipv(int j) { return S[DB * (DB + 1) + 2 * PB * DB + j]; }   // 1 / pivot of the current panel
 };
 // FP32 diagonal-block kernel: threads.  It runs on the side stream beside the
 // trailing update, whose workgroups (512 threads, 80 VGPRs, 40 KB LDS) sit
@@ -854,6 +855,89 @@ __device__ void potrf_blocked(BlkStore<T, PB> B, T* A, int64_t lda, T* X, int64_
   }
   __syncthreads();
   int bad_col = 0;                            // first non-positive pivot + 1 (wave 0)
+  // FP32: the rows of X are computed by the last wave from the stored panel
+  // while the other waves apply the panel to the trailing L tiles; the X
+  // tiles follow after a barrier.
+  constexpr bool SPLIT_X = sizeof(T) == 4 && PB == 16 && NT >= 128;
+  constexpr int XW = NT / 64 - 1;             // the wave of the X rows
+  // ---- rows jb .. jb+PB-1 of X from the panel's diagonal block (lane l holds
+  //      row jb + l in a) and the reciprocal pivots: lane owns columns lane
+  //      and lane + 64
+  auto x_rows = [&](int jb, const T (&a)[PB], const T (&ipv)[PB]) {
+    T xa[PB], xb[PB];
+    const int ca = lane, cb = lane + 64;
+#pragma unroll
+    for (int r = 0; r < PB; ++r) {
+      const int i = jb + r;
+      T va = ca < jb ? B.X(i, ca) : (ca == i ? T(1) : T(0));   // B.X holds -acc[i][c] for c < jb
+      T vb = cb < jb ? B.X(i, cb) : (cb == i ? T(1) : T(0));
+#pragma unroll
+      for (int k = 0; k < r; ++k) {
+        const T lik = lane_value(a[k], r);        // L[jb+r][jb+k]
+        va -= lik * xa[k];
+        vb -= lik * xb[k];
+      }
+      xa[r] = va * ipv[r];
+      xb[r] = vb * ipv[r];
+    }
+#pragma unroll
+    for (int r = 0; r < PB; ++r) {
+      const int i = jb + r;
+      if (ca <= i) B.X(i, ca) = xa[r];
+      if (cb <= i) B.X(i, cb) = xb[r];
+    }
+  };
+  // ---- phase 2 on MFMA (16 x 16 x 4 f32), one 16 x 16 tile per wave at a time
+  //      (waves w, w + nw, ...):
+  //      L tiles (ti >= tj >= c0/16):  L[i][c] -= sum_t L[i][jb+t] L[c][jb+t]
+  //      X tiles (ti >= c0/16 > tj):   X[i][c] -= sum_t L[i][jb+t] X[jb+t][c]
+  // Operands from the column-major panel buffer P[pb] (and the packed X rows
+  // of the panel); a lane holds C[4 (lane/16) + r][lane % 16] of a tile.
+  auto mfma_tiles = [&](auto jb, int pb, int c0, bool is_l, int w, int nw) {
+    const int a = c0 / 16, nt = DB / 16 - a;             // first trailing tile row, tile rows left
+    const int count = is_l ? nt * (nt + 1) / 2 : nt * a;
+    const int lr = lane & 15, lq = lane >> 4;
+    for (int tile = w; tile < count; tile += nw) {
+      int ti, tj;
+      if (is_l) {                                         // lower triangle of the trailing tiles
+        int t = tile, r = 0;
+        while (t > r) { t -= r + 1; ++r; }
+        ti = a + r;
+        tj = a + t;
+      } else {
+        ti = a + tile / a;
+        tj = tile % a;
+      }
+      const int i0 = ti * 16, cb = tj * 16;
+      f4v acc;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int i = i0 + lq * 4 + r, c = cb + lr;
+        acc[r] = is_l ? (c <= i ? B.L(i, c) : 0.0f) : B.X(i, c);
+      }
+#pragma unroll
+      for (int ks = 0; ks < PB / 4; ++ks) {
+        const int t = ks * 4 + lq;
+        const float av = -B.P(pb, t, i0 + lr);            // A[row][k] = L[i0+lr][jb+t]
+        float bv;
+        if (is_l) bv = B.P(pb, t, cb + lr);               // B[k][col] = L[cb+lr][jb+t]
+        else bv = (cb + lr <= jb + t) ? B.X(jb + t, cb + lr) : 0.0f;
+        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv, acc, 0, 0, 0);
+      }
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int i = i0 + lq * 4 + r, c = cb + lr;
+        if (is_l) {
+          if (c <= i) {
+            B.L(i, c) = acc[r];
+            if (c < c0 + PB) B.P(pb ^ 1, c - c0, i) = acc[r];
+          }
+        } else {
+          B.X(i, c) = acc[r];
+        }
+      }
+    }
+  };
   for (int jb = 0, pb = 0; jb < DB; jb += PB, pb ^= 1) {
     if (wave == 0) {
       // ---- panel: lane l holds rows jb + l (a) and jb + 64 + l (b)
@@ -892,82 +976,30 @@ __device__ void potrf_blocked(BlkStore<T, PB> B, T* A, int64_t lda, T* X, int64_
         if (ra < DB) B.P(pb, c, ra) = a[c];
         if (rb < DB) B.P(pb, c, rb) = b[c];
       }
-      // ---- rows jb .. jb+PB-1 of X: lane owns columns lane and lane + 64
-      T xa[PB], xb[PB];
-      const int ca = lane, cb = lane + 64;
-#pragma unroll
-      for (int r = 0; r < PB; ++r) {
-        const int i = jb + r;
-        T va = ca < jb ? B.X(i, ca) : (ca == i ? T(1) : T(0));   // B.X holds -acc[i][c] for c < jb
-        T vb = cb < jb ? B.X(i, cb) : (cb == i ? T(1) : T(0));
-#pragma unroll
-        for (int k = 0; k < r; ++k) {
-          const T lik = lane_value(a[k], r);      // L[jb+r][jb+k]
-          va -= lik * xa[k];
-          vb -= lik * xb[k];
-        }
-        xa[r] = va * ipv[r];
-        xb[r] = vb * ipv[r];
-      }
-#pragma unroll
-      for (int r = 0; r < PB; ++r) {
-        const int i = jb + r;
-        if (ca <= i) B.X(i, ca) = xa[r];
-        if (cb <= i) B.X(i, cb) = xb[r];
+      if constexpr (SPLIT_X) {
+        if (lane < PB) B.ipv(lane) = ipv[lane];
+      } else {
+        x_rows(jb, a, ipv);
       }
     }
     __syncthreads();
     const int c0 = jb + PB;
-    if constexpr (sizeof(T) == 4 && PB == 16) {
-      // ---- phase 2 on MFMA (16 x 16 x 4 f32), one 16 x 16 tile per wave at a time:
-      //      L tiles (ti >= tj >= c0/16):  L[i][c] -= sum_t L[i][jb+t] L[c][jb+t]
-      //      X tiles (ti >= c0/16 > tj):   X[i][c] -= sum_t L[i][jb+t] X[jb+t][c]
-      // Operands from the column-major panel buffer P[pb] (and the packed X rows
-      // of the panel); a lane holds C[4 (lane/16) + r][lane % 16] of a tile.
-      const int a = c0 / 16, nt = DB / 16 - a;           // first trailing tile row, tile rows left
-      const int n_l = nt * (nt + 1) / 2, n_x = nt * a;
-      const int lr = lane & 15, lq = lane >> 4;
-      for (int tile = wave; tile < n_l + n_x; tile += NT / 64) {
-        int ti, tj;
-        const bool is_l = tile < n_l;
-        if (is_l) {                                       // lower triangle of the trailing tiles
-          int t = tile, r = 0;
-          while (t > r) { t -= r + 1; ++r; }
-          ti = a + r;
-          tj = a + t;
-        } else {
-          ti = a + (tile - n_l) / a;
-          tj = (tile - n_l) % a;
-        }
-        const int i0 = ti * 16, cb = tj * 16;
-        f4v acc;
+    if constexpr (SPLIT_X) {
+      if (wave == XW) {
+        // the panel's diagonal block as wave 0 held it: lane l, row jb + l
+        T a[PB], ipv[PB];
+        const int rac = min(jb + lane, DB - 1);
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const int i = i0 + lq * 4 + r, c = cb + lr;
-          acc[r] = is_l ? (c <= i ? B.L(i, c) : 0.0f) : B.X(i, c);
+        for (int c = 0; c < PB; ++c) {
+          a[c] = B.P(pb, c, rac);
+          ipv[c] = B.ipv(c);
         }
-#pragma unroll
-        for (int ks = 0; ks < PB / 4; ++ks) {
-          const int t = ks * 4 + lq;
-          const float av = -B.P(pb, t, i0 + lr);          // A[row][k] = L[i0+lr][jb+t]
-          float bv;
-          if (is_l) bv = B.P(pb, t, cb + lr);             // B[k][col] = L[cb+lr][jb+t]
-          else bv = (cb + lr <= jb + t) ? B.X(jb + t, cb + lr) : 0.0f;
-          acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv, acc, 0, 0, 0);
-        }
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const int i = i0 + lq * 4 + r, c = cb + lr;
-          if (is_l) {
-            if (c <= i) {
-              B.L(i, c) = acc[r];
-              if (c < c0 + PB) B.P(pb ^ 1, c - c0, i) = acc[r];
-            }
-          } else {
-            B.X(i, c) = acc[r];
-          }
-        }
+        x_rows(jb, a, ipv);
+      } else {
+        mfma_tiles(jb, pb, c0, true, wave, XW);
       }
+      __syncthreads();
+      mfma_tiles(jb, pb, c0, false, wave, NT / 64);
       __syncthreads();
       continue;
     }
@@ -1251,6 +1283,58 @@ static int chol32_panel(float* A, int64_t lda, int64_t kb, float* dinv, int32_t*
   return GK_OK;
 }
 
+// The split16 panel, as a blocked solve that leaves the inverse's
+// off-diagonal quadrant for later: with the 256-block halves 1, 2 and the
+// rows below them b,
+//   X11 = L11^-1;  [L21; W1] = [A21; Ab1] X11^T  (one product);
+//   [A22; Ab2] -= [L21; W1] L21^T;  X22 = L22^-1;  W2 = Ab2 X22^T,
+// so W = [W1 W2] = Ab Dinv^T with Dinv = [X11 0; -X22 L21 X11  X22], whose
+// lower-left quadrant (needed only by the triangular solves) is formed for
+// all panels at the end (dinv_lower_left).  Six kernels in sequence where the
+// Dinv form takes eight.  V holds DB + rem rows of 256: L21 (first 128
+// columns of its first DB rows), then W.
+static int chol32_panel_split(float* A, int64_t lda, int64_t kb, float* dinv, int32_t* dinfo, float* V,
+                              hipStream_t s, _Float16* P) {
+  const int64_t k0 = kb * NB, rem = lda - k0 - NB;
+  float* D = dinv + k0 * NB;                                     // 256 x 256, ld NB
+  Potrf128<float>::launch(A + k0 * lda + k0, lda, D, NB, dinfo, k0, s);
+  GK_TRY(gemm_launch<float>(DB + rem, DB, DB, A + (k0 + DB) * lda + k0, lda, D, NB, true, 1.0f, 0.0f, V, NB, false,
+                            s));                                 // [L21; W1]
+  GK_TRY(gemm_launch<float>(DB + rem, DB, DB, V, NB, V, NB, true, -1.0f, 1.0f, A + (k0 + DB) * lda + k0 + DB, lda,
+                            false, s));                          // [A22; Ab2] -= [L21; W1] L21^T
+  Potrf128<float>::launch(A + (k0 + DB) * lda + k0 + DB, lda, D + DB * NB + DB, NB, dinfo, k0 + DB, s);
+  if (rem > 0) {
+    GK_TRY(gemm_launch<float>(rem, DB, DB, A + (k0 + NB) * lda + k0 + DB, lda, D + DB * NB + DB, NB, true, 1.0f,
+                              0.0f, V + DB * NB + DB, NB, false, s));   // W2
+    GK_TRY(split_h3(V + DB * NB, rem, P, s));
+  }
+  return GK_OK;
+}
+
+// L21 and W of chol32_panel_split back into A (columns no other kernel
+// touches until the end).
+static int chol32_panel_split_post(float* A, int64_t lda, int64_t kb, const float* V, hipStream_t s) {
+  const int64_t k0 = kb * NB, rem = lda - k0 - NB;
+  hipLaunchKernelGGL(copy_block_kernel<float>, dim3(DB * DB / 256), dim3(256), 0, s, V, (int64_t)NB,
+                     A + (k0 + DB) * lda + k0, lda, (int64_t)DB, (int64_t)DB);
+  if (rem > 0)
+    hipLaunchKernelGGL(copy_block_kernel<float>, dim3((unsigned)gk_ceil_div(rem * NB, 256)), dim3(256), 0, s,
+                       V + DB * NB, (int64_t)NB, A + (k0 + NB) * lda + k0, lda, rem, (int64_t)NB);
+  return GK_OK;
+}
+
+// Lower-left quadrants of the diagonal blocks' inverses, all panels at once:
+// Dinv21 = -X22 (L21 X11), two batched 128^3 products per panel (T: nb x DB x
+// DB scratch).
+static int dinv_lower_left(const float* A, int64_t lda, float* dinv, float* T, hipStream_t s) {
+  const int64_t nb = lda / NB, sD = (int64_t)NB * NB;
+  if (nb == 0) return GK_OK;
+  GK_TRY(gemm_launch<float>(DB, DB, DB, A + DB * lda, lda, dinv, NB, false, 1.0f, 0.0f, T, DB, false, s, nb,
+                            (int64_t)NB * lda + NB, sD, (int64_t)DB * DB));
+  return gemm_launch<float>(DB, DB, DB, dinv + DB * NB + DB, NB, T, DB, false, -1.0f, 0.0f, dinv + DB * NB, NB, false,
+                            s, nb, sD, (int64_t)DB * DB, sD);
+}
+
 // Chain planes of the triangular solves (planes of lda x 256, D =
 // GK_CHAIN_DEPTH), for all block rows kb at once from the finished factor:
 // plane D+1-d: G_d = Dinv_kb L_{kb,kb-d} (kb >= d); plane 2D+1-d: H_d =
@@ -1351,11 +1435,19 @@ static int cholesky_impl(gk_handle h, T* A, int64_t n, int64_t lda, T* dinv, int
       auto sp = [&](int64_t kb) { return Hb[kb % 4]; };
       const int64_t RB = NB * 2 * H3_K;         // halves of NB rows of a split panel
       const bool pairs = split16 && factor_pairs();
+      // panel kb on the side stream; ev[1] marks it factored (split16: its
+      // split form in sp(kb)), the copies back into A follow
+      auto panel = [&](int64_t kb) -> int {
+        T* V = Wb[kb & 1];
+        if (split16) GK_TRY(chol32_panel_split(A, lda, kb, dinv, dinfo, V, s2, sp(kb)));
+        else GK_TRY(chol32_panel(A, lda, kb, dinv, dinfo, V, Tb, s2));
+        GK_HIP(hipEventRecord(ev[1], s2));
+        if (split16) return chol32_panel_split_post(A, lda, kb, V, s2);
+        return chol32_panel_post(A, lda, kb, V, Tb, s2);
+      };
       GK_HIP(hipEventRecord(ev[0], st));
       GK_HIP(hipStreamWaitEvent(s2, ev[0], 0));
-      if (nb > 0) GK_TRY(chol32_panel(A, lda, 0, dinv, dinfo, Wb[0], Tb, s2, sp(0)));
-      GK_HIP(hipEventRecord(ev[1], s2));                    // panel 0 factored
-      if (nb > 0) GK_TRY(chol32_panel_post(A, lda, 0, Wb[0], Tb, s2));
+      if (nb > 0) GK_TRY(panel(0));
       for (int64_t kb = 0; kb + 1 < nb; ++kb) {
         const int64_t k0 = kb * NB, rem = lda - k0 - NB;
         float* Cn = A + (k0 + NB) * lda + k0 + NB;          // column block kb + 1, rows below k0 + NB
@@ -1365,9 +1457,7 @@ static int cholesky_impl(gk_handle h, T* A, int64_t n, int64_t lda, T* dinv, int
           // panel kb was factored); the side stream applies panel kb to it
           // and factors it, beside the trailing update of the last pair.
           GK_TRY(syrk_h3_launch(rem, NB, sp(kb), sp(kb), Cn, lda, false, s2));
-          GK_TRY(chol32_panel(A, lda, kb + 1, dinv, dinfo, Wb[(kb + 1) & 1], Tb, s2, sp(kb + 1)));
-          GK_HIP(hipEventRecord(ev[1], s2));
-          GK_TRY(chol32_panel_post(A, lda, kb + 1, Wb[(kb + 1) & 1], Tb, s2));
+          GK_TRY(panel(kb + 1));
         } else if (pairs) {
           // kb closes the pair (kb - 1, kb): its product with the next two
           // column blocks (rank 512; the block above the diagonal of the
@@ -1379,9 +1469,7 @@ static int cholesky_impl(gk_handle h, T* A, int64_t n, int64_t lda, T* dinv, int
                                 sp(kb - 1) + RB, sp(kb - 1) + RB));
           GK_HIP(hipEventRecord(ev[0], st));
           GK_HIP(hipStreamWaitEvent(s2, ev[0], 0));
-          GK_TRY(chol32_panel(A, lda, kb + 1, dinv, dinfo, Wb[(kb + 1) & 1], Tb, s2, sp(kb + 1)));
-          GK_HIP(hipEventRecord(ev[1], s2));
-          GK_TRY(chol32_panel_post(A, lda, kb + 1, Wb[(kb + 1) & 1], Tb, s2));
+          GK_TRY(panel(kb + 1));
           if (rem > 2 * NB)
             GK_TRY(syrk_h3_launch(rem - 2 * NB, rem - 2 * NB, sp(kb) + 2 * RB, sp(kb) + 2 * RB,
                                   A + (k0 + 3 * NB) * lda + k0 + 3 * NB, lda, true, st, tile_ctr,
@@ -1397,9 +1485,7 @@ static int cholesky_impl(gk_handle h, T* A, int64_t n, int64_t lda, T* dinv, int
                                   false, st));
           GK_HIP(hipEventRecord(ev[0], st));
           GK_HIP(hipStreamWaitEvent(s2, ev[0], 0));
-          GK_TRY(chol32_panel(A, lda, kb + 1, dinv, dinfo, Wb[(kb + 1) & 1], Tb, s2, sp(kb + 1)));
-          GK_HIP(hipEventRecord(ev[1], s2));
-          GK_TRY(chol32_panel_post(A, lda, kb + 1, Wb[(kb + 1) & 1], Tb, s2));
+          GK_TRY(panel(kb + 1));
           if (rem > NB) {                                   // the rest of the trailing update (lower)
             if (split16)
               GK_TRY(syrk_h3_launch(rem - NB, rem - NB, sp(kb) + RB, sp(kb) + RB,
@@ -1418,7 +1504,7 @@ static int cholesky_impl(gk_handle h, T* A, int64_t n, int64_t lda, T* dinv, int
       }
       GK_HIP(hipEventRecord(ev[2], s2));
       GK_HIP(hipStreamWaitEvent(st, ev[2], 0));
-
+      if (split16) GK_TRY(dinv_lower_left(A, lda, dinv, Wb[0], st));
     }
   }
   GK_TRY(chain_planes<T>(A, lda, dinv, st));   // chain planes of the triangular solves

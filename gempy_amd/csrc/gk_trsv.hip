@@ -1,5 +1,9 @@
 // Sync-free blocked triangular solves with a Cholesky factor: one launch for
-// L y = b and one for L^T x = y (in place on b).
+// L y = b (y into a workspace vector) and one for L^T x = y (x into b).  No
+// launch writes a vector that it also reads before the matching flag (the
+// forward c steps read b, the backward ones y): a copy of the old line read
+// on one XCD could otherwise stay in that XCD's L2 and be read there after
+// the new value's flag.
 //
 // L is row-major (ld = lda, a multiple of 256); its 256 x 256 diagonal blocks
 // are not read.  The factorization leaves 1 + 2 D planes of 256 x 256 blocks
@@ -319,7 +323,7 @@ __device__ __forceinline__ bool take_ticket(const TrsvCtl& ctl, Item* it, int* s
 constexpr int MAXCHUNK = 8;              // blocks per far chunk at most
 
 template <typename T>
-__device__ __forceinline__ void far_fwd(const Item& it, const T* __restrict__ L, int64_t lda, T* b, T* part,
+__device__ __forceinline__ void far_fwd(const Item& it, const T* __restrict__ L, int64_t lda, const T* b, T* part,
                                         const TrsvCtl& ctl, T* xall, int* s_ok) {
   typedef typename Vec<T>::type vec;
   constexpr int RPW = Shape<T>::RPW, NT = Shape<T>::NT, VW = Shape<T>::VW, PW = 64 * VW, PS = NB / PW;
@@ -430,8 +434,8 @@ __device__ __forceinline__ void far_bwd(const Item& it, const T* __restrict__ L,
 // of the first block is loaded into registers before the item's waits.
 template <typename T, bool PF>
 __global__ __launch_bounds__(Shape<T>::NT) __attribute__((amdgpu_waves_per_eu(Shape<T>::WPE, Shape<T>::WPE)))
-void trsv_fwd_kernel(const T* __restrict__ L, int64_t lda, const T* __restrict__ planes, T* b, T* cbuf, T* part,
-                     TrsvCtl ctl) {
+void trsv_fwd_kernel(const T* __restrict__ L, int64_t lda, const T* __restrict__ planes, const T* b, T* y_out,
+                     T* cbuf, T* part, TrsvCtl ctl) {
   __shared__ int s_id, s_ok, s_ok2;
   typedef typename Vec<T>::type vec;
   __shared__ __attribute__((aligned(32))) T xs[NB * MAXCHUNK];
@@ -441,7 +445,7 @@ void trsv_fwd_kernel(const T* __restrict__ L, int64_t lda, const T* __restrict__
   Item it;
   if (!take_ticket<T>(ctl, &it, &s_id)) return;
   if (it.s >= 0) {
-    far_fwd<T>(it, L, lda, b, part, ctl, xs, &s_ok);
+    far_fwd<T>(it, L, lda, y_out, part, ctl, xs, &s_ok);
     return;
   }
   const int64_t k0 = (int64_t)it.k * NB;
@@ -458,7 +462,7 @@ void trsv_fwd_kernel(const T* __restrict__ L, int64_t lda, const T* __restrict__
   } else {
     nblk = min(it.k, NEAR - 1);
     M = planes + (NEAR - nblk) * plane + k0 * NB; ld = NB; bstride = plane;
-    x = b + k0 - (int64_t)nblk * NB; wflag = ctl.yflag + it.k - nblk;
+    x = y_out + k0 - (int64_t)nblk * NB; wflag = ctl.yflag + it.k - nblk;
   }
   Rows<T> R;
   if constexpr (PF) {
@@ -518,7 +522,7 @@ void trsv_fwd_kernel(const T* __restrict__ L, int64_t lda, const T* __restrict__
     if (ok && l < RPW) cbuf[k0 + row] = sum;
     group_publish(ctl.cflag + it.k, ctl.epoch);
   } else {
-    if (ok && l < RPW) b[k0 + row] = y - sum;
+    if (ok && l < RPW) y_out[k0 + row] = y - sum;
     group_publish(ctl.yflag + it.k, ctl.epoch);
   }
 }
@@ -527,8 +531,8 @@ void trsv_fwd_kernel(const T* __restrict__ L, int64_t lda, const T* __restrict__
 // as in the forward kernel.
 template <typename T, bool PF>
 __global__ __launch_bounds__(Shape<T>::NT) __attribute__((amdgpu_waves_per_eu(Shape<T>::WPE, Shape<T>::WPE)))
-void trsv_bwd_kernel(const T* __restrict__ L, int64_t lda, const T* __restrict__ planes, T* b, T* cbuf, T* part,
-                     TrsvCtl ctl) {
+void trsv_bwd_kernel(const T* __restrict__ L, int64_t lda, const T* __restrict__ planes, T* b, const T* y_in,
+                     T* cbuf, T* part, TrsvCtl ctl) {
   typedef typename Vec<T>::type vec;
   __shared__ int s_id, s_ok, s_ok2;
   constexpr int RPW = Shape<T>::RPW, NW = Shape<T>::NW, NT = Shape<T>::NT, VW = Shape<T>::VW, PW = 64 * VW;
@@ -565,7 +569,7 @@ void trsv_bwd_kernel(const T* __restrict__ L, int64_t lda, const T* __restrict__
   T cv = T(0);
   if (it.s == CSTEP) {                                        // t = y_k - far partials (the nearest last)
     const int f0 = ctl.rowbase[it.k], nfar = ctl.rowbase[it.k + 1] - f0;
-    T t = threadIdx.x < NB ? b[k0 + threadIdx.x] : T(0);
+    T t = threadIdx.x < NB ? y_in[k0 + threadIdx.x] : T(0);
     if (nfar > 1) {
       ok = group_wait_all(ctl.pflag + f0 + 1, nfar - 1, ctl.epoch, ctl.err, &s_ok2);
       if (ok && threadIdx.x < NB)
@@ -677,9 +681,9 @@ void build_tables(int nb, std::vector<Item>& fwd, std::vector<int>& fbase, std::
 
 }  // namespace
 
-// Workspace: [fwd items][bwd items][fwd base][bwd base][yflag][cflag][pflag][ticket][err][c][partials]
+// Workspace: [fwd items][bwd items][fwd base][bwd base][yflag][cflag][pflag][ticket][err][c][y][partials]
 struct TrsvLayout {
-  size_t o_fi, o_bi, o_fb, o_bb, o_yf, o_cf, o_pf, o_tk, o_err, o_c, o_part, bytes;
+  size_t o_fi, o_bi, o_fb, o_bb, o_yf, o_cf, o_pf, o_tk, o_err, o_c, o_y, o_part, bytes;
   int nfwd, nbwd, slots;
 };
 
@@ -697,6 +701,7 @@ static TrsvLayout trsv_layout(int nb, int nfwd, int nbwd, int slots) {
   L.o_tk = take(sizeof(unsigned long long));
   L.o_err = take(sizeof(int));
   L.o_c = take(sizeof(double) * NB * (size_t)nb);
+  L.o_y = take(sizeof(double) * NB * (size_t)nb);
   L.o_part = take(sizeof(double) * NB * (size_t)std::max(slots, 1));
   L.bytes = o;
   L.nfwd = nfwd; L.nbwd = nbwd; L.slots = slots;
@@ -755,6 +760,7 @@ int gk_trsv_chol_impl(gk_handle h, const T* L, const T* planes, int64_t lda, T* 
   c.err = reinterpret_cast<int*>(base + lay.o_err);
   c.nb = nb;
   T* cbuf = reinterpret_cast<T*>(base + lay.o_c);
+  T* ybuf = reinterpret_cast<T*>(base + lay.o_y);
   T* part = reinterpret_cast<T*>(base + lay.o_part);
   // forward
   c.items = reinterpret_cast<const Item*>(base + lay.o_fi);
@@ -764,8 +770,8 @@ int gk_trsv_chol_impl(gk_handle h, const T* L, const T* planes, int64_t lda, T* 
   c.n_items = lay.nfwd;
   hipEvent_t pa;
   const bool prof = gk_prof_begin(st, &pa);
-  hipLaunchKernelGGL((trsv_fwd_kernel<T, PF>), dim3((unsigned)lay.nfwd), dim3(Shape<T>::NT), 0, st, L, lda, planes, b, cbuf,
-                     part, c);
+  hipLaunchKernelGGL((trsv_fwd_kernel<T, PF>), dim3((unsigned)lay.nfwd), dim3(Shape<T>::NT), 0, st, L, lda, planes, b, ybuf,
+                     cbuf, part, c);
   h->trsv_tickets += (unsigned long long)lay.nfwd;
   // backward
   c.items = reinterpret_cast<const Item*>(base + lay.o_bi);
@@ -773,8 +779,8 @@ int gk_trsv_chol_impl(gk_handle h, const T* L, const T* planes, int64_t lda, T* 
   c.ticket_base = h->trsv_tickets;
   c.epoch = ++h->trsv_epoch;
   c.n_items = lay.nbwd;
-  hipLaunchKernelGGL((trsv_bwd_kernel<T, PF>), dim3((unsigned)lay.nbwd), dim3(Shape<T>::NT), 0, st, L, lda, planes, b, cbuf,
-                     part, c);
+  hipLaunchKernelGGL((trsv_bwd_kernel<T, PF>), dim3((unsigned)lay.nbwd), dim3(Shape<T>::NT), 0, st, L, lda, planes, b, ybuf,
+                     cbuf, part, c);
   h->trsv_tickets += (unsigned long long)lay.nbwd;
   if (prof) {    // each direction reads the blocks at distance >= NEAR, Dinv and the chain planes
     const double far = nb > NEAR ? (double)(nb - NEAR) * (nb - NEAR + 1) / 2.0 : 0.0;
