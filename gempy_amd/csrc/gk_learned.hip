@@ -290,7 +290,8 @@ __global__ __launch_bounds__(256) void local_inverse_kernel(Geo G, int64_t n, co
   constexpr int LD = 3 * K;
   __shared__ double A[LD][LD + 1];
   __shared__ int64_t un[LD];
-  __shared__ int self;
+  __shared__ int orig[LD];                // compact row -> slot row (3 slot + component)
+  __shared__ int self, nc;
   const int tid = threadIdx.x;
   // a fixed grid walks the points: the kernel holds at most gridDim.x CUs'
   // worth of LDS, so a factorization running beside it keeps the rest
@@ -302,44 +303,51 @@ __global__ __launch_bounds__(256) void local_inverse_kernel(Geo G, int64_t n, co
       if (idx[p * K + j] == p) { self = j; break; }
   }
   __syncthreads();
-  // slot s < K - 1: neighbour s (s + 1 past self); slot K - 1: p itself
+  // slot s < K - 1: neighbour s (s + 1 past self); slot K - 1: p itself.
+  // A rest point has one unknown of its three slot rows; the others are
+  // identity rows, which decouple exactly, so the matrix is factored on the
+  // real unknowns only (the three rows of p itself are always kept, last).
   if (tid < LD) {
     const int sl = tid / 3, d = tid % 3;
     const int j = sl == K - 1 ? self : (sl < self ? sl : sl + 1);
     un[tid] = unknown_of(G.s, idx[p * K + j], d);
   }
   __syncthreads();
-  for (int e = tid; e < LD * LD; e += 256) {
-    const int r = e / LD, c = e % LD;
+  if (tid == 0) {
+    int c = 0;
+    for (int r = 0; r < LD; ++r)
+      if (un[r] >= 0 || r >= LD - 3) orig[c++] = r;
+    nc = c;
+  }
+  __syncthreads();
+  const int m_all = nc;
+  for (int e = tid; e < m_all * m_all; e += 256) {
+    const int r = e / m_all, c = e % m_all;
     if (c > r) continue;
-    const int64_t ur = un[r], uc = un[c];
+    const int64_t ur = un[orig[r]], uc = un[orig[c]];
     A[r][c] = (ur < 0 || uc < 0) ? (r == c ? 1.0 : 0.0) : entry(G, ur, uc) * dscale[ur] * dscale[uc];
   }
   __syncthreads();
-  for (int k = 0; k < LD; ++k) {
+  for (int k = 0; k < m_all; ++k) {
     if (tid == 0) A[k][k] = sqrt(A[k][k]);
     __syncthreads();
     const double ip = 1.0 / A[k][k];
-    for (int r = k + 1 + tid; r < LD; r += 256) A[r][k] *= ip;
+    for (int r = k + 1 + tid; r < m_all; r += 256) A[r][k] *= ip;
     __syncthreads();
-    // trailing lower triangle: rows k+1+i, columns k+1+t, t <= i
-    const int m = LD - k - 1;
-    for (int e = tid; e < m * (m + 1) / 2; e += 256) {
-      int i = (int)((sqrtf(8.0f * (float)e + 1.0f) - 1.0f) * 0.5f);
-      while ((i + 1) * (i + 2) / 2 <= e) ++i;
-      while (i * (i + 1) / 2 > e) --i;
-      const int t = e - i * (i + 1) / 2;
-      const int r = k + 1 + i, c = k + 1 + t;
-      A[r][c] -= A[r][k] * A[c][k];
+    // trailing lower triangle: rows k+1+i (16 thread groups), columns k+1+t <= row
+    const int m = m_all - k - 1;
+    for (int i = tid >> 4; i < m; i += 16) {
+      const double ark = A[k + 1 + i][k];
+      for (int t = tid & 15; t <= i; t += 16) A[k + 1 + i][k + 1 + t] -= ark * A[k + 1 + t][k];
     }
     __syncthreads();
   }
   if (tid < 64) {
-    // z = L^-1 e_{LD-3+q}: only z[LD-3 .. LD-1] are nonzero (forward substitution
+    // z = L^-1 e_{n-3+q}: only z[n-3 .. n-1] are nonzero (forward substitution
     // on L's last 3x3 block), then x = L^-T z; lane l owns rows l, l + 64
     constexpr int RPL = (LD + 63) / 64;
     double x[RPL][3];
-    const int b0 = LD - 3;
+    const int b0 = m_all - 3;
     const double l00 = A[b0][b0], l10 = A[b0 + 1][b0], l11 = A[b0 + 1][b0 + 1];
     const double l20 = A[b0 + 2][b0], l21 = A[b0 + 2][b0 + 1], l22 = A[b0 + 2][b0 + 2];
     double z[3][3];                      // z[q][t]: column q, entry b0 + t
@@ -353,9 +361,9 @@ __global__ __launch_bounds__(256) void local_inverse_kernel(Geo G, int64_t n, co
     for (int u = 0; u < RPL; ++u) {
       const int r = tid + 64 * u;
 #pragma unroll
-      for (int q = 0; q < 3; ++q) x[u][q] = r >= b0 && r < LD ? z[q][r - b0] : 0.0;
+      for (int q = 0; q < 3; ++q) x[u][q] = r >= b0 && r < m_all ? z[q][r - b0] : 0.0;
     }
-    for (int k = LD - 1; k >= 0; --k) {
+    for (int k = m_all - 1; k >= 0; --k) {
       const double ikk = 1.0 / A[k][k];
       const int owner = k & 63, uo = k >> 6;
       double xk[3];
@@ -381,16 +389,26 @@ __global__ __launch_bounds__(256) void local_inverse_kernel(Geo G, int64_t n, co
         }
       }
     }
-    // Y[p][j][3 r + c] = (H_loc^-1)[3 self + r][3 j + c] = x_r[slot(j), c]
+    // Y[p][j][3 r + c] = (H_loc^-1)[3 self + r][3 j + c] = x_r[slot(j), c];
+    // the rows of identity unknowns (not factored) hold 0
 #pragma unroll
     for (int u = 0; u < RPL; ++u) {
       const int row = tid + 64 * u;
-      if (row >= LD) continue;
-      const int sl = row / 3, c = row % 3;
+      if (row >= m_all) continue;
+      const int orow = orig[row];
+      const int sl = orow / 3, c = orow % 3;
       const int j = sl == K - 1 ? self : (sl < self ? sl : sl + 1);
 #pragma unroll
       for (int q = 0; q < 3; ++q) Y[((int64_t)p * K + j) * 9 + 3 * q + c] = (float)x[u][q];
     }
+  }
+  // zeros for the slot rows left out (identity unknowns of neighbours)
+  for (int row = tid; row < LD - 3; row += 256) {
+    if (un[row] >= 0) continue;
+    const int sl = row / 3, c = row % 3;
+    const int j = sl < self ? sl : sl + 1;
+#pragma unroll
+    for (int q = 0; q < 3; ++q) Y[((int64_t)p * K + j) * 9 + 3 * q + c] = 0.0f;
   }
   }
 }
