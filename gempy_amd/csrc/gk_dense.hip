@@ -341,7 +341,7 @@ void gemm_dispatch(dim3 grid, bool trans_b, bool lower, hipStream_t st, int64_t 
 template <typename T>
 int gemm_launch(int64_t m, int64_t n, int64_t k, const T* A, int64_t lda, const T* B, int64_t ldb,
                 bool trans_b, T alpha, T beta, T* C, int64_t ldc, bool lower, hipStream_t st, int64_t batch = 1,
-                int64_t sA = 0, int64_t sB = 0, int64_t sC = 0) {
+                int64_t sA = 0, int64_t sB = 0, int64_t sC = 0, bool row_tiles = false) {
   constexpr int BK = Mfma<T>::BK;
   GK_REQUIRE(m % 128 == 0 && n % 128 == 0 && k % BK == 0, GK_ERR_ARG,
              "gk_gemm: m, n must be multiples of 128 and k of %d (got %lld %lld %lld)", BK,
@@ -360,7 +360,11 @@ int gemm_launch(int64_t m, int64_t n, int64_t k, const T* A, int64_t lda, const 
   // m in the thousands, n = 256) and the 128^3 products of the diagonal
   // blocks have too few 128 x 128 tiles to fill the chip (one, for the
   // latter); 64 x 64 tiles of 4 waves give 4x as many workgroups.
-  const bool narrow = sizeof(T) == 4 && !lower && batch == 1 && ((n <= 256 && m >= 2048) || m * n <= 256 * 256);
+  // row_tiles (n == 128): one tile spans all of C's columns, so a workgroup
+  // reads every A row it writes before writing it (in place, C == A, is safe).
+  GK_REQUIRE(!row_tiles || (n == 128 && !lower), GK_ERR_ARG, "gk_gemm: row tiles need n == 128");
+  const bool narrow = !row_tiles && sizeof(T) == 4 && !lower && batch == 1 &&
+                      ((n <= 256 && m >= 2048) || m * n <= 256 * 256);
   const int64_t TMe = narrow ? 64 : 128;
   const int64_t tm = m / TMe, tn = n / TMe;
   const int64_t tiles = lower ? tm * (tm + 1) / 2 : tm * tn;
@@ -404,11 +408,12 @@ constexpr int64_t H3_ROWBLK = 2 * 128 * H3_K; // halves of a 128-row block (both
 // Element (r, k): ((r / 128) * H3_NK + k / 32) * 2 H3_TILE + [lo] H3_TILE
 // + (r % 128) * 32 + k % 32.  A row offset R (a multiple of 128) is R * 2 H3_K
 // halves.
-__global__ void split_h3_kernel(const float* __restrict__ W, int64_t n8, _Float16* __restrict__ P) {
+__global__ void split_h3_kernel(const float* __restrict__ W, int64_t ldw, int64_t n8, _Float16* __restrict__ P) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n8) return;
-  const f4v a = reinterpret_cast<const f4v*>(W)[2 * i];
-  const f4v b = reinterpret_cast<const f4v*>(W)[2 * i + 1];
+  const f4v* w = reinterpret_cast<const f4v*>(W + (i / (H3_K / 8)) * ldw + (i % (H3_K / 8)) * 8);
+  const f4v a = w[0];
+  const f4v b = w[1];
   h8v hi, lo;
 #pragma unroll
   for (int e = 0; e < 4; ++e) {
@@ -626,11 +631,12 @@ __global__ __launch_bounds__(256, 2) void syrk_h3_kernel(const _Float16* __restr
 
 // Split an FP32 panel (rows x 256, dense, rows a multiple of 128) into the
 // tiled fp16 layout.
-int split_h3(const float* W, int64_t rows, _Float16* P, hipStream_t st) {
-  GK_REQUIRE(rows % 128 == 0, GK_ERR_ARG, "split_h3: rows must be a multiple of 128");
+int split_h3(const float* W, int64_t rows, _Float16* P, hipStream_t st, int64_t ldw = H3_K) {
+  GK_REQUIRE(rows % 128 == 0 && ldw >= H3_K && ldw % 4 == 0, GK_ERR_ARG,
+             "split_h3: rows must be a multiple of 128");
   const int64_t n8 = rows * H3_K / 8;
   if (n8 > 0)
-    hipLaunchKernelGGL(split_h3_kernel, dim3((unsigned)gk_ceil_div(n8, 256)), dim3(256), 0, st, W, n8, P);
+    hipLaunchKernelGGL(split_h3_kernel, dim3((unsigned)gk_ceil_div(n8, 256)), dim3(256), 0, st, W, ldw, n8, P);
   GK_HIP(hipGetLastError());
   return GK_OK;
 }
@@ -1290,36 +1296,25 @@ static int chol32_panel(float* A, int64_t lda, int64_t kb, float* dinv, int32_t*
 //   [A22; Ab2] -= [L21; W1] L21^T;  X22 = L22^-1;  W2 = Ab2 X22^T,
 // so W = [W1 W2] = Ab Dinv^T with Dinv = [X11 0; -X22 L21 X11  X22], whose
 // lower-left quadrant (needed only by the triangular solves) is formed for
-// all panels at the end (dinv_lower_left).  Six kernels in sequence where the
-// Dinv form takes eight.  V holds DB + rem rows of 256: L21 (first 128
-// columns of its first DB rows), then W.
-static int chol32_panel_split(float* A, int64_t lda, int64_t kb, float* dinv, int32_t* dinfo, float* V,
-                              hipStream_t s, _Float16* P) {
+// all panels at the end (dinv_lower_left).  The products run in place in A
+// on tiles spanning their 128 columns, and W is split from A: six kernels in
+// sequence and no copies.
+static int chol32_panel_split(float* A, int64_t lda, int64_t kb, float* dinv, int32_t* dinfo, hipStream_t s,
+                              _Float16* P) {
   const int64_t k0 = kb * NB, rem = lda - k0 - NB;
   float* D = dinv + k0 * NB;                                     // 256 x 256, ld NB
+  float* C1 = A + (k0 + DB) * lda + k0;                          // [A21; Ab1]
   Potrf128<float>::launch(A + k0 * lda + k0, lda, D, NB, dinfo, k0, s);
-  GK_TRY(gemm_launch<float>(DB + rem, DB, DB, A + (k0 + DB) * lda + k0, lda, D, NB, true, 1.0f, 0.0f, V, NB, false,
-                            s));                                 // [L21; W1]
-  GK_TRY(gemm_launch<float>(DB + rem, DB, DB, V, NB, V, NB, true, -1.0f, 1.0f, A + (k0 + DB) * lda + k0 + DB, lda,
-                            false, s));                          // [A22; Ab2] -= [L21; W1] L21^T
-  Potrf128<float>::launch(A + (k0 + DB) * lda + k0 + DB, lda, D + DB * NB + DB, NB, dinfo, k0 + DB, s);
+  GK_TRY(gemm_launch<float>(DB + rem, DB, DB, C1, lda, D, NB, true, 1.0f, 0.0f, C1, lda, false, s, 1, 0, 0, 0,
+                            true));                              // [L21; W1], in place
+  GK_TRY(gemm_launch<float>(DB + rem, DB, DB, C1, lda, C1, lda, true, -1.0f, 1.0f, C1 + DB, lda, false, s));
+  Potrf128<float>::launch(C1 + DB, lda, D + DB * NB + DB, NB, dinfo, k0 + DB, s);
   if (rem > 0) {
-    GK_TRY(gemm_launch<float>(rem, DB, DB, A + (k0 + NB) * lda + k0 + DB, lda, D + DB * NB + DB, NB, true, 1.0f,
-                              0.0f, V + DB * NB + DB, NB, false, s));   // W2
-    GK_TRY(split_h3(V + DB * NB, rem, P, s));
+    float* C2 = A + (k0 + NB) * lda + k0 + DB;                   // Ab2
+    GK_TRY(gemm_launch<float>(rem, DB, DB, C2, lda, D + DB * NB + DB, NB, true, 1.0f, 0.0f, C2, lda, false, s, 1, 0, 0,
+                              0, true));                         // W2, in place
+    GK_TRY(split_h3(A + (k0 + NB) * lda + k0, rem, P, s, lda));
   }
-  return GK_OK;
-}
-
-// L21 and W of chol32_panel_split back into A (columns no other kernel
-// touches until the end).
-static int chol32_panel_split_post(float* A, int64_t lda, int64_t kb, const float* V, hipStream_t s) {
-  const int64_t k0 = kb * NB, rem = lda - k0 - NB;
-  hipLaunchKernelGGL(copy_block_kernel<float>, dim3(DB * DB / 256), dim3(256), 0, s, V, (int64_t)NB,
-                     A + (k0 + DB) * lda + k0, lda, (int64_t)DB, (int64_t)DB);
-  if (rem > 0)
-    hipLaunchKernelGGL(copy_block_kernel<float>, dim3((unsigned)gk_ceil_div(rem * NB, 256)), dim3(256), 0, s,
-                       V + DB * NB, (int64_t)NB, A + (k0 + NB) * lda + k0, lda, rem, (int64_t)NB);
   return GK_OK;
 }
 
@@ -1436,13 +1431,13 @@ static int cholesky_impl(gk_handle h, T* A, int64_t n, int64_t lda, T* dinv, int
       const int64_t RB = NB * 2 * H3_K;         // halves of NB rows of a split panel
       const bool pairs = split16 && factor_pairs();
       // panel kb on the side stream; ev[1] marks it factored (split16: its
-      // split form in sp(kb)), the copies back into A follow
+      // split form in sp(kb)); without split16 the copies back into A follow
       auto panel = [&](int64_t kb) -> int {
         T* V = Wb[kb & 1];
-        if (split16) GK_TRY(chol32_panel_split(A, lda, kb, dinv, dinfo, V, s2, sp(kb)));
+        if (split16) GK_TRY(chol32_panel_split(A, lda, kb, dinv, dinfo, s2, sp(kb)));
         else GK_TRY(chol32_panel(A, lda, kb, dinv, dinfo, V, Tb, s2));
         GK_HIP(hipEventRecord(ev[1], s2));
-        if (split16) return chol32_panel_split_post(A, lda, kb, V, s2);
+        if (split16) return GK_OK;
         return chol32_panel_post(A, lda, kb, V, Tb, s2);
       };
       GK_HIP(hipEventRecord(ev[0], st));
