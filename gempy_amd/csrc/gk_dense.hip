@@ -14,6 +14,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdlib>
+#include <type_traits>
 #include "gk_internal.h"
 
 namespace {
@@ -841,50 +842,68 @@ template <typename T, int PB, int NT>
 __device__ void potrf_blocked(BlkStore<T, PB> B, T* A, int64_t lda, T* X, int64_t ldx, int32_t* info, int64_t gbase) {
   constexpr int RS = NT / 32;                 // rows of the trailing update in flight
   const int tid = threadIdx.x, tx = tid & 31, ty = tid >> 5, lane = tid & 63, wave = tid >> 6;
-  // all loads of the block first (in flight together), then the LDS stores
-  constexpr int PER = DB * DB / NT;
-  static_assert(PER * NT == DB * DB, "the block must be a whole number of elements per thread");
-  T vin[PER];
+  // all loads of the block first (in flight together, 16 bytes each), then
+  // the LDS stores
+  constexpr int VL = 16 / sizeof(T);          // elements per 16-byte load
+  constexpr int PER = DB * DB / (NT * VL);
+  static_assert(PER * NT * VL == DB * DB, "the block must be a whole number of vectors per thread");
+  typedef T vt __attribute__((ext_vector_type(VL)));
+  vt vin[PER];
 #pragma unroll
   for (int q = 0; q < PER; ++q) {
-    const int e = tid + q * NT, i = e / DB, c = e % DB;
-    vin[q] = c <= i ? A[i * lda + c] : T(0);
+    const int e = (tid + q * NT) * VL, i = e / DB, c = e % DB;
+    if (c <= i) vin[q] = *reinterpret_cast<const vt*>(A + i * lda + c);
+    else
+#pragma unroll
+      for (int u = 0; u < VL; ++u) vin[q][u] = T(0);
   }
 #pragma unroll
   for (int q = 0; q < PER; ++q) {
-    const int e = tid + q * NT, i = e / DB, c = e % DB;
-    if (c <= i) {
-      B.L(i, c) = vin[q];
-      B.X(i, c) = T(0);
-      if (c < PB) B.P(0, c, i) = vin[q];
+    const int e = (tid + q * NT) * VL, i = e / DB, c = e % DB;
+#pragma unroll
+    for (int u = 0; u < VL; ++u) {
+      if (c + u <= i) {
+        B.L(i, c + u) = vin[q][u];
+        B.X(i, c + u) = T(0);
+        if (c + u < PB) B.P(0, c + u, i) = vin[q][u];
+      }
     }
   }
   __syncthreads();
   int bad_col = 0;                            // first non-positive pivot + 1 (wave 0)
-  // FP32: the rows of X are computed by the last wave from the stored panel
-  // while the other waves apply the panel to the trailing L tiles; the X
-  // tiles follow after a barrier.
+  // FP32: the rows of X are computed by the last two waves from the stored
+  // panel (64 columns each) while the other waves apply the panel to the
+  // trailing L tiles; the X tiles follow beside the next panel's factor.
   constexpr bool SPLIT_X = sizeof(T) == 4 && PB == 16 && NT >= 128;
-  constexpr int XW = NT / 64 - 1;             // the wave of the X rows
+  constexpr int XW = NT / 64 - 1;             // the last of the two waves of the X rows
   // ---- rows jb .. jb+PB-1 of X from the panel's diagonal block (lane l holds
   //      row jb + l in a) and the reciprocal pivots: lane owns columns lane
   //      and lane + 64
-  auto x_rows = [&](int jb, const T (&a)[PB], const T (&ipv)[PB]) {
+  // pb >= 0: the panel's entries come from the stored panel P[pb] (LDS
+  // broadcasts: a wave other than the one that factored it), else from a.
+  auto x_rows = [&](int jb, const T (&a)[PB], const T (&ipv)[PB], int pb) {
     T xa[PB], xb[PB];
     const int ca = lane, cb = lane + 64;
 #pragma unroll
     for (int r = 0; r < PB; ++r) {
       const int i = jb + r;
-      T va = ca < jb ? B.X(i, ca) : (ca == i ? T(1) : T(0));   // B.X holds -acc[i][c] for c < jb
-      T vb = cb < jb ? B.X(i, cb) : (cb == i ? T(1) : T(0));
+      xa[r] = ca < jb ? B.X(i, ca) : (ca == i ? T(1) : T(0));  // B.X holds -acc[i][c] for c < jb
+      xb[r] = cb < jb ? B.X(i, cb) : (cb == i ? T(1) : T(0));
+    }
+    // right-looking: row k is final once the rows above it are applied, and
+    // then goes into every later row (each row takes the same products in the
+    // same order as a row-by-row substitution; the dependent chain is 16
+    // steps, not 120)
 #pragma unroll
-      for (int k = 0; k < r; ++k) {
-        const T lik = lane_value(a[k], r);        // L[jb+r][jb+k]
-        va -= lik * xa[k];
-        vb -= lik * xb[k];
+    for (int k = 0; k < PB; ++k) {
+      xa[k] *= ipv[k];
+      xb[k] *= ipv[k];
+#pragma unroll
+      for (int r = k + 1; r < PB; ++r) {
+        const T lrk = pb >= 0 ? B.P(pb, k, jb + r) : lane_value(a[k], r);   // L[jb+r][jb+k]
+        xa[r] -= lrk * xa[k];
+        xb[r] -= lrk * xb[k];
       }
-      xa[r] = va * ipv[r];
-      xb[r] = vb * ipv[r];
     }
 #pragma unroll
     for (int r = 0; r < PB; ++r) {
@@ -892,6 +911,27 @@ __device__ void potrf_blocked(BlkStore<T, PB> B, T* A, int64_t lda, T* X, int64_
       if (ca <= i) B.X(i, ca) = xa[r];
       if (cb <= i) B.X(i, cb) = xb[r];
     }
+  };
+  // the same for one column per lane (c = lane + coff), the panel's entries
+  // from the stored panel P[pb] (LDS broadcasts): two waves share the rows
+  auto x_rows1 = [&](int jb, int pb, int coff) {
+    T x[PB], ipv[PB];
+    const int c = lane + coff;
+#pragma unroll
+    for (int r = 0; r < PB; ++r) {
+      const int i = jb + r;
+      x[r] = c < jb ? B.X(i, c) : (c == i ? T(1) : T(0));
+      ipv[r] = B.ipv(r);
+    }
+#pragma unroll
+    for (int k = 0; k < PB; ++k) {
+      x[k] *= ipv[k];
+#pragma unroll
+      for (int r = k + 1; r < PB; ++r) x[r] -= B.P(pb, k, jb + r) * x[k];   // L[jb+r][jb+k]
+    }
+#pragma unroll
+    for (int r = 0; r < PB; ++r)
+      if (c <= jb + r) B.X(jb + r, c) = x[r];
   };
   // ---- phase 2 on MFMA (16 x 16 x 4 f32), one 16 x 16 tile per wave at a time
   //      (waves w, w + nw, ...):
@@ -945,6 +985,11 @@ __device__ void potrf_blocked(BlkStore<T, PB> B, T* A, int64_t lda, T* X, int64_
     }
   };
   for (int jb = 0, pb = 0; jb < DB; jb += PB, pb ^= 1) {
+    if constexpr (SPLIT_X) {
+      // the previous step's X tiles beside this step's panel (they touch
+      // only X, which the panel does not read)
+      if (wave > 0 && jb > 0) mfma_tiles(jb - PB, pb ^ 1, jb, false, wave - 1, NT / 64 - 1);
+    }
     if (wave == 0) {
       // ---- panel: lane l holds rows jb + l (a) and jb + 64 + l (b)
       const int ra = jb + lane, rb = jb + 64 + lane;
@@ -956,25 +1001,32 @@ __device__ void potrf_blocked(BlkStore<T, PB> B, T* A, int64_t lda, T* X, int64_
         a[c] = (ra < DB && jb + c <= ra) ? va : T(0);
         b[c] = rb < DB ? vb : T(0);
       }
+      // column j below the diagonal (aj, 0 on and above it) goes into the later
+      // columns unmasked; the rows jb + 64 + l exist only while jb < 64
+      auto factor = [&](auto with_b) {
 #pragma unroll
-      for (int j = 0; j < PB; ++j) {
-        T d = lane_value(a[j], j);                // L[jb+j][jb+j], held by lane j
-        if (!(d > T(0))) {
-          if (bad_col == 0) bad_col = jb + j + 1;
-          d = T(1);
-        }
-        T piv, ip;
-        pivot_pair(d, piv, ip);
-        ipv[j] = ip;
-        a[j] = lane == j ? piv : (lane > j ? a[j] * ip : a[j]);
-        b[j] *= ip;
+        for (int j = 0; j < PB; ++j) {
+          T d = lane_value(a[j], j);              // L[jb+j][jb+j], held by lane j
+          if (!(d > T(0))) {
+            if (bad_col == 0) bad_col = jb + j + 1;
+            d = T(1);
+          }
+          T piv, ip;
+          pivot_pair(d, piv, ip);
+          ipv[j] = ip;
+          const T aj = lane > j ? a[j] * ip : T(0);
+          if constexpr (decltype(with_b)::value) b[j] *= ip;
 #pragma unroll
-        for (int c = j + 1; c < PB; ++c) {
-          const T lcj = lane_value(a[j], c);      // L[jb+c][jb+j]
-          if (lane > j) a[c] -= a[j] * lcj;
-          b[c] -= b[j] * lcj;
+          for (int c = j + 1; c < PB; ++c) {
+            const T lcj = lane_value(aj, c);      // L[jb+c][jb+j]
+            a[c] -= aj * lcj;
+            if constexpr (decltype(with_b)::value) b[c] -= b[j] * lcj;
+          }
+          a[j] = lane == j ? piv : (lane > j ? aj : a[j]);
         }
-      }
+      };
+      if (jb < 64) factor(std::true_type{});
+      else factor(std::false_type{});
 #pragma unroll
       for (int c = 0; c < PB; ++c) {
         if (ra < DB && jb + c <= ra) B.L(ra, jb + c) = a[c];
@@ -985,29 +1037,19 @@ __device__ void potrf_blocked(BlkStore<T, PB> B, T* A, int64_t lda, T* X, int64_
       if constexpr (SPLIT_X) {
         if (lane < PB) B.ipv(lane) = ipv[lane];
       } else {
-        x_rows(jb, a, ipv);
+        x_rows(jb, a, ipv, -1);
       }
     }
     __syncthreads();
     const int c0 = jb + PB;
     if constexpr (SPLIT_X) {
-      if (wave == XW) {
-        // the panel's diagonal block as wave 0 held it: lane l, row jb + l
-        T a[PB], ipv[PB];
-        const int rac = min(jb + lane, DB - 1);
-#pragma unroll
-        for (int c = 0; c < PB; ++c) {
-          a[c] = B.P(pb, c, rac);
-          ipv[c] = B.ipv(c);
-        }
-        x_rows(jb, a, ipv);
+      if (wave >= XW - 1) {
+        x_rows1(jb, pb, wave == XW ? 64 : 0);
       } else {
-        mfma_tiles(jb, pb, c0, true, wave, XW);
+        mfma_tiles(jb, pb, c0, true, wave, XW - 1);
       }
       __syncthreads();
-      mfma_tiles(jb, pb, c0, false, wave, NT / 64);
-      __syncthreads();
-      continue;
+      continue;                               // (the last step has no X tiles)
     }
     // ---- rows i >= c0: L[i][c] -= sum_t L[i][jb+t] L[c][jb+t]   (c0 <= c <= i;
     //      the next panel's columns also into P[pb^1])
@@ -1037,10 +1079,22 @@ __device__ void potrf_blocked(BlkStore<T, PB> B, T* A, int64_t lda, T* X, int64_
     __syncthreads();
   }
   if (wave == 0 && lane == 0 && bad_col != 0) atomicCAS(info, 0, (int32_t)(gbase + bad_col));
-  for (int e = tid; e < DB * DB; e += NT) {
+  for (int e = tid * VL; e < DB * DB; e += NT * VL) {
     const int i = e / DB, c = e % DB;
-    if (c <= i) A[i * lda + c] = B.L(i, c);
-    X[i * ldx + c] = (c <= i) ? B.X(i, c) : T(0);
+    vt l, x;
+#pragma unroll
+    for (int u = 0; u < VL; ++u) {
+      l[u] = c + u <= i ? B.L(i, c + u) : T(0);
+      x[u] = c + u <= i ? B.X(i, c + u) : T(0);
+    }
+    if (c + VL - 1 <= i) {
+      *reinterpret_cast<vt*>(A + i * lda + c) = l;
+    } else {
+#pragma unroll
+      for (int u = 0; u < VL; ++u)
+        if (c + u <= i) A[i * lda + c + u] = l[u];
+    }
+    *reinterpret_cast<vt*>(X + i * ldx + c) = x;
   }
 }
 
