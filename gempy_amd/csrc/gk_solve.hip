@@ -432,26 +432,39 @@ struct Solver {
     return setup_drift();
   }
 
-  // Y = K^-1 U with the current apply_K and the inverse of S = U^T Y (Lt x Lt, host)
+  // Y = K^-1 U with apply_K (the factor without the local correction) and the
+  // inverse of S = U^T Y (Lt x Lt, host)
   int setup_drift() {
     if (Lt > 0) {
       GK_TRY(gk_assemble_drift_impl(sys, U, st));
+      // The drift columns take the factor alone (without the local-inverse
+      // correction and its operator product): the flexible iteration does not
+      // need Y to come from the same approximation of K, and at config 3 the
+      // iteration count is the same.
       for (int l = 0; l < Lt; ++l) {
         hipLaunchKernelGGL(get_col_kernel, dim3(nblk(Nc)), dim3(256), 0, st, U, (int)Lt, l, r, Nc);
-        GK_TRY(apply_K(r, w));
+        const bool keep = pcp;
+        pcp = false;
+        const int rc_k = apply_K(r, w);
+        pcp = keep;
+        GK_TRY(rc_k);
         hipLaunchKernelGGL(set_col_kernel, dim3(nblk(Nc)), dim3(256), 0, st, w, Y, (int)Lt, l, Nc);
       }
-      // S = U^T Y (Lt x Lt), inverted on the host (Lt <= 17)
-      std::vector<double> S(Lt * Lt), col(Lt);
+      // S = U^T Y (Lt x Lt, column l at hdev + 512 + l Lt), one read-back,
+      // inverted on the host
+      GK_REQUIRE(Lt <= 22, GK_ERR_ARG, "gk_solve: at most 22 drift and fault columns (got %d)", (int)Lt);
+      std::vector<double> S(Lt * Lt), cols(Lt * Lt);
       for (int l = 0; l < Lt; ++l) {
         hipLaunchKernelGGL(get_col_kernel, dim3(nblk(Nc)), dim3(256), 0, st, Y, (int)Lt, l, w, Nc);
         const unsigned nb = nblk(Nc, RED_ROWS);
         hipLaunchKernelGGL(drift_dots_kernel, dim3(nb), dim3(256), 0, st, U, (int)Lt, w, Nc, partial);
-        hipLaunchKernelGGL(reduce_cols_kernel, dim3(1), dim3(128), 0, st, partial, (int)nb, (int)Lt, hdev);
-        GK_HIP(hipMemcpyAsync(col.data(), hdev, Lt * sizeof(double), hipMemcpyDeviceToHost, st));
-        GK_HIP(hipStreamSynchronize(st));
-        for (int k = 0; k < Lt; ++k) S[k * Lt + l] = col[k];
+        hipLaunchKernelGGL(reduce_cols_kernel, dim3(1), dim3(128), 0, st, partial, (int)nb, (int)Lt,
+                           hdev + 512 + l * Lt);
       }
+      GK_HIP(hipMemcpyAsync(cols.data(), hdev + 512, Lt * Lt * sizeof(double), hipMemcpyDeviceToHost, st));
+      GK_HIP(hipStreamSynchronize(st));
+      for (int l = 0; l < Lt; ++l)
+        for (int k = 0; k < Lt; ++k) S[k * Lt + l] = cols[l * Lt + k];
       std::vector<double> inv(Lt * Lt, 0.0);
       for (int i = 0; i < Lt; ++i) inv[i * Lt + i] = 1.0;
       for (int c = 0; c < Lt; ++c) {           // Gauss-Jordan with partial pivoting
@@ -728,7 +741,7 @@ static int init_solver(gk_handle h, const gk_system* sys, hipStream_t st, int fo
   const size_t o_Z = take((size_t)m * Npad * sizeof(double));
   const size_t o_w = take(Npad * sizeof(double));
   const size_t o_r = take(Npad * sizeof(double));
-  const size_t o_h = take(256 * sizeof(double));
+  const size_t o_h = take(1024 * sizeof(double));     // [0, 256): dots and coefficients; [512, 801): S
   const size_t o_hc = take((size_t)m * (m + 2) * sizeof(double));
   const size_t o_b = take(Npad * sizeof(double));
   const size_t o_xt = take(Npad * sizeof(double));
