@@ -533,9 +533,12 @@ struct Solver {
       *out = nr / nb;
       return GK_OK;
     };
+    bool x_zero = x0 == nullptr;           // the first cycle from x = 0 has r = b
     while (!done) {
-      double rel0 = 0.0;
-      GK_TRY(true_rel(x, &rel0));         // r = b - A x
+      double rel0 = 1.0;
+      if (x_zero) GK_HIP(hipMemcpyAsync(r, b, N * sizeof(double), hipMemcpyDeviceToDevice, st));
+      else GK_TRY(true_rel(x, &rel0));     // r = b - A x
+      x_zero = false;
       last_true = rel0;
       if (rel0 <= tol || total >= max_iter) { *ok = rel0 <= accept; break; }
       const double beta = rel0 * nb;
