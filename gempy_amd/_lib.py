@@ -111,6 +111,7 @@ SIGNATURES = {
     "gk_condition_number": (C.c_int, [P, C.POINTER(GkSystem), C.c_int32, C.c_double, C.POINTER(C.c_double),
                                       C.POINTER(C.c_double), C.POINTER(C.c_int32), P]),
     "gk_test_knn_local": (C.c_int, [P, C.POINTER(GkSystem), P, P, P]),
+    "gk_test_cholesky_f32_split16": (C.c_int, [P, P, C.c_int64, C.c_int64, P, C.POINTER(C.c_int32), P]),
     "gk_test_syrk_split16": (C.c_int, [P, C.c_int64, C.c_int64, P, P, P, P, P, C.c_int64, C.c_int32, P]),
     "gk_finite_fault_values": (C.c_int, [P, C.c_int64, P, C.POINTER(C.c_double), C.POINTER(C.c_double),
                                          C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double),

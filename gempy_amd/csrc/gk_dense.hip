@@ -1570,6 +1570,12 @@ extern "C" int gk_cholesky_f32_split16(gk_handle h, float* A, int64_t n, int64_t
   return cholesky_impl<float>(h, A, n, lda, dinv, info, (hipStream_t)stream, true);
 }
 
+extern "C" int gk_test_cholesky_f32_split16(gk_handle h, float* A, int64_t n, int64_t lda, float* dinv,
+                                            int32_t* info, gk_stream stream) {
+  GK_REQUIRE(h && A && dinv, GK_ERR_ARG, "gk_test_cholesky_f32_split16: NULL argument");
+  return cholesky_impl<float>(h, A, n, lda, dinv, info, (hipStream_t)stream, true);
+}
+
 extern "C" int gk_test_syrk_split16(gk_handle h, int64_t m, int64_t n, const float* A, const float* B,
                                     const float* A2, const float* B2, float* C, int64_t ldc, int32_t lower,
                                     gk_stream stream) {

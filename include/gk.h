@@ -427,6 +427,13 @@ int gk_test_trsv_set_error(gk_handle h, int32_t code, gk_stream stream);
  * reference. */
 #define GK_LOCAL_NEIGHBOURS 32
 int gk_test_knn_local(gk_handle h, const gk_system* sys, int32_t* idx, float* hk, gk_stream stream);
+/* Testing: the FP32 factorization with split-fp16 updates that the FP32 solve
+ * form uses (paired rank-512 trailing updates, blocked panels), with the
+ * arguments and outputs of gk_cholesky_f32 (L in A's lower triangle, the
+ * chain planes in dinv); A must be equilibrated (unit diagonal).  No
+ * counterpart in the reference. */
+int gk_test_cholesky_f32_split16(gk_handle h, float* A, int64_t n, int64_t lda, float* dinv, int32_t* info,
+                                 gk_stream stream);
 int gk_test_syrk_split16(gk_handle h, int64_t m, int64_t n, const float* A, const float* B, const float* A2,
                          const float* B2, float* C, int64_t ldc, int32_t lower, gk_stream stream);
 

@@ -85,3 +85,46 @@ def test_split16_persistent_lower_update_twice(gpu, k):
     e32 = np.abs(f32 - ref)[~blk].max()
     say(f"split16 persistent: max err {err:.3e}, FP32 GEMM {e32:.3e}")
     assert err <= 4 * e32 + 1e-6 * np.abs(ref).max()
+
+
+@pytest.mark.parametrize("nb", [1, 2, 3, 4, 5, 8])
+def test_split16_factorization_pairs(gpu, nb):
+    """The FP32 form's factorization (panels in pairs, rank-512 updates, blocked
+    panels, the inverse's lower-left quadrants formed at the end) at an odd and
+    even number of 256-panels, padded: L L^T against the unit-diagonal SPD
+    matrix, every diagonal block's stored inverse against L_kk, and the
+    triangular solves through the chain planes against numpy."""
+    from gempy_amd import _lib
+    from gempy_amd.engine import _ptr, _stream, handle
+    lib = _lib.load()
+    lda = 256 * nb
+    n = lda - 37
+    say(f"split16 factorization: nb {nb}, n {n}")
+    rng = np.random.default_rng(100 + nb)
+    M = rng.standard_normal((n, n))
+    A = M @ M.T / n + 0.5 * np.eye(n)
+    d = np.sqrt(np.diag(A))
+    A = A / d[:, None] / d[None, :]
+    tA = torch.zeros((lda, lda), dtype=torch.float32, device="cuda")
+    tA[:n, :n] = torch.tensor(A, dtype=torch.float32)
+    dinv = torch.empty((_lib.GK_CHOL_PLANES, lda, 256), dtype=torch.float32, device="cuda")
+    info = C.c_int32(-1)
+    _lib.check("gk_test_cholesky_f32_split16", lib.gk_test_cholesky_f32_split16(
+        handle(0), _ptr(tA), n, lda, _ptr(dinv), C.byref(info), _stream()))
+    assert info.value == 0
+    Lf = torch.tril(tA).double().cpu().numpy()
+    err = np.abs(Lf[:n, :n] @ Lf[:n, :n].T - A).max()
+    say(f"  max |L L^T - A| {err:.3e}")
+    assert err <= 5e-5
+    D = dinv[0].double().cpu().numpy()
+    for k in range(nb):
+        blk = slice(256 * k, 256 * (k + 1))
+        e = np.abs(D[blk] @ Lf[blk, blk] - np.eye(256)).max()
+        assert e <= 1e-4, (k, e)
+    b = rng.standard_normal(n)
+    tb = torch.zeros(lda, dtype=torch.float32, device="cuda")
+    tb[:n] = torch.tensor(b, dtype=torch.float32)
+    _lib.check("gk_cholesky_solve_f32", lib.gk_cholesky_solve_f32(handle(0), _ptr(tA), _ptr(dinv), lda, _ptr(tb),
+                                                                  _stream()))
+    x = np.linalg.solve(A, b)
+    np.testing.assert_allclose(tb[:n].double().cpu().numpy(), x, rtol=0, atol=1e-4 * np.abs(x).max())
