@@ -15,6 +15,12 @@ void gk_set_error(const char* fmt, ...) {
 extern "C" const char* gk_last_error(void) { return g_err; }
 extern "C" int gk_version(void) { return 100; }
 
+// Workspace slot of at least `bytes` (grown by 1/8 beyond the request).
+// Growing a slot synchronizes the device before freeing the old buffer (work
+// queued on any stream may still read it), so a slot that grows in the middle
+// of a pipelined solve would stall its side streams.  Entry points take their
+// slots before their first launch, and repeated solves of one size never grow
+// a slot after the first.
 int gk_workspace(gk_handle h, int slot, size_t bytes, void** out) {
   GK_REQUIRE(h && slot >= 0 && slot < WS_COUNT, GK_ERR_ARG, "gk: bad handle or workspace slot");
   if (bytes > h->ws_bytes[slot]) {
