@@ -19,6 +19,16 @@
 namespace {
 
 constexpr int TS = 256;   // sources per LDS tile == threads per block
+// relative cost of one (point, source) pair: interface source or orientation,
+// without or with the gradient
+constexpr int src_cost(bool grad, bool ori) { return grad ? (ori ? 7 : 5) : (ori ? 5 : 4); }
+// Explicit point sets too small to fill the GPU split their sources over
+// blockIdx.y: at most SPLIT_WGS workgroups in all, at most MAX_SPLIT splits
+// with the gradient and MAX_SPLIT_SCALAR without.  Measured on the
+// operator product at N = 35k (scalar at 20,000 data points, gradient at
+// 5,000 orientations, FP64): 288 us at 64 splits against 308 at 102 for the
+// scalar, 123 us at 204 splits against 135 at 64 for the gradient.
+constexpr int SPLIT_WGS = 2048, MAX_SPLIT = 256, MAX_SPLIT_SCALAR = 64;
 
 template <typename T> struct alignas(sizeof(T) * 4) V4 { T x, y, z, w; };
 template <typename T> struct alignas(sizeof(T) * 4) V8 { T x, y, z, pad0, wx, wy, wz, pad1; };
@@ -45,9 +55,13 @@ struct EvalArgs {
   double thr[GK_MAX_THRESHOLDS];
   // source splitting across blockIdx.y (small point sets): partial sums go to
   // part_s [nsplit][n_points] (and part_g [nsplit][n_points][3]) and a
-  // finalize kernel adds them in a fixed order.
+  // finalize kernel adds them in a fixed order.  The sources are the
+  // interface sources followed by the orientations, each costing
+  // src_cost(grad, orientation) units; split y takes the sources whose cost
+  // interval starts in [y cost_chunk, (y + 1) cost_chunk), so every split
+  // carries the same work.
   int32_t nsplit;
-  int64_t pts_chunk, ori_chunk;
+  int64_t cost_chunk;
   double* part_s;
   double* part_g;
 };
@@ -161,11 +175,21 @@ __global__ __launch_bounds__(TS) void eval_kernel(EvalArgs a) {
   // ---- interface + reference sources: weight * C(|x - p|) -----------------
   const V4<T>* pts = reinterpret_cast<const V4<T>*>(a.pts);
   V4<T>* s4 = reinterpret_cast<V4<T>*>(s_src);
-  const int64_t pb = (int64_t)blockIdx.y * a.pts_chunk;
-  const int64_t pe = min(a.n_pts, pb + a.pts_chunk);
-  const int64_t obeg = (int64_t)blockIdx.y * a.ori_chunk;
-  const int64_t oend = min(a.n_ori, obeg + a.ori_chunk);
+  int64_t pb = 0, pe = a.n_pts, obeg = 0, oend = a.n_ori;    // the lattice is never split
+  if constexpr (!LATTICE) {
+    constexpr int64_t CP = src_cost(GRAD, false), CO = src_cost(GRAD, true);
+    const int64_t cpts = a.n_pts * CP;
+    const int64_t clo = (int64_t)blockIdx.y * a.cost_chunk, chi = clo + a.cost_chunk;
+    pb = min(a.n_pts, (clo + CP - 1) / CP);
+    pe = min(a.n_pts, (chi + CP - 1) / CP);
+    obeg = min(a.n_ori, (max(clo - cpts, (int64_t)0) + CO - 1) / CO);
+    oend = min(a.n_ori, (max(chi - cpts, (int64_t)0) + CO - 1) / CO);
+  }
   for (int64_t base = pb; base < pe; base += TS) {
+    // sources in this tile: a split of an explicit point set need not fill
+    // it (the lattice keeps the constant trip count; its one partial tile
+    // is padded with zero weights)
+    const int nk = LATTICE ? TS : (int)min((int64_t)TS, pe - base);
     __syncthreads();
     {
       int64_t j = base + tid;
@@ -178,7 +202,7 @@ __global__ __launch_bounds__(TS) void eval_kernel(EvalArgs a) {
     if constexpr (LATTICE && !GRAD && !MASKED) {
       T wsum = T(0);
 #pragma unroll 2
-      for (int k = 0; k < TS; ++k) {
+      for (int k = 0; k < nk; ++k) {
         const V4<T> s = s4[k];
         const T dx = x0 - s.x, dy = y0 - s.y;
         const T dxy = fma(dx, dx, fma(dy, dy, Tiny<T>::v));
@@ -202,7 +226,7 @@ __global__ __launch_bounds__(TS) void eval_kernel(EvalArgs a) {
       for (int q = 0; q < PPT; ++q) part[q] += wsum;
     } else {
 #pragma unroll 2
-    for (int k = 0; k < TS; ++k) {
+    for (int k = 0; k < nk; ++k) {
       const V4<T> s = s4[k];
       if (LATTICE) {
         const T dx = x0 - s.x, dy = y0 - s.y;
@@ -262,6 +286,7 @@ __global__ __launch_bounds__(TS) void eval_kernel(EvalArgs a) {
   // ---- orientation sources: (w . (q - x)) C'(r)/r --------------------------
   const V8<T>* ori = reinterpret_cast<const V8<T>*>(a.ori);
   for (int64_t base = obeg; base < oend; base += TS) {
+    const int nk = LATTICE ? TS : (int)min((int64_t)TS, oend - base);
     __syncthreads();
     {
       int64_t j = base + tid;
@@ -277,7 +302,7 @@ __global__ __launch_bounds__(TS) void eval_kernel(EvalArgs a) {
       T d3 = T(2.625), d2 = T(-8.75), d1 = T(13.125), d0 = T(-14.0);     // d1..d3 halved
       asm volatile("" : "+v"(d3), "+v"(d2), "+v"(d1), "+v"(d0));
 #pragma unroll 2
-      for (int k = 0; k < TS; ++k) {
+      for (int k = 0; k < nk; ++k) {
         const V8<T> s = s_src[k];
         const T dx = s.x - x0, dy = s.y - y0;          // q - x
         const T dxy = fma(dx, dx, fma(dy, dy, Tiny<T>::v));
@@ -297,7 +322,7 @@ __global__ __launch_bounds__(TS) void eval_kernel(EvalArgs a) {
       }
     } else {
 #pragma unroll 2
-    for (int k = 0; k < TS; ++k) {
+    for (int k = 0; k < nk; ++k) {
       const V8<T> s = s_src[k];
       if (LATTICE) {
         const T dx = s.x - x0, dy = s.y - y0;          // q - x
@@ -397,17 +422,37 @@ __global__ __launch_bounds__(TS) void eval_kernel(EvalArgs a) {
   }
 }
 
-// Sum the per-chunk partials (fixed order), add drift, segment, store.
-__global__ void eval_finalize_kernel(EvalArgs a, int grad) {
-  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= a.n_points) return;
+// Sum the per-split partials, add drift, segment, store.  A block takes
+// FIN_PTS points with FIN_LANES threads each: lane l adds splits l, l + 16,
+// ... (its loads independent, so a hundred splits cost a few memory
+// latencies), then a tree over the lanes in LDS; the order of the additions
+// depends only on nsplit.
+constexpr int FIN_PTS = 16, FIN_LANES = 16;
+__global__ __launch_bounds__(FIN_PTS * FIN_LANES) void eval_finalize_kernel(EvalArgs a, int grad) {
+  __shared__ double red[4][FIN_LANES][FIN_PTS];
+  const int pl = threadIdx.x % FIN_PTS, lane = threadIdx.x / FIN_PTS;
+  const int64_t p = (int64_t)blockIdx.x * FIN_PTS + pl;
   double s = 0.0, g[3] = {0.0, 0.0, 0.0};
-  for (int y = 0; y < a.nsplit; ++y) {
-    const int64_t o = (int64_t)y * a.n_points + p;
-    s += a.part_s[o];
-    if (grad) { g[0] += a.part_g[3 * o]; g[1] += a.part_g[3 * o + 1]; g[2] += a.part_g[3 * o + 2]; }
+  if (p < a.n_points) {
+    for (int y = lane; y < a.nsplit; y += FIN_LANES) {
+      const int64_t o = (int64_t)y * a.n_points + p;
+      s += a.part_s[o];
+      if (grad) { g[0] += a.part_g[3 * o]; g[1] += a.part_g[3 * o + 1]; g[2] += a.part_g[3 * o + 2]; }
+    }
   }
-  g[0] *= a.inv_a; g[1] *= a.inv_a; g[2] *= a.inv_a;
+  red[0][lane][pl] = s;
+  red[1][lane][pl] = g[0];
+  red[2][lane][pl] = g[1];
+  red[3][lane][pl] = g[2];
+  __syncthreads();
+  for (int w = FIN_LANES / 2; w > 0; w >>= 1) {
+    if (lane < w)
+      for (int c = 0; c < 4; ++c) red[c][lane][pl] += red[c][lane + w][pl];
+    __syncthreads();
+  }
+  if (lane != 0 || p >= a.n_points) return;
+  s = red[0][0][pl];
+  g[0] = red[1][0][pl] * a.inv_a; g[1] = red[2][0][pl] * a.inv_a; g[2] = red[3][0][pl] * a.inv_a;
   const double* P = a.xyz + 3 * p;
   drift_terms(a, P[0], P[1], P[2], p, s, grad ? g : nullptr);
   a.scalar[p] = s;
@@ -477,10 +522,16 @@ __global__ void prep_ori_kernel(gk_system sys, const double* w, V8<T>* out) {
 
 int n_drift_of(int deg) { return deg == 0 ? 0 : (deg == 1 ? 3 : 9); }
 
+// Points per thread: a lattice z-run of 8 (FP64 with gradient or mask: 4);
+// explicit points 4 (FP64 with gradient: 2, for its 24 accumulator registers).
+constexpr int eval_ppt(bool lattice, bool f32, bool grad, bool masked) {
+  return lattice ? ((f32 || (!grad && !masked)) ? 8 : 4) : ((f32 || !grad) ? 4 : 2);
+}
+
 template <typename T, bool LATTICE, bool GRAD, bool MASKED>
 void launch_eval(const EvalArgs& a, int ppt_runs_div, hipStream_t st) {
   (void)ppt_runs_div;
-  constexpr int PPT = LATTICE ? ((sizeof(T) == 4 || (!GRAD && !MASKED)) ? 8 : 4) : (sizeof(T) == 4 ? 4 : 2);
+  constexpr int PPT = eval_ppt(LATTICE, sizeof(T) == 4, GRAD, MASKED);
   EvalArgs b = a;
   if (LATTICE) {
     b.nzr = gk_ceil_div(a.nz, PPT);
@@ -493,7 +544,8 @@ void launch_eval(const EvalArgs& a, int ppt_runs_div, hipStream_t st) {
   hipLaunchKernelGGL((eval_kernel<T, LATTICE, GRAD, MASKED, PPT>), dim3((unsigned)blocks, (unsigned)b.nsplit),
                      dim3(TS), 0, st, b);
   if (b.nsplit > 1)
-    hipLaunchKernelGGL(eval_finalize_kernel, dim3((unsigned)gk_ceil_div(b.n_points, 256)), dim3(256), 0, st,
+    hipLaunchKernelGGL(eval_finalize_kernel, dim3((unsigned)gk_ceil_div(b.n_points, FIN_PTS)),
+                       dim3(FIN_PTS * FIN_LANES), 0, st,
                        b, GRAD ? 1 : 0);
 }
 
@@ -589,25 +641,25 @@ static bool needs_mask(const gk_system* sys, const double* lo, const double* hi,
 // then launch the evaluation kernel for the requested variant.
 static int run_eval(gk_handle h, EvalArgs& a, bool lattice, int precision, bool grad, bool masked,
                     hipStream_t st, bool external = false, char* part = nullptr, size_t part_bytes = 0) {
+  const int64_t cost = a.n_pts * src_cost(grad, false) + a.n_ori * src_cost(grad, true);
   a.nsplit = 1;
-  a.pts_chunk = std::max<int64_t>(a.n_pts, 1);
-  a.ori_chunk = std::max<int64_t>(a.n_ori, 1);
+  a.cost_chunk = std::max<int64_t>(cost, 1);
   if (!lattice) {
-    const int ppt = precision == GK_F64 ? 2 : 4;
+    const int ppt = eval_ppt(false, precision != GK_F64, grad, masked);
     const int64_t blocks = gk_ceil_div(gk_ceil_div(a.n_points, ppt), TS);
-    const int64_t tiles = std::max(gk_ceil_div(a.n_pts, TS), gk_ceil_div(a.n_ori, TS));
-    int64_t want = std::min<int64_t>(std::min<int64_t>(gk_ceil_div(2048, blocks), 64), tiles);
+    // at most SPLIT_WGS workgroups in all (rounded down: a few more would
+    // run as a nearly empty last round), each split at least 64 interface
+    // sources' worth of work
+    int64_t want = std::min<int64_t>(std::min<int64_t>(SPLIT_WGS / blocks, grad ? MAX_SPLIT : MAX_SPLIT_SCALAR),
+                                     gk_ceil_div(cost, 64 * src_cost(grad, false)));
     if (external) {          // as many splits as the caller's scratch holds partials for
       const size_t per_split = (size_t)a.n_points * (grad ? 4 : 1) * sizeof(double);
       want = std::min<int64_t>(want, per_split > 0 ? (int64_t)((part_bytes > 512 ? part_bytes - 512 : 0) / per_split)
                                                    : 1);
     }
     if (want > 1) {
-      // chunks are at least one tile, so an empty source set (n = 0) never divides by zero
-      a.pts_chunk = std::max<int64_t>(gk_round_up(gk_ceil_div(a.n_pts, want), TS), TS);
-      a.ori_chunk = std::max<int64_t>(gk_round_up(gk_ceil_div(a.n_ori, want), TS), TS);
-      a.nsplit = (int32_t)std::max<int64_t>(
-          std::max(gk_ceil_div(a.n_pts, a.pts_chunk), gk_ceil_div(a.n_ori, a.ori_chunk)), 1);
+      a.cost_chunk = gk_ceil_div(cost, want);
+      a.nsplit = (int32_t)gk_ceil_div(cost, a.cost_chunk);
       if (a.nsplit > 1) {
         void* pw = part;
         const size_t per = (size_t)a.n_points * (grad ? 4 : 1);
@@ -615,8 +667,7 @@ static int run_eval(gk_handle h, EvalArgs& a, bool lattice, int precision, bool 
         a.part_s = static_cast<double*>(pw);
         a.part_g = grad ? a.part_s + (size_t)a.nsplit * a.n_points : nullptr;
       } else {
-        a.pts_chunk = std::max<int64_t>(a.n_pts, 1);
-        a.ori_chunk = std::max<int64_t>(a.n_ori, 1);
+        a.cost_chunk = std::max<int64_t>(cost, 1);
       }
     }
   }
