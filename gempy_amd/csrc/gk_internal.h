@@ -52,16 +52,21 @@ enum gk_ws_slot {
   WS_KNN,            // cells and sorted points of the grid neighbour search
   WS_COUNT
 };
+// Independent sets of tables, flags and buffers of the sync-free triangular
+// solves in WS_TRSV: up to this many solves run at once, each on its own
+// stream with its own set.
+constexpr int GK_TRSV_CTX = 3;
+
 struct gk_context {
   int device = 0;
   void* ws[WS_COUNT] = {};
   size_t ws_bytes[WS_COUNT] = {};
   void* solver_state = nullptr;            // factorization kept by the last gk_solve
   void (*solver_free)(void*) = nullptr;
-  // state of the sync-free triangular solves (tables in WS_TRSV)
+  // state of the sync-free triangular solves (GK_TRSV_CTX sets in WS_TRSV)
   int trsv_nb = 0, trsv_slots = 0, trsv_nfwd = 0, trsv_nbwd = 0;
-  unsigned trsv_epoch = 0;
-  unsigned long long trsv_tickets = 0;
+  unsigned trsv_epoch[GK_TRSV_CTX] = {};
+  unsigned long long trsv_tickets[GK_TRSV_CTX] = {};
   // side stream (default priority) and events of the look-ahead factorization
   hipStream_t side = nullptr;
   hipEvent_t side_ev[3] = {};
@@ -104,10 +109,12 @@ int gk_side_stream(gk_handle h, hipStream_t* s2, hipEvent_t** ev);
 int gk_aux_stream(gk_handle h, hipStream_t* s3, hipEvent_t** ev);
 
 int gk_workspace(gk_handle h, int slot, size_t bytes, void** out);
-// L^-T L^-1 b in place (one launch per direction); gk_trsv_check reports a
-// readiness wait that reached its spin limit since the last check.
+// L^-T L^-1 b in place (one launch per direction) with the solve set ctx
+// (0 .. GK_TRSV_CTX-1; solves running at once on different streams take
+// different sets); gk_trsv_check reports a readiness wait that reached its
+// spin limit since the last check.
 template <typename T>
-int gk_trsv_chol_impl(gk_handle h, const T* L, const T* dinv, int64_t lda, T* b, hipStream_t st);
+int gk_trsv_chol_impl(gk_handle h, const T* L, const T* dinv, int64_t lda, T* b, hipStream_t st, int ctx = 0);
 int gk_trsv_check(gk_handle h, hipStream_t st);
 
 // Optional per-launch timing of the dense kernels (gk_profile): HIP events are

@@ -261,6 +261,7 @@ struct Solver {
   double* Y = nullptr;
   double* Sinv = nullptr;
   void* uvec = nullptr;
+  float* ucol3 = nullptr;            // FP32 drift columns solved at once: uvec holds two, ucol3 the third
   double* ytmp = nullptr;
   double* ytmp2 = nullptr;
   double* lam = nullptr;
@@ -439,16 +440,54 @@ struct Solver {
       GK_TRY(gk_assemble_drift_impl(sys, U, st));
       // The drift columns take the factor alone (without the local-inverse
       // correction and its operator product): the flexible iteration does not
-      // need Y to come from the same approximation of K, and at config 3 the
+      // need Y to come from the same approximation of K, and at N = 35k the
       // iteration count is the same.
-      for (int l = 0; l < Lt; ++l) {
-        hipLaunchKernelGGL(get_col_kernel, dim3(nblk(Nc)), dim3(256), 0, st, U, (int)Lt, l, r, Nc);
-        const bool keep = pcp;
-        pcp = false;
-        const int rc_k = apply_K(r, w);
-        pcp = keep;
-        GK_TRY(rc_k);
-        hipLaunchKernelGGL(set_col_kernel, dim3(nblk(Nc)), dim3(256), 0, st, w, Y, (int)Lt, l, Nc);
+      if (!f64 && !spai && Lt > 1) {
+        // FP32 factor: three columns at a time, their solves on the caller's,
+        // the side and the auxiliary stream with solve sets 0, 1, 2 (each
+        // solve is bound by its dependency chain, not by HBM, so they overlap)
+        hipStream_t sx[3] = {st, nullptr, nullptr};
+        hipEvent_t* e2 = nullptr;
+        hipEvent_t* e3 = nullptr;
+        GK_TRY(gk_side_stream(h, &sx[1], &e2));
+        GK_TRY(gk_aux_stream(h, &sx[2], &e3));
+        float* ub[3] = {static_cast<float*>(uvec), static_cast<float*>(uvec) + Ncpad, ucol3};
+        for (int l0 = 0; l0 < Lt; l0 += 3) {
+          const int nl = (int)std::min<int64_t>(3, Lt - l0);
+          for (int c = 0; c < nl; ++c) {
+            hipLaunchKernelGGL(get_col_kernel, dim3(nblk(Nc)), dim3(256), 0, st, U, (int)Lt, l0 + c, r, Nc);
+            hipLaunchKernelGGL(to_scaled_kernel<float>, dim3(nblk(Ncpad)), dim3(256), 0, st, r, dscale, ub[c], Nc,
+                               Ncpad);
+          }
+          GK_HIP(hipEventRecord(e2[0], st));
+          for (int c = 1; c < nl; ++c) GK_HIP(hipStreamWaitEvent(sx[c], e2[0], 0));
+          for (int c = 0; c < nl; ++c)
+            GK_TRY(gk_trsv_chol_impl<float>(h, static_cast<float*>(H), static_cast<float*>(dinv), Ncpad, ub[c],
+                                            sx[c], c));
+          if (nl > 1) {
+            GK_HIP(hipEventRecord(e2[1], sx[1]));
+            GK_HIP(hipStreamWaitEvent(st, e2[1], 0));
+          }
+          if (nl > 2) {
+            GK_HIP(hipEventRecord(e3[0], sx[2]));
+            GK_HIP(hipStreamWaitEvent(st, e3[0], 0));
+          }
+          for (int c = 0; c < nl; ++c) {
+            hipLaunchKernelGGL(from_scaled_kernel<float>, dim3(nblk(Nc)), dim3(256), 0, st, ub[c], dscale, w, Nc);
+            hipLaunchKernelGGL(set_col_kernel, dim3(nblk(Nc)), dim3(256), 0, st, w, Y, (int)Lt, l0 + c, Nc);
+          }
+        }
+        GK_HIP(hipGetLastError());
+      } else {
+        for (int l = 0; l < Lt; ++l) {
+          hipLaunchKernelGGL(get_col_kernel, dim3(nblk(Nc)), dim3(256), 0, st, U, (int)Lt, l, r, Nc);
+          const bool keep = pcp;
+          pcp = false;
+          const int rc_k = apply_K(r, w);
+          pcp = keep;
+          GK_TRY(rc_k);
+          hipLaunchKernelGGL(set_col_kernel, dim3(nblk(Nc)), dim3(256), 0, st, w, Y, (int)Lt, l, Nc);
+        }
       }
       // S = U^T Y (Lt x Lt, column l at hdev + 512 + l Lt), one read-back,
       // inverted on the host
@@ -736,6 +775,7 @@ static int init_solver(gk_handle h, const gk_system* sys, hipStream_t st, int fo
   const size_t o_Y = take((size_t)S.Ncpad * std::max<int64_t>(Lt, 1) * sizeof(double));
   const size_t o_Si = take(32 * 32 * sizeof(double));
   const size_t o_u32 = take(S.Ncpad * sizeof(double));
+  const size_t o_u3 = take(S.Ncpad * sizeof(float));   // third FP32 drift column (two fit in o_u32)
   const size_t o_yt = take(Npad * sizeof(double));
   const size_t o_yt2 = take(Npad * sizeof(double));
   const size_t o_lam = take(64 * sizeof(double));
@@ -767,6 +807,7 @@ static int init_solver(gk_handle h, const gk_system* sys, hipStream_t st, int fo
   S.Y = reinterpret_cast<double*>(base + o_Y);
   S.Sinv = reinterpret_cast<double*>(base + o_Si);
   S.uvec = base + o_u32;
+  S.ucol3 = reinterpret_cast<float*>(base + o_u3);
   S.ytmp = reinterpret_cast<double*>(base + o_yt);
   S.ytmp2 = reinterpret_cast<double*>(base + o_yt2);
   S.lam = reinterpret_cast<double*>(base + o_lam);

@@ -708,7 +708,9 @@ static TrsvLayout trsv_layout(int nb, int nfwd, int nbwd, int slots) {
   return L;
 }
 
-static int trsv_prepare(gk_handle h, int nb, hipStream_t st, char** base, TrsvLayout* lay) {
+// Tables, flags and buffers of solve set ctx: WS_TRSV holds GK_TRSV_CTX
+// copies of the layout back to back.
+static int trsv_prepare(gk_handle h, int nb, hipStream_t st, char** base, TrsvLayout* lay, int ctx) {
   std::vector<Item> fwd, bwd;
   std::vector<int> fb, bb;
   const bool rebuild = h->trsv_nb != nb || h->ws[WS_TRSV] == nullptr;
@@ -724,34 +726,38 @@ static int trsv_prepare(gk_handle h, int nb, hipStream_t st, char** base, TrsvLa
   *lay = trsv_layout(nb, nfwd, nbwd, slots);
   void* p = nullptr;
   const void* before = h->ws[WS_TRSV];
-  GK_TRY(gk_workspace(h, WS_TRSV, lay->bytes, &p));
-  *base = static_cast<char*>(p);
+  GK_TRY(gk_workspace(h, WS_TRSV, lay->bytes * GK_TRSV_CTX, &p));
   if (rebuild || p != before) {
     if (!rebuild) build_tables(nb, fwd, fb, bwd, bb);
-    GK_HIP(hipMemcpyAsync(*base + lay->o_fi, fwd.data(), sizeof(Item) * fwd.size(), hipMemcpyHostToDevice, st));
-    GK_HIP(hipMemcpyAsync(*base + lay->o_bi, bwd.data(), sizeof(Item) * bwd.size(), hipMemcpyHostToDevice, st));
-    GK_HIP(hipMemcpyAsync(*base + lay->o_fb, fb.data(), sizeof(int) * fb.size(), hipMemcpyHostToDevice, st));
-    GK_HIP(hipMemcpyAsync(*base + lay->o_bb, bb.data(), sizeof(int) * bb.size(), hipMemcpyHostToDevice, st));
-    GK_HIP(hipMemsetAsync(*base + lay->o_yf, 0, lay->o_c - lay->o_yf, st));   // flags, ticket, error
+    for (int c = 0; c < GK_TRSV_CTX; ++c) {
+      char* b = static_cast<char*>(p) + c * lay->bytes;
+      GK_HIP(hipMemcpyAsync(b + lay->o_fi, fwd.data(), sizeof(Item) * fwd.size(), hipMemcpyHostToDevice, st));
+      GK_HIP(hipMemcpyAsync(b + lay->o_bi, bwd.data(), sizeof(Item) * bwd.size(), hipMemcpyHostToDevice, st));
+      GK_HIP(hipMemcpyAsync(b + lay->o_fb, fb.data(), sizeof(int) * fb.size(), hipMemcpyHostToDevice, st));
+      GK_HIP(hipMemcpyAsync(b + lay->o_bb, bb.data(), sizeof(int) * bb.size(), hipMemcpyHostToDevice, st));
+      GK_HIP(hipMemsetAsync(b + lay->o_yf, 0, lay->o_c - lay->o_yf, st));   // flags, ticket, error
+      h->trsv_epoch[c] = 0;
+      h->trsv_tickets[c] = 0;
+    }
     GK_HIP(hipStreamSynchronize(st));          // host tables go out of scope
     h->trsv_nb = nb;
     h->trsv_slots = slots;
     h->trsv_nfwd = nfwd;
     h->trsv_nbwd = nbwd;
-    h->trsv_epoch = 0;
-    h->trsv_tickets = 0;
   }
+  *base = static_cast<char*>(p) + ctx * lay->bytes;
   return GK_OK;
 }
 
 template <typename T>
-int gk_trsv_chol_impl(gk_handle h, const T* L, const T* planes, int64_t lda, T* b, hipStream_t st) {
+int gk_trsv_chol_impl(gk_handle h, const T* L, const T* planes, int64_t lda, T* b, hipStream_t st, int ctx) {
   GK_REQUIRE(h && L && planes && b && lda > 0 && lda % NB == 0, GK_ERR_ARG, "triangular solve: bad arguments");
+  GK_REQUIRE(ctx >= 0 && ctx < GK_TRSV_CTX, GK_ERR_ARG, "triangular solve: solve set %d out of range", ctx);
   constexpr bool PF = true;                     // first pass of the first block before the waits
   const int nb = (int)(lda / NB);
   char* base = nullptr;
   TrsvLayout lay;
-  GK_TRY(trsv_prepare(h, nb, st, &base, &lay));
+  GK_TRY(trsv_prepare(h, nb, st, &base, &lay, ctx));
   TrsvCtl c;
   c.yflag = reinterpret_cast<unsigned*>(base + lay.o_yf);
   c.cflag = reinterpret_cast<unsigned*>(base + lay.o_cf);
@@ -765,23 +771,23 @@ int gk_trsv_chol_impl(gk_handle h, const T* L, const T* planes, int64_t lda, T* 
   // forward
   c.items = reinterpret_cast<const Item*>(base + lay.o_fi);
   c.rowbase = reinterpret_cast<const int*>(base + lay.o_fb);
-  c.ticket_base = h->trsv_tickets;
-  c.epoch = ++h->trsv_epoch;
+  c.ticket_base = h->trsv_tickets[ctx];
+  c.epoch = ++h->trsv_epoch[ctx];
   c.n_items = lay.nfwd;
   hipEvent_t pa;
   const bool prof = gk_prof_begin(st, &pa);
   hipLaunchKernelGGL((trsv_fwd_kernel<T, PF>), dim3((unsigned)lay.nfwd), dim3(Shape<T>::NT), 0, st, L, lda, planes, b, ybuf,
                      cbuf, part, c);
-  h->trsv_tickets += (unsigned long long)lay.nfwd;
+  h->trsv_tickets[ctx] += (unsigned long long)lay.nfwd;
   // backward
   c.items = reinterpret_cast<const Item*>(base + lay.o_bi);
   c.rowbase = reinterpret_cast<const int*>(base + lay.o_bb);
-  c.ticket_base = h->trsv_tickets;
-  c.epoch = ++h->trsv_epoch;
+  c.ticket_base = h->trsv_tickets[ctx];
+  c.epoch = ++h->trsv_epoch[ctx];
   c.n_items = lay.nbwd;
   hipLaunchKernelGGL((trsv_bwd_kernel<T, PF>), dim3((unsigned)lay.nbwd), dim3(Shape<T>::NT), 0, st, L, lda, planes, b, ybuf,
                      cbuf, part, c);
-  h->trsv_tickets += (unsigned long long)lay.nbwd;
+  h->trsv_tickets[ctx] += (unsigned long long)lay.nbwd;
   if (prof) {    // each direction reads the blocks at distance >= NEAR, Dinv and the chain planes
     const double far = nb > NEAR ? (double)(nb - NEAR) * (nb - NEAR + 1) / 2.0 : 0.0;
     double nearb = nb;
@@ -796,17 +802,21 @@ int gk_trsv_chol_impl(gk_handle h, const T* L, const T* planes, int64_t lda, T* 
   return GK_OK;
 }
 
-template int gk_trsv_chol_impl<float>(gk_handle, const float*, const float*, int64_t, float*, hipStream_t);
-template int gk_trsv_chol_impl<double>(gk_handle, const double*, const double*, int64_t, double*, hipStream_t);
+template int gk_trsv_chol_impl<float>(gk_handle, const float*, const float*, int64_t, float*, hipStream_t, int);
+template int gk_trsv_chol_impl<double>(gk_handle, const double*, const double*, int64_t, double*, hipStream_t, int);
 
-// Error word of the sync-free solves (spin limit reached); cleared by reading.
+// Error words of the sync-free solves (spin limit reached) of every solve
+// set; the caller's stream has waited for the solves of the other sets.
 int gk_trsv_check(gk_handle h, hipStream_t st) {
   if (!h->ws[WS_TRSV] || h->trsv_nb == 0) return GK_OK;
   TrsvLayout lay = trsv_layout(h->trsv_nb, h->trsv_nfwd, h->trsv_nbwd, h->trsv_slots);
   char* base = static_cast<char*>(h->ws[WS_TRSV]);
-  int err = 0;
-  GK_HIP(hipMemcpyAsync(&err, base + lay.o_err, sizeof(int), hipMemcpyDeviceToHost, st));
+  int errs[GK_TRSV_CTX] = {};
+  for (int c = 0; c < GK_TRSV_CTX; ++c)
+    GK_HIP(hipMemcpyAsync(&errs[c], base + c * lay.bytes + lay.o_err, sizeof(int), hipMemcpyDeviceToHost, st));
   GK_HIP(hipStreamSynchronize(st));
+  int err = 0;
+  for (int c = 0; c < GK_TRSV_CTX; ++c) err = err ? err : errs[c];
   if (err) {
     h->trsv_nb = 0;              // reset tables, flags and counters on the next call
     GK_REQUIRE(false, GK_ERR_HIP, err == 1 ? "triangular solve: a readiness wait reached its spin limit"
