@@ -76,10 +76,11 @@ template <> struct Mfma<double> {
 // 256 x 256 tiles with 8 waves 71-72, with 4 waves (accumulators in AGPRs)
 // 49-53 (one workgroup per CU leaves the stage barriers uncovered); a second
 // register stage or BK 32 were slower.
-template <int TM_, int WM_, int WN_>
+// TM x TN tiles (TN defaults to TM) of WM x WN waves.
+template <int TM_, int WM_, int WN_, int TN_ = TM_>
 struct Tile {
-  static constexpr int TM = TM_, WM = WM_, WN = WN_, NT = 64 * WM * WN;
-  static constexpr int FM = TM / WM / 16, FN = TM / WN / 16;
+  static constexpr int TM = TM_, TN = TN_, WM = WM_, WN = WN_, NT = 64 * WM * WN;
+  static constexpr int FM = TM / WM / 16, FN = TN / WN / 16;
 };
 #ifndef GK_GEMM_GROUP
 #define GK_GEMM_GROUP 16
@@ -112,7 +113,8 @@ __global__ __launch_bounds__(TL::NT) __attribute__((amdgpu_waves_per_eu(WPE, 0))
   A += blockIdx.y * sA;
   B += blockIdx.y * sB;
   C += blockIdx.y * sC;
-  constexpr int TM = TL::TM, NT = TL::NT, FM = TL::FM, FN = TL::FN;
+  constexpr int TM = TL::TM, TN = TL::TN, NT = TL::NT, FM = TL::FM, FN = TL::FN;
+  static_assert(!LOWER || TN == TM, "lower-triangle products take square tiles");
   constexpr int BK = M::BK;
   constexpr int KS = BK / 4;              // k values per lane per stage
   constexpr int PAD = 16 / sizeof(T);     // keep rows 16-byte aligned
@@ -120,7 +122,7 @@ __global__ __launch_bounds__(TL::NT) __attribute__((amdgpu_waves_per_eu(WPE, 0))
   constexpr int VEC = 16 / sizeof(T);     // elements per 16-byte vector
   typedef T vec_t __attribute__((ext_vector_type(VEC)));
   __shared__ __attribute__((aligned(16))) T As[2][TM * LDS_LD];
-  __shared__ __attribute__((aligned(16))) T Bs[2][TM * LDS_LD];
+  __shared__ __attribute__((aligned(16))) T Bs[2][TN * LDS_LD];
 
   // XCD-aware tile order: consecutive hardware block ids go to different
   // XCDs, so give each XCD a contiguous range of logical tiles.
@@ -157,7 +159,7 @@ __global__ __launch_bounds__(TL::NT) __attribute__((amdgpu_waves_per_eu(WPE, 0))
     tm = bid / tiles_n;
     tn = bid % tiles_n;
   }
-  const int64_t m0 = tm * TM, n0 = tn * TM;
+  const int64_t m0 = tm * TM, n0 = tn * TN;
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
   const int wm = wave / TL::WN, wn = wave % TL::WN;
@@ -169,13 +171,15 @@ __global__ __launch_bounds__(TL::NT) __attribute__((amdgpu_waves_per_eu(WPE, 0))
 #pragma unroll
     for (int j = 0; j < FN; ++j) acc[i][j] = typename M::acc_t{0, 0, 0, 0};
 
-  // global -> register staging: TM x BK tile = TM*BK/VEC vectors, NV per thread
+  // global -> register staging: TM x BK tile = TM*BK/VEC vectors, NV per
+  // thread (NVB for the TN x BK tile of B)
   constexpr int VPR = BK / VEC;           // vectors per row
-  constexpr int NV = TM * BK / VEC / NT;
+  constexpr int NV = TM * BK / VEC / NT, NVB = TN * BK / VEC / NT;
   static_assert(NV >= 1 && NV * NT * VEC == TM * BK, "BK must make the tile a whole number of vectors per thread");
+  static_assert(NVB >= 1 && NVB * NT * VEC == TN * BK, "BK must make the tile a whole number of vectors per thread");
   constexpr int RS = sizeof(T) == 4 ? GK_GEMM_RS_F32 : 1;
-  vec_t ra[RS][NV], rb[RS][NV];
-  auto load_global = [&](int64_t k0, vec_t (&xa)[NV], vec_t (&xb)[NV]) {
+  vec_t ra[RS][NV], rb[RS][NVB];
+  auto load_global = [&](int64_t k0, vec_t (&xa)[NV], vec_t (&xb)[NVB]) {
 #pragma unroll
     for (int it = 0; it < NV; ++it) {
       const int idx = tid + NT * it;
@@ -184,22 +188,22 @@ __global__ __launch_bounds__(TL::NT) __attribute__((amdgpu_waves_per_eu(WPE, 0))
     }
     if (TRANS_B) {
 #pragma unroll
-      for (int it = 0; it < NV; ++it) {
+      for (int it = 0; it < NVB; ++it) {
         const int idx = tid + NT * it;
         const int row = idx / VPR, cv = idx % VPR;
         xb[it] = *reinterpret_cast<const vec_t*>(B + (n0 + row) * ldb + k0 + cv * VEC);
       }
     } else {
-      constexpr int VPRB = TM / VEC;      // vectors per k-row of the B tile
+      constexpr int VPRB = TN / VEC;      // vectors per k-row of the B tile
 #pragma unroll
-      for (int it = 0; it < NV; ++it) {
+      for (int it = 0; it < NVB; ++it) {
         const int idx = tid + NT * it;
         const int kk = idx / VPRB, cv = idx % VPRB;
         xb[it] = *reinterpret_cast<const vec_t*>(B + (k0 + kk) * ldb + n0 + cv * VEC);
       }
     }
   };
-  auto store_lds = [&](int buf, const vec_t (&xa)[NV], const vec_t (&xb)[NV]) {
+  auto store_lds = [&](int buf, const vec_t (&xa)[NV], const vec_t (&xb)[NVB]) {
 #pragma unroll
     for (int it = 0; it < NV; ++it) {
       const int idx = tid + NT * it;
@@ -208,15 +212,15 @@ __global__ __launch_bounds__(TL::NT) __attribute__((amdgpu_waves_per_eu(WPE, 0))
     }
     if (TRANS_B) {
 #pragma unroll
-      for (int it = 0; it < NV; ++it) {
+      for (int it = 0; it < NVB; ++it) {
         const int idx = tid + NT * it;
         const int row = idx / VPR, cv = idx % VPR;
         *reinterpret_cast<vec_t*>(&Bs[buf][row * LDS_LD + cv * VEC]) = xb[it];
       }
     } else {
-      constexpr int VPRB = TM / VEC;
+      constexpr int VPRB = TN / VEC;
 #pragma unroll
-      for (int it = 0; it < NV; ++it) {
+      for (int it = 0; it < NVB; ++it) {
         const int idx = tid + NT * it;
         const int kk = idx / VPRB, cv = idx % VPRB;
 #pragma unroll
@@ -328,13 +332,15 @@ void gemm_dispatch(dim3 grid, bool trans_b, bool lower, hipStream_t st, int64_t 
                    int64_t lda, const T* B, int64_t ldb, T alpha, T beta, T* C, int64_t ldc, int64_t tn,
                    int64_t tiles, int64_t sA, int64_t sB, int64_t sC) {
   dim3 block(TL::NT);
-  if (trans_b) {
-    if (lower) hipLaunchKernelGGL((gemm_kernel<T, true, true, TL, WPE>), grid, block, 0, st, m, n, k, A, lda, B, ldb, alpha, beta, C, ldc, tn, tiles, sA, sB, sC);
-    else hipLaunchKernelGGL((gemm_kernel<T, true, false, TL, WPE>), grid, block, 0, st, m, n, k, A, lda, B, ldb, alpha, beta, C, ldc, tn, tiles, sA, sB, sC);
-  } else {
-    if (lower) hipLaunchKernelGGL((gemm_kernel<T, false, true, TL, WPE>), grid, block, 0, st, m, n, k, A, lda, B, ldb, alpha, beta, C, ldc, tn, tiles, sA, sB, sC);
-    else hipLaunchKernelGGL((gemm_kernel<T, false, false, TL, WPE>), grid, block, 0, st, m, n, k, A, lda, B, ldb, alpha, beta, C, ldc, tn, tiles, sA, sB, sC);
+  if constexpr (TL::TN == TL::TM) {      // lower-triangle products: square tiles only
+    if (lower) {
+      if (trans_b) hipLaunchKernelGGL((gemm_kernel<T, true, true, TL, WPE>), grid, block, 0, st, m, n, k, A, lda, B, ldb, alpha, beta, C, ldc, tn, tiles, sA, sB, sC);
+      else hipLaunchKernelGGL((gemm_kernel<T, false, true, TL, WPE>), grid, block, 0, st, m, n, k, A, lda, B, ldb, alpha, beta, C, ldc, tn, tiles, sA, sB, sC);
+      return;
+    }
   }
+  if (trans_b) hipLaunchKernelGGL((gemm_kernel<T, true, false, TL, WPE>), grid, block, 0, st, m, n, k, A, lda, B, ldb, alpha, beta, C, ldc, tn, tiles, sA, sB, sC);
+  else hipLaunchKernelGGL((gemm_kernel<T, false, false, TL, WPE>), grid, block, 0, st, m, n, k, A, lda, B, ldb, alpha, beta, C, ldc, tn, tiles, sA, sB, sC);
 }
 
 // C = alpha A op(B) + beta C; `batch` > 1 runs that many products, the i-th at
@@ -369,7 +375,21 @@ int gemm_launch(int64_t m, int64_t n, int64_t k, const T* A, int64_t lda, const 
   const int64_t TMe = narrow ? 64 : 128;
   const int64_t tm = m / TMe, tn = n / TMe;
   const int64_t tiles = lower ? tm * (tm + 1) / 2 : tm * tn;
-  if (narrow)
+  bool launched = false;
+  if constexpr (sizeof(T) == 4) {
+    if (row_tiles) {
+      // FP32 row tiles of 64 x 128 on 4 waves: 30 KB of LDS and at most 102
+      // VGPRs, so a workgroup fits on a CU beside two workgroups of the
+      // split-fp16 trailing update (64 KB of LDS and 200 VGPRs per wave
+      // each) and the factor panel's products run while that update holds
+      // the GPU.
+      gemm_dispatch<T, Tile<64, 1, 4, 128>, 5>(dim3((unsigned)(m / 64), (unsigned)batch), trans_b, false, st, m, n,
+                                               k, A, lda, B, ldb, alpha, beta, C, ldc, 1, m / 64, sA, sB, sC);
+      launched = true;
+    }
+  }
+  if (launched) {
+  } else if (narrow)
     gemm_dispatch<T, Tile<64, 2, 2>, Mfma<T>::WPE>(dim3((unsigned)tiles, 1), trans_b, false, st, m, n, k, A, lda, B,
                                                    ldb, alpha, beta, C, ldc, tn, tiles, 0, 0, 0);
   else
