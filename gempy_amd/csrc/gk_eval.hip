@@ -114,7 +114,9 @@ template <> struct Tiny<float> { static constexpr float v = 1e-30f; };
 __device__ __forceinline__ double gk_rsq_raw(double t) { return __builtin_amdgcn_rsq(t); }
 __device__ __forceinline__ float gk_rsq_raw(float t) { return __builtin_amdgcn_rsqf(t); }
 
-template <typename T, bool LATTICE, bool GRAD, bool MASKED, int PPT>
+// SCALAR false (explicit points with gradient): the gradient only; the
+// scalar output then holds the drift terms alone.
+template <typename T, bool LATTICE, bool GRAD, bool MASKED, int PPT, bool SCALAR = true>
 __global__ __launch_bounds__(TS) void eval_kernel(EvalArgs a) {
   __shared__ V8<T> s_src[TS];
   const int tid = threadIdx.x;
@@ -260,7 +262,7 @@ __global__ __launch_bounds__(TS) void eval_kernel(EvalArgs a) {
           const T h3 = fma(-r, y, T(3));
           T c = fma(fma(r * fma(fma(kc[0], t, kc[1]), t, kc[2]), h3, T(-7)), t, T(1));
           if (MASKED) c = t < T(1) ? c : T(0);
-          part[q] = fma(s.w, c, part[q]);
+          if (SCALAR) part[q] = fma(s.w, c, part[q]);
           if (GRAD) {
             T dp = fma(r * fma(fma(kc[3], t, kc[4]), t, kc[5]), h3, T(-14));
             if (MASKED) dp = t < T(1) ? dp : T(0);
@@ -360,7 +362,7 @@ __global__ __launch_bounds__(TS) void eval_kernel(EvalArgs a) {
           T p = fma(r * fma(fma(kc[3], t, kc[4]), t, kc[5]), h3, T(-14));
           if (MASKED) p = t < T(1) ? p : T(0);
           const T dot = fma(s.wz, dz, fma(s.wy, dy, s.wx * dx));
-          part[q] = fma(dot, p, part[q]);
+          if (SCALAR) part[q] = fma(dot, p, part[q]);
           if (GRAD) {
             const T om = T(1) - t;
             T dd = T(13.125) * om * om * (y * h3);     // 26.25 (1-t)^2 / rho, 1/rho = y (3 - r y) / 2
@@ -528,7 +530,7 @@ constexpr int eval_ppt(bool lattice, bool f32, bool grad, bool masked) {
   return lattice ? ((f32 || (!grad && !masked)) ? 8 : 4) : ((f32 || !grad) ? 4 : 2);
 }
 
-template <typename T, bool LATTICE, bool GRAD, bool MASKED>
+template <typename T, bool LATTICE, bool GRAD, bool MASKED, bool SCALAR = true>
 void launch_eval(const EvalArgs& a, int ppt_runs_div, hipStream_t st) {
   (void)ppt_runs_div;
   constexpr int PPT = eval_ppt(LATTICE, sizeof(T) == 4, GRAD, MASKED);
@@ -541,7 +543,7 @@ void launch_eval(const EvalArgs& a, int ppt_runs_div, hipStream_t st) {
   }
   const int64_t blocks = gk_ceil_div(b.n_runs, TS);
   if (blocks == 0) return;
-  hipLaunchKernelGGL((eval_kernel<T, LATTICE, GRAD, MASKED, PPT>), dim3((unsigned)blocks, (unsigned)b.nsplit),
+  hipLaunchKernelGGL((eval_kernel<T, LATTICE, GRAD, MASKED, PPT, SCALAR>), dim3((unsigned)blocks, (unsigned)b.nsplit),
                      dim3(TS), 0, st, b);
   if (b.nsplit > 1)
     hipLaunchKernelGGL(eval_finalize_kernel, dim3((unsigned)gk_ceil_div(b.n_points, FIN_PTS)),
@@ -550,12 +552,14 @@ void launch_eval(const EvalArgs& a, int ppt_runs_div, hipStream_t st) {
 }
 
 template <typename T>
-void dispatch_eval(const EvalArgs& a, bool lattice, bool grad, bool masked, hipStream_t st) {
+void dispatch_eval(const EvalArgs& a, bool lattice, bool grad, bool masked, hipStream_t st, bool scalar = true) {
   if (lattice) {
     if (grad) { if (masked) launch_eval<T, true, true, true>(a, 0, st); else launch_eval<T, true, true, false>(a, 0, st); }
     else { if (masked) launch_eval<T, true, false, true>(a, 0, st); else launch_eval<T, true, false, false>(a, 0, st); }
   } else {
-    if (grad) { if (masked) launch_eval<T, false, true, true>(a, 0, st); else launch_eval<T, false, true, false>(a, 0, st); }
+    if (grad && !scalar) {
+      if (masked) launch_eval<T, false, true, true, false>(a, 0, st); else launch_eval<T, false, true, false, false>(a, 0, st);
+    } else if (grad) { if (masked) launch_eval<T, false, true, true>(a, 0, st); else launch_eval<T, false, true, false>(a, 0, st); }
     else { if (masked) launch_eval<T, false, false, true>(a, 0, st); else launch_eval<T, false, false, false>(a, 0, st); }
   }
 }
@@ -640,7 +644,8 @@ static bool needs_mask(const gk_system* sys, const double* lo, const double* hi,
 // Split the sources over blockIdx.y when the points alone cannot fill the GPU,
 // then launch the evaluation kernel for the requested variant.
 static int run_eval(gk_handle h, EvalArgs& a, bool lattice, int precision, bool grad, bool masked,
-                    hipStream_t st, bool external = false, char* part = nullptr, size_t part_bytes = 0) {
+                    hipStream_t st, bool external = false, char* part = nullptr, size_t part_bytes = 0,
+                    bool grad_only = false) {
   const int64_t cost = a.n_pts * src_cost(grad, false) + a.n_ori * src_cost(grad, true);
   a.nsplit = 1;
   a.cost_chunk = std::max<int64_t>(cost, 1);
@@ -671,8 +676,8 @@ static int run_eval(gk_handle h, EvalArgs& a, bool lattice, int precision, bool 
       }
     }
   }
-  if (precision == GK_F64) dispatch_eval<double>(a, lattice, grad, masked, st);
-  else dispatch_eval<float>(a, lattice, grad, masked, st);
+  if (precision == GK_F64) dispatch_eval<double>(a, lattice, grad, masked, st, !grad_only);
+  else dispatch_eval<float>(a, lattice, grad, masked, st, !grad_only);
   GK_HIP(hipGetLastError());
   return GK_OK;
 }
@@ -680,8 +685,11 @@ static int run_eval(gk_handle h, EvalArgs& a, bool lattice, int precision, bool 
 // ext (nullable): caller-provided scratch for the packed sources and the
 // split partials instead of the handle's workspaces (the no-allocation entry
 // point); the split is then limited to what fits in it.
+// grad_only (explicit points with gradient): the covariance part of the
+// scalar is skipped; the scalar output then holds the drift terms alone.
 static int eval_impl(gk_handle h, const gk_system* sys, const double* weights, gk_eval_desc* d, hipStream_t st,
-                     bool force_mask, const double* src_box, bool reuse_sources, char* ext, size_t ext_bytes);
+                     bool force_mask, const double* src_box, bool reuse_sources, char* ext, size_t ext_bytes,
+                     bool grad_only = false);
 
 extern "C" int gk_eval_impl(gk_handle h, const gk_system* sys, const double* weights,
                             gk_eval_desc* d, hipStream_t st, bool force_mask,
@@ -690,8 +698,11 @@ extern "C" int gk_eval_impl(gk_handle h, const gk_system* sys, const double* wei
 }
 
 static int eval_impl(gk_handle h, const gk_system* sys, const double* weights, gk_eval_desc* d, hipStream_t st,
-                     bool force_mask, const double* src_box, bool reuse_sources, char* ext, size_t ext_bytes) {
+                     bool force_mask, const double* src_box, bool reuse_sources, char* ext, size_t ext_bytes,
+                     bool grad_only) {
   GK_TRY(check_system(sys));
+  GK_REQUIRE(!grad_only || (d && d->gradient && d->kind == GK_POINTS_EXPLICIT), GK_ERR_ARG,
+             "gk_eval: a gradient-only evaluation needs explicit points and a gradient output");
   GK_REQUIRE(d && weights && d->scalar, GK_ERR_ARG, "gk_eval: desc, weights and scalar required");
   GK_REQUIRE(d->precision == GK_F32 || d->precision == GK_F64, GK_ERR_ARG, "gk_eval: bad precision");
   GK_REQUIRE(d->n_thresholds >= 0 && d->n_thresholds <= GK_MAX_THRESHOLDS, GK_ERR_ARG,
@@ -760,7 +771,7 @@ static int eval_impl(gk_handle h, const gk_system* sys, const double* weights, g
     masked = needs_mask(sys, lo, hi, sl, sh);
   }
   GK_TRY(run_eval(h, a, d->kind == GK_POINTS_LATTICE, d->precision, d->gradient != nullptr, masked, st, ext != nullptr,
-                  part, part_bytes));
+                  part, part_bytes, grad_only));
   GK_HIP(hipGetLastError());
   return GK_OK;
 }
@@ -880,9 +891,9 @@ extern "C" int gk_matvec_impl(gk_handle h, const gk_system* sys, int32_t precisi
     d.precision = precision;
     d.n_points = no;
     d.xyz = sys->ori_xyz;
-    d.scalar = s_ori;
+    d.scalar = s_ori;             // drift terms only: the rows take the gradient
     d.gradient = g_ori;
-    GK_TRY(gk_eval_impl(h, &nofault, x, &d, st, src_box == nullptr, src_box, R + S > 0));
+    GK_TRY(eval_impl(h, &nofault, x, &d, st, src_box == nullptr, src_box, R + S > 0, nullptr, 0, true));
   }
   const int64_t NG = 3 * no;
   if (NG + R > 0)
