@@ -797,14 +797,20 @@ __global__ void matvec_combine_kernel(gk_system sys, const double* x, const doub
 // Drift / fault rows: deterministic block reductions, one block per row.
 // Drift rows: y[NG+R+l] = sum over the orientation and interface unknowns of
 // the drift function's derivative / difference times x (one workgroup per row).
-__global__ __launch_bounds__(1024) void matvec_drift_rows_kernel(gk_system sys, const double* x, double* y) {
-  __shared__ double red[16];
-  const int l = blockIdx.x;                  // 0..L+F-1
+// Drift and fault rows: block (l, c) adds chunk c of row l's terms (the
+// orientations and rest points split into DRIFT_CHUNKS ranges) into
+// part[l DRIFT_CHUNKS + c]; matvec_drift_sum_kernel adds the chunks in order.
+constexpr int DRIFT_CHUNKS = 16;
+__global__ __launch_bounds__(256) void matvec_drift_rows_kernel(gk_system sys, const double* x, double* part) {
+  __shared__ double red[4];
+  const int l = blockIdx.x, c = blockIdx.y;  // row 0..L+F-1, chunk
   const int64_t no = sys.n_ori, R = sys.n_rest, NG = 3 * no;
   const int L = sys.uni_degree == 0 ? 0 : (sys.uni_degree == 1 ? 3 : 9);
+  const int64_t g0 = no * c / DRIFT_CHUNKS, g1 = no * (c + 1) / DRIFT_CHUNKS;
+  const int64_t r0 = R * c / DRIFT_CHUNKS, r1 = R * (c + 1) / DRIFT_CHUNKS;
   double acc = 0.0;
   if (l < L) {
-    for (int64_t g = threadIdx.x; g < no; g += 1024) {
+    for (int64_t g = g0 + threadIdx.x; g < g1; g += 256) {
       const double* q = sys.ori_xyz + 3 * g;
 #pragma unroll
       for (int u = 0; u < 3; ++u) {
@@ -817,7 +823,7 @@ __global__ __launch_bounds__(1024) void matvec_drift_rows_kernel(gk_system sys, 
         acc += df * x[u * no + g];
       }
     }
-    for (int64_t r = threadIdx.x; r < R; r += 1024) {
+    for (int64_t r = r0 + threadIdx.x; r < r1; r += 256) {
       const double* p = sys.rest_xyz + 3 * r;
       const double* p0 = sys.ref_xyz + 3 * sys.rest_surf[r];
       double f1, f0;
@@ -829,17 +835,21 @@ __global__ __launch_bounds__(1024) void matvec_drift_rows_kernel(gk_system sys, 
       acc += sys.gi_res * (f1 - f0) * x[NG + r];
     }
   } else {
-    for (int64_t r = threadIdx.x; r < R; r += 1024)
+    for (int64_t r = r0 + threadIdx.x; r < r1; r += 256)
       acc += sys.gi_res * sys.fault_rest[r * sys.n_fault + (l - L)] * x[NG + r];
   }
   for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
   __syncthreads();
-  if (threadIdx.x == 0) {
-    double t = 0.0;
-    for (int w = 0; w < 16; ++w) t += red[w];
-    y[NG + R + l] = t;
-  }
+  if (threadIdx.x == 0) part[l * DRIFT_CHUNKS + c] = (red[0] + red[1]) + (red[2] + red[3]);
+}
+
+__global__ void matvec_drift_sum_kernel(const double* part, int rows, double* y) {
+  const int l = threadIdx.x;
+  if (l >= rows) return;
+  double t = 0.0;
+  for (int c = 0; c < DRIFT_CHUNKS; ++c) t += part[l * DRIFT_CHUNKS + c];
+  y[l] = t;
 }
 
 // Fault-drift contribution to the interface rows (not part of the field eval).
@@ -865,12 +875,13 @@ __global__ void gather_data_points_kernel(gk_system sys, double* pts) {
 
 extern "C" int gk_matvec_impl(gk_handle h, const gk_system* sys, int32_t precision, const double* x,
                               double* y, hipStream_t st, void* scratch, const double* src_box) {
-  // scratch: >= (R+S)*3 + (R+S) + n_ori*4 doubles
+  // scratch: >= (R+S)*3 + (R+S) + n_ori*4 + 17 DRIFT_CHUNKS doubles
   const int64_t R = sys->n_rest, S = sys->n_surf, no = sys->n_ori;
   double* pts = static_cast<double*>(scratch);
   double* s_data = pts + 3 * (R + S);
   double* s_ori = s_data + (R + S);
   double* g_ori = s_ori + no;
+  double* d_part = g_ori + 3 * no;          // drift / fault row chunks
   if (R + S > 0)
     hipLaunchKernelGGL(gather_data_points_kernel, dim3((unsigned)gk_ceil_div(R + S, 256)), dim3(256),
                        0, st, *sys, pts);
@@ -903,14 +914,17 @@ extern "C" int gk_matvec_impl(gk_handle h, const gk_system* sys, int32_t precisi
     hipLaunchKernelGGL(matvec_fault_rows_kernel, dim3((unsigned)gk_ceil_div(R, 256)), dim3(256), 0,
                        st, *sys, x, y);
   const int L = n_drift_of(sys->uni_degree);
-  if (L + sys->n_fault > 0)
-    hipLaunchKernelGGL(matvec_drift_rows_kernel, dim3(L + sys->n_fault), dim3(1024), 0, st, *sys, x, y);
+  if (L + sys->n_fault > 0) {
+    hipLaunchKernelGGL(matvec_drift_rows_kernel, dim3(L + sys->n_fault, DRIFT_CHUNKS), dim3(256), 0, st, *sys, x,
+                       d_part);
+    hipLaunchKernelGGL(matvec_drift_sum_kernel, dim3(1), dim3(32), 0, st, d_part, L + sys->n_fault, y + NG + R);
+  }
   GK_HIP(hipGetLastError());
   return GK_OK;
 }
 
 extern "C" size_t gk_matvec_scratch_doubles(const gk_system* sys) {
-  return (size_t)(4 * (sys->n_rest + sys->n_surf) + 4 * sys->n_ori + 64);
+  return (size_t)(4 * (sys->n_rest + sys->n_surf) + 4 * sys->n_ori + 17 * DRIFT_CHUNKS + 64);
 }
 
 // ------------------------------------------------------------ stack combination
